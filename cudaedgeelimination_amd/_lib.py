@@ -1,0 +1,181 @@
+"""ctypes binding of ``libee_hip.so`` (the C ABI in ``include/ee_hip.h`` and
+``include/ee_gen.h``).
+
+The library is loaded from this package directory only.  There is no CPU
+fallback: when the library is missing every entry point raises
+``BackendMissing``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+import threading
+
+import numpy as np
+
+LIBRARY_PATH = pathlib.Path(__file__).resolve().with_name("libee_hip.so")
+
+EE_OK = 0
+STATUS_NAMES = {
+    1: "EE_ERR_INVALID_ARGUMENT",
+    2: "EE_ERR_OVERFLOW",
+    3: "EE_ERR_RUNTIME",
+    4: "EE_ERR_LOGIC",
+    5: "EE_ERR_UNAVAILABLE",
+}
+
+
+class BackendMissing(RuntimeError):
+    """The HIP backend library has not been built."""
+
+
+class EeError(RuntimeError):
+    """A C entry point returned a non-OK status."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+        self.message = message
+
+
+class InvalidArgument(EeError, ValueError):
+    pass
+
+
+class OverflowFailure(EeError, OverflowError):
+    pass
+
+
+class Unavailable(EeError):
+    pass
+
+
+# ---------------------------------------------------------------- structs
+KOPT_TASK_DTYPE = np.dtype([("port_nodes", np.int32, (10,)), ("deleted_cost", np.int64)],
+                           align=True)
+assert KOPT_TASK_DTYPE.itemsize == 48
+MATCHING_DTYPE = np.dtype([("endpoint_count", np.uint8), ("mate", np.uint8, (14,))])
+assert MATCHING_DTYPE.itemsize == 15
+
+
+class GraphView(C.Structure):
+    _fields_ = [
+        ("dimension", C.c_int32),
+        ("distance_type", C.c_uint8),
+        ("integer_coordinates", C.c_uint8),
+        ("reserved", C.c_uint8 * 2),
+        ("x", C.c_void_p),
+        ("y", C.c_void_p),
+        ("edge_count", C.c_int32),
+        ("edge_u", C.c_void_p),
+        ("edge_v", C.c_void_p),
+        ("edge_weight", C.c_void_p),
+        ("edge_active", C.c_void_p),
+        ("row_offsets", C.c_void_p),
+        ("neighbors", C.c_void_p),
+        ("csr_edge_ids", C.c_void_p),
+    ]
+
+
+class JvUsage(C.Structure):
+    _fields_ = [
+        ("static_hit", C.c_int32),
+        ("workspace_hit", C.c_int32),
+        ("resident_bytes", C.c_uint64),
+        ("h2d_ms", C.c_double),
+        ("kernel_ms", C.c_double),
+        ("d2h_ms", C.c_double),
+    ]
+
+
+class KoptUsage(C.Structure):
+    _fields_ = [
+        ("snapshot_hit", C.c_int32),
+        ("template_hit", C.c_int32),
+        ("workspace_hit", C.c_int32),
+        ("reserved", C.c_int32),
+        ("resident_bytes", C.c_uint64),
+        ("h2d_ms", C.c_double),
+        ("kernel_ms", C.c_double),
+        ("d2h_ms", C.c_double),
+    ]
+
+
+P = C.c_void_p
+I32, U32, I64, U64, U8, SZ, DBL = (C.c_int32, C.c_uint32, C.c_int64, C.c_uint64, C.c_uint8,
+                                   C.c_size_t, C.c_double)
+
+# name -> (restype, argtypes)
+PROTOTYPES = {
+    "ee_last_error": (C.c_char_p, []),
+    "ee_version": (C.c_char_p, []),
+    "ee_device_count": (C.c_int, []),
+    "ee_backend_available": (C.c_int, [C.c_int, C.c_char_p, SZ]),
+    "ee_ctx_create": (C.c_int, [C.c_int, C.POINTER(P)]),
+    "ee_ctx_destroy": (None, [P]),
+    "ee_ctx_device": (C.c_int, [P]),
+    "ee_jv_find_candidates": (C.c_int, [P, C.POINTER(GraphView), P, C.POINTER(JvUsage)]),
+    "ee_jv_clear_cache": (C.c_int, [P]),
+    "ee_kopt_build_templates": (C.c_int, [U32, P, U32, C.POINTER(U32), C.POINTER(U64)]),
+    "ee_kopt_eval_costs": (C.c_int, [P, P, P, I32, U8, U32, P, U32, U64, P, SZ, P,
+                                     C.POINTER(KoptUsage)]),
+    "ee_kopt_clear_cache": (C.c_int, [P]),
+    "ee_gen_points_uniform": (C.c_int, [U64, I32, I64, I64, P, P]),
+    "ee_gen_points_lattice": (C.c_int, [U64, I32, I32, I64, DBL, I64, C.POINTER(I32), P, P]),
+    "ee_gen_points_clustered": (C.c_int, [U64, I32, I32, DBL, I64, I64, P, P]),
+    "ee_gen_knn_graph": (C.c_int, [P, P, I32, U8, I32, P, P, P, I64, C.POINTER(I64)]),
+    "ee_graph_build_csr": (C.c_int, [I32, I32, P, P, P, P, P, P]),
+    "ee_graph_content_hash": (U64, [I32, U8, P, P, I32, P, P, P, P]),
+    "ee_gen_kopt_tasks": (C.c_int, [U64, P, P, I32, U8, P, P, U32, P, U64, P]),
+    "ee_gen_shuffled_ks": (C.c_int, [U64, P, P]),
+    "ee_isqrt_host": (None, [P, P, SZ]),
+    "ee_isqrt_device": (C.c_int, [P, P, P, SZ]),
+    "ee_distances_device": (C.c_int, [P, P, P, I32, U8, P, P, SZ, P]),
+}
+
+_lock = threading.Lock()
+_lib: C.CDLL | None = None
+
+
+def lib() -> C.CDLL:
+    """The loaded backend library; raises ``BackendMissing`` when absent."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIBRARY_PATH.exists():
+                raise BackendMissing(
+                    f"{LIBRARY_PATH} is missing: build the HIP backend first "
+                    "(python -c 'import __graft_entry__ as g; g.build()')")
+            handle = C.CDLL(str(LIBRARY_PATH))
+            for name, (restype, argtypes) in PROTOTYPES.items():
+                fn = getattr(handle, name)
+                fn.restype = restype
+                fn.argtypes = argtypes
+            _lib = handle
+        return _lib
+
+
+def last_error() -> str:
+    return lib().ee_last_error().decode(errors="replace")
+
+
+def check(status: int) -> None:
+    if status == EE_OK:
+        return
+    message = last_error()
+    if status == 1:
+        raise InvalidArgument(status, message)
+    if status == 2:
+        raise OverflowFailure(status, message)
+    if status == 5:
+        raise Unavailable(status, message)
+    raise EeError(status, message)
+
+
+def ptr(array: np.ndarray | None) -> int | None:
+    """Address of a C-contiguous numpy array (None for None)."""
+    if array is None:
+        return None
+    if not array.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return array.ctypes.data

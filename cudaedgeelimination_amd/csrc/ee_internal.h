@@ -1,0 +1,105 @@
+// Internal declarations shared by the HIP kernels and the host runtime.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "ee_hip.h"
+
+namespace ee {
+
+// --------------------------------------------------------------- errors
+// Thrown types map onto ee_status codes at the C boundary.
+struct Unavailable : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void CheckHip(hipError_t status, const char* what);
+
+// ------------------------------------------------------------ k-opt plan
+// Per-k reconnect plan derived from a template table: the distinct port pairs
+// (p < q) the templates use, and for every template its k pair indices in
+// ascending port order (the order the reference visits added edges,
+// src/cuda/kopt_cost.cu:370-377).
+constexpr int kMaxK = 5;
+constexpr int kMaxPorts = 2 * kMaxK;
+constexpr int kMaxPairs = kMaxPorts * (kMaxPorts - 1) / 2;  // 45
+constexpr int kMaxTemplates = 945;                           // (2k-1)!! for k = 5
+
+struct KoptPlan {
+  uint32_t k = 0;
+  uint32_t templates = 0;
+  uint32_t pairs = 0;
+  uint32_t reserved = 0;
+  uint8_t pair_p[kMaxPairs] = {};
+  uint8_t pair_q[kMaxPairs] = {};
+  uint8_t tpl[kMaxTemplates * kMaxK] = {};
+};
+
+// One rectangular block of the cost output: task rows [task_offset,
+// task_offset + task_count) of instance `instance` evaluated against the plan
+// for k, written row-major at cells [cost_offset, cost_offset + task_count*T).
+struct KoptSegment {
+  uint32_t instance;
+  uint32_t k;
+  uint64_t task_offset;
+  uint64_t task_count;
+  uint64_t cost_offset;
+  uint64_t tile_begin;  // first global tile index of this segment
+  uint32_t rows_per_tile;
+  uint32_t reserved;
+};
+
+// Coordinates of one instance: xy[base .. base + n) in the coordinate pool.
+struct KoptInstance {
+  uint64_t coord_base;
+  uint32_t n;
+  uint32_t distance_type;
+};
+
+struct KoptLaunch {
+  const longlong2* xy;         // coordinate pool
+  const KoptInstance* instances;
+  const KoptSegment* segments;
+  uint32_t segment_count;
+  uint32_t reserved;
+  uint64_t tile_count;
+  const ee_kopt_task* tasks;   // task pool
+  const KoptPlan* plans;       // [3] for k = 3, 4, 5
+  int64_t* costs;              // cost pool
+  const uint32_t* tile_segment;  // [tile_count] segment index of each tile
+};
+
+// Rows per tile for each k; `grid` = 0 launches one block per tile.
+struct KoptTuning {
+  uint32_t rows_per_tile[3] = {128, 32, 8};
+  uint32_t grid = 0;
+  uint32_t block = 256;
+  bool non_temporal = false;
+};
+
+size_t KoptSharedBytes(uint32_t k, uint32_t rows_per_tile, uint32_t pairs, uint32_t templates);
+void LaunchKopt(const KoptLaunch& args, const KoptTuning& tuning, size_t shared_bytes,
+                hipStream_t stream);
+
+// ------------------------------------------------------------------- JV
+struct JvLaunch {
+  int32_t edge_count;
+  int32_t distance_type;
+  const int2* edge_uv;          // [m]
+  const int64_t* edge_weight;   // [m]
+  const int32_t* edge_active;   // [m]
+  const int32_t* row_offsets;   // [n+1]
+  const int32_t* neighbors;     // [A]
+  const int32_t* csr_edge_ids;  // [A]
+  const longlong2* xy;          // [n]
+  int32_t* witnesses;           // [m]
+};
+
+void LaunchJv(const JvLaunch& args, uint32_t grid, hipStream_t stream);
+
+}  // namespace ee

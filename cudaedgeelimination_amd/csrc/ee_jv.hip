@@ -1,0 +1,223 @@
+// Jonker-Volgenant fast-elimination edge test on gfx950.
+//
+// Reference kernel: JvCandidatesKernel (src/cuda/jv_cuda.cu:221-288), one
+// thread per edge slot walking N(a), N(b) and every candidate's row serially.
+//
+// Here one 64-lane wavefront owns one edge slot (a, b):
+//   1. candidate scoring (jv_elimination.cpp:45-84): lane s scores the s-th
+//      entry of N(a) followed by N(b); x in {a, b} is skipped, and on the b
+//      side x is skipped when (a, x) is an edge (the reference's HasActiveEdge
+//      binary search; here a readlane scan over the a-side lanes when both
+//      rows fit one pass).  score = w(side, x) + D(x, other).
+//   2. top ten by (score, node): every valid lane computes its rank by
+//      counting smaller keys with readlane; ranks < 10 land in LDS in
+//      ascending order.  Rows longer than one pass merge into the running list.
+//   3. witness test (jv_elimination.cpp:86-117): candidates in rank order; for
+//      candidate c, lane j tests the j-th neighbour d of c; a ballot tells
+//      whether any d is compatible (w_ab + w_cd <= D(a,c) + D(d,b) or
+//      w_ab + w_cd <= D(a,d) + D(b,c)).  The first c with an empty ballot is
+//      the witness.  Testing every d at once gives the same verdict as the
+//      reference's early exit at the first compatible d.
+// Sums stay exact in uint64: the host guarantees weights <= 2^62 and
+// distances <= 2^32, so no sum wraps and unsigned comparison equals the
+// reference CPU's 128-bit comparison.
+#include "ee_exact.h"
+#include "ee_internal.h"
+
+namespace ee {
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kCandidates = 10;  // jv_elimination.cpp:28, jv_cuda.cu:21
+
+__device__ __forceinline__ void WaveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ bool KeyLess(uint64_t sa, int32_t na, uint64_t sb, int32_t nb) {
+  return sa < sb || (sa == sb && na < nb);
+}
+
+__device__ __forceinline__ int32_t ReadLane(int32_t v, int lane) {
+  return __builtin_amdgcn_readlane(v, lane);
+}
+
+__device__ __forceinline__ uint64_t ReadLane64(uint64_t v, int lane) {
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), lane));
+  const uint32_t hi =
+      static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v >> 32), lane));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// Number of lanes j < lanes with bit j of `mask` set and key_j < (s, n).
+__device__ __forceinline__ int CountLessInWave(uint64_t mask, int lanes, uint64_t my_s,
+                                               int32_t my_n, uint64_t s, int32_t n) {
+  int count = 0;
+  for (int j = 0; j < lanes; ++j) {
+    if (((mask >> j) & 1ull) == 0) continue;  // uniform branch
+    const uint64_t js = ReadLane64(my_s, j);
+    const int32_t jn = ReadLane(my_n, j);
+    count += KeyLess(js, jn, s, n) ? 1 : 0;
+  }
+  return count;
+}
+
+__device__ __forceinline__ int64_t Dist(const longlong2 p, const longlong2 q, int type) {
+  return ExactDistance(p.x, p.y, q.x, q.y, type);
+}
+
+__global__ void __launch_bounds__(kBlock) JvKernel(JvLaunch g) {
+  __shared__ uint64_t s_score[kWavesPerBlock][2][kCandidates];
+  __shared__ int32_t s_node[kWavesPerBlock][2][kCandidates];
+  __shared__ int64_t s_cac[kWavesPerBlock][kCandidates];
+  __shared__ int64_t s_cbc[kWavesPerBlock][kCandidates];
+
+  const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+  const int wib = static_cast<int>(threadIdx.x) / kWave;
+  const int32_t waves = static_cast<int32_t>(gridDim.x) * kWavesPerBlock;
+  const int type = g.distance_type;
+
+  for (int32_t e = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock + wib);
+       e < g.edge_count; e += waves) {
+    int32_t result = -1;
+    const int2 uv = g.edge_uv[e];
+    const int32_t a = uv.x, b = uv.y;
+    const int32_t ra0 = g.row_offsets[a], ra1 = g.row_offsets[a + 1];
+    const int32_t rb0 = g.row_offsets[b], rb1 = g.row_offsets[b + 1];
+    const int32_t deg_a = ra1 - ra0, deg_b = rb1 - rb0;
+    if (g.edge_active[e] != 0 && deg_a > 2 && deg_b > 2) {
+      const longlong2 pa = g.xy[a], pb = g.xy[b];
+      const uint64_t wab = static_cast<uint64_t>(g.edge_weight[e]);
+
+      // ---- 1+2: score N(a) then N(b), keep the ten smallest (score, node)
+      const int32_t total = deg_a + deg_b;
+      const bool one_pass = total <= kWave;
+      int cnt = 0, cur = 0;
+      for (int32_t base = 0; base < total; base += kWave) {
+        const int32_t s = base + lane;
+        bool valid = false;
+        uint64_t sc = ~0ull;
+        int32_t x = INT32_MAX;
+        int32_t raw = -1;
+        const bool side_b = s >= deg_a;
+        int32_t slot = 0;
+        if (s < total) {
+          slot = side_b ? rb0 + (s - deg_a) : ra0 + s;
+          raw = g.neighbors[slot];
+        }
+        bool skip = s >= total || raw == a || raw == b;
+        if (one_pass) {
+          // every lane joins the (uniform) scan; lanes 0..deg_a-1 hold N(a)
+          bool in_a = false;
+          for (int j = 0; j < deg_a; ++j) in_a |= ReadLane(raw, j) == raw;
+          skip |= side_b && in_a;
+        } else if (side_b && !skip) {
+          {
+            int32_t lo = ra0, hi = ra1;
+            while (lo < hi) {
+              const int32_t mid = lo + (hi - lo) / 2;
+              if (g.neighbors[mid] < raw) {
+                lo = mid + 1;
+              } else {
+                hi = mid;
+              }
+            }
+            skip = lo < ra1 && g.neighbors[lo] == raw;
+          }
+        }
+        if (!skip) {
+          const longlong2 px = g.xy[raw];
+          sc = static_cast<uint64_t>(g.edge_weight[g.csr_edge_ids[slot]]) +
+               static_cast<uint64_t>(Dist(px, side_b ? pa : pb, type));
+          x = raw;
+          valid = true;
+        }
+        const uint64_t vmask = __ballot(valid);
+        const int lanes = total - base < kWave ? total - base : kWave;
+        const int npass = __popcll(vmask);
+        // rank among this pass plus the running list
+        int rank = CountLessInWave(vmask, lanes, sc, x, sc, x);
+        for (int j = 0; j < cnt; ++j) {
+          rank += KeyLess(s_score[wib][cur][j], s_node[wib][cur][j], sc, x) ? 1 : 0;
+        }
+        // new positions of the running list entries
+        uint64_t ls = ~0ull;
+        int32_t ln = INT32_MAX;
+        int lrank = kCandidates;
+        if (lane < cnt) {
+          ls = s_score[wib][cur][lane];
+          ln = s_node[wib][cur][lane];
+        }
+        {
+          const int c = CountLessInWave(vmask, lanes, sc, x, ls, ln);
+          if (lane < cnt) lrank = lane + c;
+        }
+        const int nxt = cur ^ 1;
+        if (valid && rank < kCandidates) {
+          s_score[wib][nxt][rank] = sc;
+          s_node[wib][nxt][rank] = x;
+        }
+        if (lane < cnt && lrank < kCandidates) {
+          s_score[wib][nxt][lrank] = ls;
+          s_node[wib][nxt][lrank] = ln;
+        }
+        WaveSync();
+        cnt = cnt + npass < kCandidates ? cnt + npass : kCandidates;
+        cur = nxt;
+      }
+
+      // ---- 3: witness test in candidate order
+      if (lane < cnt) {
+        const longlong2 pc = g.xy[s_node[wib][cur][lane]];
+        s_cac[wib][lane] = Dist(pa, pc, type);
+        s_cbc[wib][lane] = Dist(pb, pc, type);
+      }
+      WaveSync();
+      for (int i = 0; i < cnt; ++i) {
+        const int32_t c = s_node[wib][cur][i];
+        const uint64_t cac = static_cast<uint64_t>(s_cac[wib][i]);
+        const uint64_t cbc = static_cast<uint64_t>(s_cbc[wib][i]);
+        const int32_t rc0 = g.row_offsets[c], rc1 = g.row_offsets[c + 1];
+        bool compatible = false;
+        for (int32_t base = rc0; base < rc1 && !compatible; base += kWave) {
+          const int32_t slot = base + lane;
+          bool hit = false;
+          if (slot < rc1) {
+            const int32_t d = g.neighbors[slot];
+            if (d != a && d != b) {
+              const longlong2 pd = g.xy[d];
+              const uint64_t lhs = wab + static_cast<uint64_t>(g.edge_weight[g.csr_edge_ids[slot]]);
+              const uint64_t first = cac + static_cast<uint64_t>(Dist(pd, pb, type));
+              const uint64_t second = static_cast<uint64_t>(Dist(pa, pd, type)) + cbc;
+              hit = lhs <= first || lhs <= second;
+            }
+          }
+          compatible = __ballot(hit) != 0;
+        }
+        if (!compatible) {
+          result = c;
+          break;
+        }
+      }
+      WaveSync();
+    }
+    if (lane == 0) g.witnesses[e] = result;
+  }
+}
+
+}  // namespace
+
+void LaunchJv(const JvLaunch& args, uint32_t grid, hipStream_t stream) {
+  if (args.edge_count <= 0) return;
+  const uint64_t needed = (static_cast<uint64_t>(args.edge_count) + kWavesPerBlock - 1) / kWavesPerBlock;
+  uint64_t blocks = grid == 0 ? needed : grid;
+  if (blocks > needed) blocks = needed;
+  hipLaunchKernelGGL(JvKernel, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, stream, args);
+  CheckHip(hipGetLastError(), "JV kernel launch");
+}
+
+}  // namespace ee

@@ -1,0 +1,172 @@
+// Host-side runtime types: device arrays, per-context caches, the context.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <chrono>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "ee_internal.h"
+
+namespace ee {
+
+void SetLastError(const std::string& message);
+const char* LastError();
+// Maps the in-flight exception onto a status code (call inside catch).
+ee_status StatusFromException();
+double MsSince(std::chrono::steady_clock::time_point t0);
+size_t Grow(size_t current, size_t required);
+bool DistanceSafe(const int64_t* x, const int64_t* y, int32_t n);
+int CountDevices();
+
+// Owning device allocation of `count` elements of T on one device.
+template <typename T> class DeviceArray {
+ public:
+  DeviceArray() = default;
+  DeviceArray(size_t count, int device) { Allocate(count, device); }
+  ~DeviceArray() { Reset(); }
+  DeviceArray(const DeviceArray&) = delete;
+  DeviceArray& operator=(const DeviceArray&) = delete;
+  DeviceArray(DeviceArray&& o) noexcept { Take(&o); }
+  DeviceArray& operator=(DeviceArray&& o) noexcept {
+    if (this != &o) {
+      Reset();
+      Take(&o);
+    }
+    return *this;
+  }
+
+  void Allocate(size_t count, int device) {
+    Reset();
+    if (count == 0) return;
+    if (count > SIZE_MAX / sizeof(T)) throw std::overflow_error("device allocation size overflows");
+    CheckHip(hipSetDevice(device), "hipSetDevice");
+    void* p = nullptr;
+    const hipError_t status = hipMalloc(&p, count * sizeof(T));
+    if (status != hipSuccess) {
+      throw std::runtime_error(std::string("hipMalloc(") + std::to_string(count * sizeof(T)) +
+                               " bytes): " + hipGetErrorString(status));
+    }
+    ptr_ = static_cast<T*>(p);
+    count_ = count;
+    device_ = device;
+  }
+
+  void Upload(const T* host, size_t count, hipStream_t stream) { UploadAt(0, host, count, stream); }
+  void UploadAt(size_t offset, const T* host, size_t count, hipStream_t stream) {
+    if (count == 0) return;
+    if (offset > count_ || count > count_ - offset) throw std::logic_error("H2D beyond device buffer");
+    CheckHip(hipMemcpyAsync(ptr_ + offset, host, count * sizeof(T), hipMemcpyHostToDevice, stream),
+             "hipMemcpyAsync H2D");
+  }
+  void Download(T* host, size_t count, hipStream_t stream) const {
+    DownloadAt(0, host, count, stream);
+  }
+  void DownloadAt(size_t offset, T* host, size_t count, hipStream_t stream) const {
+    if (count == 0) return;
+    if (offset > count_ || count > count_ - offset) throw std::logic_error("D2H beyond device buffer");
+    CheckHip(hipMemcpyAsync(host, ptr_ + offset, count * sizeof(T), hipMemcpyDeviceToHost, stream),
+             "hipMemcpyAsync D2H");
+  }
+
+  T* get() { return ptr_; }
+  const T* get() const { return ptr_; }
+  size_t count() const { return count_; }
+  uint64_t bytes() const { return static_cast<uint64_t>(count_) * sizeof(T); }
+
+ private:
+  void Reset() noexcept {
+    if (ptr_ != nullptr) {
+      static_cast<void>(hipSetDevice(device_));
+      static_cast<void>(hipFree(ptr_));
+    }
+    ptr_ = nullptr;
+    count_ = 0;
+    device_ = -1;
+  }
+  void Take(DeviceArray* o) noexcept {
+    ptr_ = o->ptr_;
+    count_ = o->count_;
+    device_ = o->device_;
+    o->ptr_ = nullptr;
+    o->count_ = 0;
+    o->device_ = -1;
+  }
+
+  T* ptr_ = nullptr;
+  size_t count_ = 0;
+  int device_ = -1;
+};
+
+struct JvCache {
+  bool valid = false;
+  int32_t dimension = 0;
+  uint8_t distance_type = 0;
+  // exact static key (host copies)
+  std::vector<int64_t> x, y, edge_weight;
+  std::vector<int32_t> edge_u, edge_v;
+  std::vector<int32_t> host_active;
+  // static: 8m + 8m + 16n bytes
+  DeviceArray<int2> d_uv;
+  DeviceArray<int64_t> d_weight;
+  DeviceArray<longlong2> d_xy;
+  // dynamic: 4m + 4(n+1) + 4A + 4A + 4m bytes
+  DeviceArray<int32_t> d_active;
+  DeviceArray<int32_t> d_row_offsets;
+  DeviceArray<int32_t> d_neighbors;
+  DeviceArray<int32_t> d_edge_ids;
+  DeviceArray<int32_t> d_witnesses;
+};
+
+struct KoptTables {
+  KoptInstance instance;
+  KoptSegment segment;
+};
+
+struct KoptCache {
+  struct TemplateEntry {
+    bool valid = false;
+    uint64_t hash = 0;
+    std::vector<ee_endpoint_matching> table;
+  };
+  bool snapshot_valid = false;
+  int32_t dimension = 0;
+  uint8_t distance_type = 0;
+  std::vector<int64_t> x, y;
+  DeviceArray<longlong2> d_xy;
+  std::array<TemplateEntry, 3> templates;
+  std::array<KoptPlan, 3> host_plans;
+  DeviceArray<KoptPlan> d_plans;
+  DeviceArray<ee_kopt_task> d_tasks;
+  DeviceArray<int64_t> d_costs;
+  DeviceArray<KoptTables> d_tables;
+};
+
+struct Context {
+  explicit Context(int requested_device);
+  ~Context();
+  Context(const Context&) = delete;
+  Context& operator=(const Context&) = delete;
+
+  void Activate() const;
+  void Sync() const;
+
+  int device = -1;
+  int compute_units = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  uint32_t jv_grid = 0;  // 0: one wavefront per edge slot
+  KoptTuning kopt_tuning;
+  JvCache jv;
+  KoptCache kopt;
+};
+
+std::vector<ee_endpoint_matching> CanonicalTemplates(uint32_t k, uint64_t* hash);
+KoptPlan BuildPlan(uint32_t k, const ee_endpoint_matching* templates, uint32_t count);
+void ValidateTasks(int32_t n, uint32_t k, const ee_kopt_task* tasks, size_t count);
+size_t PlanSharedBytes(const KoptPlan& plan, const KoptTuning& tuning);
+
+}  // namespace ee

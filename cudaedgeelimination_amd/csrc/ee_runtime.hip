@@ -1,0 +1,723 @@
+// Host runtime of the gfx950 backend: device selection, per-context
+// device-resident caches, host-side validation and the C entry points of the
+// reference-shaped (synchronous, host buffers in and out) paths.
+//
+// Cache semantics follow the reference so that callers observe the same hit
+// flags and resident byte counts:
+//   JV    (src/cuda/jv_cuda.cu:360-478): the static part (coordinates, edge
+//         endpoints, edge weights) is keyed by an exact element-wise compare;
+//         the dynamic part (active flags, CSR, witnesses) lives in buffers
+//         that grow to max(required, 2 x current).  Resident bytes are
+//         24m + 20n + 4 + 8A on an exact first fit.
+//   k-opt (src/cuda/kopt_cost.cu:205-295): coordinates keyed by exact
+//         compare of (dimension, distance type, x, y); templates keyed per k
+//         by (generator hash, content); task and cost buffers grow the same
+//         way.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "ee_exact.h"
+#include "ee_internal.h"
+#include "ee_runtime.h"
+
+namespace ee {
+
+void CheckHip(hipError_t status, const char* what) {
+  if (status != hipSuccess) {
+    throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(status));
+  }
+}
+
+namespace {
+thread_local std::string g_last_error;
+}  // namespace
+
+void SetLastError(const std::string& message) { g_last_error = message; }
+const char* LastError() { return g_last_error.c_str(); }
+
+ee_status StatusFromException() {
+  try {
+    throw;
+  } catch (const Unavailable& e) {
+    SetLastError(e.what());
+    return EE_ERR_UNAVAILABLE;
+  } catch (const std::invalid_argument& e) {
+    SetLastError(e.what());
+    return EE_ERR_INVALID_ARGUMENT;
+  } catch (const std::overflow_error& e) {
+    SetLastError(e.what());
+    return EE_ERR_OVERFLOW;
+  } catch (const std::logic_error& e) {
+    SetLastError(e.what());
+    return EE_ERR_LOGIC;
+  } catch (const std::bad_alloc&) {
+    SetLastError("host allocation failed");
+    return EE_ERR_RUNTIME;
+  } catch (const std::exception& e) {
+    SetLastError(e.what());
+    return EE_ERR_RUNTIME;
+  } catch (...) {
+    SetLastError("unknown error");
+    return EE_ERR_RUNTIME;
+  }
+}
+
+double MsSince(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+size_t Grow(size_t current, size_t required) {
+  if (current >= required) return current;
+  if (current == 0 || current > SIZE_MAX / 2) return required;
+  return std::max(required, current * 2);
+}
+
+// Bounding box diagonal squared <= UINT64_MAX (src/common/graph.cpp:170-181).
+bool DistanceSafe(const int64_t* x, const int64_t* y, int32_t n) {
+  if (n <= 0) return true;
+  int64_t lx = x[0], hx = x[0], ly = y[0], hy = y[0];
+  for (int32_t i = 1; i < n; ++i) {
+    lx = std::min(lx, x[i]);
+    hx = std::max(hx, x[i]);
+    ly = std::min(ly, y[i]);
+    hy = std::max(hy, y[i]);
+  }
+  const unsigned __int128 dx = static_cast<unsigned __int128>(static_cast<__int128>(hx) - lx);
+  const unsigned __int128 dy = static_cast<unsigned __int128>(static_cast<__int128>(hy) - ly);
+  return dx * dx + dy * dy <= static_cast<unsigned __int128>(UINT64_MAX);
+}
+
+std::vector<longlong2> Interleave(const int64_t* x, const int64_t* y, int32_t n) {
+  std::vector<longlong2> xy(static_cast<size_t>(n));
+  for (int32_t i = 0; i < n; ++i) {
+    xy[static_cast<size_t>(i)].x = x[i];
+    xy[static_cast<size_t>(i)].y = y[i];
+  }
+  return xy;
+}
+
+// ------------------------------------------------------------------ device
+int CountDevices() {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess) return 0;
+  return count;
+}
+
+int SelectDevice(int requested) {
+  const int count = CountDevices();
+  if (count <= 0) throw Unavailable("no visible HIP device");
+  if (requested >= 0) {
+    if (requested >= count) throw Unavailable("requested HIP device ordinal is not visible");
+    CheckHip(hipSetDevice(requested), "hipSetDevice");
+    return requested;
+  }
+  int best = -1;
+  size_t best_free = 0;
+  for (int d = 0; d < count; ++d) {
+    if (hipSetDevice(d) != hipSuccess) continue;
+    size_t free_bytes = 0, total_bytes = 0;
+    if (hipMemGetInfo(&free_bytes, &total_bytes) == hipSuccess &&
+        (best < 0 || free_bytes > best_free)) {
+      best = d;
+      best_free = free_bytes;
+    }
+  }
+  if (best < 0) throw Unavailable("no HIP device could report its free memory");
+  CheckHip(hipSetDevice(best), "hipSetDevice");
+  return best;
+}
+
+Context::Context(int requested) {
+  device = SelectDevice(requested);
+  CheckHip(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+  CheckHip(hipEventCreate(&ev_begin), "hipEventCreate");
+  CheckHip(hipEventCreate(&ev_end), "hipEventCreate");
+  hipDeviceProp_t prop{};
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    compute_units = prop.multiProcessorCount;
+  }
+}
+
+Context::~Context() {
+  static_cast<void>(hipSetDevice(device));
+  static_cast<void>(hipStreamSynchronize(stream));
+  jv = JvCache{};
+  kopt = KoptCache{};
+  static_cast<void>(hipEventDestroy(ev_begin));
+  static_cast<void>(hipEventDestroy(ev_end));
+  static_cast<void>(hipStreamDestroy(stream));
+}
+
+void Context::Activate() const { CheckHip(hipSetDevice(device), "hipSetDevice"); }
+
+void Context::Sync() const { CheckHip(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
+
+// ---------------------------------------------------------------------- JV
+namespace {
+
+void ValidateStaticJv(const ee_graph_view& g) {
+  const int32_t n = g.dimension, m = g.edge_count;
+  for (int32_t e = 0; e < m; ++e) {
+    if (g.edge_u[e] < 0 || g.edge_u[e] >= n || g.edge_v[e] < 0 || g.edge_v[e] >= n) {
+      throw std::invalid_argument("JV edge endpoint out of range");
+    }
+    if (g.edge_weight[e] < 0 || g.edge_weight[e] > (int64_t{1} << 62)) {
+      throw std::invalid_argument("JV edge weight outside [0, 2^62]");
+    }
+  }
+  if (!DistanceSafe(g.x, g.y, n)) {
+    throw std::runtime_error("HIP JV needs integer coordinates whose squared distances fit uint64");
+  }
+}
+
+void ValidateDynamicJv(const ee_graph_view& g, int64_t* adjacency) {
+  const int32_t n = g.dimension, m = g.edge_count;
+  if (g.row_offsets[0] != 0) throw std::invalid_argument("CSR row_offsets[0] must be 0");
+  for (int32_t v = 0; v < n; ++v) {
+    if (g.row_offsets[v + 1] < g.row_offsets[v]) {
+      throw std::invalid_argument("CSR row_offsets must be non-decreasing");
+    }
+  }
+  const int64_t a = g.row_offsets[n];
+  for (int64_t i = 0; i < a; ++i) {
+    if (g.neighbors[i] < 0 || g.neighbors[i] >= n) {
+      throw std::invalid_argument("CSR neighbour out of range");
+    }
+    if (g.csr_edge_ids[i] < 0 || g.csr_edge_ids[i] >= m) {
+      throw std::invalid_argument("CSR edge id out of range");
+    }
+  }
+  *adjacency = a;
+}
+
+bool StaticMatches(const JvCache& c, const ee_graph_view& g) {
+  const size_t n = static_cast<size_t>(g.dimension), m = static_cast<size_t>(g.edge_count);
+  return c.valid && c.dimension == g.dimension && c.distance_type == g.distance_type &&
+         c.x.size() == n && c.edge_u.size() == m &&
+         std::memcmp(c.x.data(), g.x, n * 8) == 0 && std::memcmp(c.y.data(), g.y, n * 8) == 0 &&
+         std::memcmp(c.edge_u.data(), g.edge_u, m * 4) == 0 &&
+         std::memcmp(c.edge_v.data(), g.edge_v, m * 4) == 0 &&
+         std::memcmp(c.edge_weight.data(), g.edge_weight, m * 8) == 0;
+}
+
+}  // namespace
+
+uint64_t JvResidentBytes(const JvCache& c) {
+  return c.d_uv.bytes() + c.d_weight.bytes() + c.d_xy.bytes() + c.d_active.bytes() +
+         c.d_row_offsets.bytes() + c.d_neighbors.bytes() + c.d_edge_ids.bytes() +
+         c.d_witnesses.bytes();
+}
+
+// PrepareStaticGraph (jv_cuda.cu:384-430).  Returns true on a cache hit.
+bool PrepareJvStatic(Context& ctx, const ee_graph_view& g) {
+  JvCache& c = ctx.jv;
+  if (StaticMatches(c, g)) return true;
+  ValidateStaticJv(g);
+  const size_t n = static_cast<size_t>(g.dimension), m = static_cast<size_t>(g.edge_count);
+  std::vector<int2> uv(m);
+  for (size_t e = 0; e < m; ++e) uv[e] = make_int2(g.edge_u[e], g.edge_v[e]);
+  const std::vector<longlong2> xy = Interleave(g.x, g.y, g.dimension);
+  DeviceArray<int2> d_uv(m, ctx.device);
+  DeviceArray<int64_t> d_weight(m, ctx.device);
+  DeviceArray<longlong2> d_xy(n, ctx.device);
+  d_uv.Upload(uv.data(), m, ctx.stream);
+  d_weight.Upload(g.edge_weight, m, ctx.stream);
+  d_xy.Upload(xy.data(), n, ctx.stream);
+  ctx.Sync();
+  c.d_uv = std::move(d_uv);
+  c.d_weight = std::move(d_weight);
+  c.d_xy = std::move(d_xy);
+  c.x.assign(g.x, g.x + n);
+  c.y.assign(g.y, g.y + n);
+  c.edge_u.assign(g.edge_u, g.edge_u + m);
+  c.edge_v.assign(g.edge_v, g.edge_v + m);
+  c.edge_weight.assign(g.edge_weight, g.edge_weight + m);
+  c.dimension = g.dimension;
+  c.distance_type = g.distance_type;
+  c.valid = true;
+  return false;
+}
+
+// PrepareWorkspace (jv_cuda.cu:442-470).  Returns true on a cache hit.
+bool PrepareJvWorkspace(Context& ctx, size_t n, size_t m, size_t adjacency) {
+  JvCache& c = ctx.jv;
+  const bool hit = c.d_active.count() >= m && c.d_row_offsets.count() >= n + 1 &&
+                   c.d_neighbors.count() >= adjacency && c.d_edge_ids.count() >= adjacency &&
+                   c.d_witnesses.count() >= m;
+  if (c.d_active.count() < m) c.d_active.Allocate(Grow(c.d_active.count(), m), ctx.device);
+  if (c.d_row_offsets.count() < n + 1) {
+    c.d_row_offsets.Allocate(Grow(c.d_row_offsets.count(), n + 1), ctx.device);
+  }
+  if (c.d_neighbors.count() < adjacency) {
+    c.d_neighbors.Allocate(Grow(c.d_neighbors.count(), adjacency), ctx.device);
+  }
+  if (c.d_edge_ids.count() < adjacency) {
+    c.d_edge_ids.Allocate(Grow(c.d_edge_ids.count(), adjacency), ctx.device);
+  }
+  if (c.d_witnesses.count() < m) c.d_witnesses.Allocate(Grow(c.d_witnesses.count(), m), ctx.device);
+  return hit;
+}
+
+void CheckGraphView(const ee_graph_view* g) {
+  if (g == nullptr) throw std::invalid_argument("graph view is NULL");
+  if (g->dimension <= 0 || g->edge_count < 0) throw std::invalid_argument("bad graph dimensions");
+  if (g->distance_type != EE_EUC_2D && g->distance_type != EE_CEIL_2D) {
+    throw std::invalid_argument("unsupported distance type");
+  }
+  if (g->integer_coordinates == 0) {
+    throw std::runtime_error("HIP JV supports integer coordinates only");
+  }
+  if (g->x == nullptr || g->y == nullptr || g->row_offsets == nullptr ||
+      (g->edge_count > 0 && (g->edge_u == nullptr || g->edge_v == nullptr ||
+                             g->edge_weight == nullptr || g->edge_active == nullptr)) ||
+      (g->row_offsets[g->dimension] > 0 && (g->neighbors == nullptr || g->csr_edge_ids == nullptr))) {
+    throw std::invalid_argument("graph view has a NULL array");
+  }
+}
+
+JvLaunch JvArgs(Context& ctx, int32_t m, int type) {
+  JvCache& c = ctx.jv;
+  JvLaunch a{};
+  a.edge_count = m;
+  a.distance_type = type;
+  a.edge_uv = c.d_uv.get();
+  a.edge_weight = c.d_weight.get();
+  a.edge_active = c.d_active.get();
+  a.row_offsets = c.d_row_offsets.get();
+  a.neighbors = c.d_neighbors.get();
+  a.csr_edge_ids = c.d_edge_ids.get();
+  a.xy = c.d_xy.get();
+  a.witnesses = c.d_witnesses.get();
+  return a;
+}
+
+// FindJvCandidatesCuda (jv_cuda.cu:488-566) without the final compaction.
+void FindJvCandidates(Context& ctx, const ee_graph_view& g, int32_t* witnesses, ee_jv_usage* usage) {
+  ee_jv_usage u{};
+  ctx.Activate();
+  const int32_t m = g.edge_count;
+  if (m == 0) {
+    if (usage) *usage = u;
+    return;
+  }
+  int64_t adjacency = 0;
+  ValidateDynamicJv(g, &adjacency);
+  const bool static_hit = PrepareJvStatic(ctx, g);
+  const bool workspace_hit =
+      PrepareJvWorkspace(ctx, static_cast<size_t>(g.dimension), static_cast<size_t>(m),
+                         static_cast<size_t>(adjacency));
+  JvCache& c = ctx.jv;
+  c.host_active.resize(static_cast<size_t>(m));
+  for (int32_t e = 0; e < m; ++e) c.host_active[static_cast<size_t>(e)] = g.edge_active[e] ? 1 : 0;
+
+  auto t0 = std::chrono::steady_clock::now();
+  c.d_active.Upload(c.host_active.data(), static_cast<size_t>(m), ctx.stream);
+  c.d_row_offsets.Upload(g.row_offsets, static_cast<size_t>(g.dimension) + 1, ctx.stream);
+  c.d_neighbors.Upload(g.neighbors, static_cast<size_t>(adjacency), ctx.stream);
+  c.d_edge_ids.Upload(g.csr_edge_ids, static_cast<size_t>(adjacency), ctx.stream);
+  ctx.Sync();
+  u.h2d_ms = MsSince(t0);
+  u.static_hit = static_hit ? 1 : 0;
+  u.workspace_hit = workspace_hit ? 1 : 0;
+  u.resident_bytes = JvResidentBytes(c);
+
+  t0 = std::chrono::steady_clock::now();
+  LaunchJv(JvArgs(ctx, m, g.distance_type), ctx.jv_grid, ctx.stream);
+  ctx.Sync();
+  u.kernel_ms = MsSince(t0);
+
+  t0 = std::chrono::steady_clock::now();
+  c.d_witnesses.Download(witnesses, static_cast<size_t>(m), ctx.stream);
+  ctx.Sync();
+  u.d2h_ms = MsSince(t0);
+  if (usage) *usage = u;
+}
+
+// ------------------------------------------------------------------- k-opt
+namespace {
+
+// Reference generator (src/cpu/kopt_search.cpp:1143-1193 with
+// src/cpu/path_system.cpp:47-71, 404-421): all perfect matchings of the 2k
+// endpoints in depth-first order (lowest free endpoint paired with each later
+// free endpoint, ascending); keep those that form one alternating cycle with
+// the component matching {2e+1 - 2e+2, 2k-1 - 0} and never pair 2e with 2e+1.
+void Matchings(uint8_t* mate, uint32_t ends, std::vector<std::vector<uint8_t>>* out) {
+  uint32_t first = ends;
+  for (uint32_t i = 0; i < ends; ++i) {
+    if (mate[i] == 0xff) {
+      first = i;
+      break;
+    }
+  }
+  if (first == ends) {
+    out->emplace_back(mate, mate + ends);
+    return;
+  }
+  for (uint32_t second = first + 1; second < ends; ++second) {
+    if (mate[second] != 0xff) continue;
+    mate[first] = static_cast<uint8_t>(second);
+    mate[second] = static_cast<uint8_t>(first);
+    Matchings(mate, ends, out);
+    mate[first] = mate[second] = 0xff;
+  }
+}
+
+bool OneAlternatingCycle(const std::vector<uint8_t>& inside, uint32_t k) {
+  const uint32_t ends = 2 * k;
+  std::vector<uint8_t> outside(ends);
+  for (uint32_t e = 0; e + 1 < k; ++e) {
+    outside[2 * e + 1] = static_cast<uint8_t>(2 * e + 2);
+    outside[2 * e + 2] = static_cast<uint8_t>(2 * e + 1);
+  }
+  outside[2 * k - 1] = 0;
+  outside[0] = static_cast<uint8_t>(2 * k - 1);
+  std::vector<bool> seen(ends, false);
+  uint32_t at = 0, steps = 0;
+  while (!seen[at]) {
+    seen[at] = true;
+    at = (steps % 2 == 0) ? inside[at] : outside[at];
+    ++steps;
+  }
+  return steps == ends && at == 0;
+}
+
+}  // namespace
+
+std::vector<ee_endpoint_matching> CanonicalTemplates(uint32_t k, uint64_t* hash) {
+  if (k < 3 || k > 5) throw std::invalid_argument("reconnect template k must be in [3,5]");
+  const uint32_t ends = 2 * k;
+  std::vector<std::vector<uint8_t>> all;
+  std::vector<uint8_t> mate(ends, 0xff);
+  Matchings(mate.data(), ends, &all);
+  std::vector<ee_endpoint_matching> out;
+  for (const auto& m : all) {
+    bool proper = OneAlternatingCycle(m, k);
+    for (uint32_t e = 0; e < k && proper; ++e) proper = m[2 * e] != 2 * e + 1;
+    if (!proper) continue;
+    ee_endpoint_matching t{};
+    t.endpoint_count = static_cast<uint8_t>(ends);
+    std::memset(t.mate, 0xff, sizeof(t.mate));
+    std::memcpy(t.mate, m.data(), ends);
+    out.push_back(t);
+  }
+  static const size_t kCounts[3] = {4, 25, 208};
+  if (out.size() != kCounts[k - 3]) throw std::logic_error("proper reconnect template count");
+  uint64_t h = 14695981039346656037ULL;
+  const auto byte = [&h](uint8_t b) {
+    h ^= b;
+    h *= 1099511628211ULL;
+  };
+  for (const char ch : std::string("CUDAEE_KOPT_RECONNECT_TEMPLATES_V1")) byte(static_cast<uint8_t>(ch));
+  for (const uint32_t v : {k, static_cast<uint32_t>(out.size())}) {
+    for (int s = 0; s < 32; s += 8) byte(static_cast<uint8_t>((v >> s) & 0xff));
+  }
+  for (const auto& t : out) {
+    for (uint32_t i = 0; i < t.endpoint_count; ++i) byte(t.mate[i]);
+  }
+  if (hash) *hash = h;
+  return out;
+}
+
+// Validates a template table and derives its pair plan.
+KoptPlan BuildPlan(uint32_t k, const ee_endpoint_matching* templates, uint32_t count) {
+  if (k < 3 || k > 5) throw std::invalid_argument("k-opt reconnect table k must be in [3,5]");
+  if (templates == nullptr || count == 0 || count > static_cast<uint32_t>(kMaxTemplates)) {
+    throw std::invalid_argument("k-opt reconnect table is empty or too large");
+  }
+  const uint32_t ends = 2 * k;
+  int pair_index[kMaxPorts][kMaxPorts];
+  for (auto& row : pair_index) std::fill(row, row + kMaxPorts, -1);
+  std::vector<std::pair<uint8_t, uint8_t>> pairs;
+  KoptPlan plan;
+  plan.k = k;
+  plan.templates = count;
+  for (uint32_t t = 0; t < count; ++t) {
+    const ee_endpoint_matching& m = templates[t];
+    if (m.endpoint_count != ends) throw std::invalid_argument("template endpoint count != 2k");
+    uint32_t used = 0;
+    for (uint32_t p = 0; p < ends; ++p) {
+      const uint32_t q = m.mate[p];
+      if (q >= ends || q == p || m.mate[q] != p) {
+        throw std::invalid_argument("template is not a perfect matching of the 2k ports");
+      }
+      if (p > q) continue;
+      if (pair_index[p][q] < 0) {
+        pair_index[p][q] = static_cast<int>(pairs.size());
+        pairs.emplace_back(static_cast<uint8_t>(p), static_cast<uint8_t>(q));
+      }
+      plan.tpl[t * k + used++] = static_cast<uint8_t>(pair_index[p][q]);
+    }
+  }
+  plan.pairs = static_cast<uint32_t>(pairs.size());
+  for (size_t i = 0; i < pairs.size(); ++i) {
+    plan.pair_p[i] = pairs[i].first;
+    plan.pair_q[i] = pairs[i].second;
+  }
+  return plan;
+}
+
+// ValidateKOptCostTask (src/cpu/kopt_search.cpp:784-807).
+void ValidateTasks(int32_t n, uint32_t k, const ee_kopt_task* tasks, size_t count) {
+  for (size_t r = 0; r < count; ++r) {
+    const ee_kopt_task& t = tasks[r];
+    if (t.deleted_cost < 0) throw std::invalid_argument("k-opt task has a negative deleted_cost");
+    uint64_t deleted[kMaxK];
+    for (uint32_t e = 0; e < k; ++e) {
+      const int32_t a = t.port_nodes[2 * e], b = t.port_nodes[2 * e + 1];
+      if (a < 0 || b < 0 || a >= n || b >= n || a == b) {
+        throw std::invalid_argument("k-opt task has an invalid or repeated deleted edge");
+      }
+      const uint64_t key = (static_cast<uint64_t>(std::min(a, b)) << 32) |
+                           static_cast<uint32_t>(std::max(a, b));
+      for (uint32_t f = 0; f < e; ++f) {
+        if (deleted[f] == key) {
+          throw std::invalid_argument("k-opt task has an invalid or repeated deleted edge");
+        }
+      }
+      deleted[e] = key;
+    }
+  }
+}
+
+uint64_t KoptResidentBytes(const KoptCache& c) {
+  return c.d_xy.bytes() + c.d_plans.bytes() + c.d_tasks.bytes() + c.d_costs.bytes() +
+         c.d_tables.bytes();
+}
+
+uint32_t TileRows(const KoptTuning& tuning, uint32_t k) { return tuning.rows_per_tile[k - 3]; }
+
+size_t PlanSharedBytes(const KoptPlan& plan, const KoptTuning& tuning) {
+  return KoptSharedBytes(plan.k, TileRows(tuning, plan.k), plan.pairs, plan.templates);
+}
+
+void EvalKoptCosts(Context& ctx, const int64_t* x, const int64_t* y, int32_t n, uint8_t type,
+                   uint32_t k, const ee_endpoint_matching* templates, uint32_t template_count,
+                   uint64_t generator_hash, const ee_kopt_task* tasks, size_t task_count,
+                   int64_t* costs, ee_kopt_usage* usage) {
+  ee_kopt_usage u{};
+  ctx.Activate();
+  if (task_count == 0) {
+    if (usage) *usage = u;
+    return;
+  }
+  if (k < 3 || k > 5 || templates == nullptr || template_count == 0) {
+    throw std::invalid_argument("HIP k-opt reconnect table is invalid");
+  }
+  if (x == nullptr || y == nullptr || n <= 0 || (type != EE_EUC_2D && type != EE_CEIL_2D)) {
+    throw std::invalid_argument("HIP k-opt coordinates are invalid");
+  }
+  if (tasks == nullptr || costs == nullptr) throw std::invalid_argument("HIP k-opt NULL buffer");
+  if (task_count > SIZE_MAX / template_count) throw std::overflow_error("k-opt cell count overflows");
+  const size_t cells = task_count * template_count;
+  ValidateTasks(n, k, tasks, task_count);
+  KoptCache& c = ctx.kopt;
+
+  // snapshot
+  const size_t nn = static_cast<size_t>(n);
+  const bool snapshot_hit = c.snapshot_valid && c.dimension == n && c.distance_type == type &&
+                            c.x.size() == nn && std::memcmp(c.x.data(), x, nn * 8) == 0 &&
+                            std::memcmp(c.y.data(), y, nn * 8) == 0;
+  if (!snapshot_hit) {
+    if (!DistanceSafe(x, y, n)) {
+      throw std::invalid_argument("k-opt cost needs distance-safe integer coordinates");
+    }
+    const std::vector<longlong2> xy = Interleave(x, y, n);
+    DeviceArray<longlong2> d_xy(nn, ctx.device);
+    d_xy.Upload(xy.data(), nn, ctx.stream);
+    ctx.Sync();
+    c.d_xy = std::move(d_xy);
+    c.x.assign(x, x + nn);
+    c.y.assign(y, y + nn);
+    c.dimension = n;
+    c.distance_type = type;
+    c.snapshot_valid = true;
+  }
+
+  // templates (one plan slot per k)
+  KoptCache::TemplateEntry& entry = c.templates[k - 3];
+  const bool template_hit =
+      entry.valid && entry.hash == generator_hash && entry.table.size() == template_count &&
+      std::memcmp(entry.table.data(), templates, template_count * sizeof(ee_endpoint_matching)) == 0;
+  if (!template_hit) {
+    const KoptPlan plan = BuildPlan(k, templates, template_count);
+    if (c.d_plans.count() < 3) c.d_plans.Allocate(3, ctx.device);
+    c.host_plans[k - 3] = plan;
+    c.d_plans.UploadAt(k - 3, &c.host_plans[k - 3], 1, ctx.stream);
+    ctx.Sync();
+    entry.table.assign(templates, templates + template_count);
+    entry.hash = generator_hash;
+    entry.valid = true;
+  }
+  const KoptPlan& plan = c.host_plans[k - 3];
+
+  // workspace
+  const bool workspace_hit = c.d_tasks.count() >= task_count && c.d_costs.count() >= cells;
+  if (c.d_tasks.count() < task_count) c.d_tasks.Allocate(Grow(c.d_tasks.count(), task_count), ctx.device);
+  if (c.d_costs.count() < cells) c.d_costs.Allocate(Grow(c.d_costs.count(), cells), ctx.device);
+  if (c.d_tables.count() < 1) c.d_tables.Allocate(1, ctx.device);
+
+  const uint32_t rows = TileRows(ctx.kopt_tuning, k);
+  KoptTables tables{};
+  tables.instance = KoptInstance{0, static_cast<uint32_t>(n), type};
+  tables.segment = KoptSegment{0, k, 0, task_count, 0, 0, rows, 0};
+  auto t0 = std::chrono::steady_clock::now();
+  c.d_tasks.Upload(tasks, task_count, ctx.stream);
+  c.d_tables.Upload(&tables, 1, ctx.stream);
+  ctx.Sync();
+  u.h2d_ms = MsSince(t0);
+
+  KoptLaunch a{};
+  a.xy = c.d_xy.get();
+  a.instances = &c.d_tables.get()->instance;
+  a.segments = &c.d_tables.get()->segment;
+  a.segment_count = 1;
+  a.tile_count = (task_count + rows - 1) / rows;
+  a.tasks = c.d_tasks.get();
+  a.plans = c.d_plans.get();
+  a.costs = c.d_costs.get();
+  a.tile_segment = nullptr;
+  t0 = std::chrono::steady_clock::now();
+  LaunchKopt(a, ctx.kopt_tuning, PlanSharedBytes(plan, ctx.kopt_tuning), ctx.stream);
+  ctx.Sync();
+  u.kernel_ms = MsSince(t0);
+
+  t0 = std::chrono::steady_clock::now();
+  c.d_costs.Download(costs, cells, ctx.stream);
+  ctx.Sync();
+  u.d2h_ms = MsSince(t0);
+  u.snapshot_hit = snapshot_hit ? 1 : 0;
+  u.template_hit = template_hit ? 1 : 0;
+  u.workspace_hit = workspace_hit ? 1 : 0;
+  u.resident_bytes = KoptResidentBytes(c);
+  if (usage) *usage = u;
+}
+
+}  // namespace ee
+
+// =================================================================== C ABI
+using ee::StatusFromException;
+
+extern "C" {
+
+const char* ee_last_error(void) { return ee::LastError(); }
+
+const char* ee_version(void) { return "ee_hip 0.1.0 (gfx950)"; }
+
+int ee_device_count(void) { return ee::CountDevices(); }
+
+ee_status ee_backend_available(int device, char* reason, size_t reason_capacity) {
+  std::string why;
+  const int count = ee::CountDevices();
+  if (count <= 0) {
+    why = "no visible HIP device";
+  } else if (device >= count) {
+    why = "requested HIP device ordinal is not visible";
+  }
+  if (reason && reason_capacity > 0) {
+    const size_t len = std::min(why.size(), reason_capacity - 1);
+    std::memcpy(reason, why.data(), len);
+    reason[len] = '\0';
+  }
+  if (!why.empty()) {
+    ee::SetLastError(why);
+    return EE_ERR_UNAVAILABLE;
+  }
+  return EE_OK;
+}
+
+ee_status ee_ctx_create(int device, ee_ctx** out) {
+  try {
+    if (out == nullptr) throw std::invalid_argument("ee_ctx_create: out is NULL");
+    *out = nullptr;
+    *out = reinterpret_cast<ee_ctx*>(new ee::Context(device));
+    return EE_OK;
+  } catch (...) {
+    return StatusFromException();
+  }
+}
+
+void ee_ctx_destroy(ee_ctx* ctx) { delete reinterpret_cast<ee::Context*>(ctx); }
+
+int ee_ctx_device(const ee_ctx* ctx) {
+  return ctx ? reinterpret_cast<const ee::Context*>(ctx)->device : -1;
+}
+
+ee_status ee_jv_find_candidates(ee_ctx* ctx, const ee_graph_view* graph, int32_t* witnesses,
+                                ee_jv_usage* usage) {
+  try {
+    if (ctx == nullptr) throw std::invalid_argument("context is NULL");
+    ee::CheckGraphView(graph);
+    if (graph->edge_count > 0 && witnesses == nullptr) {
+      throw std::invalid_argument("witness buffer is NULL");
+    }
+    ee::FindJvCandidates(*reinterpret_cast<ee::Context*>(ctx), *graph, witnesses, usage);
+    return EE_OK;
+  } catch (...) {
+    return StatusFromException();
+  }
+}
+
+ee_status ee_jv_clear_cache(ee_ctx* ctx) {
+  try {
+    if (ctx == nullptr) throw std::invalid_argument("context is NULL");
+    auto* c = reinterpret_cast<ee::Context*>(ctx);
+    c->Activate();
+    c->Sync();
+    c->jv = ee::JvCache{};
+    return EE_OK;
+  } catch (...) {
+    return StatusFromException();
+  }
+}
+
+ee_status ee_kopt_build_templates(uint32_t k, ee_endpoint_matching* templates, uint32_t capacity,
+                                  uint32_t* count, uint64_t* generator_hash) {
+  try {
+    uint64_t hash = 0;
+    const auto table = ee::CanonicalTemplates(k, &hash);
+    if (count) *count = static_cast<uint32_t>(table.size());
+    if (generator_hash) *generator_hash = hash;
+    if (templates) {
+      if (capacity < table.size()) throw std::invalid_argument("template capacity too small");
+      std::memcpy(templates, table.data(), table.size() * sizeof(ee_endpoint_matching));
+    }
+    return EE_OK;
+  } catch (...) {
+    return StatusFromException();
+  }
+}
+
+ee_status ee_kopt_eval_costs(ee_ctx* ctx, const int64_t* x, const int64_t* y, int32_t dimension,
+                             uint8_t distance_type, uint32_t k,
+                             const ee_endpoint_matching* templates, uint32_t template_count,
+                             uint64_t generator_hash, const ee_kopt_task* tasks,
+                             size_t task_count, int64_t* costs, ee_kopt_usage* usage) {
+  try {
+    if (ctx == nullptr) throw std::invalid_argument("context is NULL");
+    ee::EvalKoptCosts(*reinterpret_cast<ee::Context*>(ctx), x, y, dimension, distance_type, k,
+                      templates, template_count, generator_hash, tasks, task_count, costs, usage);
+    return EE_OK;
+  } catch (...) {
+    return StatusFromException();
+  }
+}
+
+ee_status ee_kopt_clear_cache(ee_ctx* ctx) {
+  try {
+    if (ctx == nullptr) throw std::invalid_argument("context is NULL");
+    auto* c = reinterpret_cast<ee::Context*>(ctx);
+    c->Activate();
+    c->Sync();
+    c->kopt = ee::KoptCache{};
+    return EE_OK;
+  } catch (...) {
+    return StatusFromException();
+  }
+}
+
+}  // extern "C"

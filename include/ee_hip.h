@@ -1,0 +1,176 @@
+/*
+ * ee_hip.h - C ABI of the MI355X (gfx950) backend for the two candidate
+ * generation hot paths of cudaEdgeElimination:
+ *
+ *   1. the Jonker-Volgenant fast-elimination edge test over a CSR graph, and
+ *   2. the exact 3/4/5-opt leaf cost matrix (task rows x reconnect templates,
+ *      int64 cells).
+ *
+ * The entry points below are what a reference-side binding calls in place of
+ * the reference's CUDA translation units; each declaration names the
+ * reference interface it replaces (file:line in the reference tree).  Plain
+ * pointers and sizes only.  Every call is synchronous unless it says
+ * otherwise; nothing here allocates memory that the caller must free except
+ * through the matching *_destroy call.
+ *
+ * Error convention: every function returning ee_status returns EE_OK on
+ * success; on failure the message of the failing call is available from
+ * ee_last_error() on the same host thread.  The reference throws C++
+ * exceptions instead (src/cuda/jv_cuda.cu:23-27, src/cuda/kopt_cost.cu:434-442);
+ * the status codes map one-to-one onto the exception types it throws.
+ *
+ * Arithmetic: distances are the exact TSPLIB EUC_2D / CEIL_2D integers
+ * computed from int64 coordinates (floor square root from an IEEE FP64 square
+ * root plus an integer correction, bit-identical to the reference's integer
+ * restoring method for every 64-bit radicand); every cost is int64.
+ */
+#ifndef EE_HIP_H
+#define EE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  EE_OK = 0,
+  EE_ERR_INVALID_ARGUMENT = 1, /* reference: std::invalid_argument */
+  EE_ERR_OVERFLOW = 2,         /* reference: std::overflow_error */
+  EE_ERR_RUNTIME = 3,          /* reference: std::runtime_error (device errors too) */
+  EE_ERR_LOGIC = 4,            /* reference: std::logic_error */
+  EE_ERR_UNAVAILABLE = 5       /* no usable device; reference: std::runtime_error */
+} ee_status;
+
+enum { EE_EUC_2D = 0, EE_CEIL_2D = 1 };
+
+/* Message of the last failing call on this host thread ("" if none). */
+const char* ee_last_error(void);
+/* Library version string. */
+const char* ee_version(void);
+
+/* ------------------------------------------------------------------ device */
+
+/* Number of visible HIP devices (0 when none or on error). */
+int ee_device_count(void);
+
+/* Replaces cudaee::CudaBackendAvailable (include/cuda_edge_elimination/
+ * elimination.hpp:333) and cudaee::detail::KOptCostCudaAvailable
+ * (include/cuda_edge_elimination/local_search.hpp:235).  device = -1 asks
+ * whether any device is usable.  Returns EE_OK or EE_ERR_UNAVAILABLE with the
+ * reason copied into `reason` (NUL-terminated, may be NULL). */
+ee_status ee_backend_available(int device, char* reason, size_t reason_capacity);
+
+/* A context owns the device-resident caches of one host thread on one device:
+ * the reference keeps them in thread_local per-device caches
+ * (src/cuda/jv_cuda.cu:113-139, src/cuda/kopt_cost.cu:111-134) and picks the
+ * device with SelectDevice (src/cuda/jv_cuda.cu:290-347): forced ordinal,
+ * else sticky, else the device with the most free memory.  Here the forced
+ * ordinal is the `device` argument; device = -1 picks the device with the
+ * most free memory.  A context must be used by one host thread at a time. */
+typedef struct ee_ctx ee_ctx;
+ee_status ee_ctx_create(int device, ee_ctx** out);
+void ee_ctx_destroy(ee_ctx* ctx);
+int ee_ctx_device(const ee_ctx* ctx);
+
+/* ---------------------------------------------------------------------- JV */
+
+/* Host view of a graph snapshot (reference GraphSnapshot,
+ * include/cuda_edge_elimination/graph.hpp:13-37).  Edges are in edge-id
+ * order; the CSR rows are sorted by neighbour (as RebuildCsr produces them,
+ * src/common/graph.cpp:230-282).  Coordinates must be integers whose bounding
+ * box diagonal squared fits in uint64 (integer_distance_safe). */
+typedef struct {
+  int32_t dimension;           /* n */
+  uint8_t distance_type;       /* EE_EUC_2D or EE_CEIL_2D */
+  uint8_t integer_coordinates; /* must be 1 */
+  uint8_t reserved[2];
+  const int64_t* x;            /* [n] */
+  const int64_t* y;            /* [n] */
+  int32_t edge_count;          /* m, all edge slots, active or not */
+  const int32_t* edge_u;       /* [m] */
+  const int32_t* edge_v;       /* [m] */
+  const int64_t* edge_weight;  /* [m], each in [0, 2^62] */
+  const uint8_t* edge_active;  /* [m], 0 or 1 */
+  const int32_t* row_offsets;  /* [n+1], row_offsets[0] = 0 */
+  const int32_t* neighbors;    /* [row_offsets[n]] */
+  const int32_t* csr_edge_ids; /* [row_offsets[n]] */
+} ee_graph_view;
+
+/* Reference JvCudaCacheUsage (include/cuda_edge_elimination/elimination.hpp:41-49). */
+typedef struct {
+  int32_t static_hit;
+  int32_t workspace_hit;
+  uint64_t resident_bytes; /* 24m + 20n + 4 + 8A on an exact first fit */
+  double h2d_ms;
+  double kernel_ms;
+  double d2h_ms;
+} ee_jv_usage;
+
+/* Replaces cudaee::FindJvCandidatesCuda (src/cuda/jv_cuda.cu:488-566).
+ * witnesses[e] (m entries) receives the first JV witness of edge slot e, or
+ * -1 when the edge is inactive, has an endpoint of degree <= 2, or has no
+ * witness among its ten best candidates.  The reference returns the same
+ * information compacted into Candidate{edge_id, witness, JV} in edge-id
+ * order.  usage may be NULL. */
+ee_status ee_jv_find_candidates(ee_ctx* ctx, const ee_graph_view* graph, int32_t* witnesses,
+                                ee_jv_usage* usage);
+
+/* Replaces cudaee::ClearJvCudaCache (src/cuda/jv_cuda.cu:568-571). */
+ee_status ee_jv_clear_cache(ee_ctx* ctx);
+
+/* ------------------------------------------------------------------- k-opt */
+
+/* Reference KOptCostTask (include/cuda_edge_elimination/local_search.hpp:54-57):
+ * deleted edge e joins ports 2e and 2e+1; ports >= 2k are ignored. */
+typedef struct {
+  int32_t port_nodes[10];
+  int64_t deleted_cost;
+} ee_kopt_task;
+
+/* Reference EndpointMatching (include/cuda_edge_elimination/path_system.hpp:275-280). */
+typedef struct {
+  uint8_t endpoint_count;
+  uint8_t mate[14];
+} ee_endpoint_matching;
+
+/* Reference KOptCudaCacheUsage (include/cuda_edge_elimination/local_search.hpp:59-64)
+ * plus wall-clock phase times. */
+typedef struct {
+  int32_t snapshot_hit;
+  int32_t template_hit;
+  int32_t workspace_hit;
+  int32_t reserved;
+  uint64_t resident_bytes;
+  double h2d_ms;
+  double kernel_ms;
+  double d2h_ms;
+} ee_kopt_usage;
+
+/* Replaces cudaee::BuildKOptReconnectTable (src/cpu/kopt_search.cpp:1143-1193):
+ * the canonical proper k-opt reconnect templates (4 / 25 / 208 for
+ * k = 3 / 4 / 5) and their FNV-1a generator hash. */
+ee_status ee_kopt_build_templates(uint32_t k, ee_endpoint_matching* templates, uint32_t capacity,
+                                  uint32_t* count, uint64_t* generator_hash);
+
+/* Replaces cudaee::detail::EvaluateKOptTemplateCostsCuda
+ * (src/cuda/kopt_cost.cu:415-470).  costs receives task_count x
+ * template_count int64 cells, row-major; a cell is INT64_MAX when its
+ * template adds a self loop, re-adds a deleted edge, adds an edge twice, or
+ * overflows.  Tasks are validated like ValidateKOptCostTask
+ * (src/cpu/kopt_search.cpp:784-807).  usage may be NULL. */
+ee_status ee_kopt_eval_costs(ee_ctx* ctx, const int64_t* x, const int64_t* y, int32_t dimension,
+                             uint8_t distance_type, uint32_t k,
+                             const ee_endpoint_matching* templates, uint32_t template_count,
+                             uint64_t generator_hash, const ee_kopt_task* tasks,
+                             size_t task_count, int64_t* costs, ee_kopt_usage* usage);
+
+/* Replaces cudaee::detail::ClearKOptCostCudaCache (src/cuda/kopt_cost.cu:472). */
+ee_status ee_kopt_clear_cache(ee_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EE_HIP_H */
