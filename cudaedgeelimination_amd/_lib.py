@@ -101,6 +101,29 @@ class KoptUsage(C.Structure):
     ]
 
 
+class SegmentDesc(C.Structure):
+    _fields_ = [("instance", C.c_uint32), ("k", C.c_uint32), ("task_count", C.c_uint64)]
+
+
+class KoptTuning(C.Structure):
+    _fields_ = [("rows_per_tile", C.c_uint32 * 3), ("grid", C.c_uint32), ("block", C.c_uint32),
+                ("non_temporal", C.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {"rows_per_tile": list(self.rows_per_tile), "grid": self.grid,
+                "block": self.block, "non_temporal": self.non_temporal}
+
+
+class JvRunStats(C.Structure):
+    _fields_ = [
+        ("epochs", C.c_uint32), ("reserved", C.c_uint32), ("edges_scanned", C.c_uint64),
+        ("proposed", C.c_uint64), ("committed", C.c_uint64), ("initial_hash", C.c_uint64),
+        ("final_hash", C.c_uint64), ("device_ms", C.c_double), ("h2d_ms", C.c_double),
+        ("d2h_ms", C.c_double), ("propose_ms", C.c_double), ("verify_ms", C.c_double),
+        ("commit_ms", C.c_double), ("total_ms", C.c_double),
+    ]
+
+
 P = C.c_void_p
 I32, U32, I64, U64, U8, SZ, DBL = (C.c_int32, C.c_uint32, C.c_int64, C.c_uint64, C.c_uint8,
                                    C.c_size_t, C.c_double)
@@ -114,6 +137,22 @@ PROTOTYPES = {
     "ee_ctx_create": (C.c_int, [C.c_int, C.POINTER(P)]),
     "ee_ctx_destroy": (None, [P]),
     "ee_ctx_device": (C.c_int, [P]),
+    "ee_ctx_synchronize": (C.c_int, [P]),
+    "ee_kopt_batch_create": (C.c_int, [P, C.POINTER(P)]),
+    "ee_kopt_batch_destroy": (None, [P]),
+    "ee_kopt_batch_add_instance": (C.c_int, [P, P, P, I32, U8, C.POINTER(U32)]),
+    "ee_kopt_batch_set_work": (C.c_int, [P, C.POINTER(SegmentDesc), U32, P, P]),
+    "ee_kopt_batch_set_tuning": (C.c_int, [P, C.POINTER(KoptTuning)]),
+    "ee_kopt_batch_get_tuning": (C.c_int, [P, C.POINTER(KoptTuning)]),
+    "ee_kopt_batch_autotune": (C.c_int, [P, U32, C.c_char_p, C.POINTER(KoptTuning)]),
+    "ee_kopt_batch_run": (C.c_int, [P, U32, C.c_int, C.POINTER(DBL)]),
+    "ee_kopt_batch_download": (C.c_int, [P, U64, U64, P]),
+    "ee_kopt_batch_resident_bytes": (U64, [P]),
+    "ee_jv_session_create": (C.c_int, [P, C.POINTER(GraphView), C.POINTER(P)]),
+    "ee_jv_session_destroy": (None, [P]),
+    "ee_jv_session_run": (C.c_int, [P, U32, C.POINTER(JvRunStats)]),
+    "ee_jv_session_proof": (C.c_int, [P, C.c_char_p, SZ, C.POINTER(SZ)]),
+    "ee_jv_session_active": (C.c_int, [P, P]),
     "ee_jv_find_candidates": (C.c_int, [P, C.POINTER(GraphView), P, C.POINTER(JvUsage)]),
     "ee_jv_clear_cache": (C.c_int, [P]),
     "ee_kopt_build_templates": (C.c_int, [U32, P, U32, C.POINTER(U32), C.POINTER(U64)]),

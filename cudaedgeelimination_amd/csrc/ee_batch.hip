@@ -1,0 +1,466 @@
+// Device-resident k-opt batches: every (instance, k) segment of a batch is
+// evaluated by one launch of the tiled cost kernel, with inputs and the whole
+// cost output kept in HBM between launches.  Includes the autotuner that
+// picks rows per tile, grid, block and store policy by timing candidates.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "ee_runtime.h"
+
+namespace ee {
+namespace {
+
+constexpr uint32_t kTemplateCount[3] = {4, 25, 208};
+
+struct Batch {
+  explicit Batch(Context* c) : ctx(c) {}
+  ~Batch() { DropGraph(); }
+
+  Context* ctx;
+  // instances
+  std::vector<KoptInstance> instances;
+  std::vector<longlong2> host_xy;
+  DeviceArray<longlong2> d_xy;
+  DeviceArray<KoptInstance> d_instances;
+  bool instances_dirty = false;
+  // work
+  std::vector<KoptSegment> segments;
+  DeviceArray<KoptSegment> d_segments;
+  DeviceArray<ee_kopt_task> d_tasks;
+  DeviceArray<int64_t> d_costs;
+  std::vector<uint32_t> tile_map;
+  DeviceArray<uint32_t> d_tile_map;
+  uint64_t task_count = 0, cell_count = 0, tile_count = 0;
+  bool k_present[3] = {false, false, false};
+  // plans for the canonical templates of k = 3, 4, 5
+  std::array<KoptPlan, 3> plans;
+  DeviceArray<KoptPlan> d_plans;
+  KoptTuning tuning;
+  bool layout_dirty = true;
+  // graph replay of one launch
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  void DropGraph() {
+    if (exec != nullptr) static_cast<void>(hipGraphExecDestroy(exec));
+    if (graph != nullptr) static_cast<void>(hipGraphDestroy(graph));
+    exec = nullptr;
+    graph = nullptr;
+  }
+
+  size_t SharedBytes() const {
+    size_t bytes = 0;
+    for (int i = 0; i < 3; ++i) {
+      if (k_present[i]) bytes = std::max(bytes, PlanSharedBytes(plans[i], tuning));
+    }
+    return bytes;
+  }
+
+  // Splits segments into tiles for the current tuning and uploads the maps.
+  void Prepare() {
+    ctx->Activate();
+    if (instances_dirty) {
+      d_xy.Allocate(host_xy.size(), ctx->device);
+      d_xy.Upload(host_xy.data(), host_xy.size(), ctx->stream);
+      d_instances.Allocate(instances.size(), ctx->device);
+      d_instances.Upload(instances.data(), instances.size(), ctx->stream);
+      ctx->Sync();
+      instances_dirty = false;
+    }
+    if (!layout_dirty) return;
+    tile_map.clear();
+    uint64_t tile = 0;
+    for (size_t s = 0; s < segments.size(); ++s) {
+      KoptSegment& seg = segments[s];
+      seg.rows_per_tile = tuning.rows_per_tile[seg.k - 3];
+      seg.tile_begin = tile;
+      const uint64_t tiles = (seg.task_count + seg.rows_per_tile - 1) / seg.rows_per_tile;
+      tile += tiles;
+      tile_map.insert(tile_map.end(), tiles, static_cast<uint32_t>(s));
+    }
+    tile_count = tile;
+    if (SharedBytes() > 65536) throw std::invalid_argument("k-opt tuning needs more than 64 KiB of LDS");
+    d_segments.Allocate(std::max<size_t>(segments.size(), 1), ctx->device);
+    d_segments.Upload(segments.data(), segments.size(), ctx->stream);
+    d_tile_map.Allocate(std::max<size_t>(tile_map.size(), 1), ctx->device);
+    d_tile_map.Upload(tile_map.data(), tile_map.size(), ctx->stream);
+    ctx->Sync();
+    layout_dirty = false;
+    DropGraph();
+  }
+
+  KoptLaunch Args() {
+    KoptLaunch a{};
+    a.xy = d_xy.get();
+    a.instances = d_instances.get();
+    a.segments = d_segments.get();
+    a.segment_count = static_cast<uint32_t>(segments.size());
+    a.tile_count = tile_count;
+    a.tasks = d_tasks.get();
+    a.plans = d_plans.get();
+    a.costs = d_costs.get();
+    a.tile_segment = d_tile_map.get();
+    return a;
+  }
+
+  void LaunchOnce() { LaunchKopt(Args(), tuning, SharedBytes(), ctx->stream); }
+
+  // Mean device time of one launch over `launches` back-to-back launches.
+  double Time(uint32_t launches, bool use_graph) {
+    Prepare();
+    if (tile_count == 0 || launches == 0) return 0.0;
+    if (use_graph && exec == nullptr) {
+      CheckHip(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+      LaunchOnce();
+      CheckHip(hipStreamEndCapture(ctx->stream, &graph), "end capture");
+      CheckHip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
+    }
+    CheckHip(hipEventRecord(ctx->ev_begin, ctx->stream), "event record");
+    for (uint32_t i = 0; i < launches; ++i) {
+      if (use_graph) {
+        CheckHip(hipGraphLaunch(exec, ctx->stream), "graph launch");
+      } else {
+        LaunchOnce();
+      }
+    }
+    CheckHip(hipEventRecord(ctx->ev_end, ctx->stream), "event record");
+    CheckHip(hipEventSynchronize(ctx->ev_end), "event synchronize");
+    float ms = 0.0f;
+    CheckHip(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end), "event elapsed");
+    return static_cast<double>(ms) / launches;
+  }
+
+  uint64_t ResidentBytes() const {
+    return d_xy.bytes() + d_instances.bytes() + d_segments.bytes() + d_tasks.bytes() +
+           d_costs.bytes() + d_tile_map.bytes() + d_plans.bytes();
+  }
+};
+
+Batch& AsBatch(ee_kopt_batch* b) {
+  if (b == nullptr) throw std::invalid_argument("batch is NULL");
+  return *reinterpret_cast<Batch*>(b);
+}
+
+void ApplyTuning(Batch& b, const ee_kopt_tuning& t) {
+  for (int i = 0; i < 3; ++i) {
+    if (t.rows_per_tile[i] == 0 || t.rows_per_tile[i] > 1024) {
+      throw std::invalid_argument("rows per tile must be in [1, 1024]");
+    }
+  }
+  if (t.block == 0 || t.block > 256 || t.block % 64 != 0) {
+    throw std::invalid_argument("block must be 64, 128, 192 or 256");
+  }
+  KoptTuning next;
+  for (int i = 0; i < 3; ++i) next.rows_per_tile[i] = t.rows_per_tile[i];
+  next.grid = t.grid;
+  next.block = t.block;
+  next.non_temporal = t.non_temporal != 0;
+  const KoptTuning prev = b.tuning;
+  b.tuning = next;
+  if (b.SharedBytes() > 65536) {
+    b.tuning = prev;
+    throw std::invalid_argument("k-opt tuning needs more than 64 KiB of LDS");
+  }
+  b.layout_dirty = true;
+  b.DropGraph();
+}
+
+ee_kopt_tuning Export(const KoptTuning& t) {
+  ee_kopt_tuning o{};
+  for (int i = 0; i < 3; ++i) o.rows_per_tile[i] = t.rows_per_tile[i];
+  o.grid = t.grid;
+  o.block = t.block;
+  o.non_temporal = t.non_temporal ? 1 : 0;
+  return o;
+}
+
+std::string TuningKey(const Batch& b) {
+  hipDeviceProp_t prop{};
+  static_cast<void>(hipGetDeviceProperties(&prop, b.ctx->device));
+  uint64_t rows[3] = {0, 0, 0};
+  for (const KoptSegment& s : b.segments) rows[s.k - 3] += s.task_count;
+  std::ostringstream key;
+  key << prop.gcnArchName << '/' << prop.multiProcessorCount << "cu/rows";
+  for (uint64_t r : rows) {
+    int bucket = 0;  // power-of-two bucket of the row count
+    while ((uint64_t{1} << bucket) < r) ++bucket;
+    key << '-' << (r == 0 ? -1 : bucket);
+  }
+  std::string s = key.str();
+  std::replace(s.begin(), s.end(), ' ', '_');
+  return s;
+}
+
+bool LoadCached(const char* path, const std::string& key, ee_kopt_tuning* out) {
+  std::ifstream in(path);
+  std::string line;
+  while (std::getline(in, line)) {
+    std::istringstream f(line);
+    std::string k;
+    ee_kopt_tuning t{};
+    if (f >> k >> t.rows_per_tile[0] >> t.rows_per_tile[1] >> t.rows_per_tile[2] >> t.grid >>
+            t.block >> t.non_temporal &&
+        k == key) {
+      *out = t;
+      return true;
+    }
+  }
+  return false;
+}
+
+void StoreCached(const char* path, const std::string& key, const ee_kopt_tuning& t) {
+  std::vector<std::string> keep;
+  {
+    std::ifstream in(path);
+    std::string line;
+    while (std::getline(in, line)) {
+      if (line.rfind(key + " ", 0) != 0 && !line.empty()) keep.push_back(line);
+    }
+  }
+  std::ostringstream entry;
+  entry << key << ' ' << t.rows_per_tile[0] << ' ' << t.rows_per_tile[1] << ' '
+        << t.rows_per_tile[2] << ' ' << t.grid << ' ' << t.block << ' ' << t.non_temporal;
+  keep.push_back(entry.str());
+  const std::string tmp = std::string(path) + ".tmp";
+  {
+    std::ofstream out(tmp);
+    for (const std::string& l : keep) out << l << '\n';
+  }
+  std::rename(tmp.c_str(), path);
+}
+
+// Coordinate descent over the tuning space; every candidate is timed on the
+// batch itself.
+ee_kopt_tuning Autotune(Batch& b, uint32_t launches) {
+  ee_kopt_tuning best = Export(b.tuning);
+  const auto measure = [&](const ee_kopt_tuning& t) -> double {
+    try {
+      ApplyTuning(b, t);
+    } catch (const std::invalid_argument&) {
+      return 1e30;  // does not fit LDS
+    }
+    b.Time(1, false);  // warm
+    return b.Time(launches, false);
+  };
+  double best_ms = measure(best);
+  const uint32_t cu = static_cast<uint32_t>(std::max(b.ctx->compute_units, 1));
+  const std::vector<uint32_t> rows_options[3] = {
+      {16, 32, 64, 128, 256}, {4, 8, 16, 32, 64, 128}, {1, 2, 4, 8, 16, 32}};
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int k = 2; k >= 0; --k) {
+      if (!b.k_present[k]) continue;
+      for (uint32_t r : rows_options[k]) {
+        ee_kopt_tuning t = best;
+        t.rows_per_tile[k] = r;
+        const double ms = measure(t);
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = t;
+        }
+      }
+    }
+    for (uint32_t block : {256u, 128u, 64u}) {
+      ee_kopt_tuning t = best;
+      t.block = block;
+      const double ms = measure(t);
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = t;
+      }
+    }
+    for (uint32_t grid : {0u, 2 * cu, 4 * cu, 8 * cu, 16 * cu}) {
+      ee_kopt_tuning t = best;
+      t.grid = grid;
+      const double ms = measure(t);
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = t;
+      }
+    }
+    for (uint32_t nt : {0u, 1u}) {
+      ee_kopt_tuning t = best;
+      t.non_temporal = nt;
+      const double ms = measure(t);
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = t;
+      }
+    }
+  }
+  ApplyTuning(b, best);
+  return best;
+}
+
+template <typename F> ee_status Guarded(F&& body) {
+  try {
+    body();
+    return EE_OK;
+  } catch (...) {
+    return StatusFromException();
+  }
+}
+
+}  // namespace
+}  // namespace ee
+
+using ee::Guarded;
+
+extern "C" {
+
+ee_status ee_ctx_synchronize(ee_ctx* handle) {
+  return Guarded([&] {
+    if (handle == nullptr) throw std::invalid_argument("context is NULL");
+    auto& ctx = *reinterpret_cast<ee::Context*>(handle);
+    ctx.Activate();
+    ctx.Sync();
+  });
+}
+
+ee_status ee_kopt_batch_create(ee_ctx* handle, ee_kopt_batch** out) {
+  return Guarded([&] {
+    if (handle == nullptr || out == nullptr) throw std::invalid_argument("NULL argument");
+    *out = nullptr;
+    auto* ctx = reinterpret_cast<ee::Context*>(handle);
+    ctx->Activate();
+    auto batch = std::make_unique<ee::Batch>(ctx);
+    for (uint32_t k = 3; k <= 5; ++k) {
+      uint64_t hash = 0;
+      const auto table = ee::CanonicalTemplates(k, &hash);
+      batch->plans[k - 3] = ee::BuildPlan(k, table.data(), static_cast<uint32_t>(table.size()));
+    }
+    batch->d_plans.Allocate(3, ctx->device);
+    batch->d_plans.Upload(batch->plans.data(), 3, ctx->stream);
+    ctx->Sync();
+    *out = reinterpret_cast<ee_kopt_batch*>(batch.release());
+  });
+}
+
+void ee_kopt_batch_destroy(ee_kopt_batch* batch) {
+  if (batch == nullptr) return;
+  auto* b = reinterpret_cast<ee::Batch*>(batch);
+  static_cast<void>(hipSetDevice(b->ctx->device));
+  static_cast<void>(hipStreamSynchronize(b->ctx->stream));
+  delete b;
+}
+
+ee_status ee_kopt_batch_add_instance(ee_kopt_batch* batch, const int64_t* x, const int64_t* y,
+                                     int32_t dimension, uint8_t distance_type,
+                                     uint32_t* instance) {
+  return Guarded([&] {
+    ee::Batch& b = ee::AsBatch(batch);
+    if (x == nullptr || y == nullptr || dimension <= 0) throw std::invalid_argument("bad coordinates");
+    if (distance_type != EE_EUC_2D && distance_type != EE_CEIL_2D) {
+      throw std::invalid_argument("unsupported distance type");
+    }
+    if (!ee::DistanceSafe(x, y, dimension)) {
+      throw std::invalid_argument("k-opt cost needs distance-safe integer coordinates");
+    }
+    ee::KoptInstance inst{b.host_xy.size(), static_cast<uint32_t>(dimension), distance_type};
+    for (int32_t i = 0; i < dimension; ++i) b.host_xy.push_back(longlong2{x[i], y[i]});
+    b.instances.push_back(inst);
+    b.instances_dirty = true;
+    if (instance) *instance = static_cast<uint32_t>(b.instances.size() - 1);
+  });
+}
+
+ee_status ee_kopt_batch_set_work(ee_kopt_batch* batch, const ee_kopt_segment_desc* segments,
+                                 uint32_t segment_count, const ee_kopt_task* tasks,
+                                 uint64_t* cell_offsets) {
+  return Guarded([&] {
+    ee::Batch& b = ee::AsBatch(batch);
+    if (segment_count > 0 && segments == nullptr) throw std::invalid_argument("segments is NULL");
+    std::vector<ee::KoptSegment> segs;
+    uint64_t rows = 0, cells = 0;
+    bool present[3] = {false, false, false};
+    for (uint32_t s = 0; s < segment_count; ++s) {
+      const ee_kopt_segment_desc& d = segments[s];
+      if (d.instance >= b.instances.size()) throw std::invalid_argument("segment instance id");
+      if (d.k < 3 || d.k > 5) throw std::invalid_argument("segment k must be in [3,5]");
+      if (d.task_count > 0 && tasks == nullptr) throw std::invalid_argument("tasks is NULL");
+      ee::ValidateTasks(static_cast<int32_t>(b.instances[d.instance].n), d.k, tasks + rows,
+                        d.task_count);
+      if (cell_offsets) cell_offsets[s] = cells;
+      ee::KoptSegment seg{d.instance, d.k, rows, d.task_count, cells, 0, 1, 0};
+      segs.push_back(seg);
+      present[d.k - 3] = present[d.k - 3] || d.task_count > 0;
+      rows += d.task_count;
+      cells += d.task_count * ee::kTemplateCount[d.k - 3];
+    }
+    if (cell_offsets) cell_offsets[segment_count] = cells;
+    b.ctx->Activate();
+    b.DropGraph();
+    b.d_tasks.Allocate(std::max<uint64_t>(rows, 1), b.ctx->device);
+    b.d_tasks.Upload(tasks, rows, b.ctx->stream);
+    b.d_costs.Allocate(std::max<uint64_t>(cells, 1), b.ctx->device);
+    b.ctx->Sync();
+    b.segments = std::move(segs);
+    b.task_count = rows;
+    b.cell_count = cells;
+    for (int i = 0; i < 3; ++i) b.k_present[i] = present[i];
+    b.layout_dirty = true;
+  });
+}
+
+ee_status ee_kopt_batch_set_tuning(ee_kopt_batch* batch, const ee_kopt_tuning* tuning) {
+  return Guarded([&] {
+    if (tuning == nullptr) throw std::invalid_argument("tuning is NULL");
+    ee::ApplyTuning(ee::AsBatch(batch), *tuning);
+  });
+}
+
+ee_status ee_kopt_batch_get_tuning(const ee_kopt_batch* batch, ee_kopt_tuning* tuning) {
+  return Guarded([&] {
+    if (batch == nullptr || tuning == nullptr) throw std::invalid_argument("NULL argument");
+    *tuning = ee::Export(reinterpret_cast<const ee::Batch*>(batch)->tuning);
+  });
+}
+
+ee_status ee_kopt_batch_autotune(ee_kopt_batch* batch, uint32_t launches_per_candidate,
+                                 const char* cache_path, ee_kopt_tuning* chosen) {
+  return Guarded([&] {
+    ee::Batch& b = ee::AsBatch(batch);
+    const std::string key = ee::TuningKey(b);
+    ee_kopt_tuning t{};
+    if (cache_path != nullptr && ee::LoadCached(cache_path, key, &t)) {
+      ee::ApplyTuning(b, t);
+    } else {
+      t = ee::Autotune(b, std::max<uint32_t>(launches_per_candidate, 1));
+      if (cache_path != nullptr) ee::StoreCached(cache_path, key, t);
+    }
+    if (chosen) *chosen = t;
+  });
+}
+
+ee_status ee_kopt_batch_run(ee_kopt_batch* batch, uint32_t launches, int use_graph,
+                            double* device_ms) {
+  return Guarded([&] {
+    const double ms = ee::AsBatch(batch).Time(launches, use_graph != 0);
+    if (device_ms) *device_ms = ms;
+  });
+}
+
+ee_status ee_kopt_batch_download(ee_kopt_batch* batch, uint64_t first_cell, uint64_t cell_count,
+                                 int64_t* costs) {
+  return Guarded([&] {
+    ee::Batch& b = ee::AsBatch(batch);
+    if (first_cell > b.cell_count || cell_count > b.cell_count - first_cell) {
+      throw std::invalid_argument("cell range outside the batch output");
+    }
+    if (cell_count > 0 && costs == nullptr) throw std::invalid_argument("costs is NULL");
+    b.ctx->Activate();
+    b.d_costs.DownloadAt(first_cell, costs, cell_count, b.ctx->stream);
+    b.ctx->Sync();
+  });
+}
+
+uint64_t ee_kopt_batch_resident_bytes(const ee_kopt_batch* batch) {
+  return batch == nullptr ? 0 : reinterpret_cast<const ee::Batch*>(batch)->ResidentBytes();
+}
+
+}  // extern "C"

@@ -1,0 +1,162 @@
+"""Device-resident forms of the two hot paths: many k-opt segments per launch
+(``KoptBatch``) and multi-epoch JV with the snapshot kept on the device
+(``JvSession``).  Thin owners over the C ABI in include/ee_hip.h."""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+
+import numpy as np
+
+from . import _lib
+from .backend import Context
+from .graph import GraphSnapshot
+
+TEMPLATES_PER_K = {3: 4, 4: 25, 5: 208}
+
+
+class KoptBatch:
+    """Instances (coordinates) plus (instance, k, rows) segments evaluated in
+    one launch; the cost output stays on the device until downloaded."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        handle = C.c_void_p()
+        _lib.check(_lib.lib().ee_kopt_batch_create(ctx.handle, C.byref(handle)))
+        self._h = handle
+        self.cell_offsets = np.zeros(1, np.uint64)
+        self.segments: list[tuple[int, int, int]] = []
+
+    def close(self):
+        if self._h is not None:
+            _lib.lib().ee_kopt_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_instance(self, graph: GraphSnapshot) -> int:
+        inst = C.c_uint32()
+        _lib.check(_lib.lib().ee_kopt_batch_add_instance(
+            self._h, _lib.ptr(graph.x), _lib.ptr(graph.y), graph.dimension, graph.distance_type,
+            C.byref(inst)))
+        return inst.value
+
+    def set_work(self, segments, tasks: np.ndarray) -> np.ndarray:
+        """segments: [(instance, k, task_count)], tasks concatenated in
+        segment order.  Returns the first cell of every segment (+ total)."""
+        descs = (_lib.SegmentDesc * max(len(segments), 1))()
+        for i, (inst, k, count) in enumerate(segments):
+            descs[i].instance, descs[i].k, descs[i].task_count = inst, k, count
+        tasks = np.ascontiguousarray(tasks, dtype=_lib.KOPT_TASK_DTYPE)
+        offsets = np.zeros(len(segments) + 1, np.uint64)
+        _lib.check(_lib.lib().ee_kopt_batch_set_work(
+            self._h, descs, len(segments), _lib.ptr(tasks) if len(tasks) else None,
+            _lib.ptr(offsets)))
+        self.cell_offsets = offsets
+        self.segments = list(segments)
+        return offsets
+
+    @property
+    def cells(self) -> int:
+        return int(self.cell_offsets[-1])
+
+    def tuning(self) -> dict:
+        t = _lib.KoptTuning()
+        _lib.check(_lib.lib().ee_kopt_batch_get_tuning(self._h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, rows_per_tile, grid=0, block=256, non_temporal=0):
+        t = _lib.KoptTuning()
+        for i, r in enumerate(rows_per_tile):
+            t.rows_per_tile[i] = r
+        t.grid, t.block, t.non_temporal = grid, block, non_temporal
+        _lib.check(_lib.lib().ee_kopt_batch_set_tuning(self._h, C.byref(t)))
+
+    def autotune(self, launches_per_candidate: int = 10, cache_path: str | None = None) -> dict:
+        t = _lib.KoptTuning()
+        _lib.check(_lib.lib().ee_kopt_batch_autotune(
+            self._h, launches_per_candidate, cache_path.encode() if cache_path else None,
+            C.byref(t)))
+        return t.as_dict()
+
+    def run(self, launches: int = 1, use_graph: bool = False) -> float:
+        """Mean device milliseconds of one launch (HIP events)."""
+        ms = C.c_double()
+        _lib.check(_lib.lib().ee_kopt_batch_run(self._h, launches, 1 if use_graph else 0,
+                                                C.byref(ms)))
+        return ms.value
+
+    def download(self, first: int = 0, count: int | None = None) -> np.ndarray:
+        count = self.cells - first if count is None else count
+        out = np.empty(count, np.int64)
+        _lib.check(_lib.lib().ee_kopt_batch_download(self._h, first, count,
+                                                     _lib.ptr(out) if count else None))
+        return out
+
+    def segment_costs(self, index: int) -> np.ndarray:
+        lo, hi = int(self.cell_offsets[index]), int(self.cell_offsets[index + 1])
+        return self.download(lo, hi - lo)
+
+    def resident_bytes(self) -> int:
+        return int(_lib.lib().ee_kopt_batch_resident_bytes(self._h))
+
+
+@dataclasses.dataclass
+class JvRun:
+    epochs: int
+    edges_scanned: int
+    proposed: int
+    committed: int
+    initial_hash: int
+    final_hash: int
+    device_ms: float
+    h2d_ms: float
+    d2h_ms: float
+    propose_ms: float
+    verify_ms: float
+    commit_ms: float
+    total_ms: float
+    proof: str
+
+
+class JvSession:
+    """RunJvElimination with device proposal over a resident snapshot."""
+
+    def __init__(self, ctx: Context, graph: GraphSnapshot):
+        self.ctx = ctx
+        self.graph = graph
+        handle = C.c_void_p()
+        view = graph.view()
+        _lib.check(_lib.lib().ee_jv_session_create(ctx.handle, C.byref(view), C.byref(handle)))
+        self._h = handle
+
+    def close(self):
+        if self._h is not None:
+            _lib.lib().ee_jv_session_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, max_rounds: int = 100) -> JvRun:
+        st = _lib.JvRunStats()
+        _lib.check(_lib.lib().ee_jv_session_run(self._h, max_rounds, C.byref(st)))
+        length = C.c_size_t()
+        _lib.check(_lib.lib().ee_jv_session_proof(self._h, None, 0, C.byref(length)))
+        buf = C.create_string_buffer(length.value + 1)
+        _lib.check(_lib.lib().ee_jv_session_proof(self._h, buf, len(buf), C.byref(length)))
+        return JvRun(st.epochs, st.edges_scanned, st.proposed, st.committed, st.initial_hash,
+                     st.final_hash, st.device_ms, st.h2d_ms, st.d2h_ms, st.propose_ms,
+                     st.verify_ms, st.commit_ms, st.total_ms, buf.value.decode())
+
+    def active(self) -> np.ndarray:
+        out = np.empty(self.graph.edge_count, np.uint8)
+        _lib.check(_lib.lib().ee_jv_session_active(self._h, _lib.ptr(out)))
+        return out

@@ -73,6 +73,8 @@ typedef struct ee_ctx ee_ctx;
 ee_status ee_ctx_create(int device, ee_ctx** out);
 void ee_ctx_destroy(ee_ctx* ctx);
 int ee_ctx_device(const ee_ctx* ctx);
+/* Waits for all work the context enqueued. */
+ee_status ee_ctx_synchronize(ee_ctx* ctx);
 
 /* ---------------------------------------------------------------------- JV */
 
@@ -168,6 +170,97 @@ ee_status ee_kopt_eval_costs(ee_ctx* ctx, const int64_t* x, const int64_t* y, in
 
 /* Replaces cudaee::detail::ClearKOptCostCudaCache (src/cuda/kopt_cost.cu:472). */
 ee_status ee_kopt_clear_cache(ee_ctx* ctx);
+
+/* --------------------------------------------------- resident k-opt batch */
+/* No reference counterpart.  The reference evaluates one (graph, k) matrix
+ * per synchronous call with a full upload and download
+ * (src/cpu/kopt_search.cpp:2619-2692, src/cuda/kopt_cost.cu:437-462).  A
+ * batch keeps the coordinates of many instances, all task rows and the whole
+ * cost output resident on the device and evaluates every (instance, k)
+ * segment in one kernel launch.  Cells are bit-identical to
+ * ee_kopt_eval_costs with the canonical templates. */
+typedef struct ee_kopt_batch ee_kopt_batch;
+
+typedef struct {
+  uint32_t instance;   /* id from ee_kopt_batch_add_instance */
+  uint32_t k;          /* 3, 4 or 5 */
+  uint64_t task_count; /* rows of this segment, consecutive in the task array */
+} ee_kopt_segment_desc;
+
+typedef struct {
+  uint32_t rows_per_tile[3]; /* task rows per workgroup tile for k = 3, 4, 5 */
+  uint32_t grid;             /* 0: one workgroup per tile; else a persistent grid */
+  uint32_t block;            /* threads per workgroup: 64, 128, 192 or 256 */
+  uint32_t non_temporal;     /* 1: cost stores bypass the caches */
+} ee_kopt_tuning;
+
+ee_status ee_kopt_batch_create(ee_ctx* ctx, ee_kopt_batch** out);
+void ee_kopt_batch_destroy(ee_kopt_batch* batch);
+/* Uploads one instance's coordinates; *instance receives its id. */
+ee_status ee_kopt_batch_add_instance(ee_kopt_batch* batch, const int64_t* x, const int64_t* y,
+                                     int32_t dimension, uint8_t distance_type,
+                                     uint32_t* instance);
+/* Validates and uploads the task rows of every segment (segment order) and
+ * sizes the resident cost output.  cell_offsets (nullable, count + 1
+ * entries) receives the first cell of every segment and the total. */
+ee_status ee_kopt_batch_set_work(ee_kopt_batch* batch, const ee_kopt_segment_desc* segments,
+                                 uint32_t segment_count, const ee_kopt_task* tasks,
+                                 uint64_t* cell_offsets);
+ee_status ee_kopt_batch_set_tuning(ee_kopt_batch* batch, const ee_kopt_tuning* tuning);
+ee_status ee_kopt_batch_get_tuning(const ee_kopt_batch* batch, ee_kopt_tuning* tuning);
+/* Times `candidates_launches` launches of each tuning candidate (coordinate
+ * descent over rows per tile, grid, block and store policy), keeps the
+ * fastest, and, when cache_path is non-NULL, reads the choice from / writes
+ * it to that file under a key made of the device name and the batch shape. */
+ee_status ee_kopt_batch_autotune(ee_kopt_batch* batch, uint32_t launches_per_candidate,
+                                 const char* cache_path, ee_kopt_tuning* chosen);
+/* Enqueues `launches` back-to-back evaluations of the whole batch on the
+ * context's stream and waits; *device_ms receives the mean device time of one
+ * launch measured with HIP events on that stream.  use_graph = 1 replays a
+ * captured HIP graph of one launch instead of launching directly. */
+ee_status ee_kopt_batch_run(ee_kopt_batch* batch, uint32_t launches, int use_graph,
+                            double* device_ms);
+ee_status ee_kopt_batch_download(ee_kopt_batch* batch, uint64_t first_cell, uint64_t cell_count,
+                                 int64_t* costs);
+/* Device bytes held by the batch. */
+uint64_t ee_kopt_batch_resident_bytes(const ee_kopt_batch* batch);
+
+/* ------------------------------------------------------- resident JV epochs */
+/* Replaces the propose step of cudaee::RunJvElimination with backend cuda
+ * (src/cpu/jv_elimination.cpp:250-322): the snapshot stays resident on the
+ * device across epochs; each epoch proposes on the device, re-verifies every
+ * candidate on the host (an epoch with a rejected candidate fails closed,
+ * nothing committed), commits through the deterministic degree gate, rebuilds
+ * the CSR and records the proof.  The proof is the reference's V1 text
+ * (src/cpu/jv_elimination.cpp:324-368) with backend name "hip". */
+typedef struct ee_jv_session ee_jv_session;
+
+typedef struct {
+  uint32_t epochs;
+  uint32_t reserved;
+  uint64_t edges_scanned;   /* sum over epochs of active edges before the epoch */
+  uint64_t proposed;
+  uint64_t committed;
+  uint64_t initial_hash;
+  uint64_t final_hash;
+  double device_ms;         /* JV kernel time, HIP events */
+  double h2d_ms;
+  double d2h_ms;
+  double propose_ms;        /* host wall time of the propose step */
+  double verify_ms;
+  double commit_ms;
+  double total_ms;
+} ee_jv_run_stats;
+
+ee_status ee_jv_session_create(ee_ctx* ctx, const ee_graph_view* graph, ee_jv_session** out);
+void ee_jv_session_destroy(ee_jv_session* session);
+ee_status ee_jv_session_run(ee_jv_session* session, uint32_t max_rounds, ee_jv_run_stats* stats);
+/* V1 proof text of the last run (NUL-terminated when capacity > 0);
+ * *length receives the full length. */
+ee_status ee_jv_session_proof(const ee_jv_session* session, char* buffer, size_t capacity,
+                              size_t* length);
+/* Active flags after the last run (edge_count entries). */
+ee_status ee_jv_session_active(const ee_jv_session* session, uint8_t* active);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,202 @@
+"""Resident forms on the device: multi-segment k-opt batches (every tuning,
+graph replay, many instances) and the multi-epoch JV session, against the
+plain path, the CPU oracle and the reference-generated fixtures."""
+from __future__ import annotations
+
+import hashlib
+import json
+import pathlib
+import sys
+
+import numpy as np
+import pytest
+
+import _oracle as O
+from cudaedgeelimination_amd import backend, workloads as W
+from cudaedgeelimination_amd.graph import CEIL_2D, EUC_2D, GraphSnapshot
+from cudaedgeelimination_amd.resident import JvSession, KoptBatch
+
+pytestmark = pytest.mark.gpu
+
+ROOT = pathlib.Path(__file__).resolve().parent.parent
+GOLDEN = ROOT / "tests" / "golden"
+G = json.loads((GOLDEN / "goldens.json").read_text())
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def config3_small():
+    return W.config3(seed=0xEE0000C3, n=2000, rows=4096)
+
+
+def single_instance_batch(ctx, wl):
+    b = KoptBatch(ctx)
+    inst = b.add_instance(wl.graph)
+    segs = [(inst, k, int((wl.ks == k).sum())) for k in (3, 4, 5)]
+    b.set_work(segs, np.concatenate([wl.bucket(k) for k in (3, 4, 5)]))
+    return b
+
+
+TUNINGS = [
+    ((128, 32, 8), 0, 256, 0),
+    ((1, 1, 1), 0, 256, 0),
+    ((7, 3, 5), 0, 64, 1),
+    ((256, 64, 16), 512, 128, 0),
+    ((33, 17, 2), 37, 192, 1),
+]
+
+
+@pytest.mark.parametrize("rows,grid,block,nt", TUNINGS)
+def test_batch_equals_reference_for_every_tuning(ctx, config3_small, rows, grid, block, nt):
+    say(f"batch tuning rows={rows} grid={grid} block={block} nt={nt}")
+    b = single_instance_batch(ctx, config3_small)
+    b.set_tuning(rows, grid, block, nt)
+    b.run(2)
+    for i, k in enumerate((3, 4, 5)):
+        assert sha(b.segment_costs(i)) == G["config3_small"][f"k{k}"]["sha256"], f"k={k}"
+    b.close()
+
+
+def test_batch_full_config3_graph_replay(ctx):
+    say("batch config3 graph replay")
+    wl = W.config3()
+    b = single_instance_batch(ctx, wl)
+    for i, k in enumerate((3, 4, 5)):
+        b.run(1)
+        assert sha(b.segment_costs(i)) == G["config3"][f"k{k}"]["sha256"]
+    # graph replays interleaved with eager work and downloads
+    for rep in range(3):
+        b.run(2, use_graph=True)
+        for i, k in enumerate((3, 4, 5)):
+            assert sha(b.segment_costs(i)) == G["config3"][f"k{k}"]["sha256"], f"replay {rep}"
+        b.run(1, use_graph=False)
+    b.close()
+
+
+def test_batch_autotune_and_cache(ctx, config3_small, tmp_path):
+    say("batch autotune")
+    b = single_instance_batch(ctx, config3_small)
+    cache = tmp_path / "tuning.txt"
+    chosen = b.autotune(3, str(cache))
+    assert cache.exists() and cache.read_text().strip()
+    b.run(1)
+    for i, k in enumerate((3, 4, 5)):
+        assert sha(b.segment_costs(i)) == G["config3_small"][f"k{k}"]["sha256"]
+    b2 = single_instance_batch(ctx, config3_small)
+    assert b2.autotune(3, str(cache)) == chosen  # read back, no re-tuning
+    b.close()
+    b2.close()
+
+
+def test_batch_many_instances_match_oracle(ctx):
+    say("batch many instances")
+    graphs, wls = [], []
+    for i in range(8):
+        g, wl = W.config4_instance(i, rows=256)
+        graphs.append(g)
+        wls.append(wl)
+    b = KoptBatch(ctx)
+    ids = [b.add_instance(g) for g in graphs]
+    segs, tasks = [], []
+    for i, wl in enumerate(wls):
+        for k in ((5, 3, 4) if i % 2 else (3, 4, 5)):
+            rows = wl.bucket(k)
+            if i == 3 and k == 4:
+                rows = rows[:0]  # an empty segment
+            segs.append((ids[i], k, len(rows)))
+            tasks.append(rows)
+    offsets = b.set_work(segs, np.concatenate(tasks))
+    b.autotune(2)
+    b.run(1)
+    out = b.download()
+    for s, (inst, k, count) in enumerate(segs):
+        g = graphs[inst]
+        want, _, _ = O.kopt_costs(g.x, g.y, g.distance_type, k, tasks[s])
+        assert np.array_equal(out[int(offsets[s]):int(offsets[s + 1])], want), (inst, k)
+    b.close()
+
+
+def test_batch_matches_plain_path(ctx, config3_small):
+    b = single_instance_batch(ctx, config3_small)
+    b.run(1)
+    for i, k in enumerate((3, 4, 5)):
+        plain, _, _ = backend.evaluate_kopt_template_costs_cuda(
+            config3_small.graph, backend.build_kopt_reconnect_table(k), config3_small.bucket(k), ctx)
+        assert np.array_equal(b.segment_costs(i), plain)
+    b.close()
+
+
+def test_batch_rejects_bad_work(ctx):
+    from cudaedgeelimination_amd import _lib
+
+    wl = W.config3(seed=77, n=300, rows=30)
+    b = KoptBatch(ctx)
+    inst = b.add_instance(wl.graph)
+    with pytest.raises(_lib.InvalidArgument):
+        b.set_work([(inst + 1, 3, 0)], wl.tasks[:0])
+    bad = wl.bucket(3).copy()
+    bad["port_nodes"][0, 0] = wl.graph.dimension
+    with pytest.raises(_lib.InvalidArgument):
+        b.set_work([(inst, 3, len(bad))], bad)
+    with pytest.raises(_lib.InvalidArgument):
+        b.set_tuning((4096, 8, 8))
+    b.close()
+
+
+# ---------------------------------------------------------------- sessions
+@pytest.mark.parametrize("name", ["config1", "config2", "config1_ceil"])
+def test_session_proof_equals_reference(ctx, name):
+    say(f"session {name}")
+    if name == "config1_ceil":
+        base = W.config1(seed=0xEE0000C1, n=400)
+        g = W.knn_graph(base.x, base.y, CEIL_2D, 12)
+    else:
+        g = W.config1() if name == "config1" else W.config2()
+    s = JvSession(ctx, g)
+    run = s.run(100)
+    assert O.canonical_proof(run.proof) == (GOLDEN / f"{name}.ref.proof").read_text()
+    og = O.OGraph.from_snapshot(g)
+    O.jv_replay(og, run.proof)
+    assert run.device_ms > 0 and run.edges_scanned > 0
+    # the final snapshot equals the oracle's after the same run
+    final = og.clone()
+    O.oracle().orc_jv_replay(final.h, run.proof.encode())
+    assert np.array_equal(s.active(), final.arrays()["edge_active"])
+    s.close()
+
+
+def test_session_reference_fixture_and_rounds_limit(ctx):
+    rp = GOLDEN / "recursive-point"
+    g = GraphSnapshot.load(rp / "recursive-point.tsp", rp / "recursive-point.edg")
+    s = JvSession(ctx, g)
+    run = s.run(100)
+    assert O.canonical_proof(run.proof) == O.canonical_proof((rp / "recursive-point.ref.proof").read_text())
+    assert run.epochs == 2
+    s2 = JvSession(ctx, g)
+    one = s2.run(1)
+    assert one.epochs == 1
+    og = O.OGraph.from_snapshot(g)
+    assert O.canonical_proof(one.proof) == O.canonical_proof(O.jv_run(og, max_rounds=1))
+    s.close()
+    s2.close()
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_session_random_graphs(ctx, seed):
+    rng = np.random.default_rng(seed)
+    n = 600
+    x = rng.integers(0, 2000, n).astype(np.int64)
+    y = rng.integers(0, 2000, n).astype(np.int64)
+    g = W.knn_graph(x, y, seed % 2, 8 + 4 * seed)
+    s = JvSession(ctx, g)
+    run = s.run(100)
+    assert O.canonical_proof(run.proof) == O.canonical_proof(O.jv_run(O.OGraph.from_snapshot(g)))
+    s.close()
