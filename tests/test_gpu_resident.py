@@ -49,7 +49,7 @@ TUNINGS = [
     ((128, 32, 8), 0, 256, 0),
     ((1, 1, 1), 0, 256, 0),
     ((7, 3, 5), 0, 64, 1),
-    ((256, 64, 16), 512, 128, 0),
+    ((192, 64, 16), 512, 128, 0),
     ((33, 17, 2), 37, 192, 1),
 ]
 
