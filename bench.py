@@ -28,7 +28,7 @@ sys.path.insert(0, str(ROOT / "tests"))
 import numpy as np  # noqa: E402
 
 from cudaedgeelimination_amd import backend, workloads as W  # noqa: E402
-from cudaedgeelimination_amd.resident import JvSession, KoptBatch  # noqa: E402
+from cudaedgeelimination_amd.resident import JvSession, KoptBatch, KoptServer  # noqa: E402
 
 METRIC = "exact k-opt cost cells/s + JV edges/s; p50/p99 query latency vs plain GPU path"
 HBM_PEAK_GBS = 8000.0
@@ -108,6 +108,90 @@ def cpu_baseline_jv(graph):
             "median_ms": statistics.median(samples) * 1e3}, text
 
 
+def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005):
+    """Open-loop Poisson queries against four resident graphs (n = 1k, 5k,
+    15k, 30k, uniform, 10-NN): sizes lognormal (median 2,896 cells,
+    sigma 1.04) clipped to [128, 65,536] cells, k = 3 / 4 / 5 at 50/30/20%.
+    Latency runs from the scheduled arrival to completion, so queueing counts
+    (no coordinated omission).  Same request stream for the batched server and
+    the plain per-query path."""
+    import threading
+
+    graphs = W.config5_graphs()
+    pools = []
+    for i, g in enumerate(graphs):
+        per_k = {}
+        for k in (3, 4, 5):
+            ks = np.full(2048, k, np.uint8)
+            per_k[k] = W.kopt_tasks(seed + 17 * i + k, g, ks)
+        pools.append(per_k)
+    out = {"graphs": [g.dimension for g in graphs], "points": []}
+    for qps in qps_list:
+        rng = np.random.default_rng(seed + qps)
+        count = max(int(qps * seconds), 50)
+        arrivals = np.cumsum(rng.exponential(1.0 / qps, count))
+        cells = np.clip(np.exp(rng.normal(np.log(2896.0), 1.04, count)), 128, 65536)
+        kinds = rng.choice([3, 4, 5], size=count, p=[0.5, 0.3, 0.2])
+        which = rng.integers(0, len(graphs), count)
+        requests = []
+        for i in range(count):
+            k = int(kinds[i])
+            rows = max(1, int(np.ceil(cells[i] / TEMPLATES[k])))
+            pool = pools[which[i]][k]
+            rows = min(rows, len(pool))
+            start = int(rng.integers(0, len(pool) - rows + 1))
+            requests.append((int(which[i]), k, pool[start:start + rows]))
+        point = {"qps": qps, "requests": count,
+                 "mean_cells": float(np.mean([len(t) * TEMPLATES[k] for _, k, t in requests]))}
+        for mode in ("batched", "plain"):
+            server = KoptServer(ctx, batching=(mode == "batched"), hbm_budget_bytes=2 << 30,
+                                max_batch_cells=1 << 21)
+            gids = [server.add_graph(g) for g in graphs]
+            warm = server.submit(gids[0], 3, pools[0][3][:8])
+            server.wait(warm)
+            latencies = np.zeros(count)
+            ids = [0] * count
+            submitted = threading.Semaphore(0)
+
+            def collect():
+                for i in range(count):
+                    submitted.acquire()
+                    _, info = server.wait(ids[i])
+                    latencies[i] = info["latency_ms"] + info_offsets[i]
+
+            info_offsets = np.zeros(count)
+            waiter = threading.Thread(target=collect)
+            waiter.start()
+            t0 = time.perf_counter()
+            for i, (g, k, tasks) in enumerate(requests):
+                due = t0 + arrivals[i]
+                while True:
+                    now = time.perf_counter()
+                    if now >= due:
+                        break
+                    if due - now > 2e-4:
+                        time.sleep(1e-4)
+                ids[i] = server.submit(gids[g], k, tasks)
+                info_offsets[i] = (time.perf_counter() - due) * 1e3  # late submit counts
+                submitted.release()
+            waiter.join()
+            wall = time.perf_counter() - t0
+            stats = server.stats()
+            server.close()
+            total_cells = sum(len(t) * TEMPLATES[k] for _, k, t in requests)
+            point[mode] = {
+                "p50_ms": float(np.percentile(latencies, 50)),
+                "p99_ms": float(np.percentile(latencies, 99)),
+                "mean_ms": float(latencies.mean()),
+                "cells_per_s": total_cells / wall,
+                "launches": stats["batches"],
+                "device_cells_per_s": stats["cells"] / (stats["device_ms"] * 1e-3)
+                if stats["device_ms"] > 0 else None,
+            }
+        out["points"].append(point)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +200,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-jv", action="store_true", help="skip the JV workload")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
+    ap.add_argument("--serve-qps", default="2000,8000,20000")
+    ap.add_argument("--serve-seconds", type=float, default=1.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -221,6 +308,11 @@ def main() -> int:
             jv["cpu_baseline"] = cpu_jv
             jv["proof_equals_cpu"] = O.canonical_proof(text) == O.canonical_proof(best.proof)
         result["jv"] = jv
+
+    if not args.no_serve:
+        log("serving workload (open-loop Poisson queries)")
+        result["serving"] = serving_leg(ctx, [int(q) for q in args.serve_qps.split(",")],
+                                        args.serve_seconds)
 
     batch.close()
     ctx.close()

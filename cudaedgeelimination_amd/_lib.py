@@ -124,6 +124,22 @@ class JvRunStats(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch_cells", C.c_uint64),
+                ("max_batch_requests", C.c_uint32), ("pipeline_depth", C.c_uint32)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("request_id", C.c_uint64), ("status", C.c_int32), ("batch_size", C.c_uint32),
+                ("batch_id", C.c_uint64), ("queue_ms", C.c_double), ("service_ms", C.c_double),
+                ("latency_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests", C.c_uint64), ("batches", C.c_uint64), ("cells", C.c_uint64),
+                ("peak_inflight_bytes", C.c_uint64), ("device_ms", C.c_double)]
+
+
 P = C.c_void_p
 I32, U32, I64, U64, U8, SZ, DBL = (C.c_int32, C.c_uint32, C.c_int64, C.c_uint64, C.c_uint8,
                                    C.c_size_t, C.c_double)
@@ -153,6 +169,12 @@ PROTOTYPES = {
     "ee_jv_session_run": (C.c_int, [P, U32, C.POINTER(JvRunStats)]),
     "ee_jv_session_proof": (C.c_int, [P, C.c_char_p, SZ, C.POINTER(SZ)]),
     "ee_jv_session_active": (C.c_int, [P, P]),
+    "ee_server_create": (C.c_int, [P, C.POINTER(ServerConfig), C.POINTER(P)]),
+    "ee_server_destroy": (None, [P]),
+    "ee_server_add_graph": (C.c_int, [P, P, P, I32, U8, C.POINTER(U32)]),
+    "ee_server_submit": (C.c_int, [P, U32, U32, P, U64, P, C.POINTER(U64)]),
+    "ee_server_wait": (C.c_int, [P, U64, DBL, C.POINTER(RequestResult)]),
+    "ee_server_stats_get": (C.c_int, [P, C.POINTER(ServerStats)]),
     "ee_jv_find_candidates": (C.c_int, [P, C.POINTER(GraphView), P, C.POINTER(JvUsage)]),
     "ee_jv_clear_cache": (C.c_int, [P]),
     "ee_kopt_build_templates": (C.c_int, [U32, P, U32, C.POINTER(U32), C.POINTER(U64)]),

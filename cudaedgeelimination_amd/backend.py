@@ -24,6 +24,7 @@ from __future__ import annotations
 import ctypes as C
 import dataclasses
 import threading
+import weakref
 
 import numpy as np
 
@@ -40,6 +41,12 @@ class Context:
         handle = C.c_void_p()
         _lib.check(_lib.lib().ee_ctx_create(device, C.byref(handle)))
         self._handle = handle
+        # batches, sessions and servers built on this context; they are
+        # closed before the context itself
+        self._children: weakref.WeakSet = weakref.WeakSet()
+
+    def adopt(self, child) -> None:
+        self._children.add(child)
 
     @property
     def handle(self) -> C.c_void_p:
@@ -53,6 +60,8 @@ class Context:
 
     def close(self) -> None:
         if self._handle is not None:
+            for child in list(self._children):
+                child.close()
             _lib.lib().ee_ctx_destroy(self._handle)
             self._handle = None
 

@@ -55,11 +55,8 @@ struct Batch {
   }
 
   size_t SharedBytes() const {
-    size_t bytes = 0;
-    for (int i = 0; i < 3; ++i) {
-      if (k_present[i]) bytes = std::max(bytes, PlanSharedBytes(plans[i], tuning));
-    }
-    return bytes;
+    KoptLaunch probe{};
+    return KoptPrepareLayout(&probe, plans.data(), k_present, tuning);
   }
 
   // Splits segments into tiles for the current tuning and uploads the maps.
@@ -109,7 +106,11 @@ struct Batch {
     return a;
   }
 
-  void LaunchOnce() { LaunchKopt(Args(), tuning, SharedBytes(), ctx->stream); }
+  void LaunchOnce() {
+    KoptLaunch a = Args();
+    const size_t shared = KoptPrepareLayout(&a, plans.data(), k_present, tuning);
+    LaunchKopt(a, tuning, shared, ctx->stream);
+  }
 
   // Mean device time of one launch over `launches` back-to-back launches.
   double Time(uint32_t launches, bool use_graph) {
@@ -163,7 +164,14 @@ void ApplyTuning(Batch& b, const ee_kopt_tuning& t) {
   next.non_temporal = t.non_temporal != 0;
   const KoptTuning prev = b.tuning;
   b.tuning = next;
-  if (b.SharedBytes() > 65536) {
+  size_t shared = 0;
+  try {
+    shared = b.SharedBytes();
+  } catch (...) {
+    b.tuning = prev;
+    throw;
+  }
+  if (shared > 65536) {
     b.tuning = prev;
     throw std::invalid_argument("k-opt tuning needs more than 64 KiB of LDS");
   }
@@ -251,7 +259,7 @@ ee_kopt_tuning Autotune(Batch& b, uint32_t launches) {
   double best_ms = measure(best);
   const uint32_t cu = static_cast<uint32_t>(std::max(b.ctx->compute_units, 1));
   const std::vector<uint32_t> rows_options[3] = {
-      {16, 32, 64, 128, 256}, {4, 8, 16, 32, 64, 128}, {1, 2, 4, 8, 16, 32}};
+      {8, 16, 24, 32, 48, 64}, {4, 8, 12, 16, 24, 32}, {2, 4, 6, 8, 12, 16}};
   for (int pass = 0; pass < 2; ++pass) {
     for (int k = 2; k >= 0; --k) {
       if (!b.k_present[k]) continue;

@@ -61,6 +61,16 @@ struct KoptInstance {
   uint32_t distance_type;
 };
 
+// LDS layout of one k inside a launch: the block-shared plan (pair ports and
+// packed template pair indices) and the per-wave tile arrays.
+struct KoptKLayout {
+  uint32_t present;
+  uint32_t P, T, rows;
+  uint32_t step_r, step_t;  // 64 = step_r * T + step_t
+  uint32_t pp_off, pq_off, tpl_off;
+  uint32_t xy_off, node_off, cls_off, del_off, pair_off;
+};
+
 struct KoptLaunch {
   const longlong2* xy;         // coordinate pool
   const KoptInstance* instances;
@@ -72,17 +82,24 @@ struct KoptLaunch {
   const KoptPlan* plans;       // [3] for k = 3, 4, 5
   int64_t* costs;              // cost pool
   const uint32_t* tile_segment;  // [tile_count] segment index of each tile
+  KoptKLayout layout[3];
+  uint32_t wave_base;   // LDS offset of the first wave's tile arrays
+  uint32_t wave_bytes;  // LDS bytes per wave
 };
 
-// Rows per tile for each k; `grid` = 0 launches one block per tile.
+// Rows per wavefront tile for each k; `grid` = 0 launches enough workgroups
+// for one tile per wavefront.
 struct KoptTuning {
-  uint32_t rows_per_tile[3] = {128, 32, 8};
+  uint32_t rows_per_tile[3] = {32, 16, 8};
   uint32_t grid = 0;
   uint32_t block = 256;
   bool non_temporal = false;
 };
 
-size_t KoptSharedBytes(uint32_t k, uint32_t rows_per_tile, uint32_t pairs, uint32_t templates);
+// Fills the layout of `a` for the plans present and returns the dynamic LDS
+// bytes of one workgroup.
+size_t KoptPrepareLayout(KoptLaunch* a, const KoptPlan plans[3], const bool present[3],
+                         const KoptTuning& tuning);
 void LaunchKopt(const KoptLaunch& args, const KoptTuning& tuning, size_t shared_bytes,
                 hipStream_t stream);
 

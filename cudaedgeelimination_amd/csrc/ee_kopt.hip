@@ -3,204 +3,264 @@
 // Reference kernel: KOptTemplateCostsKernel (src/cuda/kopt_cost.cu:349-409),
 // one thread per (row, template) cell recomputing k exact distances per cell.
 //
-// Here a workgroup owns a tile of task rows of one segment (one instance, one
-// k).  Three LDS phases per tile:
-//   A. stage the 2k port node ids and their coordinates (one gather per port);
+// Here every 64-lane wavefront owns a tile of task rows of one segment (one
+// instance, one k) and a private slice of LDS; the reconnect plans of the
+// launch are loaded into LDS once per workgroup.  Per tile, with wave-level
+// synchronisation only:
+//   A. stage the 2k port node ids and coordinates (one gather per port);
+//   A2. give every port the index of the first port holding the same node
+//       (its class) and key every deleted edge by its class pair;
 //   B. for every distinct port pair the templates use, compute once per row
-//      its canonical node pair and its exact distance, or INT64_MAX when it is
-//      a self loop or re-adds one of the row's deleted edges;
-//   C. every thread walks cells of the tile in row-major order: a cell sums
-//      the k pair costs of its template and is INT64_MAX if a pair is invalid
-//      or two pairs join the same node pair.  Consecutive threads write
-//      consecutive int64 cells, so the store stream is fully coalesced.
-// The cell rules are the reference's (kopt_cost.cu:370-408); the sum of at most
-// five exact distances of a distance-safe graph is below 2^35, so the
-// reference's overflow branch cannot fire and is not evaluated per cell.
+//      the class-pair key of the added edge and its exact distance; a pair
+//      that is a self loop or re-adds a deleted edge gets the sentinel 2^40;
+//   C. lanes walk the tile's cells in row-major order: a cell sums the k pair
+//      records of its template; a sentinel in the sum or two equal keys
+//      (the same added edge twice) make it INT64_MAX.  Consecutive lanes write
+//      consecutive int64 cells.
+// The cell rules are the reference's (kopt_cost.cu:370-408).  Exact distances
+// of a distance-safe graph are at most 2^32, so a pair record (40-bit cost or
+// sentinel, 8-bit key) fits in 64 bits, the sum of five costs stays below
+// 2^35 < 2^40, and the reference's overflow branch cannot fire.
+#include <algorithm>
+
 #include "ee_exact.h"
 #include "ee_internal.h"
 
 namespace ee {
 namespace {
 
+constexpr int kWave = 64;
 constexpr int64_t kInvalid = INT64_MAX;
+constexpr uint64_t kSentinel = uint64_t{1} << 40;
+constexpr uint64_t kCostMask = (uint64_t{1} << 48) - 1;
 
-struct PairCell {
-  int64_t cost;
-  uint64_t key;  // (u << 32) | v of the canonical added edge
-};
+__device__ __forceinline__ void WaveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // q = c / d for c < 2^24 via a float reciprocal plus one correction step.
-__device__ __forceinline__ uint32_t DivSmall(uint32_t c, uint32_t d, float inv) {
-  uint32_t q = static_cast<uint32_t>(static_cast<float>(c) * inv);
+__device__ __forceinline__ uint32_t DivSmall(uint32_t c, uint32_t d) {
+  uint32_t q = static_cast<uint32_t>(static_cast<float>(c) * (1.0f / static_cast<float>(d)));
   if (q * d > c) --q;
   if ((q + 1) * d <= c) ++q;
   return q;
 }
 
-__device__ __forceinline__ uint64_t EdgeKey(int32_t a, int32_t b) {
-  const uint32_t lo = static_cast<uint32_t>(a < b ? a : b);
-  const uint32_t hi = static_cast<uint32_t>(a < b ? b : a);
-  return (static_cast<uint64_t>(lo) << 32) | hi;
-}
-
-// LDS layout of one tile (all offsets 16-byte aligned).
-struct TileLayout {
-  uint32_t xy_off, pairs_off, nodes_off, plan_off, bytes;
-};
-
-__host__ __device__ inline uint32_t Align16(uint32_t v) { return (v + 15u) & ~15u; }
-
-__host__ __device__ inline TileLayout Layout(uint32_t k, uint32_t rows, uint32_t pairs,
-                                             uint32_t templates) {
-  TileLayout l;
-  const uint32_t ports = 2 * k;
-  l.xy_off = 0;
-  l.pairs_off = Align16(l.xy_off + rows * ports * 16u);
-  l.nodes_off = Align16(l.pairs_off + rows * pairs * 16u);
-  l.plan_off = Align16(l.nodes_off + rows * ports * 4u);
-  // plan: pair_p[45], pair_q[45], then templates padded to 8 bytes each
-  l.bytes = Align16(l.plan_off + 2u * kMaxPairs + templates * 8u);
-  return l;
-}
-
-template <int K>
-__device__ __forceinline__ void LoadPlan(const KoptPlan& plan, uint8_t* s_plan) {
-  for (uint32_t i = threadIdx.x; i < kMaxPairs; i += blockDim.x) {
-    s_plan[i] = plan.pair_p[i];
-    s_plan[kMaxPairs + i] = plan.pair_q[i];
-  }
-  uint8_t* s_tpl = s_plan + 2 * kMaxPairs;
-  for (uint32_t i = threadIdx.x; i < plan.templates * 8u; i += blockDim.x) {
-    const uint32_t t = i >> 3, j = i & 7u;
-    s_tpl[i] = j < static_cast<uint32_t>(K) ? plan.tpl[t * K + j] : 0;
+// Loads the plans of every k present into the block-shared LDS prefix.
+__device__ void LoadPlans(const KoptLaunch& a, char* smem) {
+  for (int i = 0; i < 3; ++i) {
+    const KoptKLayout& L = a.layout[i];
+    if (!L.present) continue;
+    const KoptPlan& plan = a.plans[i];
+    uint8_t* pp = reinterpret_cast<uint8_t*>(smem + L.pp_off);
+    uint8_t* pq = reinterpret_cast<uint8_t*>(smem + L.pq_off);
+    uint64_t* tpl = reinterpret_cast<uint64_t*>(smem + L.tpl_off);
+    for (uint32_t j = threadIdx.x; j < L.P; j += blockDim.x) {
+      pp[j] = plan.pair_p[j];
+      pq[j] = plan.pair_q[j];
+    }
+    const uint32_t k = i + 3;
+    for (uint32_t t = threadIdx.x; t < L.T; t += blockDim.x) {
+      uint64_t packed = 0;
+      for (uint32_t j = 0; j < k; ++j) packed |= static_cast<uint64_t>(plan.tpl[t * k + j]) << (8 * j);
+      tpl[t] = packed;
+    }
   }
 }
 
 template <int K, bool kNonTemporal>
-__device__ void KoptTile(const KoptLaunch& a, const KoptSegment& seg, uint64_t tile, char* smem,
-                         uint32_t P, uint32_t T) {
+__device__ void WaveTile(const KoptLaunch& a, const KoptKLayout& L, const KoptSegment& seg,
+                         uint64_t tile, const char* smem, char* wave, int lane) {
   constexpr uint32_t kPorts = 2 * K;
-  const TileLayout lay = Layout(K, seg.rows_per_tile, P, T);
-  longlong2* s_xy = reinterpret_cast<longlong2*>(smem + lay.xy_off);
-  PairCell* s_pairs = reinterpret_cast<PairCell*>(smem + lay.pairs_off);
-  int32_t* s_nodes = reinterpret_cast<int32_t*>(smem + lay.nodes_off);
-  const uint8_t* s_pp = reinterpret_cast<const uint8_t*>(smem + lay.plan_off);
-  const uint8_t* s_pq = s_pp + kMaxPairs;
-  const uint64_t* s_tpl = reinterpret_cast<const uint64_t*>(s_pp + 2 * kMaxPairs);
+  const uint8_t* s_pp = reinterpret_cast<const uint8_t*>(smem + L.pp_off);
+  const uint8_t* s_pq = reinterpret_cast<const uint8_t*>(smem + L.pq_off);
+  const uint64_t* s_tpl = reinterpret_cast<const uint64_t*>(smem + L.tpl_off);
+  longlong2* w_xy = reinterpret_cast<longlong2*>(wave + L.xy_off);
+  int32_t* w_node = reinterpret_cast<int32_t*>(wave + L.node_off);
+  uint32_t* w_cls = reinterpret_cast<uint32_t*>(wave + L.cls_off);
+  uint32_t* w_del = reinterpret_cast<uint32_t*>(wave + L.del_off);
+  uint64_t* w_pair = reinterpret_cast<uint64_t*>(wave + L.pair_off);
+  const uint32_t P = L.P, T = L.T;
 
   const uint64_t row0 = (tile - seg.tile_begin) * seg.rows_per_tile;
   const uint64_t left = seg.task_count - row0;
   const uint32_t rows = left < seg.rows_per_tile ? static_cast<uint32_t>(left) : seg.rows_per_tile;
   const KoptInstance inst = a.instances[seg.instance];
   const longlong2* xy = a.xy + inst.coord_base;
-  const int32_t* ports =
-      reinterpret_cast<const int32_t*>(a.tasks + seg.task_offset + row0);  // 12 ints per task
+  const int32_t* ports = reinterpret_cast<const int32_t*>(a.tasks + seg.task_offset + row0);
 
-  // A. port nodes and coordinates
-  for (uint32_t s = threadIdx.x; s < rows * kPorts; s += blockDim.x) {
+  // A. port nodes and coordinates (a task row is 12 int32 words)
+  for (uint32_t s = lane; s < rows * kPorts; s += kWave) {
     const uint32_t r = s / kPorts, p = s - r * kPorts;
     const int32_t node = ports[r * 12 + p];
-    s_nodes[s] = node;
-    s_xy[s] = xy[node];
+    w_node[s] = node;
+    w_xy[s] = xy[node];
   }
-  __syncthreads();
+  WaveSync();
 
-  // B. distinct port-pair costs of every row
-  const float invP = 1.0f / static_cast<float>(P);
-  const int type = static_cast<int>(inst.distance_type);
-  for (uint32_t s = threadIdx.x; s < rows * P; s += blockDim.x) {
-    const uint32_t r = DivSmall(s, P, invP), pi = s - r * P;
-    const uint32_t p = s_pp[pi], q = s_pq[pi];
-    const int32_t* nodes = s_nodes + r * kPorts;
-    const int32_t na = nodes[p], nb = nodes[q];
-    const uint64_t key = EdgeKey(na, nb);
-    bool bad = na == nb;
-#pragma unroll
-    for (int e = 0; e < K; ++e) bad |= key == EdgeKey(nodes[2 * e], nodes[2 * e + 1]);
-    int64_t cost = kInvalid;
-    if (!bad) {
-      const longlong2 pa = s_xy[r * kPorts + p], pb = s_xy[r * kPorts + q];
-      cost = ExactDistance(pa.x, pa.y, pb.x, pb.y, type);
+  // A2. port classes and deleted-edge keys
+  for (uint32_t s = lane; s < rows * kPorts; s += kWave) {
+    const uint32_t r = s / kPorts, p = s - r * kPorts;
+    const int32_t* nd = w_node + r * kPorts;
+    uint32_t c = p;
+    for (uint32_t j = 0; j < p; ++j) {
+      if (nd[j] == nd[p]) {
+        c = j;
+        break;
+      }
     }
-    s_pairs[s] = PairCell{cost, key};
+    w_cls[s] = c;
+    if (p & 1u) {
+      uint32_t c0 = p - 1;
+      for (uint32_t j = 0; j + 1 < p; ++j) {
+        if (nd[j] == nd[p - 1]) {
+          c0 = j;
+          break;
+        }
+      }
+      w_del[r * K + (p >> 1)] = c0 < c ? c0 * 16 + c : c * 16 + c0;
+    }
   }
-  __syncthreads();
+  WaveSync();
+
+  // B. pair records of every row
+  const int type = static_cast<int>(inst.distance_type);
+  for (uint32_t s = lane; s < rows * P; s += kWave) {
+    const uint32_t r = DivSmall(s, P), pi = s - r * P;
+    const uint32_t p = s_pp[pi], q = s_pq[pi];
+    const uint32_t cp = w_cls[r * kPorts + p], cq = w_cls[r * kPorts + q];
+    const uint32_t key = cp < cq ? cp * 16 + cq : cq * 16 + cp;
+    bool bad = cp == cq;
+#pragma unroll
+    for (int e = 0; e < K; ++e) bad |= key == w_del[r * K + e];
+    uint64_t rec = kSentinel;
+    if (!bad) {
+      const longlong2 pa = w_xy[r * kPorts + p], pb = w_xy[r * kPorts + q];
+      rec = static_cast<uint64_t>(ExactDistance(pa.x, pa.y, pb.x, pb.y, type));
+    }
+    w_pair[s] = rec | (static_cast<uint64_t>(key) << 48);
+  }
+  WaveSync();
 
   // C. cells, row-major, coalesced stores
   int64_t* out = a.costs + seg.cost_offset + row0 * T;
-  const float invT = 1.0f / static_cast<float>(T);
-  for (uint32_t c = threadIdx.x; c < rows * T; c += blockDim.x) {
-    const uint32_t r = DivSmall(c, T, invT), t = c - r * T;
+  const uint32_t cells = rows * T;
+  uint32_t r = DivSmall(static_cast<uint32_t>(lane), T);
+  uint32_t t = static_cast<uint32_t>(lane) - r * T;
+  for (uint32_t c = lane; c < cells; c += kWave) {
     const uint64_t tp = s_tpl[t];
-    const PairCell* row = s_pairs + r * P;
-    uint64_t keys[K];
-    uint64_t total = 0;
-    bool bad = false;
+    const uint64_t* row = w_pair + r * P;
+    uint64_t sum = 0;
+    uint32_t key[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-      const PairCell pc = row[(tp >> (8 * j)) & 0xffu];
-      keys[j] = pc.key;
-      bad |= pc.cost == kInvalid;
-      total += static_cast<uint64_t>(pc.cost);
+      const uint64_t rec = row[(tp >> (8 * j)) & 0xffu];
+      sum += rec & kCostMask;
+      key[j] = static_cast<uint32_t>(rec >> 48);
     }
+    bool bad = sum >= kSentinel;
 #pragma unroll
     for (int i = 0; i < K; ++i) {
 #pragma unroll
-      for (int j = i + 1; j < K; ++j) bad |= keys[i] == keys[j];
+      for (int j = i + 1; j < K; ++j) bad |= key[i] == key[j];
     }
-    const int64_t value = bad ? kInvalid : static_cast<int64_t>(total);
+    const int64_t value = bad ? kInvalid : static_cast<int64_t>(sum);
     if (kNonTemporal) {
       __builtin_nontemporal_store(value, out + c);
     } else {
       out[c] = value;
     }
+    t += L.step_t;
+    r += L.step_r;
+    if (t >= T) {
+      t -= T;
+      ++r;
+    }
   }
-  __syncthreads();
+  WaveSync();
 }
 
 template <bool kNonTemporal>
 __global__ void __launch_bounds__(256) KoptKernel(KoptLaunch a) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
-  uint32_t loaded_k = 0;
-  for (uint64_t tile = blockIdx.x; tile < a.tile_count; tile += gridDim.x) {
+  LoadPlans(a, smem);
+  __syncthreads();
+  const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+  const uint32_t wib = threadIdx.x / kWave;
+  const uint32_t waves_per_block = blockDim.x / kWave;
+  char* wave = smem + a.wave_base + wib * a.wave_bytes;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * waves_per_block;
+  for (uint64_t tile = __builtin_amdgcn_readfirstlane(blockIdx.x * waves_per_block + wib);
+       tile < a.tile_count; tile += stride) {
     const KoptSegment seg = a.segments[a.tile_segment ? a.tile_segment[tile] : 0u];
-    const KoptPlan& plan = a.plans[seg.k - 3];
-    const uint32_t P = plan.pairs, T = plan.templates;
-    if (seg.k != loaded_k) {
-      uint8_t* s_plan = reinterpret_cast<uint8_t*>(
-          smem + Layout(seg.k, seg.rows_per_tile, P, T).plan_off);
-      // every tile of a segment shares one layout; a new k starts a new plan
-      switch (seg.k) {
-        case 3: LoadPlan<3>(plan, s_plan); break;
-        case 4: LoadPlan<4>(plan, s_plan); break;
-        default: LoadPlan<5>(plan, s_plan); break;
-      }
-      loaded_k = seg.k;
-      __syncthreads();
-    }
     switch (seg.k) {
-      case 3: KoptTile<3, kNonTemporal>(a, seg, tile, smem, P, T); break;
-      case 4: KoptTile<4, kNonTemporal>(a, seg, tile, smem, P, T); break;
-      default: KoptTile<5, kNonTemporal>(a, seg, tile, smem, P, T); break;
+      case 3: WaveTile<3, kNonTemporal>(a, a.layout[0], seg, tile, smem, wave, lane); break;
+      case 4: WaveTile<4, kNonTemporal>(a, a.layout[1], seg, tile, smem, wave, lane); break;
+      default: WaveTile<5, kNonTemporal>(a, a.layout[2], seg, tile, smem, wave, lane); break;
     }
   }
 }
 
+uint32_t Align16(uint32_t v) { return (v + 15u) & ~15u; }
+
 }  // namespace
 
-size_t KoptSharedBytes(uint32_t k, uint32_t rows_per_tile, uint32_t pairs, uint32_t templates) {
-  return Layout(k, rows_per_tile, pairs, templates).bytes;
+size_t KoptWaveBytes(uint32_t k, uint32_t rows, uint32_t pairs) {
+  const uint32_t ports = 2 * k;
+  return Align16(rows * ports * 16) + Align16(rows * ports * 4) + Align16(rows * ports * 4) +
+         Align16(rows * k * 4) + Align16(rows * pairs * 8);
+}
+
+size_t KoptPrepareLayout(KoptLaunch* a, const KoptPlan plans[3], const bool present[3],
+                         const KoptTuning& tuning) {
+  if (tuning.block == 0 || tuning.block > 256 || tuning.block % kWave != 0) {
+    throw std::invalid_argument("k-opt block size must be a multiple of 64 up to 256");
+  }
+  uint32_t off = 0;
+  uint32_t wave_bytes = 0;
+  for (int i = 0; i < 3; ++i) {
+    KoptKLayout& L = a->layout[i];
+    L = KoptKLayout{};
+    if (!present[i]) continue;
+    const KoptPlan& plan = plans[i];
+    const uint32_t k = i + 3, rows = tuning.rows_per_tile[i];
+    if (rows == 0) throw std::invalid_argument("rows per tile must be positive");
+    if (static_cast<uint64_t>(rows) * std::max(plan.pairs, plan.templates) >= (1u << 24)) {
+      throw std::invalid_argument("k-opt tile too large");
+    }
+    L.present = 1;
+    L.P = plan.pairs;
+    L.T = plan.templates;
+    L.rows = rows;
+    L.step_r = kWave / L.T;
+    L.step_t = kWave - L.step_r * L.T;
+    L.pp_off = off;
+    L.pq_off = off + kMaxPairs;
+    L.tpl_off = Align16(off + 2 * kMaxPairs);
+    off = Align16(L.tpl_off + L.T * 8);
+    const uint32_t ports = 2 * k;
+    L.xy_off = 0;
+    L.node_off = Align16(rows * ports * 16);
+    L.cls_off = L.node_off + Align16(rows * ports * 4);
+    L.del_off = L.cls_off + Align16(rows * ports * 4);
+    L.pair_off = L.del_off + Align16(rows * k * 4);
+    wave_bytes = std::max<uint32_t>(wave_bytes, static_cast<uint32_t>(KoptWaveBytes(k, rows, L.P)));
+  }
+  a->wave_base = off;
+  a->wave_bytes = wave_bytes;
+  return static_cast<size_t>(off) + static_cast<size_t>(wave_bytes) * (tuning.block / kWave);
 }
 
 void LaunchKopt(const KoptLaunch& args, const KoptTuning& tuning, size_t shared_bytes,
                 hipStream_t stream) {
   if (args.tile_count == 0) return;
-  if (shared_bytes > 65536) throw std::logic_error("k-opt tile needs more than 64 KiB of LDS");
-  if (tuning.block == 0 || tuning.block > 256 || tuning.block % 64 != 0) {
-    throw std::logic_error("k-opt block size must be a multiple of 64 up to 256");
-  }
-  uint64_t grid = tuning.grid == 0 ? args.tile_count : tuning.grid;
-  if (grid > args.tile_count) grid = args.tile_count;
+  if (shared_bytes > 65536) throw std::invalid_argument("k-opt launch needs more than 64 KiB of LDS");
+  const uint32_t waves_per_block = tuning.block / kWave;
+  const uint64_t needed = (args.tile_count + waves_per_block - 1) / waves_per_block;
+  uint64_t grid = tuning.grid == 0 ? needed : tuning.grid;
+  if (grid > needed) grid = needed;
   if (grid > 0x7fffffffull) grid = 0x7fffffffull;
   const dim3 g(static_cast<uint32_t>(grid)), b(tuning.block);
   if (tuning.non_temporal) {

@@ -167,6 +167,6 @@ struct Context {
 std::vector<ee_endpoint_matching> CanonicalTemplates(uint32_t k, uint64_t* hash);
 KoptPlan BuildPlan(uint32_t k, const ee_endpoint_matching* templates, uint32_t count);
 void ValidateTasks(int32_t n, uint32_t k, const ee_kopt_task* tasks, size_t count);
-size_t PlanSharedBytes(const KoptPlan& plan, const KoptTuning& tuning);
+
 
 }  // namespace ee

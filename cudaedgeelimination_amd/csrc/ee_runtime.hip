@@ -491,10 +491,6 @@ uint64_t KoptResidentBytes(const KoptCache& c) {
 
 uint32_t TileRows(const KoptTuning& tuning, uint32_t k) { return tuning.rows_per_tile[k - 3]; }
 
-size_t PlanSharedBytes(const KoptPlan& plan, const KoptTuning& tuning) {
-  return KoptSharedBytes(plan.k, TileRows(tuning, plan.k), plan.pairs, plan.templates);
-}
-
 void EvalKoptCosts(Context& ctx, const int64_t* x, const int64_t* y, int32_t n, uint8_t type,
                    uint32_t k, const ee_endpoint_matching* templates, uint32_t template_count,
                    uint64_t generator_hash, const ee_kopt_task* tasks, size_t task_count,
@@ -553,7 +549,6 @@ void EvalKoptCosts(Context& ctx, const int64_t* x, const int64_t* y, int32_t n, 
     entry.hash = generator_hash;
     entry.valid = true;
   }
-  const KoptPlan& plan = c.host_plans[k - 3];
 
   // workspace
   const bool workspace_hit = c.d_tasks.count() >= task_count && c.d_costs.count() >= cells;
@@ -581,8 +576,11 @@ void EvalKoptCosts(Context& ctx, const int64_t* x, const int64_t* y, int32_t n, 
   a.plans = c.d_plans.get();
   a.costs = c.d_costs.get();
   a.tile_segment = nullptr;
+  bool present[3] = {false, false, false};
+  present[k - 3] = true;
+  const size_t shared = KoptPrepareLayout(&a, c.host_plans.data(), present, ctx.kopt_tuning);
   t0 = std::chrono::steady_clock::now();
-  LaunchKopt(a, ctx.kopt_tuning, PlanSharedBytes(plan, ctx.kopt_tuning), ctx.stream);
+  LaunchKopt(a, ctx.kopt_tuning, shared, ctx.stream);
   ctx.Sync();
   u.kernel_ms = MsSince(t0);
 

@@ -24,6 +24,7 @@ class KoptBatch:
         handle = C.c_void_p()
         _lib.check(_lib.lib().ee_kopt_batch_create(ctx.handle, C.byref(handle)))
         self._h = handle
+        ctx.adopt(self)
         self.cell_offsets = np.zeros(1, np.uint64)
         self.segments: list[tuple[int, int, int]] = []
 
@@ -133,6 +134,7 @@ class JvSession:
         view = graph.view()
         _lib.check(_lib.lib().ee_jv_session_create(ctx.handle, C.byref(view), C.byref(handle)))
         self._h = handle
+        ctx.adopt(self)
 
     def close(self):
         if self._h is not None:
@@ -160,3 +162,71 @@ class JvSession:
         out = np.empty(self.graph.edge_count, np.uint8)
         _lib.check(_lib.lib().ee_jv_session_active(self._h, _lib.ptr(out)))
         return out
+
+
+class KoptServer:
+    """Serving loop over device-resident graphs (ee_server_* in ee_hip.h).
+
+    batching=True: continuous batching with two pipeline slots;
+    batching=False: the plain per-query path (one request per launch, one
+    synchronous upload / launch / download at a time)."""
+
+    def __init__(self, ctx: Context, hbm_budget_bytes: int = 1 << 30,
+                 max_batch_cells: int = 1 << 21, max_batch_requests: int = 256,
+                 pipeline_depth: int = 2, batching: bool = True):
+        cfg = _lib.ServerConfig()
+        cfg.hbm_budget_bytes = hbm_budget_bytes
+        cfg.max_batch_cells = max_batch_cells
+        cfg.max_batch_requests = max_batch_requests if batching else 1
+        cfg.pipeline_depth = pipeline_depth if batching else 1
+        handle = C.c_void_p()
+        _lib.check(_lib.lib().ee_server_create(ctx.handle, C.byref(cfg), C.byref(handle)))
+        self._h = handle
+        self.ctx = ctx
+        ctx.adopt(self)
+        self._buffers: dict[int, tuple[np.ndarray, np.ndarray]] = {}
+
+    def close(self):
+        if self._h is not None:
+            _lib.lib().ee_server_destroy(self._h)
+            self._h = None
+            self._buffers.clear()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_graph(self, graph: GraphSnapshot) -> int:
+        gid = C.c_uint32()
+        _lib.check(_lib.lib().ee_server_add_graph(self._h, _lib.ptr(graph.x), _lib.ptr(graph.y),
+                                                  graph.dimension, graph.distance_type,
+                                                  C.byref(gid)))
+        return gid.value
+
+    def submit(self, graph: int, k: int, tasks: np.ndarray, out: np.ndarray | None = None) -> int:
+        tasks = np.ascontiguousarray(tasks, dtype=_lib.KOPT_TASK_DTYPE)
+        if out is None:
+            out = np.empty(len(tasks) * TEMPLATES_PER_K[k], np.int64)
+        rid = C.c_uint64()
+        _lib.check(_lib.lib().ee_server_submit(self._h, graph, k, _lib.ptr(tasks), len(tasks),
+                                               _lib.ptr(out), C.byref(rid)))
+        self._buffers[rid.value] = (tasks, out)
+        return rid.value
+
+    def wait(self, request_id: int, timeout_ms: float = -1.0):
+        """(costs, result dict) of a completed request."""
+        res = _lib.RequestResult()
+        status = _lib.lib().ee_server_wait(self._h, request_id, timeout_ms, C.byref(res))
+        _, out = self._buffers.pop(request_id, (None, None))
+        _lib.check(status)
+        return out, {"batch_size": res.batch_size, "batch_id": res.batch_id,
+                     "queue_ms": res.queue_ms, "service_ms": res.service_ms,
+                     "latency_ms": res.latency_ms}
+
+    def stats(self) -> dict:
+        st = _lib.ServerStats()
+        _lib.check(_lib.lib().ee_server_stats_get(self._h, C.byref(st)))
+        return {"requests": st.requests, "batches": st.batches, "cells": st.cells,
+                "peak_inflight_bytes": st.peak_inflight_bytes, "device_ms": st.device_ms}

@@ -1,0 +1,205 @@
+// Drop-in C++ backend for cudaEdgeElimination over the gfx950 C ABI.
+//
+// This translation unit takes the place of the reference's CUDA units for
+// the two candidate-generation paths (src/cuda/jv_cuda.cu and
+// src/cuda/kopt_cost.cu): it defines the same C++ symbols, declared in the
+// reference headers, so the reference's callers (RunJvElimination,
+// EvaluateKOptTemplateCosts, the HT leaf batches, the CLI, the unit tests)
+// link unchanged against libee_hip.so:
+//
+//   cudaee::CudaBackendAvailable / FindJvCandidatesCuda / ClearJvCudaCache
+//     (include/cuda_edge_elimination/elimination.hpp:333-338)
+//   cudaee::detail::KOptCostCudaAvailable / EvaluateKOptTemplateCostsCuda /
+//     ClearKOptCostCudaCache (include/cuda_edge_elimination/local_search.hpp:235-242)
+//
+// It is compiled against the reference headers, so it builds only where the
+// reference tree is present.  Device caches live in one ee_ctx per host
+// thread (the reference keeps thread_local per-device caches,
+// src/cuda/jv_cuda.cu:138, src/cuda/kopt_cost.cu:133); the device is the
+// thread's forced ordinal when set (CudaDevicePreferenceForCurrentThread),
+// otherwise the device with the most free memory, kept for later calls.
+// Status codes become the exception types the reference throws.
+#include "cuda_edge_elimination/cuda_device_affinity.hpp"
+#include "cuda_edge_elimination/elimination.hpp"
+#include "cuda_edge_elimination/local_search.hpp"
+
+#include <cstddef>
+#include <cstdint>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ee_hip.h"
+
+static_assert(sizeof(cudaee::KOptCostTask) == sizeof(ee_kopt_task), "task layout");
+static_assert(offsetof(cudaee::KOptCostTask, deleted_cost) == offsetof(ee_kopt_task, deleted_cost),
+              "task layout");
+static_assert(sizeof(cudaee::EndpointMatching) == sizeof(ee_endpoint_matching), "template layout");
+
+namespace cudaee {
+namespace {
+
+void Throw(ee_status status, const std::string& what) {
+  const std::string message = what + ": " + ee_last_error();
+  switch (status) {
+    case EE_OK: return;
+    case EE_ERR_INVALID_ARGUMENT: throw std::invalid_argument(message);
+    case EE_ERR_OVERFLOW: throw std::overflow_error(message);
+    case EE_ERR_LOGIC: throw std::logic_error(message);
+    default: throw std::runtime_error(message);
+  }
+}
+
+struct ThreadContext {
+  ee_ctx* ctx = nullptr;
+  int device = -1;
+  ~ThreadContext() { Reset(); }
+  void Reset() {
+    if (ctx != nullptr) ee_ctx_destroy(ctx);
+    ctx = nullptr;
+    device = -1;
+  }
+};
+
+thread_local ThreadContext t_jv;
+thread_local ThreadContext t_kopt;
+
+// SelectDevice (src/cuda/jv_cuda.cu:290-347): forced > sticky > most free.
+ee_ctx* Acquire(ThreadContext* slot, std::string* reason) {
+  const int forced = detail::CudaDevicePreferenceForCurrentThread();
+  if (slot->ctx != nullptr && (forced < 0 || forced == slot->device)) return slot->ctx;
+  slot->Reset();
+  ee_ctx* ctx = nullptr;
+  const ee_status status = ee_ctx_create(forced >= 0 ? forced : -1, &ctx);
+  if (status != EE_OK) {
+    if (reason != nullptr) *reason = ee_last_error();
+    return nullptr;
+  }
+  slot->ctx = ctx;
+  slot->device = ee_ctx_device(ctx);
+  return ctx;
+}
+
+struct Coordinates {
+  std::vector<std::int64_t> x, y;
+  explicit Coordinates(const GraphSnapshot& graph) {
+    x.reserve(graph.points.size());
+    y.reserve(graph.points.size());
+    for (const Point& p : graph.points) {
+      x.push_back(p.integer_x);
+      y.push_back(p.integer_y);
+    }
+  }
+};
+
+}  // namespace
+
+bool CudaBackendAvailable(std::string* const reason) {
+  return Acquire(&t_jv, reason) != nullptr;
+}
+
+std::vector<Candidate> FindJvCandidatesCuda(const GraphSnapshot& graph, int* const selected_device,
+                                            JvCudaCacheUsage* const cache_usage) {
+  if (cache_usage != nullptr) *cache_usage = {};
+  if (!graph.integer_coordinates || !graph.integer_distance_safe) {
+    throw std::runtime_error("HIP JV supports distance-safe integer coordinates only");
+  }
+  if (graph.edges.size() > static_cast<std::size_t>(std::numeric_limits<std::int32_t>::max())) {
+    throw std::overflow_error("HIP JV edge count exceeds int32 kernel indexing");
+  }
+  std::string reason;
+  ee_ctx* ctx = Acquire(&t_jv, &reason);
+  if (ctx == nullptr) throw std::runtime_error("HIP device unavailable: " + reason);
+  if (selected_device != nullptr) *selected_device = t_jv.device;
+  if (graph.edges.empty()) return {};
+
+  const Coordinates xy(graph);
+  const std::size_t m = graph.edges.size();
+  std::vector<std::int32_t> u(m), v(m), witnesses(m, -1);
+  std::vector<std::int64_t> w(m);
+  std::vector<std::uint8_t> active(m);
+  for (std::size_t e = 0; e < m; ++e) {
+    u[e] = graph.edges[e].u;
+    v[e] = graph.edges[e].v;
+    w[e] = graph.edges[e].weight;
+    active[e] = graph.edges[e].active ? 1 : 0;
+  }
+  ee_graph_view view{};
+  view.dimension = graph.dimension;
+  view.distance_type = static_cast<std::uint8_t>(graph.distance_type);
+  view.integer_coordinates = 1;
+  view.x = xy.x.data();
+  view.y = xy.y.data();
+  view.edge_count = static_cast<std::int32_t>(m);
+  view.edge_u = u.data();
+  view.edge_v = v.data();
+  view.edge_weight = w.data();
+  view.edge_active = active.data();
+  view.row_offsets = graph.row_offsets.data();
+  view.neighbors = graph.neighbors.data();
+  view.csr_edge_ids = graph.csr_edge_ids.data();
+  ee_jv_usage usage{};
+  Throw(ee_jv_find_candidates(ctx, &view, witnesses.data(), &usage), "HIP JV propose");
+  if (cache_usage != nullptr) {
+    cache_usage->static_hit = usage.static_hit != 0;
+    cache_usage->workspace_hit = usage.workspace_hit != 0;
+    cache_usage->resident_bytes = usage.resident_bytes;
+    cache_usage->h2d_ms = usage.h2d_ms;
+    cache_usage->kernel_ms = usage.kernel_ms;
+    cache_usage->d2h_ms = usage.d2h_ms;
+  }
+  std::vector<Candidate> candidates;
+  for (std::size_t e = 0; e < m; ++e) {
+    if (witnesses[e] >= 0) {
+      candidates.push_back({static_cast<std::int32_t>(e), witnesses[e], EliminationMethod::kJv});
+    }
+  }
+  return candidates;
+}
+
+void ClearJvCudaCache() { t_jv.Reset(); }
+
+namespace detail {
+
+bool KOptCostCudaAvailable(std::string* const reason) {
+  return Acquire(&t_kopt, reason) != nullptr;
+}
+
+std::vector<std::int64_t> EvaluateKOptTemplateCostsCuda(const GraphSnapshot& graph,
+                                                        const KOptReconnectTable& table,
+                                                        const std::vector<KOptCostTask>& tasks,
+                                                        int* const selected_device,
+                                                        KOptCudaCacheUsage* const cache_usage) {
+  if (cache_usage != nullptr) *cache_usage = {};
+  std::string reason;
+  ee_ctx* ctx = Acquire(&t_kopt, &reason);
+  if (ctx == nullptr) throw std::runtime_error("HIP k-opt cost backend unavailable: " + reason);
+  if (selected_device != nullptr) *selected_device = t_kopt.device;
+  if (tasks.empty()) return {};
+  if (table.k < 3U || table.k > 5U || table.templates.empty()) {
+    throw std::invalid_argument("HIP k-opt reconnect table is invalid");
+  }
+  const Coordinates xy(graph);
+  std::vector<std::int64_t> costs(tasks.size() * table.templates.size());
+  ee_kopt_usage usage{};
+  Throw(ee_kopt_eval_costs(ctx, xy.x.data(), xy.y.data(), graph.dimension,
+                           static_cast<std::uint8_t>(graph.distance_type), table.k,
+                           reinterpret_cast<const ee_endpoint_matching*>(table.templates.data()),
+                           static_cast<std::uint32_t>(table.templates.size()), table.generator_hash,
+                           reinterpret_cast<const ee_kopt_task*>(tasks.data()), tasks.size(),
+                           costs.data(), &usage),
+        "HIP k-opt cost matrix");
+  if (cache_usage != nullptr) {
+    cache_usage->snapshot_hit = usage.snapshot_hit != 0;
+    cache_usage->template_hit = usage.template_hit != 0;
+    cache_usage->workspace_hit = usage.workspace_hit != 0;
+    cache_usage->resident_bytes = usage.resident_bytes;
+  }
+  return costs;
+}
+
+void ClearKOptCostCudaCache() { t_kopt.Reset(); }
+
+}  // namespace detail
+}  // namespace cudaee

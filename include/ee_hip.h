@@ -68,7 +68,9 @@ ee_status ee_backend_available(int device, char* reason, size_t reason_capacity)
  * device with SelectDevice (src/cuda/jv_cuda.cu:290-347): forced ordinal,
  * else sticky, else the device with the most free memory.  Here the forced
  * ordinal is the `device` argument; device = -1 picks the device with the
- * most free memory.  A context must be used by one host thread at a time. */
+ * most free memory.  A context must be used by one host thread at a time.
+ * Batches, sessions and servers created on a context must be destroyed
+ * before the context. */
 typedef struct ee_ctx ee_ctx;
 ee_status ee_ctx_create(int device, ee_ctx** out);
 void ee_ctx_destroy(ee_ctx* ctx);
@@ -261,6 +263,62 @@ ee_status ee_jv_session_proof(const ee_jv_session* session, char* buffer, size_t
                               size_t* length);
 /* Active flags after the last run (edge_count entries). */
 ee_status ee_jv_session_active(const ee_jv_session* session, uint8_t* active);
+
+/* ------------------------------------------------------- serving k-opt */
+/* No reference counterpart.  A request loop over device-resident graph
+ * snapshots: callers submit leaf-cost queries (one graph, one k, task rows)
+ * from any thread; a worker thread packs whatever has arrived into the next
+ * batched launch (continuous batching) as long as the batch stays within the
+ * cell and HBM budgets, runs up to `pipeline_depth` batches concurrently on
+ * separate streams (upload, kernel and download of one batch overlap with the
+ * next), copies each request's cells to the caller's buffer and marks it
+ * complete.  Cells are bit-identical to ee_kopt_eval_costs with the canonical
+ * templates.  With max_batch_requests = 1 and pipeline_depth = 1 the server
+ * is the plain per-query path: one synchronous upload, launch and download
+ * per request, in arrival order. */
+typedef struct ee_server ee_server;
+
+typedef struct {
+  uint64_t hbm_budget_bytes;   /* device bytes for all in-flight batch buffers */
+  uint64_t max_batch_cells;    /* cells per launch (a larger request runs alone) */
+  uint32_t max_batch_requests; /* requests per launch */
+  uint32_t pipeline_depth;     /* batches in flight: 1 or 2 */
+} ee_server_config;
+
+typedef struct {
+  uint64_t request_id;
+  int32_t status;       /* ee_status of the request */
+  uint32_t batch_size;  /* requests in the launch that served it */
+  uint64_t batch_id;
+  double queue_ms;      /* submit -> its batch was issued */
+  double service_ms;    /* issue -> cells in the caller's buffer */
+  double latency_ms;    /* submit -> complete */
+} ee_request_result;
+
+typedef struct {
+  uint64_t requests;
+  uint64_t batches;
+  uint64_t cells;
+  uint64_t peak_inflight_bytes;
+  double device_ms;     /* sum of batch kernel times (HIP events) */
+} ee_server_stats;
+
+ee_status ee_server_create(ee_ctx* ctx, const ee_server_config* config, ee_server** out);
+/* Stops the worker (finishing queued requests) and frees everything. */
+void ee_server_destroy(ee_server* server);
+/* Uploads a graph snapshot's coordinates (before or while serving). */
+ee_status ee_server_add_graph(ee_server* server, const int64_t* x, const int64_t* y,
+                              int32_t dimension, uint8_t distance_type, uint32_t* graph);
+/* Queues a request; validates it first (EE_ERR_INVALID_ARGUMENT when a task
+ * is malformed or the request alone exceeds the HBM budget).  costs (count x
+ * templates(k) cells) must stay valid until the request completes. */
+ee_status ee_server_submit(ee_server* server, uint32_t graph, uint32_t k,
+                           const ee_kopt_task* tasks, uint64_t count, int64_t* costs,
+                           uint64_t* request_id);
+/* Blocks until the request completes (timeout_ms < 0: no limit). */
+ee_status ee_server_wait(ee_server* server, uint64_t request_id, double timeout_ms,
+                         ee_request_result* result);
+ee_status ee_server_stats_get(const ee_server* server, ee_server_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -45,11 +45,12 @@ def single_instance_batch(ctx, wl):
     return b
 
 
+# rows per wavefront tile for k = 3, 4, 5; grid; workgroup size; stores
 TUNINGS = [
-    ((128, 32, 8), 0, 256, 0),
+    ((32, 16, 8), 0, 256, 0),
     ((1, 1, 1), 0, 256, 0),
     ((7, 3, 5), 0, 64, 1),
-    ((192, 64, 16), 512, 128, 0),
+    ((48, 24, 12), 512, 128, 0),
     ((33, 17, 2), 37, 192, 1),
 ]
 

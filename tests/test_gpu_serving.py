@@ -1,0 +1,131 @@
+"""Serving loop on the device: every request's cells equal the same request
+served alone (plain per-query server) and the CPU oracle, under continuous
+batching with concurrent submitters; admission limits are enforced."""
+from __future__ import annotations
+
+import sys
+import threading
+
+import numpy as np
+import pytest
+
+import _oracle as O
+from cudaedgeelimination_amd import _lib, workloads as W
+from cudaedgeelimination_amd.resident import KoptServer, TEMPLATES_PER_K
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def graphs_and_rows():
+    graphs, rows = [], []
+    for i, n in enumerate((300, 1200, 2500)):
+        wl = W.config3(seed=0xEE0005A0 + i, n=n, rows=600)
+        graphs.append(wl.graph)
+        rows.append({k: wl.bucket(k) for k in (3, 4, 5)})
+    return graphs, rows
+
+
+def make_requests(rows, count, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        g = int(rng.integers(len(rows)))
+        k = int(rng.choice([3, 4, 5], p=[0.5, 0.3, 0.2]))
+        pool = rows[g][k]
+        size = int(rng.integers(1, min(len(pool), 90) + 1))
+        start = int(rng.integers(0, len(pool) - size + 1))
+        out.append((g, k, pool[start:start + size]))
+    return out
+
+
+def expected(graphs, g, k, tasks):
+    graph = graphs[g]
+    costs, _, _ = O.kopt_costs(graph.x, graph.y, graph.distance_type, k, tasks)
+    return costs
+
+
+@pytest.mark.parametrize("batching", [True, False])
+def test_requests_equal_oracle_alone_and_in_the_mix(ctx, graphs_and_rows, batching):
+    say(f"serving batching={batching}")
+    graphs, rows = graphs_and_rows
+    server = KoptServer(ctx, batching=batching, max_batch_cells=40000)
+    ids = [server.add_graph(g) for g in graphs]
+    reqs = make_requests(rows, 300, seed=1)
+    submitted = [(server.submit(ids[g], k, t), g, k, t) for g, k, t in reqs]
+    sizes = []
+    for rid, g, k, t in submitted:
+        costs, info = server.wait(rid, 60000)
+        sizes.append(info["batch_size"])
+        assert np.array_equal(costs, expected(graphs, g, k, t)), (g, k, len(t))
+    stats = server.stats()
+    assert stats["requests"] == len(reqs)
+    if batching:
+        assert max(sizes) > 1  # requests really shared launches
+    else:
+        assert max(sizes) == 1
+    server.close()
+
+
+def test_concurrent_submitters(ctx, graphs_and_rows):
+    say("serving concurrent submitters")
+    graphs, rows = graphs_and_rows
+    server = KoptServer(ctx, batching=True, max_batch_cells=20000)
+    ids = [server.add_graph(g) for g in graphs]
+    failures = []
+
+    def client(seed):
+        try:
+            for g, k, t in make_requests(rows, 60, seed):
+                rid = server.submit(ids[g], k, t)
+                costs, _ = server.wait(rid, 60000)
+                if not np.array_equal(costs, expected(graphs, g, k, t)):
+                    failures.append((seed, g, k))
+        except Exception as e:  # noqa: BLE001
+            failures.append(repr(e))
+
+    threads = [threading.Thread(target=client, args=(s,)) for s in range(6)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(120)
+    assert not failures
+    server.close()
+
+
+def test_graph_added_while_serving(ctx, graphs_and_rows):
+    graphs, rows = graphs_and_rows
+    server = KoptServer(ctx, batching=True)
+    g0 = server.add_graph(graphs[0])
+    rid = server.submit(g0, 5, rows[0][5][:10])
+    g1 = server.add_graph(graphs[1])
+    rid1 = server.submit(g1, 4, rows[1][4][:10])
+    c0, _ = server.wait(rid)
+    c1, _ = server.wait(rid1)
+    assert np.array_equal(c0, expected(graphs, 0, 5, rows[0][5][:10]))
+    assert np.array_equal(c1, expected(graphs, 1, 4, rows[1][4][:10]))
+    server.close()
+
+
+def test_admission_rejects_bad_requests(ctx, graphs_and_rows):
+    graphs, rows = graphs_and_rows
+    server = KoptServer(ctx, hbm_budget_bytes=64 * 1024, batching=True)
+    g = server.add_graph(graphs[0])
+    big = np.repeat(rows[0][5][:1], 100, axis=0)  # 20800 cells > 32 KiB slot budget
+    with pytest.raises(_lib.InvalidArgument):
+        server.submit(g, 5, big)
+    bad = rows[0][3][:2].copy()
+    bad["port_nodes"][0, 0] = graphs[0].dimension
+    with pytest.raises(_lib.InvalidArgument):
+        server.submit(g, 3, bad)
+    with pytest.raises(_lib.InvalidArgument):
+        server.submit(g + 7, 3, rows[0][3][:2])
+    rid = server.submit(g, 3, rows[0][3][:4])
+    costs, _ = server.wait(rid)
+    assert np.array_equal(costs, expected(graphs, 0, 3, rows[0][3][:4]))
+    server.close()
