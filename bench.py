@@ -192,6 +192,37 @@ def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005):
     return out
 
 
+COPY_CELLS = 3348853  # int64 cells of the k-opt workload
+
+
+def copy_calibration(launches: int) -> int:
+    """Device-to-device copy of exactly COPY_CELLS int64 values, `launches`
+    times: the reference kernel of known traffic (26,790,824 bytes read and
+    written per launch) that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    src = torch.arange(COPY_CELLS, dtype=torch.int64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(launches):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    assert torch.equal(dst, src)
+    print(json.dumps({"copy_bytes_per_launch": COPY_CELLS * 8, "launches": launches}), flush=True)
+    return 0
+
+
+def measured_traffic():
+    """HBM bytes per KoptKernel launch from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the copy calibration), or None."""
+    path = ROOT / "profiles" / "kopt_kernel_config3_traffic.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":", 1)[1].strip())
+    return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,11 +234,18 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--serve-qps", default="2000,8000,20000")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
+    ap.add_argument("--tuning", default=None,
+                    help="r3,r4,r5,grid,block,nt: skip the autotuner and use this tuning")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device copy of the cost-matrix size (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the path is sharded by row range across GPUs, see DESIGN.md)", file=sys.stderr)
         return 2
+
+    if args.copy_calibration:
+        return copy_calibration(args.steps)
 
     log("building the k-opt workload (n=15000, 65536 rows)")
     wl = W.config3()
@@ -216,8 +254,13 @@ def main() -> int:
     inst = batch.add_instance(wl.graph)
     segments = [(inst, k, int((wl.ks == k).sum())) for k in (3, 4, 5)]
     batch.set_work(segments, np.concatenate([wl.bucket(k) for k in (3, 4, 5)]))
-    log("autotuning")
-    tuning = batch.autotune(10)
+    if args.tuning:
+        v = [int(t) for t in args.tuning.split(",")]
+        batch.set_tuning(v[:3], v[3], v[4], v[5])
+        tuning = batch.tuning()
+    else:
+        log("autotuning")
+        tuning = batch.autotune(10)
     batch.run(max(args.warmup, 1))
     log(f"timing {args.steps} steps")
     ctx_sync = backend._lib.lib().ee_ctx_synchronize
@@ -265,7 +308,7 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
             "kernel": "KoptKernel", "algorithmic_bytes_per_launch": algo_bytes,
             "kernel_ms": device_ms, "graph_replay_kernel_ms": graph_ms,
         },

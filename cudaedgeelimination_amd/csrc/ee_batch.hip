@@ -17,6 +17,8 @@ namespace ee {
 namespace {
 
 constexpr uint32_t kTemplateCount[3] = {4, 25, 208};
+// Bumped whenever the k-opt kernel changes, so cached tunings are retuned.
+constexpr int kKernelRevision = 3;
 
 struct Batch {
   explicit Batch(Context* c) : ctx(c) {}
@@ -36,6 +38,7 @@ struct Batch {
   DeviceArray<int64_t> d_costs;
   std::vector<uint32_t> tile_map;
   DeviceArray<uint32_t> d_tile_map;
+  KoptLaunch ranges{};  // per-k tile ranges of the current tile map
   uint64_t task_count = 0, cell_count = 0, tile_count = 0;
   bool k_present[3] = {false, false, false};
   // plans for the canonical templates of k = 3, 4, 5
@@ -71,17 +74,9 @@ struct Batch {
       instances_dirty = false;
     }
     if (!layout_dirty) return;
-    tile_map.clear();
-    uint64_t tile = 0;
-    for (size_t s = 0; s < segments.size(); ++s) {
-      KoptSegment& seg = segments[s];
-      seg.rows_per_tile = tuning.rows_per_tile[seg.k - 3];
-      seg.tile_begin = tile;
-      const uint64_t tiles = (seg.task_count + seg.rows_per_tile - 1) / seg.rows_per_tile;
-      tile += tiles;
-      tile_map.insert(tile_map.end(), tiles, static_cast<uint32_t>(s));
-    }
-    tile_count = tile;
+    tile_map.assign(KoptCountTiles(segments.data(), segments.size(), tuning), 0u);
+    KoptAssignTiles(segments.data(), segments.size(), tuning, tile_map.data(), &ranges);
+    tile_count = ranges.tile_count;
     if (SharedBytes() > 65536) throw std::invalid_argument("k-opt tuning needs more than 64 KiB of LDS");
     d_segments.Allocate(std::max<size_t>(segments.size(), 1), ctx->device);
     d_segments.Upload(segments.data(), segments.size(), ctx->stream);
@@ -93,12 +88,11 @@ struct Batch {
   }
 
   KoptLaunch Args() {
-    KoptLaunch a{};
+    KoptLaunch a = ranges;
     a.xy = d_xy.get();
     a.instances = d_instances.get();
     a.segments = d_segments.get();
     a.segment_count = static_cast<uint32_t>(segments.size());
-    a.tile_count = tile_count;
     a.tasks = d_tasks.get();
     a.plans = d_plans.get();
     a.costs = d_costs.get();
@@ -150,8 +144,8 @@ Batch& AsBatch(ee_kopt_batch* b) {
 
 void ApplyTuning(Batch& b, const ee_kopt_tuning& t) {
   for (int i = 0; i < 3; ++i) {
-    if (t.rows_per_tile[i] == 0 || t.rows_per_tile[i] > 1024) {
-      throw std::invalid_argument("rows per tile must be in [1, 1024]");
+    if (t.rows_per_tile[i] == 0 || t.rows_per_tile[i] > 64) {
+      throw std::invalid_argument("rows per tile must be in [1, 64]");
     }
   }
   if (t.block == 0 || t.block > 256 || t.block % 64 != 0) {
@@ -194,7 +188,8 @@ std::string TuningKey(const Batch& b) {
   uint64_t rows[3] = {0, 0, 0};
   for (const KoptSegment& s : b.segments) rows[s.k - 3] += s.task_count;
   std::ostringstream key;
-  key << prop.gcnArchName << '/' << prop.multiProcessorCount << "cu/rows";
+  key << "kopt" << kKernelRevision << '/' << prop.gcnArchName << '/' << prop.multiProcessorCount
+      << "cu/rows";
   for (uint64_t r : rows) {
     int bucket = 0;  // power-of-two bucket of the row count
     while ((uint64_t{1} << bucket) < r) ++bucket;
@@ -259,7 +254,7 @@ ee_kopt_tuning Autotune(Batch& b, uint32_t launches) {
   double best_ms = measure(best);
   const uint32_t cu = static_cast<uint32_t>(std::max(b.ctx->compute_units, 1));
   const std::vector<uint32_t> rows_options[3] = {
-      {8, 16, 24, 32, 48, 64}, {4, 8, 12, 16, 24, 32}, {2, 4, 6, 8, 12, 16}};
+      {16, 32, 48, 64}, {8, 16, 24, 32, 48, 64}, {4, 8, 12, 16, 24, 32}};
   for (int pass = 0; pass < 2; ++pass) {
     for (int k = 2; k >= 0; --k) {
       if (!b.k_present[k]) continue;
@@ -369,7 +364,7 @@ ee_status ee_kopt_batch_add_instance(ee_kopt_batch* batch, const int64_t* x, con
     if (!ee::DistanceSafe(x, y, dimension)) {
       throw std::invalid_argument("k-opt cost needs distance-safe integer coordinates");
     }
-    ee::KoptInstance inst{b.host_xy.size(), static_cast<uint32_t>(dimension), distance_type};
+    const ee::KoptInstance inst = ee::MakeKoptInstance(b.host_xy.size(), x, y, dimension, distance_type);
     for (int32_t i = 0; i < dimension; ++i) b.host_xy.push_back(longlong2{x[i], y[i]});
     b.instances.push_back(inst);
     b.instances_dirty = true;

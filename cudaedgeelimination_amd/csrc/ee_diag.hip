@@ -16,12 +16,19 @@ __global__ void IsqrtKernel(const uint64_t* s, uint64_t* out, size_t count) {
   }
 }
 
+// `near`: coordinate extents below 2^25, the FP64 path the k-opt kernel takes.
 __global__ void DistanceKernel(const longlong2* xy, const int32_t* a, const int32_t* b,
-                               size_t count, int type, int64_t* out) {
+                               size_t count, int type, int near, int64_t* out) {
   for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < count;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
     const longlong2 p = xy[a[i]], q = xy[b[i]];
-    out[i] = ExactDistance(p.x, p.y, q.x, q.y, type);
+    if (near) {
+      const int32_t dx = static_cast<int32_t>(static_cast<uint32_t>(p.x) - static_cast<uint32_t>(q.x));
+      const int32_t dy = static_cast<int32_t>(static_cast<uint32_t>(p.y) - static_cast<uint32_t>(q.y));
+      out[i] = ExactDistanceNear(dx, dy, type);
+    } else {
+      out[i] = ExactDistance(p.x, p.y, q.x, q.y, type);
+    }
   }
 }
 
@@ -91,7 +98,7 @@ ee_status ee_distances_device(ee_ctx* handle, const int64_t* x, const int64_t* y
     d_b.Upload(b, count, ctx.stream);
     hipLaunchKernelGGL(ee::DistanceKernel, dim3(ee::Blocks(count)), dim3(256), 0, ctx.stream,
                        d_xy.get(), d_a.get(), d_b.get(), count, static_cast<int>(distance_type),
-                       d_out.get());
+                       ee::NearExtent(x, y, n) ? 1 : 0, d_out.get());
     ee::CheckHip(hipGetLastError(), "distance kernel launch");
     d_out.Download(out, count, ctx.stream);
     ctx.Sync();

@@ -48,4 +48,22 @@ EE_HD int64_t ExactDistance(int64_t ax, int64_t ay, int64_t bx, int64_t by, int 
   return static_cast<int64_t>(r + up);
 }
 
+// Largest coordinate extent (max - min, per axis) for ExactDistanceNear.
+constexpr int64_t kNearExtent = int64_t{1} << 25;
+
+// Same result as ExactDistance for |dx|, |dy| < 2^25, entirely in FP64:
+// s = dx^2 + dy^2 < 2^51, so s, r*r, (r+1)^2 and s - r*r are all exact
+// doubles, and the two integer-valued corrections move any root estimate
+// within one unit of floor(sqrt(s)) onto it.
+EE_HD int64_t ExactDistanceNear(int32_t dx, int32_t dy, int type) {
+  const double fx = static_cast<double>(dx), fy = static_cast<double>(dy);
+  const double s = fma(fx, fx, fy * fy);
+  double r = floor(sqrt(s));
+  if (r * r > s) r -= 1.0;
+  if ((r + 1.0) * (r + 1.0) <= s) r += 1.0;
+  const double rem = s - r * r;
+  const bool up = type == 0 ? rem > r : rem != 0.0;
+  return static_cast<int64_t>(r) + (up ? 1 : 0);
+}
+
 }  // namespace ee

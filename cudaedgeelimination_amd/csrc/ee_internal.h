@@ -55,22 +55,33 @@ struct KoptSegment {
 };
 
 // Coordinates of one instance: xy[base .. base + n) in the coordinate pool.
+// `near` = 1 when both coordinate extents are below 2^25 (NearExtent), so the
+// kernels may use the FP64 exact distance.
 struct KoptInstance {
   uint64_t coord_base;
   uint32_t n;
-  uint32_t distance_type;
+  uint16_t distance_type;
+  uint16_t near;
 };
 
 // LDS layout of one k inside a launch: the block-shared plan (pair ports and
-// packed template pair indices) and the per-wave tile arrays.
+// packed template pair indices) and the per-wave tile arrays.  Cells are
+// walked template-stationary: when T <= 64 the wave holds `group` = 64 / T
+// copies of the templates, copy g walking rows [g * run, (g + 1) * run);
+// otherwise `passes` blocks of 64 templates each walk all rows.  Pair records
+// are stored pair-major with `pair_stride` (odd) records per pair.
 struct KoptKLayout {
   uint32_t present;
   uint32_t P, T, rows;
-  uint32_t step_r, step_t;  // 64 = step_r * T + step_t
+  uint32_t group, passes, run, pair_stride;
   uint32_t pp_off, pq_off, tpl_off;
-  uint32_t xy_off, node_off, cls_off, del_off, pair_off;
+  uint32_t xy_off, eq_off, row_off, pair_off;
 };
 
+// Tiles are numbered k by k: tiles [k_tile_begin[i], k_tile_begin[i] +
+// k_tile_count[i]) belong to segments with k = 3 + i, and workgroups
+// [k_block_begin[i], k_block_begin[i + 1]) process exactly those tiles, so
+// every workgroup runs code for one k only.
 struct KoptLaunch {
   const longlong2* xy;         // coordinate pool
   const KoptInstance* instances;
@@ -78,17 +89,20 @@ struct KoptLaunch {
   uint32_t segment_count;
   uint32_t reserved;
   uint64_t tile_count;
+  uint64_t k_tile_begin[3];
+  uint64_t k_tile_count[3];
+  uint32_t k_block_begin[4];   // filled by LaunchKopt
   const ee_kopt_task* tasks;   // task pool
   const KoptPlan* plans;       // [3] for k = 3, 4, 5
   int64_t* costs;              // cost pool
-  const uint32_t* tile_segment;  // [tile_count] segment index of each tile
+  const uint32_t* tile_segment;  // [tile_count] segment index of each tile, or NULL for one segment
   KoptKLayout layout[3];
   uint32_t wave_base;   // LDS offset of the first wave's tile arrays
   uint32_t wave_bytes;  // LDS bytes per wave
 };
 
-// Rows per wavefront tile for each k; `grid` = 0 launches enough workgroups
-// for one tile per wavefront.
+// Rows per wavefront tile for each k (at most 64); `grid` = 0 launches
+// enough workgroups for one tile per wavefront.
 struct KoptTuning {
   uint32_t rows_per_tile[3] = {32, 16, 8};
   uint32_t grid = 0;
@@ -96,6 +110,13 @@ struct KoptTuning {
   bool non_temporal = false;
 };
 
+// Number of tiles the segments split into under `tuning`.
+uint64_t KoptCountTiles(const KoptSegment* segments, size_t count, const KoptTuning& tuning);
+// Sets rows_per_tile and tile_begin of every segment, numbering tiles k by k,
+// writes the segment index of every tile to tile_map (KoptCountTiles
+// entries) and the per-k tile ranges to `a`.
+void KoptAssignTiles(KoptSegment* segments, size_t count, const KoptTuning& tuning,
+                     uint32_t* tile_map, KoptLaunch* a);
 // Fills the layout of `a` for the plans present and returns the dynamic LDS
 // bytes of one workgroup.
 size_t KoptPrepareLayout(KoptLaunch* a, const KoptPlan plans[3], const bool present[3],
@@ -107,6 +128,8 @@ void LaunchKopt(const KoptLaunch& args, const KoptTuning& tuning, size_t shared_
 struct JvLaunch {
   int32_t edge_count;
   int32_t distance_type;
+  int32_t near;                 // NearExtent of the coordinates: FP64 distances
+  int32_t reserved;
   const int2* edge_uv;          // [m]
   const int64_t* edge_weight;   // [m]
   const int32_t* edge_active;   // [m]

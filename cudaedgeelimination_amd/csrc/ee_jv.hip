@@ -66,7 +66,13 @@ __device__ __forceinline__ int CountLessInWave(uint64_t mask, int lanes, uint64_
   return count;
 }
 
-__device__ __forceinline__ int64_t Dist(const longlong2 p, const longlong2 q, int type) {
+// The FP64 path when the coordinate extents are below 2^25 (same results).
+__device__ __forceinline__ int64_t Dist(const longlong2 p, const longlong2 q, int type, bool near) {
+  if (near) {
+    const int32_t dx = static_cast<int32_t>(static_cast<uint32_t>(p.x) - static_cast<uint32_t>(q.x));
+    const int32_t dy = static_cast<int32_t>(static_cast<uint32_t>(p.y) - static_cast<uint32_t>(q.y));
+    return ExactDistanceNear(dx, dy, type);
+  }
   return ExactDistance(p.x, p.y, q.x, q.y, type);
 }
 
@@ -80,6 +86,7 @@ __global__ void __launch_bounds__(kBlock) JvKernel(JvLaunch g) {
   const int wib = static_cast<int>(threadIdx.x) / kWave;
   const int32_t waves = static_cast<int32_t>(gridDim.x) * kWavesPerBlock;
   const int type = g.distance_type;
+  const bool near = g.near != 0;
 
   for (int32_t e = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock + wib);
        e < g.edge_count; e += waves) {
@@ -132,7 +139,7 @@ __global__ void __launch_bounds__(kBlock) JvKernel(JvLaunch g) {
         if (!skip) {
           const longlong2 px = g.xy[raw];
           sc = static_cast<uint64_t>(g.edge_weight[g.csr_edge_ids[slot]]) +
-               static_cast<uint64_t>(Dist(px, side_b ? pa : pb, type));
+               static_cast<uint64_t>(Dist(px, side_b ? pa : pb, type, near));
           x = raw;
           valid = true;
         }
@@ -173,8 +180,8 @@ __global__ void __launch_bounds__(kBlock) JvKernel(JvLaunch g) {
       // ---- 3: witness test in candidate order
       if (lane < cnt) {
         const longlong2 pc = g.xy[s_node[wib][cur][lane]];
-        s_cac[wib][lane] = Dist(pa, pc, type);
-        s_cbc[wib][lane] = Dist(pb, pc, type);
+        s_cac[wib][lane] = Dist(pa, pc, type, near);
+        s_cbc[wib][lane] = Dist(pb, pc, type, near);
       }
       WaveSync();
       for (int i = 0; i < cnt; ++i) {
@@ -191,8 +198,8 @@ __global__ void __launch_bounds__(kBlock) JvKernel(JvLaunch g) {
             if (d != a && d != b) {
               const longlong2 pd = g.xy[d];
               const uint64_t lhs = wab + static_cast<uint64_t>(g.edge_weight[g.csr_edge_ids[slot]]);
-              const uint64_t first = cac + static_cast<uint64_t>(Dist(pd, pb, type));
-              const uint64_t second = static_cast<uint64_t>(Dist(pa, pd, type)) + cbc;
+              const uint64_t first = cac + static_cast<uint64_t>(Dist(pd, pb, type, near));
+              const uint64_t second = static_cast<uint64_t>(Dist(pa, pd, type, near)) + cbc;
               hit = lhs <= first || lhs <= second;
             }
           }

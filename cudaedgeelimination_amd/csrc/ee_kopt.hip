@@ -4,23 +4,32 @@
 // one thread per (row, template) cell recomputing k exact distances per cell.
 //
 // Here every 64-lane wavefront owns a tile of task rows of one segment (one
-// instance, one k) and a private slice of LDS; the reconnect plans of the
-// launch are loaded into LDS once per workgroup.  Per tile, with wave-level
-// synchronisation only:
+// instance, one k) and a private slice of LDS; each workgroup runs the tiles
+// of one k and loads that k's reconnect plan into LDS once.  Per tile, with
+// wave-level synchronisation only:
 //   A. stage the 2k port node ids and coordinates (one gather per port);
-//   A2. give every port the index of the first port holding the same node
-//       (its class) and key every deleted edge by its class pair;
-//   B. for every distinct port pair the templates use, compute once per row
-//      the class-pair key of the added edge and its exact distance; a pair
-//      that is a self loop or re-adds a deleted edge gets the sentinel 2^40;
-//   C. lanes walk the tile's cells in row-major order: a cell sums the k pair
-//      records of its template; a sentinel in the sum or two equal keys
-//      (the same added edge twice) make it INT64_MAX.  Consecutive lanes write
-//      consecutive int64 cells.
-// The cell rules are the reference's (kopt_cost.cu:370-408).  Exact distances
-// of a distance-safe graph are at most 2^32, so a pair record (40-bit cost or
-// sentinel, 8-bit key) fits in 64 bits, the sum of five costs stays below
-// 2^35 < 2^40, and the reference's overflow branch cannot fire.
+//   A2. give every port the bit mask of the ports holding the same node, and
+//       mark per row the first port of every node held by two or more ports;
+//   B. for every distinct port pair the templates use, build once per row a
+//      64-bit pair record (below), stored pair-major so that one pair's
+//      records of consecutive rows are adjacent;
+//   C. lanes keep one template each and walk a run of rows: a cell is the sum
+//      of its k pair records plus one mask test, written as int64.
+//
+// Pair record: bits 0..34 hold the exact distance of the added edge, bits
+// 61..63 count sentinels (a self loop, or an added edge equal to a deleted
+// one), and bits 35..54 hold ten 2-bit counters, one per unordered pair of
+// nodes that each hold two or more ports of the row.  A perfect matching adds
+// the same edge twice only between two such nodes, so a cell is invalid
+// exactly when its summed record has a sentinel or a counter >= 2; these are
+// the reference's cell rules (kopt_cost.cu:370-408).  Field widths:
+//  - exact distances of a distance-safe graph are at most 2^32, so five of
+//    them stay below 2^35 (the reference's overflow branch cannot fire);
+//  - tasks are validated to have k distinct deleted edges without self loops,
+//    so a node holding c ports has c distinct neighbours among the other
+//    ports; two nodes holding four ports each would need at least 14 ports,
+//    hence a counter never exceeds 3 for k <= 5, and at most five nodes hold
+//    two or more of the 2k <= 10 ports (ten counters).
 #include <algorithm>
 
 #include "ee_exact.h"
@@ -31,8 +40,11 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int64_t kInvalid = INT64_MAX;
-constexpr uint64_t kSentinel = uint64_t{1} << 40;
-constexpr uint64_t kCostMask = (uint64_t{1} << 48) - 1;
+constexpr int kCounterBase = 35;
+constexpr uint64_t kCostMask = (uint64_t{1} << kCounterBase) - 1;
+constexpr uint64_t kSentinel = uint64_t{1} << 61;
+constexpr uint64_t kBadMask = (uint64_t{0xAAAAA} << kCounterBase) | (uint64_t{7} << 61);
+constexpr uint32_t kUnroll = 2;  // rows per step of the cell loop (one ds_read2_b64 per pair)
 
 __device__ __forceinline__ void WaveSync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -48,25 +60,36 @@ __device__ __forceinline__ uint32_t DivSmall(uint32_t c, uint32_t d) {
   return q;
 }
 
-// Loads the plans of every k present into the block-shared LDS prefix.
-__device__ void LoadPlans(const KoptLaunch& a, char* smem) {
-  for (int i = 0; i < 3; ++i) {
-    const KoptKLayout& L = a.layout[i];
-    if (!L.present) continue;
-    const KoptPlan& plan = a.plans[i];
-    uint8_t* pp = reinterpret_cast<uint8_t*>(smem + L.pp_off);
-    uint8_t* pq = reinterpret_cast<uint8_t*>(smem + L.pq_off);
-    uint64_t* tpl = reinterpret_cast<uint64_t*>(smem + L.tpl_off);
-    for (uint32_t j = threadIdx.x; j < L.P; j += blockDim.x) {
-      pp[j] = plan.pair_p[j];
-      pq[j] = plan.pair_q[j];
-    }
-    const uint32_t k = i + 3;
-    for (uint32_t t = threadIdx.x; t < L.T; t += blockDim.x) {
-      uint64_t packed = 0;
-      for (uint32_t j = 0; j < k; ++j) packed |= static_cast<uint64_t>(plan.tpl[t * k + j]) << (8 * j);
-      tpl[t] = packed;
-    }
+// Loads the plan of k = 3 + ki into the block-shared LDS prefix.
+__device__ void LoadPlan(const KoptLaunch& a, int ki, char* smem) {
+  const KoptKLayout& L = a.layout[ki];
+  const KoptPlan& plan = a.plans[ki];
+  uint8_t* pp = reinterpret_cast<uint8_t*>(smem + L.pp_off);
+  uint8_t* pq = reinterpret_cast<uint8_t*>(smem + L.pq_off);
+  uint64_t* tpl = reinterpret_cast<uint64_t*>(smem + L.tpl_off);
+  for (uint32_t j = threadIdx.x; j < L.P; j += blockDim.x) {
+    pp[j] = plan.pair_p[j];
+    pq[j] = plan.pair_q[j];
+  }
+  const uint32_t k = static_cast<uint32_t>(ki) + 3;
+  for (uint32_t t = threadIdx.x; t < L.T; t += blockDim.x) {
+    uint64_t packed = 0;
+    for (uint32_t j = 0; j < k; ++j) packed |= static_cast<uint64_t>(plan.tpl[t * k + j]) << (8 * j);
+    tpl[t] = packed;
+  }
+}
+
+// Writes one cell at byte offset `at` of the tile's output.
+template <bool kNonTemporal>
+__device__ __forceinline__ void StoreCell(int64_t* out, uint32_t at, uint64_t sum) {
+  const uint32_t hi = static_cast<uint32_t>(sum >> 32);
+  const bool bad = (hi & static_cast<uint32_t>(kBadMask >> 32)) != 0;  // kBadMask has no low bits
+  const int64_t value = bad ? kInvalid : static_cast<int64_t>(sum & kCostMask);
+  int64_t* dst = reinterpret_cast<int64_t*>(reinterpret_cast<char*>(out) + at);
+  if (kNonTemporal) {
+    __builtin_nontemporal_store(value, dst);
+  } else {
+    *dst = value;
   }
 }
 
@@ -78,11 +101,11 @@ __device__ void WaveTile(const KoptLaunch& a, const KoptKLayout& L, const KoptSe
   const uint8_t* s_pq = reinterpret_cast<const uint8_t*>(smem + L.pq_off);
   const uint64_t* s_tpl = reinterpret_cast<const uint64_t*>(smem + L.tpl_off);
   longlong2* w_xy = reinterpret_cast<longlong2*>(wave + L.xy_off);
-  int32_t* w_node = reinterpret_cast<int32_t*>(wave + L.node_off);
-  uint32_t* w_cls = reinterpret_cast<uint32_t*>(wave + L.cls_off);
-  uint32_t* w_del = reinterpret_cast<uint32_t*>(wave + L.del_off);
+  uint32_t* w_eq = reinterpret_cast<uint32_t*>(wave + L.eq_off);
+  uint32_t* w_row = reinterpret_cast<uint32_t*>(wave + L.row_off);
   uint64_t* w_pair = reinterpret_cast<uint64_t*>(wave + L.pair_off);
-  const uint32_t P = L.P, T = L.T;
+  int32_t* w_node = reinterpret_cast<int32_t*>(wave + L.pair_off);  // dead before B writes pairs
+  const uint32_t P = L.P, T = L.T, stride = L.pair_stride;
 
   const uint64_t row0 = (tile - seg.tile_begin) * seg.rows_per_tile;
   const uint64_t left = seg.task_count - row0;
@@ -97,109 +120,146 @@ __device__ void WaveTile(const KoptLaunch& a, const KoptKLayout& L, const KoptSe
     const int32_t node = ports[r * 12 + p];
     w_node[s] = node;
     w_xy[s] = xy[node];
+    if (p == 0) w_row[r] = 0;
   }
   WaveSync();
 
-  // A2. port classes and deleted-edge keys
+  // A2. same-node masks; per row, the first port of every node held by two
+  // or more ports
   for (uint32_t s = lane; s < rows * kPorts; s += kWave) {
     const uint32_t r = s / kPorts, p = s - r * kPorts;
     const int32_t* nd = w_node + r * kPorts;
-    uint32_t c = p;
-    for (uint32_t j = 0; j < p; ++j) {
-      if (nd[j] == nd[p]) {
-        c = j;
-        break;
-      }
-    }
-    w_cls[s] = c;
-    if (p & 1u) {
-      uint32_t c0 = p - 1;
-      for (uint32_t j = 0; j + 1 < p; ++j) {
-        if (nd[j] == nd[p - 1]) {
-          c0 = j;
-          break;
-        }
-      }
-      w_del[r * K + (p >> 1)] = c0 < c ? c0 * 16 + c : c * 16 + c0;
-    }
+    const int32_t me = nd[p];
+    uint32_t eq = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kPorts; ++j) eq |= static_cast<uint32_t>(nd[j] == me) << j;
+    w_eq[s] = eq;
+    if (eq != (1u << p) && __builtin_ctz(eq) == p) atomicOr(&w_row[r], 1u << p);
   }
   WaveSync();
 
-  // B. pair records of every row
+  // B. pair records, pair-major: record (pair pi, row r) at pi * stride + r
   const int type = static_cast<int>(inst.distance_type);
+  const bool near = inst.near != 0;
   for (uint32_t s = lane; s < rows * P; s += kWave) {
     const uint32_t r = DivSmall(s, P), pi = s - r * P;
     const uint32_t p = s_pp[pi], q = s_pq[pi];
-    const uint32_t cp = w_cls[r * kPorts + p], cq = w_cls[r * kPorts + q];
-    const uint32_t key = cp < cq ? cp * 16 + cq : cq * 16 + cp;
-    bool bad = cp == cq;
-#pragma unroll
-    for (int e = 0; e < K; ++e) bad |= key == w_del[r * K + e];
+    const uint32_t eqp = w_eq[r * kPorts + p], eqq = w_eq[r * kPorts + q];
+    // self loop, or the node pair of deleted edge e = (2e, 2e+1) in either order
+    const bool self = ((eqp >> q) & 1u) != 0;
+    const bool readd = (((eqp & (eqq >> 1)) | ((eqp >> 1) & eqq)) & 0x5555u) != 0;
     uint64_t rec = kSentinel;
-    if (!bad) {
+    if (!self && !readd) {
       const longlong2 pa = w_xy[r * kPorts + p], pb = w_xy[r * kPorts + q];
-      rec = static_cast<uint64_t>(ExactDistance(pa.x, pa.y, pb.x, pb.y, type));
+      if (near) {
+        const int32_t dx = static_cast<int32_t>(static_cast<uint32_t>(pa.x) - static_cast<uint32_t>(pb.x));
+        const int32_t dy = static_cast<int32_t>(static_cast<uint32_t>(pa.y) - static_cast<uint32_t>(pb.y));
+        rec = static_cast<uint64_t>(ExactDistanceNear(dx, dy, type));
+      } else {
+        rec = static_cast<uint64_t>(ExactDistance(pa.x, pa.y, pb.x, pb.y, type));
+      }
+      if ((eqp & (eqp - 1)) != 0 && (eqq & (eqq - 1)) != 0) {
+        // both nodes hold two or more ports: count the edge in its counter
+        const uint32_t multi = w_row[r];
+        const uint32_t ia = __builtin_popcount(multi & (eqp ^ (eqp - 1)) & ~eqp);
+        const uint32_t ib = __builtin_popcount(multi & (eqq ^ (eqq - 1)) & ~eqq);
+        const uint32_t lo = ia < ib ? ia : ib, hi = ia < ib ? ib : ia;
+        // counter of node ranks lo < hi < 5
+        const uint32_t c = lo * (7 - lo) / 2 + hi - 1;
+        rec |= uint64_t{1} << (kCounterBase + 2 * c);
+      }
     }
-    w_pair[s] = rec | (static_cast<uint64_t>(key) << 48);
+    w_pair[pi * stride + r] = rec;
   }
   WaveSync();
 
-  // C. cells, row-major, coalesced stores
-  int64_t* out = a.costs + seg.cost_offset + row0 * T;
-  const uint32_t cells = rows * T;
-  uint32_t r = DivSmall(static_cast<uint32_t>(lane), T);
-  uint32_t t = static_cast<uint32_t>(lane) - r * T;
-  for (uint32_t c = lane; c < cells; c += kWave) {
-    const uint64_t tp = s_tpl[t];
-    const uint64_t* row = w_pair + r * P;
-    uint64_t sum = 0;
-    uint32_t key[K];
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-      const uint64_t rec = row[(tp >> (8 * j)) & 0xffu];
-      sum += rec & kCostMask;
-      key[j] = static_cast<uint32_t>(rec >> 48);
+  // C. cells: each lane keeps template t and walks rows [rb, rb + run)
+  int64_t* const out = a.costs + seg.cost_offset + row0 * T;  // uniform
+  const char* const pairs = reinterpret_cast<const char*>(w_pair);
+  // rows every lane of the pass has: all of them unless this is the last,
+  // partial tile of a segment with T <= 64
+  const uint32_t n = __builtin_amdgcn_readfirstlane(
+      T > static_cast<uint32_t>(kWave) ? rows : (rows == L.group * L.run ? L.run : 0u));
+  for (uint32_t pass = 0; pass < L.passes; ++pass) {
+    uint32_t g = 0, t = pass * kWave + static_cast<uint32_t>(lane);
+    if (T <= static_cast<uint32_t>(kWave)) {
+      g = DivSmall(static_cast<uint32_t>(lane), T);
+      t = static_cast<uint32_t>(lane) - g * T;
     }
-    bool bad = sum >= kSentinel;
+    const bool active = g < L.group && t < T;
+    const uint32_t rb = active ? g * L.run : 0u;
+    const uint64_t tp = s_tpl[active ? t : 0u];
+    uint32_t off[K];  // byte offset of record (pair j of the template, row rb)
 #pragma unroll
-    for (int i = 0; i < K; ++i) {
+    for (int j = 0; j < K; ++j) off[j] = (static_cast<uint32_t>((tp >> (8 * j)) & 0xffu) * stride + rb) * 8;
+    // byte offset of the lane's next cell; a tile's cells stay below 2^24
+    uint32_t at = (rb * T + t) * 8;
+    const uint32_t row_bytes = T * 8;
+    if (!active) continue;
+    uint32_t i = 0;
+    for (; i + kUnroll <= n; i += kUnroll) {
 #pragma unroll
-      for (int j = i + 1; j < K; ++j) bad |= key[i] == key[j];
+      for (uint32_t u = 0; u < kUnroll; ++u) {
+        uint64_t sum = 0;
+#pragma unroll
+        for (int j = 0; j < K; ++j) sum += *reinterpret_cast<const uint64_t*>(pairs + off[j] + (i + u) * 8);
+        StoreCell<kNonTemporal>(out, at, sum);
+        at += row_bytes;
+      }
     }
-    const int64_t value = bad ? kInvalid : static_cast<int64_t>(sum);
-    if (kNonTemporal) {
-      __builtin_nontemporal_store(value, out + c);
-    } else {
-      out[c] = value;
+    for (; i < n; ++i) {
+      uint64_t sum = 0;
+#pragma unroll
+      for (int j = 0; j < K; ++j) sum += *reinterpret_cast<const uint64_t*>(pairs + off[j] + i * 8);
+      StoreCell<kNonTemporal>(out, at, sum);
+      at += row_bytes;
     }
-    t += L.step_t;
-    r += L.step_r;
-    if (t >= T) {
-      t -= T;
-      ++r;
+    if (n == 0) {
+      // last tile of a segment: lanes hold runs of different lengths
+      const uint32_t re = rows < rb + L.run ? rows : rb + L.run;
+      for (uint32_t r = rb; r < re; ++r) {
+        uint64_t sum = 0;
+#pragma unroll
+        for (int j = 0; j < K; ++j) sum += *reinterpret_cast<const uint64_t*>(pairs + off[j] + (r - rb) * 8);
+        StoreCell<kNonTemporal>(out, at, sum);
+        at += row_bytes;
+      }
     }
   }
   WaveSync();
 }
 
-template <bool kNonTemporal>
-__global__ void __launch_bounds__(256) KoptKernel(KoptLaunch a) {
-  extern __shared__ __attribute__((aligned(16))) char smem[];
-  LoadPlans(a, smem);
+// The tiles of one k handed to this workgroup, one per wavefront at a time.
+template <int K, bool kNonTemporal>
+__device__ void TileLoop(const KoptLaunch& a, char* smem) {
+  constexpr int ki = K - 3;
+  LoadPlan(a, ki, smem);
   __syncthreads();
   const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
   const uint32_t wib = threadIdx.x / kWave;
   const uint32_t waves_per_block = blockDim.x / kWave;
   char* wave = smem + a.wave_base + wib * a.wave_bytes;
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * waves_per_block;
-  for (uint64_t tile = __builtin_amdgcn_readfirstlane(blockIdx.x * waves_per_block + wib);
-       tile < a.tile_count; tile += stride) {
+  const uint32_t block = blockIdx.x - a.k_block_begin[ki];
+  const uint32_t blocks = a.k_block_begin[ki + 1] - a.k_block_begin[ki];
+  const uint64_t stride = static_cast<uint64_t>(blocks) * waves_per_block;
+  const uint64_t end = a.k_tile_begin[ki] + a.k_tile_count[ki];
+  for (uint64_t tile = a.k_tile_begin[ki] +
+                       __builtin_amdgcn_readfirstlane(block * waves_per_block + wib);
+       tile < end; tile += stride) {
     const KoptSegment seg = a.segments[a.tile_segment ? a.tile_segment[tile] : 0u];
-    switch (seg.k) {
-      case 3: WaveTile<3, kNonTemporal>(a, a.layout[0], seg, tile, smem, wave, lane); break;
-      case 4: WaveTile<4, kNonTemporal>(a, a.layout[1], seg, tile, smem, wave, lane); break;
-      default: WaveTile<5, kNonTemporal>(a, a.layout[2], seg, tile, smem, wave, lane); break;
-    }
+    WaveTile<K, kNonTemporal>(a, a.layout[ki], seg, tile, smem, wave, lane);
+  }
+}
+
+template <bool kNonTemporal>
+__global__ void __launch_bounds__(256) KoptKernel(KoptLaunch a) {
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  if (blockIdx.x < a.k_block_begin[1]) {
+    TileLoop<3, kNonTemporal>(a, smem);
+  } else if (blockIdx.x < a.k_block_begin[2]) {
+    TileLoop<4, kNonTemporal>(a, smem);
+  } else {
+    TileLoop<5, kNonTemporal>(a, smem);
   }
 }
 
@@ -207,18 +267,12 @@ uint32_t Align16(uint32_t v) { return (v + 15u) & ~15u; }
 
 }  // namespace
 
-size_t KoptWaveBytes(uint32_t k, uint32_t rows, uint32_t pairs) {
-  const uint32_t ports = 2 * k;
-  return Align16(rows * ports * 16) + Align16(rows * ports * 4) + Align16(rows * ports * 4) +
-         Align16(rows * k * 4) + Align16(rows * pairs * 8);
-}
-
 size_t KoptPrepareLayout(KoptLaunch* a, const KoptPlan plans[3], const bool present[3],
                          const KoptTuning& tuning) {
   if (tuning.block == 0 || tuning.block > 256 || tuning.block % kWave != 0) {
     throw std::invalid_argument("k-opt block size must be a multiple of 64 up to 256");
   }
-  uint32_t off = 0;
+  uint32_t plan_bytes = 0;  // every workgroup holds the plan of its one k
   uint32_t wave_bytes = 0;
   for (int i = 0; i < 3; ++i) {
     KoptKLayout& L = a->layout[i];
@@ -226,47 +280,110 @@ size_t KoptPrepareLayout(KoptLaunch* a, const KoptPlan plans[3], const bool pres
     if (!present[i]) continue;
     const KoptPlan& plan = plans[i];
     const uint32_t k = i + 3, rows = tuning.rows_per_tile[i];
-    if (rows == 0) throw std::invalid_argument("rows per tile must be positive");
-    if (static_cast<uint64_t>(rows) * std::max(plan.pairs, plan.templates) >= (1u << 24)) {
-      throw std::invalid_argument("k-opt tile too large");
+    if (rows == 0 || rows > static_cast<uint32_t>(kWave)) {
+      throw std::invalid_argument("rows per tile must be in [1, 64]");
     }
+    if (plan.templates == 0 || plan.pairs == 0) throw std::invalid_argument("empty k-opt plan");
     L.present = 1;
     L.P = plan.pairs;
     L.T = plan.templates;
     L.rows = rows;
-    L.step_r = kWave / L.T;
-    L.step_t = kWave - L.step_r * L.T;
-    L.pp_off = off;
-    L.pq_off = off + kMaxPairs;
-    L.tpl_off = Align16(off + 2 * kMaxPairs);
-    off = Align16(L.tpl_off + L.T * 8);
+    if (L.T <= static_cast<uint32_t>(kWave)) {
+      L.group = kWave / L.T;
+      L.passes = 1;
+      L.run = (rows + L.group - 1) / L.group;
+    } else {
+      L.group = 1;
+      L.passes = (L.T + kWave - 1) / kWave;
+      L.run = rows;
+    }
+    L.pair_stride = ((rows + 1) & ~1u) + 1;
+    L.pp_off = 0;
+    L.pq_off = kMaxPairs;
+    L.tpl_off = Align16(2 * kMaxPairs);
+    plan_bytes = std::max(plan_bytes, Align16(L.tpl_off + L.T * 8));
     const uint32_t ports = 2 * k;
     L.xy_off = 0;
-    L.node_off = Align16(rows * ports * 16);
-    L.cls_off = L.node_off + Align16(rows * ports * 4);
-    L.del_off = L.cls_off + Align16(rows * ports * 4);
-    L.pair_off = L.del_off + Align16(rows * k * 4);
-    wave_bytes = std::max<uint32_t>(wave_bytes, static_cast<uint32_t>(KoptWaveBytes(k, rows, L.P)));
+    L.eq_off = Align16(rows * ports * 16);
+    L.row_off = L.eq_off + Align16(rows * ports * 4);
+    L.pair_off = L.row_off + Align16(rows * 4);
+    // the port node ids of phase A share the pair record area
+    const uint32_t pair_bytes = std::max(L.P * L.pair_stride * 8, rows * ports * 4);
+    wave_bytes = std::max<uint32_t>(wave_bytes, L.pair_off + Align16(pair_bytes));
   }
-  a->wave_base = off;
+  a->wave_base = plan_bytes;
   a->wave_bytes = wave_bytes;
-  return static_cast<size_t>(off) + static_cast<size_t>(wave_bytes) * (tuning.block / kWave);
+  return static_cast<size_t>(plan_bytes) + static_cast<size_t>(wave_bytes) * (tuning.block / kWave);
+}
+
+uint64_t KoptCountTiles(const KoptSegment* segments, size_t count, const KoptTuning& tuning) {
+  uint64_t tiles = 0;
+  for (size_t i = 0; i < count; ++i) {
+    const uint32_t rows = tuning.rows_per_tile[segments[i].k - 3];
+    tiles += (segments[i].task_count + rows - 1) / rows;
+  }
+  return tiles;
+}
+
+void KoptAssignTiles(KoptSegment* segments, size_t count, const KoptTuning& tuning,
+                     uint32_t* tile_map, KoptLaunch* a) {
+  uint64_t tile = 0;
+  for (int i = 0; i < 3; ++i) {
+    a->k_tile_begin[i] = tile;
+    for (size_t s = 0; s < count; ++s) {
+      KoptSegment& seg = segments[s];
+      if (seg.k != static_cast<uint32_t>(3 + i)) continue;
+      seg.rows_per_tile = tuning.rows_per_tile[i];
+      seg.tile_begin = tile;
+      const uint64_t tiles = (seg.task_count + seg.rows_per_tile - 1) / seg.rows_per_tile;
+      if (tile_map != nullptr) std::fill(tile_map + tile, tile_map + tile + tiles, static_cast<uint32_t>(s));
+      tile += tiles;
+    }
+    a->k_tile_count[i] = tile - a->k_tile_begin[i];
+  }
+  a->tile_count = tile;
 }
 
 void LaunchKopt(const KoptLaunch& args, const KoptTuning& tuning, size_t shared_bytes,
                 hipStream_t stream) {
   if (args.tile_count == 0) return;
   if (shared_bytes > 65536) throw std::invalid_argument("k-opt launch needs more than 64 KiB of LDS");
+  uint64_t tiles = 0;
+  for (int i = 0; i < 3; ++i) tiles += args.k_tile_count[i];
+  if (tiles != args.tile_count) throw std::logic_error("k-opt tile ranges do not cover the launch");
   const uint32_t waves_per_block = tuning.block / kWave;
-  const uint64_t needed = (args.tile_count + waves_per_block - 1) / waves_per_block;
-  uint64_t grid = tuning.grid == 0 ? needed : tuning.grid;
-  if (grid > needed) grid = needed;
-  if (grid > 0x7fffffffull) grid = 0x7fffffffull;
-  const dim3 g(static_cast<uint32_t>(grid)), b(tuning.block);
+  // workgroups per k: one tile per wavefront, or a share of tuning.grid in
+  // proportion to the cells of each k
+  uint64_t need[3], blocks[3];
+  uint64_t total_need = 0;
+  double work[3], total_work = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    need[i] = (args.k_tile_count[i] + waves_per_block - 1) / waves_per_block;
+    total_need += need[i];
+    work[i] = args.k_tile_count[i] == 0
+                  ? 0.0
+                  : static_cast<double>(args.k_tile_count[i]) * args.layout[i].rows *
+                        (args.layout[i].T + args.layout[i].P);
+    total_work += work[i];
+  }
+  for (int i = 0; i < 3; ++i) {
+    blocks[i] = need[i];
+    if (tuning.grid != 0 && tuning.grid < total_need && need[i] > 0) {
+      const double share = static_cast<double>(tuning.grid) * work[i] / total_work;
+      blocks[i] = std::min<uint64_t>(need[i], std::max<uint64_t>(1, static_cast<uint64_t>(share + 0.5)));
+    }
+  }
+  if (blocks[0] + blocks[1] + blocks[2] > 0x7fffffffull) {
+    throw std::invalid_argument("k-opt launch needs too many workgroups");
+  }
+  KoptLaunch a = args;
+  a.k_block_begin[0] = 0;
+  for (int i = 0; i < 3; ++i) a.k_block_begin[i + 1] = a.k_block_begin[i] + static_cast<uint32_t>(blocks[i]);
+  const dim3 g(a.k_block_begin[3]), b(tuning.block);
   if (tuning.non_temporal) {
-    hipLaunchKernelGGL(KoptKernel<true>, g, b, shared_bytes, stream, args);
+    hipLaunchKernelGGL(KoptKernel<true>, g, b, shared_bytes, stream, a);
   } else {
-    hipLaunchKernelGGL(KoptKernel<false>, g, b, shared_bytes, stream, args);
+    hipLaunchKernelGGL(KoptKernel<false>, g, b, shared_bytes, stream, a);
   }
   CheckHip(hipGetLastError(), "k-opt kernel launch");
 }

@@ -20,6 +20,9 @@ ee_status StatusFromException();
 double MsSince(std::chrono::steady_clock::time_point t0);
 size_t Grow(size_t current, size_t required);
 bool DistanceSafe(const int64_t* x, const int64_t* y, int32_t n);
+bool NearExtent(const int64_t* x, const int64_t* y, int32_t n);
+KoptInstance MakeKoptInstance(uint64_t coord_base, const int64_t* x, const int64_t* y, int32_t n,
+                              uint8_t distance_type);
 int CountDevices();
 
 // Owning device allocation of `count` elements of T on one device.
@@ -105,6 +108,7 @@ struct JvCache {
   bool valid = false;
   int32_t dimension = 0;
   uint8_t distance_type = 0;
+  bool near = false;  // NearExtent of the cached coordinates
   // exact static key (host copies)
   std::vector<int64_t> x, y, edge_weight;
   std::vector<int32_t> edge_u, edge_v;
@@ -135,6 +139,7 @@ struct KoptCache {
   bool snapshot_valid = false;
   int32_t dimension = 0;
   uint8_t distance_type = 0;
+  bool near = false;  // NearExtent of the snapshot
   std::vector<int64_t> x, y;
   DeviceArray<longlong2> d_xy;
   std::array<TemplateEntry, 3> templates;

@@ -92,6 +92,30 @@ bool DistanceSafe(const int64_t* x, const int64_t* y, int32_t n) {
   return dx * dx + dy * dy <= static_cast<unsigned __int128>(UINT64_MAX);
 }
 
+// Per-axis coordinate extent below 2^25: every distance of the instance can
+// take the FP64 path (ExactDistanceNear).
+bool NearExtent(const int64_t* x, const int64_t* y, int32_t n) {
+  if (n <= 0) return true;
+  int64_t lx = x[0], hx = x[0], ly = y[0], hy = y[0];
+  for (int32_t i = 1; i < n; ++i) {
+    lx = std::min(lx, x[i]);
+    hx = std::max(hx, x[i]);
+    ly = std::min(ly, y[i]);
+    hy = std::max(hy, y[i]);
+  }
+  return static_cast<__int128>(hx) - lx < kNearExtent && static_cast<__int128>(hy) - ly < kNearExtent;
+}
+
+KoptInstance MakeKoptInstance(uint64_t coord_base, const int64_t* x, const int64_t* y, int32_t n,
+                              uint8_t distance_type) {
+  KoptInstance inst{};
+  inst.coord_base = coord_base;
+  inst.n = static_cast<uint32_t>(n);
+  inst.distance_type = distance_type;
+  inst.near = NearExtent(x, y, n) ? 1 : 0;
+  return inst;
+}
+
 std::vector<longlong2> Interleave(const int64_t* x, const int64_t* y, int32_t n) {
   std::vector<longlong2> xy(static_cast<size_t>(n));
   for (int32_t i = 0; i < n; ++i) {
@@ -239,6 +263,7 @@ bool PrepareJvStatic(Context& ctx, const ee_graph_view& g) {
   c.edge_weight.assign(g.edge_weight, g.edge_weight + m);
   c.dimension = g.dimension;
   c.distance_type = g.distance_type;
+  c.near = NearExtent(g.x, g.y, g.dimension);
   c.valid = true;
   return false;
 }
@@ -285,6 +310,7 @@ JvLaunch JvArgs(Context& ctx, int32_t m, int type) {
   JvLaunch a{};
   a.edge_count = m;
   a.distance_type = type;
+  a.near = c.near ? 1 : 0;
   a.edge_uv = c.d_uv.get();
   a.edge_weight = c.d_weight.get();
   a.edge_active = c.d_active.get();
@@ -531,6 +557,7 @@ void EvalKoptCosts(Context& ctx, const int64_t* x, const int64_t* y, int32_t n, 
     c.y.assign(y, y + nn);
     c.dimension = n;
     c.distance_type = type;
+    c.near = NearExtent(x, y, n);
     c.snapshot_valid = true;
   }
 
@@ -558,20 +585,23 @@ void EvalKoptCosts(Context& ctx, const int64_t* x, const int64_t* y, int32_t n, 
 
   const uint32_t rows = TileRows(ctx.kopt_tuning, k);
   KoptTables tables{};
-  tables.instance = KoptInstance{0, static_cast<uint32_t>(n), type};
+  tables.instance = KoptInstance{};
+  tables.instance.n = static_cast<uint32_t>(n);
+  tables.instance.distance_type = type;
+  tables.instance.near = c.near ? 1 : 0;
   tables.segment = KoptSegment{0, k, 0, task_count, 0, 0, rows, 0};
+  KoptLaunch a{};
+  KoptAssignTiles(&tables.segment, 1, ctx.kopt_tuning, nullptr, &a);
   auto t0 = std::chrono::steady_clock::now();
   c.d_tasks.Upload(tasks, task_count, ctx.stream);
   c.d_tables.Upload(&tables, 1, ctx.stream);
   ctx.Sync();
   u.h2d_ms = MsSince(t0);
 
-  KoptLaunch a{};
   a.xy = c.d_xy.get();
   a.instances = &c.d_tables.get()->instance;
   a.segments = &c.d_tables.get()->segment;
   a.segment_count = 1;
-  a.tile_count = (task_count + rows - 1) / rows;
   a.tasks = c.d_tasks.get();
   a.plans = c.d_plans.get();
   a.costs = c.d_costs.get();

@@ -179,26 +179,23 @@ bool Server::Issue(Slot& slot, std::vector<std::shared_ptr<Request>> batch) {
       slot.d_segments.Allocate(Grow(slot.d_segments.count(), batch.size()), ctx->device);
     }
     if (slot.d_tiles.count() < tiles) slot.d_tiles.Allocate(Grow(slot.d_tiles.count(), tiles), ctx->device);
-    uint64_t row = 0, tile = 0;
+    uint64_t row = 0;
     for (size_t i = 0; i < batch.size(); ++i) {
       const Request& r = *batch[i];
       std::memcpy(slot.h_tasks.get() + row, r.tasks.data(), r.tasks.size() * sizeof(ee_kopt_task));
-      const uint32_t rpt = tuning.rows_per_tile[r.k - 3];
-      const uint64_t n_tiles = (r.tasks.size() + rpt - 1) / rpt;
-      slot.h_segments.get()[i] = KoptSegment{r.graph, r.k, row, r.tasks.size(), slot.offsets[i], tile, rpt, 0};
-      std::fill(slot.h_tiles.get() + tile, slot.h_tiles.get() + tile + n_tiles, static_cast<uint32_t>(i));
+      slot.h_segments.get()[i] = KoptSegment{r.graph, r.k, row, r.tasks.size(), slot.offsets[i], 0, 0, 0};
       row += r.tasks.size();
-      tile += n_tiles;
     }
+    KoptLaunch a{};
+    KoptAssignTiles(slot.h_segments.get(), batch.size(), tuning, slot.h_tiles.get(), &a);
+    if (a.tile_count != tiles) throw std::logic_error("server tile count mismatch");
     slot.d_tasks.Upload(slot.h_tasks.get(), rows, slot.stream);
     slot.d_segments.Upload(slot.h_segments.get(), batch.size(), slot.stream);
     slot.d_tiles.Upload(slot.h_tiles.get(), tiles, slot.stream);
-    KoptLaunch a{};
     a.xy = d_xy.get();
     a.instances = d_instances.get();
     a.segments = slot.d_segments.get();
     a.segment_count = static_cast<uint32_t>(batch.size());
-    a.tile_count = tiles;
     a.tasks = slot.d_tasks.get();
     a.plans = d_plans.get();
     a.costs = slot.d_costs.get();
@@ -400,7 +397,7 @@ ee_status ee_server_add_graph(ee_server* server, const int64_t* x, const int64_t
       throw std::invalid_argument("k-opt cost needs distance-safe integer coordinates");
     }
     std::lock_guard<std::mutex> lk(s.mu);
-    ee::KoptInstance inst{s.host_xy.size(), static_cast<uint32_t>(dimension), distance_type};
+    const ee::KoptInstance inst = ee::MakeKoptInstance(s.host_xy.size(), x, y, dimension, distance_type);
     for (int32_t i = 0; i < dimension; ++i) s.host_xy.push_back(longlong2{x[i], y[i]});
     s.instances.push_back(inst);
     s.graphs_dirty = true;

@@ -31,6 +31,7 @@ struct Session {
   Context* ctx;
   int32_t n = 0, m = 0;
   uint8_t type = 0;
+  bool near = false;  // coordinate extents below 2^25: FP64 distances on the device
   std::vector<int64_t> x, y, weight;
   std::vector<int32_t> u, v;
   std::vector<uint8_t> active;
@@ -124,6 +125,7 @@ struct Session {
       JvLaunch a{};
       a.edge_count = m;
       a.distance_type = type;
+      a.near = near ? 1 : 0;
       a.edge_uv = d_uv.get();
       a.edge_weight = d_weight.get();
       a.edge_active = d_active.get();
@@ -267,6 +269,7 @@ ee_status ee_jv_session_create(ee_ctx* handle, const ee_graph_view* g, ee_jv_ses
     if (!ee::DistanceSafe(s->x.data(), s->y.data(), s->n)) {
       throw std::runtime_error("HIP JV needs integer coordinates whose squared distances fit uint64");
     }
+    s->near = ee::NearExtent(s->x.data(), s->y.data(), s->n);
     if (m > 0) {
       s->u.assign(g->edge_u, g->edge_u + m);
       s->v.assign(g->edge_v, g->edge_v + m);

@@ -80,7 +80,9 @@ void ee_isqrt_host(const uint64_t* s, uint64_t* root, size_t count);
 ee_status ee_isqrt_device(ee_ctx* ctx, const uint64_t* s, uint64_t* root, size_t count);
 
 /* Exact distances of node pairs (a[i], b[i]) on the context's device
- * (synchronous); coordinates must be distance safe. */
+ * (synchronous); coordinates must be distance safe.  When both coordinate
+ * extents are below 2^25 this runs the FP64 distance the k-opt kernel uses
+ * for such instances, otherwise the 64-bit integer one. */
 ee_status ee_distances_device(ee_ctx* ctx, const int64_t* x, const int64_t* y, int32_t n,
                               uint8_t distance_type, const int32_t* a, const int32_t* b,
                               size_t count, int64_t* out);

@@ -190,8 +190,9 @@ typedef struct {
 } ee_kopt_segment_desc;
 
 typedef struct {
-  uint32_t rows_per_tile[3]; /* task rows per workgroup tile for k = 3, 4, 5 */
-  uint32_t grid;             /* 0: one workgroup per tile; else a persistent grid */
+  uint32_t rows_per_tile[3]; /* task rows per wavefront tile for k = 3, 4, 5, in [1, 64] */
+  uint32_t grid;             /* 0: one tile per wavefront; else a persistent grid of
+                                this many workgroups, shared among k by work */
   uint32_t block;            /* threads per workgroup: 64, 128, 192 or 256 */
   uint32_t non_temporal;     /* 1: cost stores bypass the caches */
 } ee_kopt_tuning;

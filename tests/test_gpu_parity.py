@@ -81,6 +81,27 @@ def test_device_distances_match_oracle(ctx, dtype):
         assert out[i] == v.value
 
 
+@pytest.mark.parametrize("dtype", [EUC_2D, CEIL_2D])
+def test_device_near_distances_are_exact(ctx, dtype):
+    """Coordinate extents below 2^25 take the FP64 path: perfect squares,
+    their neighbours and the EUC_2D half-way points at the top of the range."""
+    say(f"device FP64 distances dtype={dtype}")
+    lim = (1 << 25) - 1
+    rng = np.random.default_rng(23)
+    m = np.arange(lim - 4000, lim + 1, dtype=np.int64)
+    # node 0 at the origin; nodes on both axes and random points in the box
+    x = np.concatenate([[0], m, np.zeros_like(m), rng.integers(0, lim + 1, 20000)]).astype(np.int64)
+    y = np.concatenate([[0], np.zeros_like(m), m, rng.integers(0, lim + 1, 20000)]).astype(np.int64)
+    y[1:len(m) + 1:3] = 1  # s = m^2 + 1
+    n = len(x)
+    a = np.concatenate([np.zeros(n - 1, np.int32), rng.integers(0, n, 200000).astype(np.int32)])
+    b = np.concatenate([np.arange(1, n, dtype=np.int32), rng.integers(0, n, 200000).astype(np.int32)])
+    out = np.empty(len(a), np.int64)
+    _lib.check(_lib.lib().ee_distances_device(ctx.handle, O.p(x), O.p(y), n, dtype, O.p(a),
+                                              O.p(b), len(a), O.p(out)))
+    assert np.array_equal(out, exact_distances(x[a], y[a], x[b], y[b], dtype))
+
+
 # ------------------------------------------------------------------- k-opt
 def seventeen_graph(dtype=EUC_2D):
     x = np.array([(7 * i) % 23 for i in range(17)], np.int64)
@@ -126,14 +147,17 @@ def test_kopt_reference_unit_test_scenario(ctx):
     backend.clear_kopt_cost_cuda_cache(ctx)
 
 
+@pytest.mark.parametrize("half", [5000, (1 << 24) - 1, 1 << 30])
 @pytest.mark.parametrize("dtype", [EUC_2D, CEIL_2D])
 @pytest.mark.parametrize("k", [3, 4, 5])
-def test_kopt_random_rows_match_oracle(ctx, k, dtype):
-    say(f"k-opt random rows k={k} dtype={dtype}")
-    rng = random.Random(31 * k + dtype)
+def test_kopt_random_rows_match_oracle(ctx, k, dtype, half):
+    """Coordinates in [-half, half): the FP64 distance path up to extents of
+    2^25 - 2, the 64-bit integer path beyond."""
+    say(f"k-opt random rows k={k} dtype={dtype} half={half}")
+    rng = random.Random(31 * k + dtype + half)
     n = 64
-    x = np.array([rng.randrange(-5000, 5000) for _ in range(n)], np.int64)
-    y = np.array([rng.randrange(-5000, 5000) for _ in range(n)], np.int64)
+    x = np.array([rng.randrange(-half, half) for _ in range(n)], np.int64)
+    y = np.array([rng.randrange(-half, half) for _ in range(n)], np.int64)
     g = GraphSnapshot.from_edges(x, y, dtype, np.zeros(0, np.int32), np.zeros(0, np.int32))
     tasks = np.zeros(777, _lib.KOPT_TASK_DTYPE)
     for t in tasks:
@@ -309,6 +333,20 @@ def test_jv_random_graphs_with_inactive_edges(ctx, seed, n, k):
     g.edge_active[rng.random(g.edge_count) < 0.15] = 0
     g.rebuild_csr()
     assert g.degrees().max() > (32 if k >= 40 else 0)
+    w, _ = backend.find_jv_witnesses(g, ctx)
+    assert np.array_equal(w, O.OGraph.from_snapshot(g).witnesses())
+
+
+@pytest.mark.parametrize("box", [1 << 25, (1 << 25) + 1, (1 << 31) - 1])
+def test_jv_coordinate_ranges(ctx, box):
+    """Coordinate extents just inside and outside the FP64 distance path, and
+    at the distance-safe limit."""
+    say(f"JV box={box}")
+    rng = np.random.default_rng(11)
+    x = rng.integers(0, box, 700).astype(np.int64)
+    y = rng.integers(0, box, 700).astype(np.int64)
+    x[:2], y[:2] = (0, box - 1), (box - 1, 0)  # extents exactly box - 1
+    g = W.knn_graph(x, y, EUC_2D, 9)
     w, _ = backend.find_jv_witnesses(g, ctx)
     assert np.array_equal(w, O.OGraph.from_snapshot(g).witnesses())
 
