@@ -29,16 +29,14 @@ struct Batch {
   std::vector<KoptInstance> instances;
   std::vector<longlong2> host_xy;
   DeviceArray<longlong2> d_xy;
-  DeviceArray<KoptInstance> d_instances;
   bool instances_dirty = false;
   // work
   std::vector<KoptSegment> segments;
-  DeviceArray<KoptSegment> d_segments;
   DeviceArray<ee_kopt_task> d_tasks;
   DeviceArray<int64_t> d_costs;
-  std::vector<uint32_t> tile_map;
-  DeviceArray<uint32_t> d_tile_map;
-  KoptLaunch ranges{};  // per-k tile ranges of the current tile map
+  std::vector<KoptTile> tiles;
+  DeviceArray<KoptTile> d_tiles;
+  KoptLaunch ranges{};  // per-k tile ranges of the current tiles
   uint64_t task_count = 0, cell_count = 0, tile_count = 0;
   bool k_present[3] = {false, false, false};
   // plans for the canonical templates of k = 3, 4, 5
@@ -68,20 +66,17 @@ struct Batch {
     if (instances_dirty) {
       d_xy.Allocate(host_xy.size(), ctx->device);
       d_xy.Upload(host_xy.data(), host_xy.size(), ctx->stream);
-      d_instances.Allocate(instances.size(), ctx->device);
-      d_instances.Upload(instances.data(), instances.size(), ctx->stream);
       ctx->Sync();
       instances_dirty = false;
     }
     if (!layout_dirty) return;
-    tile_map.assign(KoptCountTiles(segments.data(), segments.size(), tuning), 0u);
-    KoptAssignTiles(segments.data(), segments.size(), tuning, tile_map.data(), &ranges);
+    tiles.assign(KoptCountTiles(segments.data(), segments.size(), tuning), KoptTile{});
+    KoptAssignTiles(segments.data(), segments.size(), instances.data(), plans.data(), tuning,
+                    tiles.data(), &ranges);
     tile_count = ranges.tile_count;
     if (SharedBytes() > 65536) throw std::invalid_argument("k-opt tuning needs more than 64 KiB of LDS");
-    d_segments.Allocate(std::max<size_t>(segments.size(), 1), ctx->device);
-    d_segments.Upload(segments.data(), segments.size(), ctx->stream);
-    d_tile_map.Allocate(std::max<size_t>(tile_map.size(), 1), ctx->device);
-    d_tile_map.Upload(tile_map.data(), tile_map.size(), ctx->stream);
+    d_tiles.Allocate(std::max<size_t>(tiles.size(), 1), ctx->device);
+    d_tiles.Upload(tiles.data(), tiles.size(), ctx->stream);
     ctx->Sync();
     layout_dirty = false;
     DropGraph();
@@ -90,13 +85,10 @@ struct Batch {
   KoptLaunch Args() {
     KoptLaunch a = ranges;
     a.xy = d_xy.get();
-    a.instances = d_instances.get();
-    a.segments = d_segments.get();
-    a.segment_count = static_cast<uint32_t>(segments.size());
+    a.tiles = d_tiles.get();
     a.tasks = d_tasks.get();
     a.plans = d_plans.get();
     a.costs = d_costs.get();
-    a.tile_segment = d_tile_map.get();
     return a;
   }
 
@@ -132,8 +124,7 @@ struct Batch {
   }
 
   uint64_t ResidentBytes() const {
-    return d_xy.bytes() + d_instances.bytes() + d_segments.bytes() + d_tasks.bytes() +
-           d_costs.bytes() + d_tile_map.bytes() + d_plans.bytes();
+    return d_xy.bytes() + d_tasks.bytes() + d_costs.bytes() + d_tiles.bytes() + d_plans.bytes();
   }
 };
 

@@ -44,7 +44,8 @@ EE_HD int64_t ExactDistance(int64_t ax, int64_t ay, int64_t bx, int64_t by, int 
   const uint64_t s = dx * dx + dy * dy;
   const uint64_t r = IsqrtFloor(s);
   const uint64_t rr = r * r;
-  const uint64_t up = type == 0 ? static_cast<uint64_t>(s - rr > r) : static_cast<uint64_t>(rr != s);
+  // rr <= s, so for CEIL_2D "rr != s" is "s - rr > 0"
+  const uint64_t up = static_cast<uint64_t>(s - rr > (type == 0 ? r : 0));
   return static_cast<int64_t>(r + up);
 }
 
@@ -52,18 +53,30 @@ EE_HD int64_t ExactDistance(int64_t ax, int64_t ay, int64_t bx, int64_t by, int 
 constexpr int64_t kNearExtent = int64_t{1} << 25;
 
 // Same result as ExactDistance for |dx|, |dy| < 2^25, entirely in FP64:
-// s = dx^2 + dy^2 < 2^51, so s, r*r, (r+1)^2 and s - r*r are all exact
-// doubles, and the two integer-valued corrections move any root estimate
-// within one unit of floor(sqrt(s)) onto it.
+// s = dx^2 + dy^2 < 2^51, so s, r*r and s - r*r are exact doubles for every
+// r below 2^26.  The root estimate is within one unit of m = floor(sqrt(s)),
+// so floor(estimate) is m - 1, m or m + 1, and one step moves it onto m:
+// s - r*r < 0 means r = m + 1, s - r*r > 2r means r = m - 1 (m itself has
+// 0 <= s - m*m <= 2m).  On the device the estimate is the hardware reciprocal
+// square root (relative error about 2^-23) refined by one Newton step (about
+// 2^-45, i.e. below 2^-19 absolute for roots below 2^26).
 EE_HD int64_t ExactDistanceNear(int32_t dx, int32_t dy, int type) {
   const double fx = static_cast<double>(dx), fy = static_cast<double>(dy);
   const double s = fma(fx, fx, fy * fy);
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double y = __builtin_amdgcn_rsq(s);
+  const double g = s * y, h = 0.5 * y;
+  const double est = floor(fma(g, fma(-h, g, 0.5), g));
+  double r = s > 0.0 ? est : 0.0;  // rsq(0) is +inf
+#else
   double r = floor(sqrt(s));
-  if (r * r > s) r -= 1.0;
-  if ((r + 1.0) * (r + 1.0) <= s) r += 1.0;
-  const double rem = s - r * r;
-  const bool up = type == 0 ? rem > r : rem != 0.0;
-  return static_cast<int64_t>(r) + (up ? 1 : 0);
+#endif
+  const double rem0 = fma(-r, r, s);
+  r += rem0 < 0.0 ? -1.0 : (rem0 > r + r ? 1.0 : 0.0);
+  const double rem = fma(-r, r, s);
+  // rem >= 0 now, so for CEIL_2D "rem != 0" is "rem > 0"
+  const bool up = rem > (type == 0 ? r : 0.0);
+  return static_cast<int64_t>(static_cast<uint32_t>(r)) + (up ? 1 : 0);
 }
 
 }  // namespace ee

@@ -35,9 +35,8 @@ struct KoptPlan {
   uint32_t templates = 0;
   uint32_t pairs = 0;
   uint32_t reserved = 0;
-  uint8_t pair_p[kMaxPairs] = {};
-  uint8_t pair_q[kMaxPairs] = {};
-  uint8_t tpl[kMaxTemplates * kMaxK] = {};
+  uint16_t pair_pq[kMaxPairs] = {};  // ports p | q << 8 of pair i, p < q
+  uint64_t tpl[kMaxTemplates] = {};  // pair index of added edge j in byte j
 };
 
 // One rectangular block of the cost output: task rows [task_offset,
@@ -52,6 +51,18 @@ struct KoptSegment {
   uint64_t tile_begin;  // first global tile index of this segment
   uint32_t rows_per_tile;
   uint32_t reserved;
+};
+
+// One wavefront tile: `rows` task rows starting at task_index in the task
+// pool, whose cells start at cost_index in the cost pool, on the instance
+// whose coordinates start at coord_base in the coordinate pool.
+struct KoptTile {
+  uint64_t task_index;
+  uint64_t cost_index;
+  uint64_t coord_base;
+  uint32_t rows;
+  uint16_t distance_type;
+  uint16_t near;
 };
 
 // Coordinates of one instance: xy[base .. base + n) in the coordinate pool.
@@ -74,7 +85,8 @@ struct KoptKLayout {
   uint32_t present;
   uint32_t P, T, rows;
   uint32_t group, passes, run, pair_stride;
-  uint32_t pp_off, pq_off, tpl_off;
+  uint32_t pair_step_r, pair_step_p;  // 64 = pair_step_r * P + pair_step_p
+  uint32_t pq_off, tpl_off;
   uint32_t xy_off, eq_off, row_off, pair_off;
 };
 
@@ -84,10 +96,7 @@ struct KoptKLayout {
 // every workgroup runs code for one k only.
 struct KoptLaunch {
   const longlong2* xy;         // coordinate pool
-  const KoptInstance* instances;
-  const KoptSegment* segments;
-  uint32_t segment_count;
-  uint32_t reserved;
+  const KoptTile* tiles;       // [tile_count]
   uint64_t tile_count;
   uint64_t k_tile_begin[3];
   uint64_t k_tile_count[3];
@@ -95,7 +104,6 @@ struct KoptLaunch {
   const ee_kopt_task* tasks;   // task pool
   const KoptPlan* plans;       // [3] for k = 3, 4, 5
   int64_t* costs;              // cost pool
-  const uint32_t* tile_segment;  // [tile_count] segment index of each tile, or NULL for one segment
   KoptKLayout layout[3];
   uint32_t wave_base;   // LDS offset of the first wave's tile arrays
   uint32_t wave_bytes;  // LDS bytes per wave
@@ -113,10 +121,11 @@ struct KoptTuning {
 // Number of tiles the segments split into under `tuning`.
 uint64_t KoptCountTiles(const KoptSegment* segments, size_t count, const KoptTuning& tuning);
 // Sets rows_per_tile and tile_begin of every segment, numbering tiles k by k,
-// writes the segment index of every tile to tile_map (KoptCountTiles
-// entries) and the per-k tile ranges to `a`.
-void KoptAssignTiles(KoptSegment* segments, size_t count, const KoptTuning& tuning,
-                     uint32_t* tile_map, KoptLaunch* a);
+// writes the descriptor of every tile to `tiles` (KoptCountTiles entries; the
+// segments' instances index `instances`) and the per-k tile ranges to `a`.
+void KoptAssignTiles(KoptSegment* segments, size_t count, const KoptInstance* instances,
+                     const KoptPlan plans[3], const KoptTuning& tuning, KoptTile* tiles,
+                     KoptLaunch* a);
 // Fills the layout of `a` for the plans present and returns the dynamic LDS
 // bytes of one workgroup.
 size_t KoptPrepareLayout(KoptLaunch* a, const KoptPlan plans[3], const bool present[3],

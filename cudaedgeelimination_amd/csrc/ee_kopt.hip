@@ -60,25 +60,6 @@ __device__ __forceinline__ uint32_t DivSmall(uint32_t c, uint32_t d) {
   return q;
 }
 
-// Loads the plan of k = 3 + ki into the block-shared LDS prefix.
-__device__ void LoadPlan(const KoptLaunch& a, int ki, char* smem) {
-  const KoptKLayout& L = a.layout[ki];
-  const KoptPlan& plan = a.plans[ki];
-  uint8_t* pp = reinterpret_cast<uint8_t*>(smem + L.pp_off);
-  uint8_t* pq = reinterpret_cast<uint8_t*>(smem + L.pq_off);
-  uint64_t* tpl = reinterpret_cast<uint64_t*>(smem + L.tpl_off);
-  for (uint32_t j = threadIdx.x; j < L.P; j += blockDim.x) {
-    pp[j] = plan.pair_p[j];
-    pq[j] = plan.pair_q[j];
-  }
-  const uint32_t k = static_cast<uint32_t>(ki) + 3;
-  for (uint32_t t = threadIdx.x; t < L.T; t += blockDim.x) {
-    uint64_t packed = 0;
-    for (uint32_t j = 0; j < k; ++j) packed |= static_cast<uint64_t>(plan.tpl[t * k + j]) << (8 * j);
-    tpl[t] = packed;
-  }
-}
-
 // Writes one cell at byte offset `at` of the tile's output.
 template <bool kNonTemporal>
 __device__ __forceinline__ void StoreCell(int64_t* out, uint32_t at, uint64_t sum) {
@@ -93,36 +74,63 @@ __device__ __forceinline__ void StoreCell(int64_t* out, uint32_t at, uint64_t su
   }
 }
 
-template <int K, bool kNonTemporal>
-__device__ void WaveTile(const KoptLaunch& a, const KoptKLayout& L, const KoptSegment& seg,
-                         uint64_t tile, const char* smem, char* wave, int lane) {
+// Phase A: the tile's port node ids (into the pair record area, which is
+// free until phase B) and their coordinates, two gathers in flight per lane.
+template <int K>
+__device__ __forceinline__ void StagePorts(const KoptLaunch& a, const KoptKLayout& L,
+                                           const KoptTile& d, char* wave, int lane) {
   constexpr uint32_t kPorts = 2 * K;
-  const uint8_t* s_pp = reinterpret_cast<const uint8_t*>(smem + L.pp_off);
-  const uint8_t* s_pq = reinterpret_cast<const uint8_t*>(smem + L.pq_off);
-  const uint64_t* s_tpl = reinterpret_cast<const uint64_t*>(smem + L.tpl_off);
+  constexpr uint32_t kBatch = 2;
   longlong2* w_xy = reinterpret_cast<longlong2*>(wave + L.xy_off);
+  uint32_t* w_row = reinterpret_cast<uint32_t*>(wave + L.row_off);
+  int32_t* w_node = reinterpret_cast<int32_t*>(wave + L.pair_off);
+  const uint32_t total = d.rows * kPorts;
+  const longlong2* xy = a.xy + d.coord_base;
+  const int32_t* ports = reinterpret_cast<const int32_t*>(a.tasks + d.task_index);
+  for (uint32_t base = 0; base < total; base += kBatch * kWave) {
+    // lanes past the end re-read port 0 of the tile, so every load is in range
+    int32_t node[kBatch];
+#pragma unroll
+    for (uint32_t i = 0; i < kBatch; ++i) {
+      const uint32_t s = base + i * kWave + static_cast<uint32_t>(lane);
+      const uint32_t r = s / kPorts, p = s - r * kPorts;
+      node[i] = ports[s < total ? r * 12 + p : 0u];  // a task row is 12 int32 words
+    }
+    int64_t cx[kBatch], cy[kBatch];
+#pragma unroll
+    for (uint32_t i = 0; i < kBatch; ++i) {
+      const longlong2 c = xy[node[i]];
+      cx[i] = c.x;
+      cy[i] = c.y;
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < kBatch; ++i) {
+      const uint32_t s = base + i * kWave + static_cast<uint32_t>(lane);
+      if (s < total) {
+        const uint32_t r = s / kPorts, p = s - r * kPorts;
+        w_node[s] = node[i];
+        w_xy[s] = longlong2{cx[i], cy[i]};
+        if (p == 0) w_row[r] = 0;
+      }
+    }
+  }
+  WaveSync();
+}
+
+// Phases A2, B and C of one staged tile.
+template <int K, bool kNear, bool kNonTemporal>
+__device__ void TileCells(const KoptLaunch& a, const KoptKLayout& L, const KoptTile& d,
+                          const char* smem, char* wave, int lane) {
+  constexpr uint32_t kPorts = 2 * K;
+  const uint16_t* s_pq = reinterpret_cast<const uint16_t*>(smem + L.pq_off);
+  const uint64_t* s_tpl = reinterpret_cast<const uint64_t*>(smem + L.tpl_off);
+  const longlong2* w_xy = reinterpret_cast<const longlong2*>(wave + L.xy_off);
   uint32_t* w_eq = reinterpret_cast<uint32_t*>(wave + L.eq_off);
   uint32_t* w_row = reinterpret_cast<uint32_t*>(wave + L.row_off);
   uint64_t* w_pair = reinterpret_cast<uint64_t*>(wave + L.pair_off);
-  int32_t* w_node = reinterpret_cast<int32_t*>(wave + L.pair_off);  // dead before B writes pairs
+  const int32_t* w_node = reinterpret_cast<const int32_t*>(wave + L.pair_off);
   const uint32_t P = L.P, T = L.T, stride = L.pair_stride;
-
-  const uint64_t row0 = (tile - seg.tile_begin) * seg.rows_per_tile;
-  const uint64_t left = seg.task_count - row0;
-  const uint32_t rows = left < seg.rows_per_tile ? static_cast<uint32_t>(left) : seg.rows_per_tile;
-  const KoptInstance inst = a.instances[seg.instance];
-  const longlong2* xy = a.xy + inst.coord_base;
-  const int32_t* ports = reinterpret_cast<const int32_t*>(a.tasks + seg.task_offset + row0);
-
-  // A. port nodes and coordinates (a task row is 12 int32 words)
-  for (uint32_t s = lane; s < rows * kPorts; s += kWave) {
-    const uint32_t r = s / kPorts, p = s - r * kPorts;
-    const int32_t node = ports[r * 12 + p];
-    w_node[s] = node;
-    w_xy[s] = xy[node];
-    if (p == 0) w_row[r] = 0;
-  }
-  WaveSync();
+  const uint32_t rows = d.rows;
 
   // A2. same-node masks; per row, the first port of every node held by two
   // or more ports
@@ -138,43 +146,50 @@ __device__ void WaveTile(const KoptLaunch& a, const KoptKLayout& L, const KoptSe
   }
   WaveSync();
 
-  // B. pair records, pair-major: record (pair pi, row r) at pi * stride + r
-  const int type = static_cast<int>(inst.distance_type);
-  const bool near = inst.near != 0;
-  for (uint32_t s = lane; s < rows * P; s += kWave) {
-    const uint32_t r = DivSmall(s, P), pi = s - r * P;
-    const uint32_t p = s_pp[pi], q = s_pq[pi];
-    const uint32_t eqp = w_eq[r * kPorts + p], eqq = w_eq[r * kPorts + q];
-    // self loop, or the node pair of deleted edge e = (2e, 2e+1) in either order
-    const bool self = ((eqp >> q) & 1u) != 0;
-    const bool readd = (((eqp & (eqq >> 1)) | ((eqp >> 1) & eqq)) & 0x5555u) != 0;
-    uint64_t rec = kSentinel;
-    if (!self && !readd) {
-      const longlong2 pa = w_xy[r * kPorts + p], pb = w_xy[r * kPorts + q];
-      if (near) {
-        const int32_t dx = static_cast<int32_t>(static_cast<uint32_t>(pa.x) - static_cast<uint32_t>(pb.x));
-        const int32_t dy = static_cast<int32_t>(static_cast<uint32_t>(pa.y) - static_cast<uint32_t>(pb.y));
-        rec = static_cast<uint64_t>(ExactDistanceNear(dx, dy, type));
-      } else {
-        rec = static_cast<uint64_t>(ExactDistance(pa.x, pa.y, pb.x, pb.y, type));
-      }
-      if ((eqp & (eqp - 1)) != 0 && (eqq & (eqq - 1)) != 0) {
-        // both nodes hold two or more ports: count the edge in its counter
-        const uint32_t multi = w_row[r];
-        const uint32_t ia = __builtin_popcount(multi & (eqp ^ (eqp - 1)) & ~eqp);
-        const uint32_t ib = __builtin_popcount(multi & (eqq ^ (eqq - 1)) & ~eqq);
-        const uint32_t lo = ia < ib ? ia : ib, hi = ia < ib ? ib : ia;
-        // counter of node ranks lo < hi < 5
-        const uint32_t c = lo * (7 - lo) / 2 + hi - 1;
-        rec |= uint64_t{1} << (kCounterBase + 2 * c);
-      }
+  // B. pair records, pair-major: record (pair pi, row r) at pi * stride + r.
+  // Every lane computes the distance, sentinel or not, and the duplicate-edge
+  // counter bit without branches, so the loop body does not diverge.
+  const int type = d.distance_type;
+  uint32_t r = DivSmall(static_cast<uint32_t>(lane), P);
+  uint32_t pi = static_cast<uint32_t>(lane) - r * P;
+  while (r < rows) {
+    const uint32_t pq = s_pq[pi];
+    const uint32_t p = pq & 0xffu, q = pq >> 8;
+    const uint32_t base = __umul24(r, kPorts);
+    const uint32_t eqp = w_eq[base + p], eqq = w_eq[base + q];
+    const longlong2 pa = w_xy[base + p], pb = w_xy[base + q];
+    uint64_t rec;
+    if (kNear) {
+      const int32_t dx = static_cast<int32_t>(static_cast<uint32_t>(pa.x) - static_cast<uint32_t>(pb.x));
+      const int32_t dy = static_cast<int32_t>(static_cast<uint32_t>(pa.y) - static_cast<uint32_t>(pb.y));
+      rec = static_cast<uint64_t>(ExactDistanceNear(dx, dy, type));
+    } else {
+      rec = static_cast<uint64_t>(ExactDistance(pa.x, pa.y, pb.x, pb.y, type));
     }
-    w_pair[pi * stride + r] = rec;
+    // both nodes hold two or more ports: count the edge in the counter of
+    // their ranks lo < hi < 5 among such nodes
+    const uint32_t multi = w_row[r];
+    const uint32_t ia = __builtin_popcount(multi & (eqp ^ (eqp - 1)) & ~eqp);
+    const uint32_t ib = __builtin_popcount(multi & (eqq ^ (eqq - 1)) & ~eqq);
+    const uint32_t lo = ia < ib ? ia : ib, hi = ia < ib ? ib : ia;
+    const bool both_multi = (eqp & (eqp - 1)) != 0 && (eqq & (eqq - 1)) != 0;
+    const uint32_t counter = kCounterBase + 2 * (lo * (7 - lo) / 2 + hi - 1);
+    rec |= static_cast<uint64_t>(both_multi ? 1u : 0u) << (both_multi ? counter : 0u);
+    // self loop, or the node pair of deleted edge e = (2e, 2e+1) in either order
+    const uint32_t readd = ((eqp & (eqq >> 1)) | ((eqp >> 1) & eqq)) & 0x5555u;
+    const bool sentinel = (((eqp >> q) & 1u) | readd) != 0;
+    w_pair[__umul24(pi, stride) + r] = sentinel ? kSentinel : rec;
+    r += L.pair_step_r;
+    pi += L.pair_step_p;
+    if (pi >= P) {
+      pi -= P;
+      ++r;
+    }
   }
   WaveSync();
 
   // C. cells: each lane keeps template t and walks rows [rb, rb + run)
-  int64_t* const out = a.costs + seg.cost_offset + row0 * T;  // uniform
+  int64_t* const out = a.costs + d.cost_index;  // uniform
   const char* const pairs = reinterpret_cast<const char*>(w_pair);
   // rows every lane of the pass has: all of them unless this is the last,
   // partial tile of a segment with T <= 64
@@ -217,10 +232,10 @@ __device__ void WaveTile(const KoptLaunch& a, const KoptKLayout& L, const KoptSe
     if (n == 0) {
       // last tile of a segment: lanes hold runs of different lengths
       const uint32_t re = rows < rb + L.run ? rows : rb + L.run;
-      for (uint32_t r = rb; r < re; ++r) {
+      for (uint32_t rr = rb; rr < re; ++rr) {
         uint64_t sum = 0;
 #pragma unroll
-        for (int j = 0; j < K; ++j) sum += *reinterpret_cast<const uint64_t*>(pairs + off[j] + (r - rb) * 8);
+        for (int j = 0; j < K; ++j) sum += *reinterpret_cast<const uint64_t*>(pairs + off[j] + (rr - rb) * 8);
         StoreCell<kNonTemporal>(out, at, sum);
         at += row_bytes;
       }
@@ -230,24 +245,53 @@ __device__ void WaveTile(const KoptLaunch& a, const KoptKLayout& L, const KoptSe
 }
 
 // The tiles of one k handed to this workgroup, one per wavefront at a time.
+// The plan is fetched into registers first, so its latency overlaps the
+// first tile's gathers; it reaches LDS before the only block barrier.
 template <int K, bool kNonTemporal>
 __device__ void TileLoop(const KoptLaunch& a, char* smem) {
   constexpr int ki = K - 3;
-  LoadPlan(a, ki, smem);
-  __syncthreads();
-  const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
-  const uint32_t wib = threadIdx.x / kWave;
-  const uint32_t waves_per_block = blockDim.x / kWave;
+  constexpr uint32_t kPlanRegs = 4;
+  const KoptKLayout& L = a.layout[ki];
+  const KoptPlan& plan = a.plans[ki];
+  const uint32_t tid = threadIdx.x, bd = blockDim.x;
+  uint64_t tv[kPlanRegs];
+#pragma unroll
+  for (uint32_t i = 0; i < kPlanRegs; ++i) {
+    const uint32_t t = tid + i * bd;
+    tv[i] = t < L.T ? plan.tpl[t] : 0;
+  }
+  const uint16_t pqv = tid < L.P ? plan.pair_pq[tid] : 0;
+
+  const int lane = static_cast<int>(tid) & (kWave - 1);
+  const uint32_t wib = tid / kWave;
+  const uint32_t waves_per_block = bd / kWave;
   char* wave = smem + a.wave_base + wib * a.wave_bytes;
   const uint32_t block = blockIdx.x - a.k_block_begin[ki];
   const uint32_t blocks = a.k_block_begin[ki + 1] - a.k_block_begin[ki];
   const uint64_t stride = static_cast<uint64_t>(blocks) * waves_per_block;
   const uint64_t end = a.k_tile_begin[ki] + a.k_tile_count[ki];
-  for (uint64_t tile = a.k_tile_begin[ki] +
-                       __builtin_amdgcn_readfirstlane(block * waves_per_block + wib);
-       tile < end; tile += stride) {
-    const KoptSegment seg = a.segments[a.tile_segment ? a.tile_segment[tile] : 0u];
-    WaveTile<K, kNonTemporal>(a, a.layout[ki], seg, tile, smem, wave, lane);
+  uint64_t tile = a.k_tile_begin[ki] + __builtin_amdgcn_readfirstlane(block * waves_per_block + wib);
+  if (tile < end) StagePorts<K>(a, L, a.tiles[tile], wave, lane);
+
+  uint16_t* s_pq = reinterpret_cast<uint16_t*>(smem + L.pq_off);
+  uint64_t* s_tpl = reinterpret_cast<uint64_t*>(smem + L.tpl_off);
+#pragma unroll
+  for (uint32_t i = 0; i < kPlanRegs; ++i) {
+    const uint32_t t = tid + i * bd;
+    if (t < L.T) s_tpl[t] = tv[i];
+  }
+  for (uint32_t t = tid + kPlanRegs * bd; t < L.T; t += bd) s_tpl[t] = plan.tpl[t];
+  if (tid < L.P) s_pq[tid] = pqv;
+  __syncthreads();
+
+  for (bool first = true; tile < end; tile += stride, first = false) {
+    const KoptTile d = a.tiles[tile];
+    if (!first) StagePorts<K>(a, L, d, wave, lane);
+    if (d.near != 0) {
+      TileCells<K, true, kNonTemporal>(a, L, d, smem, wave, lane);
+    } else {
+      TileCells<K, false, kNonTemporal>(a, L, d, smem, wave, lane);
+    }
   }
 }
 
@@ -298,8 +342,9 @@ size_t KoptPrepareLayout(KoptLaunch* a, const KoptPlan plans[3], const bool pres
       L.run = rows;
     }
     L.pair_stride = ((rows + 1) & ~1u) + 1;
-    L.pp_off = 0;
-    L.pq_off = kMaxPairs;
+    L.pair_step_r = kWave / L.P;
+    L.pair_step_p = kWave - L.pair_step_r * L.P;
+    L.pq_off = 0;
     L.tpl_off = Align16(2 * kMaxPairs);
     plan_bytes = std::max(plan_bytes, Align16(L.tpl_off + L.T * 8));
     const uint32_t ports = 2 * k;
@@ -325,8 +370,9 @@ uint64_t KoptCountTiles(const KoptSegment* segments, size_t count, const KoptTun
   return tiles;
 }
 
-void KoptAssignTiles(KoptSegment* segments, size_t count, const KoptTuning& tuning,
-                     uint32_t* tile_map, KoptLaunch* a) {
+void KoptAssignTiles(KoptSegment* segments, size_t count, const KoptInstance* instances,
+                     const KoptPlan plans[3], const KoptTuning& tuning, KoptTile* tiles,
+                     KoptLaunch* a) {
   uint64_t tile = 0;
   for (int i = 0; i < 3; ++i) {
     a->k_tile_begin[i] = tile;
@@ -335,9 +381,22 @@ void KoptAssignTiles(KoptSegment* segments, size_t count, const KoptTuning& tuni
       if (seg.k != static_cast<uint32_t>(3 + i)) continue;
       seg.rows_per_tile = tuning.rows_per_tile[i];
       seg.tile_begin = tile;
-      const uint64_t tiles = (seg.task_count + seg.rows_per_tile - 1) / seg.rows_per_tile;
-      if (tile_map != nullptr) std::fill(tile_map + tile, tile_map + tile + tiles, static_cast<uint32_t>(s));
-      tile += tiles;
+      const uint64_t n_tiles = (seg.task_count + seg.rows_per_tile - 1) / seg.rows_per_tile;
+      if (tiles != nullptr) {
+        const KoptInstance& inst = instances[seg.instance];
+        const uint64_t T = plans[i].templates;
+        for (uint64_t j = 0; j < n_tiles; ++j) {
+          const uint64_t row0 = j * seg.rows_per_tile;
+          KoptTile& d = tiles[tile + j];
+          d.task_index = seg.task_offset + row0;
+          d.cost_index = seg.cost_offset + row0 * T;
+          d.coord_base = inst.coord_base;
+          d.rows = static_cast<uint32_t>(std::min<uint64_t>(seg.rows_per_tile, seg.task_count - row0));
+          d.distance_type = inst.distance_type;
+          d.near = inst.near;
+        }
+      }
+      tile += n_tiles;
     }
     a->k_tile_count[i] = tile - a->k_tile_begin[i];
   }

@@ -125,11 +125,6 @@ struct JvCache {
   DeviceArray<int32_t> d_witnesses;
 };
 
-struct KoptTables {
-  KoptInstance instance;
-  KoptSegment segment;
-};
-
 struct KoptCache {
   struct TemplateEntry {
     bool valid = false;
@@ -147,7 +142,7 @@ struct KoptCache {
   DeviceArray<KoptPlan> d_plans;
   DeviceArray<ee_kopt_task> d_tasks;
   DeviceArray<int64_t> d_costs;
-  DeviceArray<KoptTables> d_tables;
+  DeviceArray<KoptTile> d_tiles;
 };
 
 struct Context {

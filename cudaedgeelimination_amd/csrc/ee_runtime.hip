@@ -476,13 +476,12 @@ KoptPlan BuildPlan(uint32_t k, const ee_endpoint_matching* templates, uint32_t c
         pair_index[p][q] = static_cast<int>(pairs.size());
         pairs.emplace_back(static_cast<uint8_t>(p), static_cast<uint8_t>(q));
       }
-      plan.tpl[t * k + used++] = static_cast<uint8_t>(pair_index[p][q]);
+      plan.tpl[t] |= static_cast<uint64_t>(pair_index[p][q]) << (8 * used++);
     }
   }
   plan.pairs = static_cast<uint32_t>(pairs.size());
   for (size_t i = 0; i < pairs.size(); ++i) {
-    plan.pair_p[i] = pairs[i].first;
-    plan.pair_q[i] = pairs[i].second;
+    plan.pair_pq[i] = static_cast<uint16_t>(pairs[i].first | (pairs[i].second << 8));
   }
   return plan;
 }
@@ -512,7 +511,7 @@ void ValidateTasks(int32_t n, uint32_t k, const ee_kopt_task* tasks, size_t coun
 
 uint64_t KoptResidentBytes(const KoptCache& c) {
   return c.d_xy.bytes() + c.d_plans.bytes() + c.d_tasks.bytes() + c.d_costs.bytes() +
-         c.d_tables.bytes();
+         c.d_tiles.bytes();
 }
 
 uint32_t TileRows(const KoptTuning& tuning, uint32_t k) { return tuning.rows_per_tile[k - 3]; }
@@ -581,31 +580,28 @@ void EvalKoptCosts(Context& ctx, const int64_t* x, const int64_t* y, int32_t n, 
   const bool workspace_hit = c.d_tasks.count() >= task_count && c.d_costs.count() >= cells;
   if (c.d_tasks.count() < task_count) c.d_tasks.Allocate(Grow(c.d_tasks.count(), task_count), ctx.device);
   if (c.d_costs.count() < cells) c.d_costs.Allocate(Grow(c.d_costs.count(), cells), ctx.device);
-  if (c.d_tables.count() < 1) c.d_tables.Allocate(1, ctx.device);
 
-  const uint32_t rows = TileRows(ctx.kopt_tuning, k);
-  KoptTables tables{};
-  tables.instance = KoptInstance{};
-  tables.instance.n = static_cast<uint32_t>(n);
-  tables.instance.distance_type = type;
-  tables.instance.near = c.near ? 1 : 0;
-  tables.segment = KoptSegment{0, k, 0, task_count, 0, 0, rows, 0};
+  KoptInstance inst{};
+  inst.n = static_cast<uint32_t>(n);
+  inst.distance_type = type;
+  inst.near = c.near ? 1 : 0;
+  KoptSegment segment{0, k, 0, task_count, 0, 0, TileRows(ctx.kopt_tuning, k), 0};
+  const uint64_t n_tiles = KoptCountTiles(&segment, 1, ctx.kopt_tuning);
+  std::vector<KoptTile> tiles(n_tiles);
   KoptLaunch a{};
-  KoptAssignTiles(&tables.segment, 1, ctx.kopt_tuning, nullptr, &a);
+  KoptAssignTiles(&segment, 1, &inst, c.host_plans.data(), ctx.kopt_tuning, tiles.data(), &a);
+  if (c.d_tiles.count() < n_tiles) c.d_tiles.Allocate(Grow(c.d_tiles.count(), n_tiles), ctx.device);
   auto t0 = std::chrono::steady_clock::now();
   c.d_tasks.Upload(tasks, task_count, ctx.stream);
-  c.d_tables.Upload(&tables, 1, ctx.stream);
+  c.d_tiles.Upload(tiles.data(), n_tiles, ctx.stream);
   ctx.Sync();
   u.h2d_ms = MsSince(t0);
 
   a.xy = c.d_xy.get();
-  a.instances = &c.d_tables.get()->instance;
-  a.segments = &c.d_tables.get()->segment;
-  a.segment_count = 1;
+  a.tiles = c.d_tiles.get();
   a.tasks = c.d_tasks.get();
   a.plans = c.d_plans.get();
   a.costs = c.d_costs.get();
-  a.tile_segment = nullptr;
   bool present[3] = {false, false, false};
   present[k - 3] = true;
   const size_t shared = KoptPrepareLayout(&a, c.host_plans.data(), present, ctx.kopt_tuning);

@@ -77,12 +77,11 @@ struct Slot {
   hipEvent_t ev_begin = nullptr, ev_end = nullptr, ev_done = nullptr;
   DeviceArray<ee_kopt_task> d_tasks;
   DeviceArray<int64_t> d_costs;
-  DeviceArray<KoptSegment> d_segments;
-  DeviceArray<uint32_t> d_tiles;
+  DeviceArray<KoptTile> d_tiles;
   Pinned<ee_kopt_task> h_tasks;
   Pinned<int64_t> h_costs;
-  Pinned<KoptSegment> h_segments;
-  Pinned<uint32_t> h_tiles;
+  Pinned<KoptTile> h_tiles;
+  std::vector<KoptSegment> segments;
   std::vector<std::shared_ptr<Request>> requests;
   std::vector<uint64_t> offsets;
   uint64_t bytes = 0;
@@ -100,8 +99,8 @@ struct Server {
   std::vector<KoptInstance> instances;
   std::vector<longlong2> host_xy;
   DeviceArray<longlong2> d_xy;
-  DeviceArray<KoptInstance> d_instances;
   bool graphs_dirty = false;
+  std::vector<KoptInstance> worker_instances;  // the worker thread's copy
   // queue
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -155,9 +154,8 @@ bool Server::Issue(Slot& slot, std::vector<std::shared_ptr<Request>> batch) {
       }
       d_xy.Allocate(xy_copy.size(), ctx->device);
       d_xy.Upload(xy_copy.data(), xy_copy.size(), slot.stream);
-      d_instances.Allocate(inst_copy.size(), ctx->device);
-      d_instances.Upload(inst_copy.data(), inst_copy.size(), slot.stream);
       CheckHip(hipStreamSynchronize(slot.stream), "hipStreamSynchronize");
+      worker_instances = std::move(inst_copy);
     }
     uint64_t rows = 0, cells = 0, tiles = 0;
     bool present[3] = {false, false, false};
@@ -171,35 +169,29 @@ bool Server::Issue(Slot& slot, std::vector<std::shared_ptr<Request>> batch) {
     }
     slot.h_tasks.Reserve(rows);
     slot.h_costs.Reserve(cells);
-    slot.h_segments.Reserve(batch.size());
+    slot.segments.resize(batch.size());
     slot.h_tiles.Reserve(tiles);
     if (slot.d_tasks.count() < rows) slot.d_tasks.Allocate(Grow(slot.d_tasks.count(), rows), ctx->device);
     if (slot.d_costs.count() < cells) slot.d_costs.Allocate(Grow(slot.d_costs.count(), cells), ctx->device);
-    if (slot.d_segments.count() < batch.size()) {
-      slot.d_segments.Allocate(Grow(slot.d_segments.count(), batch.size()), ctx->device);
-    }
     if (slot.d_tiles.count() < tiles) slot.d_tiles.Allocate(Grow(slot.d_tiles.count(), tiles), ctx->device);
     uint64_t row = 0;
     for (size_t i = 0; i < batch.size(); ++i) {
       const Request& r = *batch[i];
       std::memcpy(slot.h_tasks.get() + row, r.tasks.data(), r.tasks.size() * sizeof(ee_kopt_task));
-      slot.h_segments.get()[i] = KoptSegment{r.graph, r.k, row, r.tasks.size(), slot.offsets[i], 0, 0, 0};
+      slot.segments[i] = KoptSegment{r.graph, r.k, row, r.tasks.size(), slot.offsets[i], 0, 0, 0};
       row += r.tasks.size();
     }
     KoptLaunch a{};
-    KoptAssignTiles(slot.h_segments.get(), batch.size(), tuning, slot.h_tiles.get(), &a);
+    KoptAssignTiles(slot.segments.data(), batch.size(), worker_instances.data(), plans.data(), tuning,
+                    slot.h_tiles.get(), &a);
     if (a.tile_count != tiles) throw std::logic_error("server tile count mismatch");
     slot.d_tasks.Upload(slot.h_tasks.get(), rows, slot.stream);
-    slot.d_segments.Upload(slot.h_segments.get(), batch.size(), slot.stream);
     slot.d_tiles.Upload(slot.h_tiles.get(), tiles, slot.stream);
     a.xy = d_xy.get();
-    a.instances = d_instances.get();
-    a.segments = slot.d_segments.get();
-    a.segment_count = static_cast<uint32_t>(batch.size());
+    a.tiles = slot.d_tiles.get();
     a.tasks = slot.d_tasks.get();
     a.plans = d_plans.get();
     a.costs = slot.d_costs.get();
-    a.tile_segment = slot.d_tiles.get();
     const size_t shared = KoptPrepareLayout(&a, plans.data(), present, tuning);
     CheckHip(hipEventRecord(slot.ev_begin, slot.stream), "event record");
     LaunchKopt(a, tuning, shared, slot.stream);
