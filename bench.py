@@ -51,6 +51,34 @@ def kopt_algorithmic_bytes(wl) -> int:
     return 48 * rows + 8 * cells + 16 * wl.graph.dimension + 15 * sum(TEMPLATES.values())
 
 
+def host_info() -> dict:
+    """CPU model and the cores this process may use (the GPU host's share)."""
+    model = "unknown"
+    try:
+        for line in pathlib.Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "usable_cores": len(os.sched_getaffinity(0)),
+            "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_DYNAMIC",
+                                                     "OMP_PROC_BIND", "OMP_PLACES")}}
+
+
+def timed(fn, budget_s: float, min_runs: int = 5, max_runs: int = 50):
+    """One warm-up call, then timed calls until the budget is spent (at least
+    min_runs): (median s, p95 s, runs, last result)."""
+    result = fn()
+    samples = []
+    t_end = time.perf_counter() + budget_s
+    while len(samples) < min_runs or (time.perf_counter() < t_end and len(samples) < max_runs):
+        t0 = time.perf_counter()
+        result = fn()
+        samples.append(time.perf_counter() - t0)
+    return statistics.median(samples), float(np.percentile(samples, 95)), len(samples), result
+
+
 def cpu_baseline_kopt(wl, budget_s: float):
     """The reference's own CPU cost matrix (EvaluateKOptTemplateCosts with
     the CPU backend, OpenMP over rows, at most 8 threads), built from the
@@ -59,25 +87,25 @@ def cpu_baseline_kopt(wl, budget_s: float):
 
     use_ref = O.ref_available()
     fn = O.ref_kopt_costs if use_ref else O.kopt_costs
-    samples, threads, outputs = [], 0, {}
-    t_end = time.perf_counter() + budget_s
-    while True:
-        t0 = time.perf_counter()
+
+    def full():
+        outputs, threads = {}, 0
         for k in (3, 4, 5):
             out, th, _ = fn(wl.graph.x, wl.graph.y, wl.graph.distance_type, k, wl.bucket(k))
             threads = max(threads, th)
             outputs[k] = out
-        samples.append(time.perf_counter() - t0)
-        if time.perf_counter() > t_end or len(samples) >= 50:
-            break
-    median = statistics.median(samples)
+        return outputs, threads
+
+    median, p95, runs, (outputs, threads) = timed(full, budget_s)
     return {
         "value": wl.cells() / median,
         "unit": "cells/s",
         "cores": threads,
         "kind": "reference" if use_ref else "port",
-        "sample": f"full cost matrix of the same workload ({wl.cells()} cells), "
-                  f"{len(samples)} timed repetitions, median {median * 1e3:.1f} ms",
+        "sample": f"full cost matrix of the same workload ({wl.cells()} cells), one warm-up and "
+                  f"{runs} timed repetitions, median {median * 1e3:.1f} ms, p95 {p95 * 1e3:.1f} ms",
+        "p95_cells_per_s": wl.cells() / p95,
+        "host": host_info(),
     }, outputs
 
 
@@ -91,30 +119,104 @@ def cpu_baseline_jv(graph):
         tsp, edg = pathlib.Path(tmp) / "g.tsp", pathlib.Path(tmp) / "g.edg"
         graph.write(tsp, edg)
         buf = C.create_string_buffer(1 << 24)
-        samples = []
-        for _ in range(3):
-            t0 = time.perf_counter()
+
+        def run():
             if O.ref_available():
                 size = O.ref().ref_jv_run_files(str(tsp).encode(), str(edg).encode(), 100, buf,
                                                 len(buf))
                 assert size > 0
-                text = buf.value.decode()
-            else:
-                text = O.jv_run(O.OGraph.load(tsp, edg))
-            samples.append(time.perf_counter() - t0)
+                return buf.value.decode()
+            return O.jv_run(O.OGraph.load(tsp, edg))
+
+        median, p95, runs, text = timed(run, 2.0)
     scanned = sum(int(line.split()[2]) for line in text.splitlines() if line.startswith("metrics "))
-    return {"edges_per_s": scanned / statistics.median(samples),
+    return {"edges_per_s": scanned / median, "p95_edges_per_s": scanned / p95,
             "kind": "reference" if O.ref_available() else "port", "cores": 1,
-            "median_ms": statistics.median(samples) * 1e3}, text
+            "median_ms": median * 1e3, "p95_ms": p95 * 1e3, "runs": runs}, text
 
 
-def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005):
+def many_instance_leg(ctx, cpu: bool, sample: int = 16):
+    """256 independent instances (n log-uniform in [200, 4000], uniform or
+    8-cluster Gaussian coordinates, CEIL_2D for odd indices, 10-NN): JV epochs
+    of all graphs in lockstep as one replayed HIP graph per epoch, then 4,096
+    k-opt rows per instance (2,048 / 1,229 / 819) in one batched launch.  The
+    first `sample` instances are checked against the reference CPU path,
+    which is also timed on them."""
+    import _oracle as O
+
+    from cudaedgeelimination_amd.resident import JvFleet
+
+    t0 = time.perf_counter()
+    instances = [W.config4_instance(i) for i in range(256)]
+    gen_s = time.perf_counter() - t0
+    graphs = [g for g, _ in instances]
+    out = {"instances": len(graphs), "nodes": sum(g.dimension for g in graphs),
+           "edge_slots": sum(g.edge_count for g in graphs), "generate_s": gen_s}
+
+    runs = []
+    for _ in range(2):
+        fleet = JvFleet(ctx, graphs)
+        st = fleet.run(100, use_graph=True)
+        runs.append((st, fleet))
+    st, fleet = min(runs, key=lambda r: r[0]["total_ms"])
+    jv = {"epochs": st["epochs"], "edges_scanned": st["edges_scanned"],
+          "committed": st["committed"], "graph_launches": st["graph_launches"],
+          "epoch_device_ms": st["device_ms"] / max(st["epochs"], 1),
+          "device_edges_per_s": st["edges_scanned"] / (st["device_ms"] * 1e-3),
+          "algorithm_edges_per_s": st["edges_scanned"] / (st["total_ms"] * 1e-3),
+          "total_ms": st["total_ms"], "host_ms": st["host_ms"], "assemble_ms": st["assemble_ms"]}
+    if cpu:
+        t1 = time.perf_counter()
+        want = [O.jv_run(O.OGraph.from_snapshot(g)) for g in graphs[:sample]]
+        cpu_s = time.perf_counter() - t1
+        scanned = sum(int(line.split()[2]) for text in want for line in text.splitlines()
+                      if line.startswith("metrics "))
+        jv["proofs_equal_cpu"] = f"{sum(O.canonical_proof(fleet.proof(i)) == O.canonical_proof(w) for i, w in enumerate(want))}/{sample}"
+        jv["cpu_baseline"] = {"edges_per_s": scanned / cpu_s, "kind": "port", "cores": 1,
+                              "sample": f"first {sample} instances, JV to fixed point"}
+    for _, f in runs:
+        f.close()
+    out["jv"] = jv
+
+    batch = KoptBatch(ctx)
+    segs, tasks = [], []
+    for g, wl in instances:
+        inst = batch.add_instance(g)
+        for k in (3, 4, 5):
+            rows = wl.bucket(k)
+            segs.append((inst, k, len(rows)))
+            tasks.append(rows)
+    batch.set_work(segs, np.concatenate(tasks))
+    tuning = batch.autotune(5)
+    batch.run(3)
+    ms = batch.run(10)
+    graph_ms = batch.run(10, use_graph=True)
+    cells = batch.cells
+    kopt = {"rows": int(sum(len(t) for t in tasks)), "cells": cells, "tuning": tuning,
+            "kernel_ms": ms, "graph_replay_kernel_ms": graph_ms,
+            "cells_per_s": cells / (min(ms, graph_ms) * 1e-3)}
+    if cpu:
+        ok = 0
+        for i in range(3 * sample):
+            g = graphs[i // 3]
+            k = segs[i][1]
+            want, _, _ = O.kopt_costs(g.x, g.y, g.distance_type, k, tasks[i])
+            ok += bool(np.array_equal(batch.segment_costs(i), want))
+        kopt["segments_equal_cpu"] = f"{ok}/{3 * sample}"
+    batch.close()
+    out["kopt"] = kopt
+    return out
+
+
+def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005, jv_share: float = 0.2):
     """Open-loop Poisson queries against four resident graphs (n = 1k, 5k,
-    15k, 30k, uniform, 10-NN): sizes lognormal (median 2,896 cells,
-    sigma 1.04) clipped to [128, 65,536] cells, k = 3 / 4 / 5 at 50/30/20%.
-    Latency runs from the scheduled arrival to completion, so queueing counts
-    (no coordinated omission).  Same request stream for the batched server and
-    the plain per-query path."""
+    15k, 30k, uniform, 10-NN).  Leaf-cost queries: sizes lognormal (median
+    2,896 cells, sigma 1.04) clipped to [128, 65,536] cells, k = 3 / 4 / 5 at
+    50/30/20%.  A share `jv_share` of the requests are JV rechecks of random
+    edge slots of the graph (count lognormal, median 256, sigma 1.0, clipped
+    to [16, 4,096]).  Latency runs from the scheduled arrival to completion,
+    so queueing counts (no coordinated omission).  Same request stream for the
+    batched server and the plain per-query path."""
     import threading
 
     graphs = W.config5_graphs()
@@ -133,20 +235,31 @@ def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005):
         cells = np.clip(np.exp(rng.normal(np.log(2896.0), 1.04, count)), 128, 65536)
         kinds = rng.choice([3, 4, 5], size=count, p=[0.5, 0.3, 0.2])
         which = rng.integers(0, len(graphs), count)
+        is_jv = rng.random(count) < jv_share
+        jv_sizes = np.clip(np.exp(rng.normal(np.log(256.0), 1.0, count)), 16, 4096).astype(int)
         requests = []
         for i in range(count):
+            g = int(which[i])
+            if is_jv[i]:
+                m = graphs[g].edge_count
+                requests.append((g, 0, rng.choice(m, size=min(int(jv_sizes[i]), m),
+                                                  replace=False).astype(np.int32)))
+                continue
             k = int(kinds[i])
             rows = max(1, int(np.ceil(cells[i] / TEMPLATES[k])))
-            pool = pools[which[i]][k]
+            pool = pools[g][k]
             rows = min(rows, len(pool))
             start = int(rng.integers(0, len(pool) - rows + 1))
-            requests.append((int(which[i]), k, pool[start:start + rows]))
-        point = {"qps": qps, "requests": count,
-                 "mean_cells": float(np.mean([len(t) * TEMPLATES[k] for _, k, t in requests]))}
+            requests.append((g, k, pool[start:start + rows]))
+        kopt_cells = [len(t) * TEMPLATES[k] for _, k, t in requests if k]
+        point = {"qps": qps, "requests": count, "jv_rechecks": int(is_jv.sum()),
+                 "mean_cells": float(np.mean(kopt_cells)),
+                 "mean_jv_edges": float(np.mean([len(t) for _, k, t in requests if not k] or [0]))}
         for mode in ("batched", "plain"):
             server = KoptServer(ctx, batching=(mode == "batched"), hbm_budget_bytes=2 << 30,
                                 max_batch_cells=1 << 21)
             gids = [server.add_graph(g) for g in graphs]
+            jids = [server.add_jv_graph(g) for g in graphs]
             warm = server.submit(gids[0], 3, pools[0][3][:8])
             server.wait(warm)
             latencies = np.zeros(count)
@@ -171,19 +284,21 @@ def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005):
                         break
                     if due - now > 2e-4:
                         time.sleep(1e-4)
-                ids[i] = server.submit(gids[g], k, tasks)
+                ids[i] = server.submit(gids[g], k, tasks) if k else server.submit_jv(jids[g], tasks)
                 info_offsets[i] = (time.perf_counter() - due) * 1e3  # late submit counts
                 submitted.release()
             waiter.join()
             wall = time.perf_counter() - t0
             stats = server.stats()
             server.close()
-            total_cells = sum(len(t) * TEMPLATES[k] for _, k, t in requests)
+            total_cells = sum(kopt_cells)
             point[mode] = {
                 "p50_ms": float(np.percentile(latencies, 50)),
                 "p99_ms": float(np.percentile(latencies, 99)),
                 "mean_ms": float(latencies.mean()),
+                "jv_p99_ms": float(np.percentile(latencies[is_jv], 99)) if is_jv.any() else None,
                 "cells_per_s": total_cells / wall,
+                "jv_edges_per_s": stats["jv_edges"] / wall,
                 "launches": stats["batches"],
                 "device_cells_per_s": stats["cells"] / (stats["device_ms"] * 1e-3)
                 if stats["device_ms"] > 0 else None,
@@ -224,6 +339,10 @@ def measured_traffic():
 
 
 def main() -> int:
+    # the reference protocol for its OpenMP CPU path (set before it loads)
+    for key, value in (("OMP_DYNAMIC", "FALSE"), ("OMP_PROC_BIND", "spread"),
+                       ("OMP_PLACES", "cores")):
+        os.environ.setdefault(key, value)
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -232,6 +351,7 @@ def main() -> int:
     ap.add_argument("--no-jv", action="store_true", help="skip the JV workload")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
+    ap.add_argument("--no-many", action="store_true", help="skip the many-instance workload")
     ap.add_argument("--serve-qps", default="2000,8000,20000")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--tuning", default=None,
@@ -351,6 +471,10 @@ def main() -> int:
             jv["cpu_baseline"] = cpu_jv
             jv["proof_equals_cpu"] = O.canonical_proof(text) == O.canonical_proof(best.proof)
         result["jv"] = jv
+
+    if not args.no_many:
+        log("many-instance workload (256 instances: JV fleet + k-opt batch)")
+        result["many_instance"] = many_instance_leg(ctx, cpu=not args.no_cpu)
 
     if not args.no_serve:
         log("serving workload (open-loop Poisson queries)")

@@ -124,6 +124,19 @@ class JvRunStats(C.Structure):
     ]
 
 
+class JvFleetStats(C.Structure):
+    _fields_ = [
+        ("instances", C.c_uint32), ("epochs", C.c_uint32), ("edge_slots", C.c_uint64),
+        ("edges_scanned", C.c_uint64), ("proposed", C.c_uint64), ("committed", C.c_uint64),
+        ("device_ms", C.c_double), ("assemble_ms", C.c_double), ("propose_ms", C.c_double),
+        ("host_ms", C.c_double), ("total_ms", C.c_double), ("graph_launches", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch_cells", C.c_uint64),
                 ("max_batch_requests", C.c_uint32), ("pipeline_depth", C.c_uint32)]
@@ -137,7 +150,8 @@ class RequestResult(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("requests", C.c_uint64), ("batches", C.c_uint64), ("cells", C.c_uint64),
-                ("peak_inflight_bytes", C.c_uint64), ("device_ms", C.c_double)]
+                ("jv_edges", C.c_uint64), ("peak_inflight_bytes", C.c_uint64),
+                ("device_ms", C.c_double)]
 
 
 P = C.c_void_p
@@ -169,10 +183,18 @@ PROTOTYPES = {
     "ee_jv_session_run": (C.c_int, [P, U32, C.POINTER(JvRunStats)]),
     "ee_jv_session_proof": (C.c_int, [P, C.c_char_p, SZ, C.POINTER(SZ)]),
     "ee_jv_session_active": (C.c_int, [P, P]),
+    "ee_jv_fleet_create": (C.c_int, [P, C.POINTER(GraphView), U32, C.POINTER(P)]),
+    "ee_jv_fleet_destroy": (None, [P]),
+    "ee_jv_fleet_run": (C.c_int, [P, U32, C.c_int, C.POINTER(JvFleetStats)]),
+    "ee_jv_fleet_instance_stats": (C.c_int, [P, U32, C.POINTER(JvRunStats)]),
+    "ee_jv_fleet_proof": (C.c_int, [P, U32, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "ee_jv_fleet_active": (C.c_int, [P, U32, P]),
     "ee_server_create": (C.c_int, [P, C.POINTER(ServerConfig), C.POINTER(P)]),
     "ee_server_destroy": (None, [P]),
     "ee_server_add_graph": (C.c_int, [P, P, P, I32, U8, C.POINTER(U32)]),
     "ee_server_submit": (C.c_int, [P, U32, U32, P, U64, P, C.POINTER(U64)]),
+    "ee_server_add_jv_graph": (C.c_int, [P, C.POINTER(GraphView), C.POINTER(U32)]),
+    "ee_server_submit_jv": (C.c_int, [P, U32, P, U64, P, C.POINTER(U64)]),
     "ee_server_wait": (C.c_int, [P, U64, DBL, C.POINTER(RequestResult)]),
     "ee_server_stats_get": (C.c_int, [P, C.POINTER(ServerStats)]),
     "ee_jv_find_candidates": (C.c_int, [P, C.POINTER(GraphView), P, C.POINTER(JvUsage)]),

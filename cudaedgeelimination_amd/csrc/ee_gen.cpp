@@ -195,6 +195,48 @@ template <typename F> ee_status Guarded(F&& body) {
 
 using ee::Guarded;
 
+namespace ee {
+
+uint64_t Fnv1a(uint64_t h, const void* p, size_t len) {
+  const auto* b = static_cast<const unsigned char*>(p);
+  for (size_t i = 0; i < len; ++i) {
+    h ^= b[i];
+    h *= 1099511628211ULL;
+  }
+  return h;
+}
+
+// ContentHash (src/common/graph.cpp:323-346): FNV-1a-64 over the raw bytes
+// of dimension (int32), distance type (uint8), the integer-coordinates flag,
+// every point's int64 x, y, then every edge's int32 u, int32 v, int64
+// weight and bool active.  The prefix covers everything before the edges.
+uint64_t ContentHashPrefix(int32_t n, uint8_t distance_type, const int64_t* x, const int64_t* y) {
+  uint64_t h = 14695981039346656037ULL;
+  const uint8_t integer = 1;
+  h = Fnv1a(h, &n, 4);
+  h = Fnv1a(h, &distance_type, 1);
+  h = Fnv1a(h, &integer, 1);
+  for (int32_t i = 0; i < n; ++i) {
+    h = Fnv1a(h, &x[i], 8);
+    h = Fnv1a(h, &y[i], 8);
+  }
+  return h;
+}
+
+uint64_t ContentHashEdges(uint64_t h, int32_t m, const int32_t* edge_u, const int32_t* edge_v,
+                          const int64_t* edge_weight, const uint8_t* edge_active) {
+  for (int32_t e = 0; e < m; ++e) {
+    const uint8_t active = edge_active == nullptr || edge_active[e] != 0 ? 1 : 0;
+    h = Fnv1a(h, &edge_u[e], 4);
+    h = Fnv1a(h, &edge_v[e], 4);
+    h = Fnv1a(h, &edge_weight[e], 8);
+    h = Fnv1a(h, &active, 1);
+  }
+  return h;
+}
+
+}  // namespace ee
+
 extern "C" {
 
 ee_status ee_gen_points_uniform(uint64_t seed, int32_t n, int64_t lo, int64_t hi, int64_t* x,
@@ -340,30 +382,8 @@ uint64_t ee_graph_content_hash(int32_t n, uint8_t distance_type, const int64_t* 
                                const int64_t* y, int32_t m, const int32_t* edge_u,
                                const int32_t* edge_v, const int64_t* edge_weight,
                                const uint8_t* edge_active) {
-  uint64_t h = 14695981039346656037ULL;
-  const auto bytes = [&h](const void* p, size_t len) {
-    const auto* b = static_cast<const unsigned char*>(p);
-    for (size_t i = 0; i < len; ++i) {
-      h ^= b[i];
-      h *= 1099511628211ULL;
-    }
-  };
-  const uint8_t integer = 1;
-  bytes(&n, 4);
-  bytes(&distance_type, 1);
-  bytes(&integer, 1);
-  for (int32_t i = 0; i < n; ++i) {
-    bytes(&x[i], 8);
-    bytes(&y[i], 8);
-  }
-  for (int32_t e = 0; e < m; ++e) {
-    const uint8_t active = edge_active == nullptr || edge_active[e] != 0 ? 1 : 0;
-    bytes(&edge_u[e], 4);
-    bytes(&edge_v[e], 4);
-    bytes(&edge_weight[e], 8);
-    bytes(&active, 1);
-  }
-  return h;
+  return ee::ContentHashEdges(ee::ContentHashPrefix(n, distance_type, x, y), m, edge_u, edge_v,
+                              edge_weight, edge_active);
 }
 
 ee_status ee_gen_kopt_tasks(uint64_t seed, const int64_t* x, const int64_t* y, int32_t n,

@@ -80,7 +80,7 @@ struct KoptInstance {
 // walked template-stationary: when T <= 64 the wave holds `group` = 64 / T
 // copies of the templates, copy g walking rows [g * run, (g + 1) * run);
 // otherwise `passes` blocks of 64 templates each walk all rows.  Pair records
-// are stored pair-major with `pair_stride` (odd) records per pair.
+// are stored pair-major with `pair_stride` records per pair.
 struct KoptKLayout {
   uint32_t present;
   uint32_t P, T, rows;
@@ -134,6 +134,8 @@ void LaunchKopt(const KoptLaunch& args, const KoptTuning& tuning, size_t shared_
                 hipStream_t stream);
 
 // ------------------------------------------------------------------- JV
+// Edge slots edge_list[0..edge_count) (or 0..edge_count when edge_list is
+// NULL) are tested; item i's witness goes to witnesses[i].
 struct JvLaunch {
   int32_t edge_count;
   int32_t distance_type;
@@ -147,6 +149,10 @@ struct JvLaunch {
   const int32_t* csr_edge_ids;  // [A]
   const longlong2* xy;          // [n]
   int32_t* witnesses;           // [m]
+  // optional, for several graphs in one launch: per edge slot, bit 0 the
+  // distance type and bit 1 the near flag of the slot's graph
+  const uint8_t* edge_mode;     // [m] or NULL
+  const int32_t* edge_list;     // [edge_count] or NULL
 };
 
 void LaunchJv(const JvLaunch& args, uint32_t grid, hipStream_t stream);

@@ -85,12 +85,18 @@ __global__ void __launch_bounds__(kBlock) JvKernel(JvLaunch g) {
   const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
   const int wib = static_cast<int>(threadIdx.x) / kWave;
   const int32_t waves = static_cast<int32_t>(gridDim.x) * kWavesPerBlock;
-  const int type = g.distance_type;
-  const bool near = g.near != 0;
 
-  for (int32_t e = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock + wib);
-       e < g.edge_count; e += waves) {
+  for (int32_t item = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock + wib);
+       item < g.edge_count; item += waves) {
+    const int32_t e = g.edge_list != nullptr ? g.edge_list[item] : item;
     int32_t result = -1;
+    int type = g.distance_type;
+    bool near = g.near != 0;
+    if (g.edge_mode != nullptr) {
+      const uint32_t mode = g.edge_mode[e];
+      type = static_cast<int>(mode & 1u);
+      near = (mode & 2u) != 0;
+    }
     const int2 uv = g.edge_uv[e];
     const int32_t a = uv.x, b = uv.y;
     const int32_t ra0 = g.row_offsets[a], ra1 = g.row_offsets[a + 1];
@@ -212,7 +218,7 @@ __global__ void __launch_bounds__(kBlock) JvKernel(JvLaunch g) {
       }
       WaveSync();
     }
-    if (lane == 0) g.witnesses[e] = result;
+    if (lane == 0) g.witnesses[item] = result;
   }
 }
 

@@ -1,0 +1,210 @@
+// Host side of a resident JV run; see ee_jvhost.h.
+#include "ee_jvhost.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <sstream>
+#include <stdexcept>
+#include <tuple>
+#include <utility>
+
+#include "ee_exact.h"
+#include "ee_gen.h"
+#include "ee_runtime.h"
+
+namespace ee {
+namespace {
+
+void CheckStatus(ee_status s) {
+  if (s != EE_OK) throw std::runtime_error(LastError());
+}
+
+std::string Hex(uint64_t h) {
+  char buf[17];
+  std::snprintf(buf, sizeof(buf), "%016llx", static_cast<unsigned long long>(h));
+  return std::string(buf);
+}
+
+}  // namespace
+
+void JvHost::Load(const ee_graph_view* g) {
+  n = g->dimension;
+  m = g->edge_count;
+  type = g->distance_type;
+  const size_t nn = static_cast<size_t>(n), mm = static_cast<size_t>(m);
+  x.assign(g->x, g->x + nn);
+  y.assign(g->y, g->y + nn);
+  if (!DistanceSafe(x.data(), y.data(), n)) {
+    throw std::runtime_error("HIP JV needs integer coordinates whose squared distances fit uint64");
+  }
+  near = NearExtent(x.data(), y.data(), n);
+  prefix_hash_ = ContentHashPrefix(n, type, x.data(), y.data());
+  u.clear();
+  v.clear();
+  weight.clear();
+  active.clear();
+  if (mm > 0) {
+    u.assign(g->edge_u, g->edge_u + mm);
+    v.assign(g->edge_v, g->edge_v + mm);
+    weight.assign(g->edge_weight, g->edge_weight + mm);
+    active.assign(g->edge_active, g->edge_active + mm);
+  }
+  for (size_t e = 0; e < mm; ++e) {
+    if (u[e] < 0 || u[e] >= n || v[e] < 0 || v[e] >= n) {
+      throw std::invalid_argument("JV edge endpoint out of range");
+    }
+    if (weight[e] < 0 || weight[e] > (int64_t{1} << 62)) {
+      throw std::invalid_argument("JV edge weight outside [0, 2^62]");
+    }
+    active[e] = active[e] ? 1 : 0;
+  }
+  ++generation_;
+  RebuildCsr();
+}
+
+int64_t JvHost::Dist(int32_t a, int32_t b) const { return ExactDistance(x[a], y[a], x[b], y[b], type); }
+
+void JvHost::RebuildCsr() {
+  row_offsets.assign(static_cast<size_t>(n) + 1, 0);
+  int64_t live = 0;
+  for (int32_t e = 0; e < m; ++e) live += active[e] != 0;
+  neighbors.assign(static_cast<size_t>(2 * live), 0);
+  edge_ids.assign(static_cast<size_t>(2 * live), 0);
+  CheckStatus(ee_graph_build_csr(n, m, u.data(), v.data(), active.data(), row_offsets.data(),
+                                 neighbors.data(), edge_ids.data()));
+}
+
+uint64_t JvHost::Hash() const {
+  if (hash_generation_ != generation_) {
+    hash_ = ContentHashEdges(prefix_hash_, m, u.data(), v.data(), weight.data(), active.data());
+    hash_generation_ = generation_;
+  }
+  return hash_;
+}
+
+bool JvHost::Verify(int32_t e, int32_t c) const {
+  if (e < 0 || e >= m || !active[e]) return false;
+  const int32_t a = u[e], b = v[e];
+  if (Degree(a) <= 2 || Degree(b) <= 2) return false;
+  if (c < 0 || c >= n || c == a || c == b) return false;
+  const __int128 cab = weight[e], cac = Dist(a, c), cbc = Dist(b, c);
+  for (int32_t s = row_offsets[c]; s < row_offsets[c + 1]; ++s) {
+    const int32_t d = neighbors[s];
+    if (d == a || d == b) continue;
+    const __int128 left = cab + weight[edge_ids[s]];
+    if (left <= cac + Dist(d, b) || left <= Dist(a, d) + cbc) return false;
+  }
+  return true;
+}
+
+void JvHost::StartRun() {
+  records_.clear();
+  metrics_.clear();
+  stats = ee_jv_run_stats{};
+  stats.initial_hash = Hash();
+  done = false;
+  proof.clear();
+}
+
+void JvHost::BeginEpoch(uint32_t epoch) {
+  cur_ = Metrics{};
+  cur_.epoch = epoch;
+  for (int32_t e = 0; e < m; ++e) cur_.before += active[e] != 0;
+  snap_ = Hash();
+  snap_generation_ = generation_;
+}
+
+size_t JvHost::FinishEpoch(const int32_t* witnesses, double propose_ms) {
+  Metrics& mt = cur_;
+  std::vector<std::pair<int32_t, int32_t>> proposed;
+  for (int32_t e = 0; e < m; ++e) {
+    if (witnesses[e] >= 0) proposed.emplace_back(e, witnesses[e]);
+  }
+  mt.propose_ms = propose_ms;
+  mt.proposed = proposed.size();
+
+  // host re-verification on the same snapshot
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<std::pair<int32_t, int32_t>> verified;
+  for (const auto& c : proposed) {
+    if (Verify(c.first, c.second)) {
+      verified.push_back(c);
+    } else {
+      ++mt.rejected;
+    }
+  }
+  mt.verify_ms = MsSince(t0);
+  mt.verified = verified.size();
+  stats.propose_ms += mt.propose_ms;
+  stats.verify_ms += mt.verify_ms;
+  if (mt.rejected != 0) {
+    throw std::runtime_error("HIP JV candidate failed CPU re-verification; epoch stopped, nothing committed");
+  }
+
+  // deterministic commit: sort by (u, v, method, witness, edge), one per edge,
+  // running degree gate > 2 at both ends (src/cpu/elimination_commit.cpp:13-75)
+  t0 = std::chrono::steady_clock::now();
+  // the snapshot must be the one proposed on (elimination_commit.cpp:16-18);
+  // it changes only through commits, which bump the generation
+  if (generation_ != snap_generation_ || Hash() != snap_) {
+    throw std::runtime_error("snapshot changed before commit");
+  }
+  std::sort(verified.begin(), verified.end(), [&](const auto& l, const auto& r) {
+    return std::make_tuple(u[l.first], v[l.first], l.second, l.first) <
+           std::make_tuple(u[r.first], v[r.first], r.second, r.first);
+  });
+  verified.erase(std::unique(verified.begin(), verified.end(),
+                             [](const auto& l, const auto& r) { return l.first == r.first; }),
+                 verified.end());
+  std::vector<int32_t> degree(static_cast<size_t>(n));
+  for (int32_t vtx = 0; vtx < n; ++vtx) degree[vtx] = Degree(vtx);
+  std::vector<std::pair<int32_t, int32_t>> committed;
+  for (const auto& c : verified) {
+    int32_t& du = degree[u[c.first]];
+    int32_t& dv = degree[v[c.first]];
+    if (du <= 2 || dv <= 2) continue;
+    --du;
+    --dv;
+    committed.push_back(c);
+  }
+  if (!committed.empty()) {
+    for (const auto& c : committed) active[c.first] = 0;
+    ++generation_;
+    RebuildCsr();
+  }
+  stats.commit_ms += MsSince(t0);
+  mt.committed = committed.size();
+  for (const auto& c : committed) {
+    records_.push_back({mt.epoch, snap_, c.first, u[c.first], v[c.first], c.second});
+  }
+  metrics_.push_back(mt);
+  stats.edges_scanned += mt.before;
+  stats.proposed += mt.proposed;
+  stats.committed += mt.committed;
+  ++stats.epochs;
+  if (committed.empty()) done = true;
+  return committed.size();
+}
+
+void JvHost::EndRun(double total_ms) {
+  done = true;
+  stats.final_hash = Hash();
+  stats.total_ms = total_ms;
+  // V1 proof text (src/cpu/jv_elimination.cpp:324-368)
+  std::ostringstream out;
+  out << "CUDAEE_PROOF_V1\nbackend hip\ninitial_hash " << Hex(stats.initial_hash) << '\n';
+  for (const Record& r : records_) {
+    out << "record " << r.epoch << ' ' << Hex(r.hash) << ' ' << r.edge << ' ' << r.a << ' ' << r.b
+        << " JV " << r.witness << '\n';
+  }
+  for (const Metrics& mt : metrics_) {
+    out << "metrics " << mt.epoch << ' ' << mt.before << ' ' << mt.proposed << ' ' << mt.verified
+        << ' ' << mt.rejected << ' ' << mt.committed << ' ' << mt.propose_ms << ' ' << mt.verify_ms
+        << '\n';
+  }
+  out << "final_hash " << Hex(stats.final_hash) << "\nEND\n";
+  proof = out.str();
+}
+
+}  // namespace ee

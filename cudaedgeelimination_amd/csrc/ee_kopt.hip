@@ -76,12 +76,13 @@ __device__ __forceinline__ void StoreCell(int64_t* out, uint32_t at, uint64_t su
 
 // Phase A: the tile's port node ids (into the pair record area, which is
 // free until phase B) and their coordinates, two gathers in flight per lane.
-template <int K>
+template <int K, bool kNear>
 __device__ __forceinline__ void StagePorts(const KoptLaunch& a, const KoptKLayout& L,
                                            const KoptTile& d, char* wave, int lane) {
   constexpr uint32_t kPorts = 2 * K;
   constexpr uint32_t kBatch = 2;
   longlong2* w_xy = reinterpret_cast<longlong2*>(wave + L.xy_off);
+  uint2* w_xy2 = reinterpret_cast<uint2*>(wave + L.xy_off);
   uint32_t* w_row = reinterpret_cast<uint32_t*>(wave + L.row_off);
   int32_t* w_node = reinterpret_cast<int32_t*>(wave + L.pair_off);
   const uint32_t total = d.rows * kPorts;
@@ -109,7 +110,12 @@ __device__ __forceinline__ void StagePorts(const KoptLaunch& a, const KoptKLayou
       if (s < total) {
         const uint32_t r = s / kPorts, p = s - r * kPorts;
         w_node[s] = node[i];
-        w_xy[s] = longlong2{cx[i], cy[i]};
+        if (kNear) {
+          // coordinate extents below 2^25: differences of the low words are exact
+          w_xy2[s] = uint2{static_cast<uint32_t>(cx[i]), static_cast<uint32_t>(cy[i])};
+        } else {
+          w_xy[s] = longlong2{cx[i], cy[i]};
+        }
         if (p == 0) w_row[r] = 0;
       }
     }
@@ -125,6 +131,7 @@ __device__ void TileCells(const KoptLaunch& a, const KoptKLayout& L, const KoptT
   const uint16_t* s_pq = reinterpret_cast<const uint16_t*>(smem + L.pq_off);
   const uint64_t* s_tpl = reinterpret_cast<const uint64_t*>(smem + L.tpl_off);
   const longlong2* w_xy = reinterpret_cast<const longlong2*>(wave + L.xy_off);
+  const uint2* w_xy2 = reinterpret_cast<const uint2*>(wave + L.xy_off);  // near tiles: low words
   uint32_t* w_eq = reinterpret_cast<uint32_t*>(wave + L.eq_off);
   uint32_t* w_row = reinterpret_cast<uint32_t*>(wave + L.row_off);
   uint64_t* w_pair = reinterpret_cast<uint64_t*>(wave + L.pair_off);
@@ -148,27 +155,36 @@ __device__ void TileCells(const KoptLaunch& a, const KoptKLayout& L, const KoptT
 
   // B. pair records, pair-major: record (pair pi, row r) at pi * stride + r.
   // Every lane computes the distance, sentinel or not, and the duplicate-edge
-  // counter bit without branches, so the loop body does not diverge.
+  // counter bit without branches, so the loop body does not diverge.  The
+  // next iteration's pair ports are read at the top of the current one.
   const int type = d.distance_type;
   uint32_t r = DivSmall(static_cast<uint32_t>(lane), P);
   uint32_t pi = static_cast<uint32_t>(lane) - r * P;
+  uint32_t pq = s_pq[pi];
   while (r < rows) {
-    const uint32_t pq = s_pq[pi];
     const uint32_t p = pq & 0xffu, q = pq >> 8;
-    const uint32_t base = __umul24(r, kPorts);
+    const uint32_t row = r, slot = pi;
+    r += L.pair_step_r;
+    pi += L.pair_step_p;
+    if (pi >= P) {
+      pi -= P;
+      ++r;
+    }
+    pq = s_pq[pi];  // pi < P always, so this read is in range even past the last row
+    const uint32_t base = __umul24(row, kPorts);
     const uint32_t eqp = w_eq[base + p], eqq = w_eq[base + q];
-    const longlong2 pa = w_xy[base + p], pb = w_xy[base + q];
     uint64_t rec;
     if (kNear) {
-      const int32_t dx = static_cast<int32_t>(static_cast<uint32_t>(pa.x) - static_cast<uint32_t>(pb.x));
-      const int32_t dy = static_cast<int32_t>(static_cast<uint32_t>(pa.y) - static_cast<uint32_t>(pb.y));
-      rec = static_cast<uint64_t>(ExactDistanceNear(dx, dy, type));
+      const uint2 pa = w_xy2[base + p], pb = w_xy2[base + q];
+      rec = static_cast<uint64_t>(ExactDistanceNear(static_cast<int32_t>(pa.x - pb.x),
+                                                    static_cast<int32_t>(pa.y - pb.y), type));
     } else {
+      const longlong2 pa = w_xy[base + p], pb = w_xy[base + q];
       rec = static_cast<uint64_t>(ExactDistance(pa.x, pa.y, pb.x, pb.y, type));
     }
     // both nodes hold two or more ports: count the edge in the counter of
     // their ranks lo < hi < 5 among such nodes
-    const uint32_t multi = w_row[r];
+    const uint32_t multi = w_row[row];
     const uint32_t ia = __builtin_popcount(multi & (eqp ^ (eqp - 1)) & ~eqp);
     const uint32_t ib = __builtin_popcount(multi & (eqq ^ (eqq - 1)) & ~eqq);
     const uint32_t lo = ia < ib ? ia : ib, hi = ia < ib ? ib : ia;
@@ -178,13 +194,7 @@ __device__ void TileCells(const KoptLaunch& a, const KoptKLayout& L, const KoptT
     // self loop, or the node pair of deleted edge e = (2e, 2e+1) in either order
     const uint32_t readd = ((eqp & (eqq >> 1)) | ((eqp >> 1) & eqq)) & 0x5555u;
     const bool sentinel = (((eqp >> q) & 1u) | readd) != 0;
-    w_pair[__umul24(pi, stride) + r] = sentinel ? kSentinel : rec;
-    r += L.pair_step_r;
-    pi += L.pair_step_p;
-    if (pi >= P) {
-      pi -= P;
-      ++r;
-    }
+    w_pair[__umul24(slot, stride) + row] = sentinel ? kSentinel : rec;
   }
   WaveSync();
 
@@ -271,7 +281,14 @@ __device__ void TileLoop(const KoptLaunch& a, char* smem) {
   const uint64_t stride = static_cast<uint64_t>(blocks) * waves_per_block;
   const uint64_t end = a.k_tile_begin[ki] + a.k_tile_count[ki];
   uint64_t tile = a.k_tile_begin[ki] + __builtin_amdgcn_readfirstlane(block * waves_per_block + wib);
-  if (tile < end) StagePorts<K>(a, L, a.tiles[tile], wave, lane);
+  if (tile < end) {
+    const KoptTile d0 = a.tiles[tile];
+    if (d0.near != 0) {
+      StagePorts<K, true>(a, L, d0, wave, lane);
+    } else {
+      StagePorts<K, false>(a, L, d0, wave, lane);
+    }
+  }
 
   uint16_t* s_pq = reinterpret_cast<uint16_t*>(smem + L.pq_off);
   uint64_t* s_tpl = reinterpret_cast<uint64_t*>(smem + L.tpl_off);
@@ -286,10 +303,11 @@ __device__ void TileLoop(const KoptLaunch& a, char* smem) {
 
   for (bool first = true; tile < end; tile += stride, first = false) {
     const KoptTile d = a.tiles[tile];
-    if (!first) StagePorts<K>(a, L, d, wave, lane);
     if (d.near != 0) {
+      if (!first) StagePorts<K, true>(a, L, d, wave, lane);
       TileCells<K, true, kNonTemporal>(a, L, d, smem, wave, lane);
     } else {
+      if (!first) StagePorts<K, false>(a, L, d, wave, lane);
       TileCells<K, false, kNonTemporal>(a, L, d, smem, wave, lane);
     }
   }
@@ -341,7 +359,10 @@ size_t KoptPrepareLayout(KoptLaunch* a, const KoptPlan plans[3], const bool pres
       L.passes = (L.T + kWave - 1) / kWave;
       L.run = rows;
     }
-    L.pair_stride = ((rows + 1) & ~1u) + 1;
+    // records per pair: at least rows, and 2 mod 4, so that one pass of 16
+    // lanes reading two adjacent records (4 dwords) of 16 different pairs
+    // covers the 64 LDS banks without conflicts
+    L.pair_stride = rows + ((6u - rows % 4u) % 4u);
     L.pair_step_r = kWave / L.P;
     L.pair_step_p = kWave - L.pair_step_r * L.P;
     L.pq_off = 0;

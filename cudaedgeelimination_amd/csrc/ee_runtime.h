@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <array>
 #include <chrono>
 #include <cstdint>
@@ -20,10 +21,59 @@ ee_status StatusFromException();
 double MsSince(std::chrono::steady_clock::time_point t0);
 size_t Grow(size_t current, size_t required);
 bool DistanceSafe(const int64_t* x, const int64_t* y, int32_t n);
+// The graph content hash in two parts (ee_graph_content_hash = edges part
+// continued from the prefix part).
+uint64_t ContentHashPrefix(int32_t n, uint8_t distance_type, const int64_t* x, const int64_t* y);
+uint64_t ContentHashEdges(uint64_t prefix, int32_t m, const int32_t* edge_u, const int32_t* edge_v,
+                          const int64_t* edge_weight, const uint8_t* edge_active);
 bool NearExtent(const int64_t* x, const int64_t* y, int32_t n);
 KoptInstance MakeKoptInstance(uint64_t coord_base, const int64_t* x, const int64_t* y, int32_t n,
                               uint8_t distance_type);
 int CountDevices();
+
+// Page-locked host staging of at least `count` elements (hipHostMalloc);
+// growing discards the contents.
+template <typename T> class Pinned {
+ public:
+  Pinned() = default;
+  ~Pinned() { Reset(); }
+  Pinned(const Pinned&) = delete;
+  Pinned& operator=(const Pinned&) = delete;
+  Pinned(Pinned&& o) noexcept : ptr_(o.ptr_), count_(o.count_) {
+    o.ptr_ = nullptr;
+    o.count_ = 0;
+  }
+  Pinned& operator=(Pinned&& o) noexcept {
+    if (this != &o) {
+      Reset();
+      ptr_ = o.ptr_;
+      count_ = o.count_;
+      o.ptr_ = nullptr;
+      o.count_ = 0;
+    }
+    return *this;
+  }
+  void Reserve(size_t count) {
+    if (count <= count_) return;
+    Reset();
+    void* p = nullptr;
+    CheckHip(hipHostMalloc(&p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocDefault),
+             "hipHostMalloc");
+    ptr_ = static_cast<T*>(p);
+    count_ = count;
+  }
+  T* get() { return ptr_; }
+  size_t count() const { return count_; }
+
+ private:
+  void Reset() {
+    if (ptr_ != nullptr) static_cast<void>(hipHostFree(ptr_));
+    ptr_ = nullptr;
+    count_ = 0;
+  }
+  T* ptr_ = nullptr;
+  size_t count_ = 0;
+};
 
 // Owning device allocation of `count` elements of T on one device.
 template <typename T> class DeviceArray {

@@ -1,14 +1,20 @@
-// Serving loop for leaf-cost queries over device-resident graph snapshots.
+// Serving loop for leaf-cost queries and JV rechecks over device-resident
+// graph snapshots.
 //
-// Callers submit requests (graph, k, task rows, output buffer) from any
-// thread.  One worker thread forms batches from the queue in arrival order
+// Callers submit requests from any thread: leaf-cost queries (graph, k, task
+// rows, output buffer) and JV rechecks (JV graph, edge slots, witness
+// buffer).  One worker thread forms batches from the queue in arrival order
 // under a cell limit, a request limit and an HBM budget, and issues each
 // batch on one of `pipeline_depth` slots (own stream, device buffers and
-// pinned staging): upload tasks and tables, one launch of the tiled k-opt
-// kernel over all the batch's (graph, k) segments, download the cells.  While
-// a batch is in flight the worker keeps admitting and issuing into a free
-// slot, so requests that arrive meanwhile overlap on the device.  Completion
-// copies each request's cells into the caller's buffer and wakes waiters.
+// pinned staging): upload tasks, tiles and edge lists, one launch of the
+// tiled k-opt kernel over all the batch's (graph, k) segments and one JV
+// launch over all its listed edge slots, download the results.  While a batch
+// is in flight the worker keeps admitting and issuing into a free slot, so
+// requests that arrive meanwhile overlap on the device.  A second thread
+// blocks on the oldest batch in flight, copies each request's results into
+// the caller's buffer, wakes waiters and frees the slot.  With one slot there
+// is no second thread: the worker issues, waits and completes each batch in
+// turn (the plain per-query path).
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -21,9 +27,13 @@
 #include <unordered_map>
 #include <vector>
 
+#include "ee_jvhost.h"
 #include "ee_runtime.h"
 
 namespace ee {
+
+void CheckGraphView(const ee_graph_view* g);
+
 namespace {
 
 using Clock = std::chrono::steady_clock;
@@ -33,36 +43,16 @@ double Ms(Clock::time_point a, Clock::time_point b) {
   return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
-template <typename T> class Pinned {
- public:
-  ~Pinned() { Reset(); }
-  void Reserve(size_t count) {
-    if (count <= count_) return;
-    Reset();
-    void* p = nullptr;
-    CheckHip(hipHostMalloc(&p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocDefault),
-             "hipHostMalloc");
-    ptr_ = static_cast<T*>(p);
-    count_ = count;
-  }
-  T* get() { return ptr_; }
-  size_t count() const { return count_; }
-
- private:
-  void Reset() {
-    if (ptr_ != nullptr) static_cast<void>(hipHostFree(ptr_));
-    ptr_ = nullptr;
-    count_ = 0;
-  }
-  T* ptr_ = nullptr;
-  size_t count_ = 0;
-};
 
 struct Request {
   uint64_t id = 0;
+  bool jv = false;  // a JV recheck rather than a leaf-cost query
   uint32_t graph = 0, k = 0;
   std::vector<ee_kopt_task> tasks;
   int64_t* out = nullptr;
+  std::vector<int32_t> edges;  // JV: union edge slots
+  int32_t* witnesses = nullptr;
+  int32_t node_base = 0;       // JV: first union node of the graph
   uint64_t cells = 0, bytes = 0;
   Clock::time_point submit, issue, done;
   int32_t status = EE_OK;
@@ -82,6 +72,8 @@ struct Slot {
   Pinned<int64_t> h_costs;
   Pinned<KoptTile> h_tiles;
   std::vector<KoptSegment> segments;
+  DeviceArray<int32_t> d_jv_list, d_jv_out;
+  Pinned<int32_t> h_jv_list, h_jv_out;
   std::vector<std::shared_ptr<Request>> requests;
   std::vector<uint64_t> offsets;
   uint64_t bytes = 0;
@@ -101,6 +93,24 @@ struct Server {
   DeviceArray<longlong2> d_xy;
   bool graphs_dirty = false;
   std::vector<KoptInstance> worker_instances;  // the worker thread's copy
+  // JV snapshots, back to back in one union (node ids, edge slots and CSR
+  // offset by graph); mutated under mu, uploaded by the worker when dirty
+  struct JvUnion {
+    std::vector<int2> uv;
+    std::vector<int64_t> weight;
+    std::vector<uint8_t> mode;
+    std::vector<longlong2> xy;
+    std::vector<int32_t> active, row_offsets{0}, neighbors, edge_ids;
+    std::vector<int32_t> node_base, edge_base, edge_count;
+  };
+  JvUnion jv;
+  bool jv_dirty = false;
+  DeviceArray<int2> d_jv_uv;
+  DeviceArray<int64_t> d_jv_weight;
+  DeviceArray<uint8_t> d_jv_mode;
+  DeviceArray<longlong2> d_jv_xy;
+  DeviceArray<int32_t> d_jv_active, d_jv_row_offsets, d_jv_neighbors, d_jv_edge_ids;
+  int32_t worker_jv_slots = 0;
   // queue
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -108,14 +118,20 @@ struct Server {
   std::unordered_map<uint64_t, std::shared_ptr<Request>> live;
   uint64_t next_id = 1, next_batch = 1;
   bool stopping = false;
+  bool issuer_done = false;
   ee_server_stats stats{};
   uint64_t inflight_bytes = 0;
   std::vector<Slot> slots;
-  std::thread worker;
+  std::deque<Slot*> inflight;  // issued batches, oldest first (guarded by mu)
+  std::condition_variable cv_inflight;
+  std::thread worker, completer;
 
   uint64_t SlotBudget() const { return cfg.hbm_budget_bytes / std::max<uint32_t>(cfg.pipeline_depth, 1); }
 
   void Loop();
+  void CompleteLoop();
+  // Waits for a slot's batch and completes (or fails) its requests.
+  void Finish(Slot& slot);
   bool Issue(Slot& slot, std::vector<std::shared_ptr<Request>> batch);
   void Complete(Slot& slot);
   void Fail(std::vector<std::shared_ptr<Request>>& batch, const std::string& why);
@@ -139,6 +155,7 @@ bool Server::Issue(Slot& slot, std::vector<std::shared_ptr<Request>> batch) {
     CheckHip(hipSetDevice(ctx->device), "hipSetDevice");
     std::vector<longlong2> xy_copy;
     std::vector<KoptInstance> inst_copy;
+    std::unique_ptr<JvUnion> jv_copy;
     {
       std::lock_guard<std::mutex> lk(mu);
       if (graphs_dirty) {
@@ -146,57 +163,124 @@ bool Server::Issue(Slot& slot, std::vector<std::shared_ptr<Request>> batch) {
         inst_copy = instances;
         graphs_dirty = false;
       }
+      if (jv_dirty) {
+        jv_copy = std::make_unique<JvUnion>(jv);
+        jv_dirty = false;
+      }
+    }
+    if (!inst_copy.empty() || jv_copy) {
+      // new graphs: wait for every batch that reads the old pools, then replace them
+      for (Slot& s : slots) CheckHip(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
     }
     if (!inst_copy.empty()) {
-      // new graphs: wait for every batch that reads the old pool, then replace it
-      for (Slot& s : slots) {
-        if (s.busy) CheckHip(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
-      }
       d_xy.Allocate(xy_copy.size(), ctx->device);
       d_xy.Upload(xy_copy.data(), xy_copy.size(), slot.stream);
       CheckHip(hipStreamSynchronize(slot.stream), "hipStreamSynchronize");
       worker_instances = std::move(inst_copy);
     }
-    uint64_t rows = 0, cells = 0, tiles = 0;
+    if (jv_copy) {
+      const JvUnion& u = *jv_copy;
+      const int dev = ctx->device;
+      d_jv_uv.Allocate(std::max<size_t>(u.uv.size(), 1), dev);
+      d_jv_weight.Allocate(std::max<size_t>(u.weight.size(), 1), dev);
+      d_jv_mode.Allocate(std::max<size_t>(u.mode.size(), 1), dev);
+      d_jv_active.Allocate(std::max<size_t>(u.active.size(), 1), dev);
+      d_jv_xy.Allocate(std::max<size_t>(u.xy.size(), 1), dev);
+      d_jv_row_offsets.Allocate(u.row_offsets.size(), dev);
+      d_jv_neighbors.Allocate(std::max<size_t>(u.neighbors.size(), 1), dev);
+      d_jv_edge_ids.Allocate(std::max<size_t>(u.edge_ids.size(), 1), dev);
+      d_jv_uv.Upload(u.uv.data(), u.uv.size(), slot.stream);
+      d_jv_weight.Upload(u.weight.data(), u.weight.size(), slot.stream);
+      d_jv_mode.Upload(u.mode.data(), u.mode.size(), slot.stream);
+      d_jv_active.Upload(u.active.data(), u.active.size(), slot.stream);
+      d_jv_xy.Upload(u.xy.data(), u.xy.size(), slot.stream);
+      d_jv_row_offsets.Upload(u.row_offsets.data(), u.row_offsets.size(), slot.stream);
+      d_jv_neighbors.Upload(u.neighbors.data(), u.neighbors.size(), slot.stream);
+      d_jv_edge_ids.Upload(u.edge_ids.data(), u.edge_ids.size(), slot.stream);
+      CheckHip(hipStreamSynchronize(slot.stream), "hipStreamSynchronize");
+      worker_jv_slots = static_cast<int32_t>(u.uv.size());
+    }
+    // leaf-cost requests: one tiled k-opt launch; JV rechecks: one JV launch
+    uint64_t rows = 0, cells = 0, tiles = 0, items = 0;
+    size_t kopt_requests = 0;
     bool present[3] = {false, false, false};
     slot.offsets.clear();
     for (const auto& r : batch) {
+      if (r->jv) {
+        slot.offsets.push_back(items);
+        items += r->edges.size();
+        continue;
+      }
       slot.offsets.push_back(cells);
       rows += r->tasks.size();
       cells += r->cells;
       tiles += (r->tasks.size() + tuning.rows_per_tile[r->k - 3] - 1) / tuning.rows_per_tile[r->k - 3];
       present[r->k - 3] = true;
+      ++kopt_requests;
     }
-    slot.h_tasks.Reserve(rows);
-    slot.h_costs.Reserve(cells);
-    slot.segments.resize(batch.size());
-    slot.h_tiles.Reserve(tiles);
-    if (slot.d_tasks.count() < rows) slot.d_tasks.Allocate(Grow(slot.d_tasks.count(), rows), ctx->device);
-    if (slot.d_costs.count() < cells) slot.d_costs.Allocate(Grow(slot.d_costs.count(), cells), ctx->device);
-    if (slot.d_tiles.count() < tiles) slot.d_tiles.Allocate(Grow(slot.d_tiles.count(), tiles), ctx->device);
-    uint64_t row = 0;
-    for (size_t i = 0; i < batch.size(); ++i) {
-      const Request& r = *batch[i];
-      std::memcpy(slot.h_tasks.get() + row, r.tasks.data(), r.tasks.size() * sizeof(ee_kopt_task));
-      slot.segments[i] = KoptSegment{r.graph, r.k, row, r.tasks.size(), slot.offsets[i], 0, 0, 0};
-      row += r.tasks.size();
-    }
-    KoptLaunch a{};
-    KoptAssignTiles(slot.segments.data(), batch.size(), worker_instances.data(), plans.data(), tuning,
-                    slot.h_tiles.get(), &a);
-    if (a.tile_count != tiles) throw std::logic_error("server tile count mismatch");
-    slot.d_tasks.Upload(slot.h_tasks.get(), rows, slot.stream);
-    slot.d_tiles.Upload(slot.h_tiles.get(), tiles, slot.stream);
-    a.xy = d_xy.get();
-    a.tiles = slot.d_tiles.get();
-    a.tasks = slot.d_tasks.get();
-    a.plans = d_plans.get();
-    a.costs = slot.d_costs.get();
-    const size_t shared = KoptPrepareLayout(&a, plans.data(), present, tuning);
     CheckHip(hipEventRecord(slot.ev_begin, slot.stream), "event record");
-    LaunchKopt(a, tuning, shared, slot.stream);
+    if (kopt_requests > 0) {
+      slot.h_tasks.Reserve(rows);
+      slot.h_costs.Reserve(cells);
+      slot.segments.resize(kopt_requests);
+      slot.h_tiles.Reserve(tiles);
+      if (slot.d_tasks.count() < rows) slot.d_tasks.Allocate(Grow(slot.d_tasks.count(), rows), ctx->device);
+      if (slot.d_costs.count() < cells) slot.d_costs.Allocate(Grow(slot.d_costs.count(), cells), ctx->device);
+      if (slot.d_tiles.count() < tiles) slot.d_tiles.Allocate(Grow(slot.d_tiles.count(), tiles), ctx->device);
+      uint64_t row = 0;
+      size_t seg = 0;
+      for (size_t i = 0; i < batch.size(); ++i) {
+        const Request& r = *batch[i];
+        if (r.jv) continue;
+        std::memcpy(slot.h_tasks.get() + row, r.tasks.data(), r.tasks.size() * sizeof(ee_kopt_task));
+        slot.segments[seg++] = KoptSegment{r.graph, r.k, row, r.tasks.size(), slot.offsets[i], 0, 0, 0};
+        row += r.tasks.size();
+      }
+      KoptLaunch a{};
+      KoptAssignTiles(slot.segments.data(), kopt_requests, worker_instances.data(), plans.data(), tuning,
+                      slot.h_tiles.get(), &a);
+      if (a.tile_count != tiles) throw std::logic_error("server tile count mismatch");
+      slot.d_tasks.Upload(slot.h_tasks.get(), rows, slot.stream);
+      slot.d_tiles.Upload(slot.h_tiles.get(), tiles, slot.stream);
+      a.xy = d_xy.get();
+      a.tiles = slot.d_tiles.get();
+      a.tasks = slot.d_tasks.get();
+      a.plans = d_plans.get();
+      a.costs = slot.d_costs.get();
+      const size_t shared = KoptPrepareLayout(&a, plans.data(), present, tuning);
+      LaunchKopt(a, tuning, shared, slot.stream);
+    }
+    if (items > 0) {
+      slot.h_jv_list.Reserve(items);
+      slot.h_jv_out.Reserve(items);
+      if (slot.d_jv_list.count() < items) slot.d_jv_list.Allocate(Grow(slot.d_jv_list.count(), items), ctx->device);
+      if (slot.d_jv_out.count() < items) slot.d_jv_out.Allocate(Grow(slot.d_jv_out.count(), items), ctx->device);
+      for (size_t i = 0; i < batch.size(); ++i) {
+        const Request& r = *batch[i];
+        if (!r.jv) continue;
+        for (int32_t e : r.edges) {
+          if (e < 0 || e >= worker_jv_slots) throw std::logic_error("JV recheck slot outside the resident union");
+        }
+        std::memcpy(slot.h_jv_list.get() + slot.offsets[i], r.edges.data(), r.edges.size() * sizeof(int32_t));
+      }
+      slot.d_jv_list.Upload(slot.h_jv_list.get(), items, slot.stream);
+      JvLaunch j{};
+      j.edge_count = static_cast<int32_t>(items);
+      j.edge_uv = d_jv_uv.get();
+      j.edge_weight = d_jv_weight.get();
+      j.edge_active = d_jv_active.get();
+      j.row_offsets = d_jv_row_offsets.get();
+      j.neighbors = d_jv_neighbors.get();
+      j.csr_edge_ids = d_jv_edge_ids.get();
+      j.xy = d_jv_xy.get();
+      j.witnesses = slot.d_jv_out.get();
+      j.edge_mode = d_jv_mode.get();
+      j.edge_list = slot.d_jv_list.get();
+      LaunchJv(j, 0, slot.stream);
+    }
     CheckHip(hipEventRecord(slot.ev_end, slot.stream), "event record");
-    slot.d_costs.Download(slot.h_costs.get(), cells, slot.stream);
+    if (kopt_requests > 0) slot.d_costs.Download(slot.h_costs.get(), cells, slot.stream);
+    if (items > 0) slot.d_jv_out.Download(slot.h_jv_out.get(), items, slot.stream);
     CheckHip(hipEventRecord(slot.ev_done, slot.stream), "event record");
     {
       std::lock_guard<std::mutex> lk(mu);
@@ -212,8 +296,11 @@ bool Server::Issue(Slot& slot, std::vector<std::shared_ptr<Request>> batch) {
       inflight_bytes += slot.bytes;
       stats.peak_inflight_bytes = std::max(stats.peak_inflight_bytes, inflight_bytes);
     }
-    slot.requests = std::move(batch);
-    slot.busy = true;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      slot.requests = std::move(batch);
+      slot.busy = true;
+    }
     return true;
   } catch (const std::exception& e) {
     Fail(batch, e.what());
@@ -226,14 +313,24 @@ void Server::Complete(Slot& slot) {
   static_cast<void>(hipEventElapsedTime(&ms, slot.ev_begin, slot.ev_end));
   for (size_t i = 0; i < slot.requests.size(); ++i) {
     Request& r = *slot.requests[i];
-    std::memcpy(r.out, slot.h_costs.get() + slot.offsets[i], r.cells * sizeof(int64_t));
+    if (r.jv) {
+      // union node ids back to the graph's own ids
+      const int32_t* w = slot.h_jv_out.get() + slot.offsets[i];
+      for (size_t j = 0; j < r.edges.size(); ++j) r.witnesses[j] = w[j] >= 0 ? w[j] - r.node_base : -1;
+    } else {
+      std::memcpy(r.out, slot.h_costs.get() + slot.offsets[i], r.cells * sizeof(int64_t));
+    }
   }
   std::lock_guard<std::mutex> lk(mu);
   const auto now = Clock::now();
   for (auto& r : slot.requests) {
     r->done = now;
     r->complete = true;
-    stats.cells += r->cells;
+    if (r->jv) {
+      stats.jv_edges += r->edges.size();
+    } else {
+      stats.cells += r->cells;
+    }
   }
   stats.requests += slot.requests.size();
   stats.batches += 1;
@@ -244,62 +341,89 @@ void Server::Complete(Slot& slot) {
   cv_done.notify_all();
 }
 
+// Issuing side: whenever a slot is free and requests are pending, packs the
+// next batch in arrival order and issues it.
 void Server::Loop() {
   const uint64_t budget = SlotBudget();
   for (;;) {
     Slot* free_slot = nullptr;
-    Slot* oldest = nullptr;
-    for (Slot& s : slots) {
-      if (!s.busy && free_slot == nullptr) free_slot = &s;
-      if (s.busy && (oldest == nullptr || s.batch_id < oldest->batch_id)) oldest = &s;
-    }
     std::vector<std::shared_ptr<Request>> batch;
     {
       std::unique_lock<std::mutex> lk(mu);
-      if (free_slot != nullptr && pending.empty() && oldest == nullptr) {
-        cv_work.wait(lk, [&] { return stopping || !pending.empty(); });
-      }
-      if (stopping && pending.empty() && oldest == nullptr) return;
-      if (free_slot != nullptr && !pending.empty()) {
-        uint64_t cells = 0, bytes = 0;
-        while (!pending.empty() && batch.size() < cfg.max_batch_requests) {
-          const auto& r = pending.front();
-          if (!batch.empty() &&
-              (cells + r->cells > cfg.max_batch_cells || bytes + r->bytes > budget)) {
-            break;
-          }
-          cells += r->cells;
-          bytes += r->bytes;
-          batch.push_back(r);
-          pending.pop_front();
+      cv_work.wait(lk, [&] {
+        if (stopping && pending.empty()) return true;
+        if (pending.empty()) return false;
+        for (Slot& s : slots) {
+          if (!s.busy) return true;
+        }
+        return false;
+      });
+      if (stopping && pending.empty()) break;
+      for (Slot& s : slots) {
+        if (!s.busy) {
+          free_slot = &s;
+          break;
         }
       }
-    }
-    if (!batch.empty()) {
-      Issue(*free_slot, std::move(batch));
-      continue;
-    }
-    if (oldest != nullptr) {
-      // wait briefly for the oldest batch, waking early for new requests
-      const hipError_t q = hipEventQuery(oldest->ev_done);
-      if (q == hipSuccess) {
-        Complete(*oldest);
-      } else if (q == hipErrorNotReady) {
-        if (free_slot == nullptr) {
-          static_cast<void>(hipEventSynchronize(oldest->ev_done));
-          Complete(*oldest);
-        } else {
-          std::unique_lock<std::mutex> lk(mu);
-          cv_work.wait_for(lk, std::chrono::microseconds(20),
-                           [&] { return stopping || !pending.empty(); });
+      uint64_t cells = 0, bytes = 0;
+      while (!pending.empty() && batch.size() < cfg.max_batch_requests) {
+        const auto& r = pending.front();
+        if (!batch.empty() && (cells + r->cells > cfg.max_batch_cells || bytes + r->bytes > budget)) {
+          break;
         }
-      } else {
-        std::vector<std::shared_ptr<Request>> failed = std::move(oldest->requests);
-        oldest->busy = false;
-        Fail(failed, std::string("batch failed: ") + hipGetErrorString(q));
+        cells += r->cells;
+        bytes += r->bytes;
+        batch.push_back(r);
+        pending.pop_front();
       }
+    }
+    if (Issue(*free_slot, std::move(batch))) {
+      if (slots.size() == 1) {
+        // one slot: the plain path, upload, launch and download in turn
+        Finish(*free_slot);
+        continue;
+      }
+      std::lock_guard<std::mutex> lk(mu);
+      inflight.push_back(free_slot);
+      cv_inflight.notify_one();
     }
   }
+  std::lock_guard<std::mutex> lk(mu);
+  issuer_done = true;
+  cv_inflight.notify_one();
+}
+
+// Completing side: blocks on the oldest issued batch, hands its results to
+// the callers and frees its slot.
+void Server::CompleteLoop() {
+  for (;;) {
+    Slot* slot = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv_inflight.wait(lk, [&] { return !inflight.empty() || issuer_done; });
+      if (inflight.empty()) return;  // the issuer stopped and everything completed
+      slot = inflight.front();
+      inflight.pop_front();
+    }
+    Finish(*slot);
+    cv_work.notify_one();
+  }
+}
+
+void Server::Finish(Slot& slot) {
+  const hipError_t st = hipEventSynchronize(slot.ev_done);
+  if (st == hipSuccess) {
+    Complete(slot);
+    return;
+  }
+  std::vector<std::shared_ptr<Request>> failed;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    failed = std::move(slot.requests);
+    inflight_bytes -= slot.bytes;
+    slot.busy = false;
+  }
+  Fail(failed, std::string("batch failed: ") + hipGetErrorString(st));
 }
 
 Server& AsServer(ee_server* s) {
@@ -353,6 +477,12 @@ ee_status ee_server_create(ee_ctx* handle, const ee_server_config* config, ee_se
     }
     ee::Server* raw = s.get();
     raw->worker = std::thread([raw] { raw->Loop(); });
+    if (raw->slots.size() > 1) {
+      raw->completer = std::thread([raw] {
+        static_cast<void>(hipSetDevice(raw->ctx->device));
+        raw->CompleteLoop();
+      });
+    }
     *out = reinterpret_cast<ee_server*>(s.release());
   });
 }
@@ -366,6 +496,7 @@ void ee_server_destroy(ee_server* server) {
   }
   s->cv_work.notify_all();
   if (s->worker.joinable()) s->worker.join();
+  if (s->completer.joinable()) s->completer.join();
   static_cast<void>(hipSetDevice(s->ctx->device));
   for (ee::Slot& slot : s->slots) {
     static_cast<void>(hipStreamSynchronize(slot.stream));
@@ -423,6 +554,79 @@ ee_status ee_server_submit(ee_server* server, uint32_t graph, uint32_t k,
     r->tasks.assign(tasks, tasks + count);
     {
       std::lock_guard<std::mutex> lk(s.mu);
+      if (s.stopping) throw std::runtime_error("server is stopping");
+      r->id = s.next_id++;
+      r->submit = ee::Clock::now();
+      s.live[r->id] = r;
+      s.pending.push_back(r);
+    }
+    s.cv_work.notify_one();
+    if (request_id) *request_id = r->id;
+  });
+}
+
+ee_status ee_server_add_jv_graph(ee_server* server, const ee_graph_view* graph, uint32_t* jv_graph) {
+  return Guarded([&] {
+    ee::Server& s = ee::AsServer(server);
+    ee::CheckGraphView(graph);
+    ee::JvHost h;
+    h.Load(graph);  // validates and builds the CSR
+    std::lock_guard<std::mutex> lk(s.mu);
+    ee::Server::JvUnion& u = s.jv;
+    const size_t nodes = u.xy.size() + static_cast<size_t>(h.n);
+    const size_t slots = u.uv.size() + static_cast<size_t>(h.m);
+    const size_t adjacency = u.neighbors.size() + h.neighbors.size();
+    if (nodes >= (size_t{1} << 31) || slots >= (size_t{1} << 31) || adjacency >= (size_t{1} << 31)) {
+      throw std::invalid_argument("resident JV graphs exceed int32 ids");
+    }
+    const int32_t nb = static_cast<int32_t>(u.xy.size()), eb = static_cast<int32_t>(u.uv.size());
+    const int32_t ab = static_cast<int32_t>(u.neighbors.size());
+    const uint8_t md = static_cast<uint8_t>((h.type & 1u) | (h.near ? 2u : 0u));
+    for (int32_t e = 0; e < h.m; ++e) {
+      u.uv.push_back(make_int2(nb + h.u[e], nb + h.v[e]));
+      u.weight.push_back(h.weight[e]);
+      u.mode.push_back(md);
+      u.active.push_back(h.active[e] ? 1 : 0);
+    }
+    for (int32_t v = 0; v < h.n; ++v) u.xy.push_back(longlong2{h.x[v], h.y[v]});
+    for (int32_t v = 1; v <= h.n; ++v) u.row_offsets.push_back(ab + h.row_offsets[v]);
+    for (size_t j = 0; j < h.neighbors.size(); ++j) {
+      u.neighbors.push_back(nb + h.neighbors[j]);
+      u.edge_ids.push_back(eb + h.edge_ids[j]);
+    }
+    u.node_base.push_back(nb);
+    u.edge_base.push_back(eb);
+    u.edge_count.push_back(h.m);
+    s.jv_dirty = true;
+    if (jv_graph) *jv_graph = static_cast<uint32_t>(u.node_base.size() - 1);
+  });
+}
+
+ee_status ee_server_submit_jv(ee_server* server, uint32_t jv_graph, const int32_t* edges,
+                              uint64_t count, int32_t* witnesses, uint64_t* request_id) {
+  return Guarded([&] {
+    ee::Server& s = ee::AsServer(server);
+    if (count == 0 || edges == nullptr || witnesses == nullptr) {
+      throw std::invalid_argument("a JV recheck needs at least one edge slot and an output buffer");
+    }
+    if (count >= (uint64_t{1} << 31)) throw std::invalid_argument("JV recheck too large");
+    auto r = std::make_shared<ee::Request>();
+    r->jv = true;
+    r->graph = jv_graph;
+    r->cells = count;  // counts against the batch limit like cells
+    r->bytes = count * 2 * sizeof(int32_t);
+    r->witnesses = witnesses;
+    if (r->bytes > s.SlotBudget()) throw std::invalid_argument("request exceeds the HBM budget");
+    {
+      std::lock_guard<std::mutex> lk(s.mu);
+      if (jv_graph >= s.jv.node_base.size()) throw std::invalid_argument("unknown JV graph id");
+      const int32_t eb = s.jv.edge_base[jv_graph], m = s.jv.edge_count[jv_graph];
+      r->node_base = s.jv.node_base[jv_graph];
+      r->edges.resize(count);
+      for (uint64_t i = 0; i < count; ++i) {
+        if (edges[i] < 0 || edges[i] >= m) throw std::invalid_argument("JV recheck edge slot out of range");
+        r->edges[i] = eb + edges[i];
+      }
       if (s.stopping) throw std::runtime_error("server is stopping");
       r->id = s.next_id++;
       r->submit = ee::Clock::now();

@@ -164,6 +164,56 @@ class JvSession:
         return out
 
 
+class JvFleet:
+    """Many independent JV runs in lockstep: one upload, launch and download
+    per epoch for all graphs (optionally a replayed HIP graph); each graph's
+    proof equals its standalone JvSession proof."""
+
+    def __init__(self, ctx: Context, graphs: list[GraphSnapshot]):
+        self.ctx = ctx
+        self.graphs = list(graphs)
+        self._views = (_lib.GraphView * len(self.graphs))(*[g.view() for g in self.graphs])
+        handle = C.c_void_p()
+        _lib.check(_lib.lib().ee_jv_fleet_create(ctx.handle, self._views, len(self.graphs),
+                                                 C.byref(handle)))
+        self._h = handle
+        ctx.adopt(self)
+
+    def close(self):
+        if self._h is not None:
+            _lib.lib().ee_jv_fleet_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, max_rounds: int = 100, use_graph: bool = False) -> dict:
+        st = _lib.JvFleetStats()
+        _lib.check(_lib.lib().ee_jv_fleet_run(self._h, max_rounds, 1 if use_graph else 0,
+                                              C.byref(st)))
+        return st.as_dict()
+
+    def proof(self, index: int) -> str:
+        length = C.c_size_t()
+        _lib.check(_lib.lib().ee_jv_fleet_proof(self._h, index, None, 0, C.byref(length)))
+        buf = C.create_string_buffer(length.value + 1)
+        _lib.check(_lib.lib().ee_jv_fleet_proof(self._h, index, buf, len(buf), C.byref(length)))
+        return buf.value.decode()
+
+    def instance_stats(self, index: int) -> dict:
+        st = _lib.JvRunStats()
+        _lib.check(_lib.lib().ee_jv_fleet_instance_stats(self._h, index, C.byref(st)))
+        return {name: getattr(st, name) for name, _ in st._fields_ if name != "reserved"}
+
+    def active(self, index: int) -> np.ndarray:
+        out = np.empty(self.graphs[index].edge_count, np.uint8)
+        _lib.check(_lib.lib().ee_jv_fleet_active(self._h, index, _lib.ptr(out)))
+        return out
+
+
 class KoptServer:
     """Serving loop over device-resident graphs (ee_server_* in ee_hip.h).
 
@@ -215,8 +265,28 @@ class KoptServer:
         self._buffers[rid.value] = (tasks, out)
         return rid.value
 
+    def add_jv_graph(self, graph: GraphSnapshot) -> int:
+        """Makes a JV snapshot resident for recheck requests."""
+        gid = C.c_uint32()
+        view = graph.view()
+        _lib.check(_lib.lib().ee_server_add_jv_graph(self._h, C.byref(view), C.byref(gid)))
+        return gid.value
+
+    def submit_jv(self, jv_graph: int, edges: np.ndarray, out: np.ndarray | None = None) -> int:
+        """Queues a JV recheck of the given edge slots; wait() returns the
+        witness per slot (-1: not eliminable)."""
+        edges = np.ascontiguousarray(edges, dtype=np.int32)
+        if out is None:
+            out = np.empty(len(edges), np.int32)
+        rid = C.c_uint64()
+        _lib.check(_lib.lib().ee_server_submit_jv(self._h, jv_graph, _lib.ptr(edges), len(edges),
+                                                  _lib.ptr(out), C.byref(rid)))
+        self._buffers[rid.value] = (edges, out)
+        return rid.value
+
     def wait(self, request_id: int, timeout_ms: float = -1.0):
-        """(costs, result dict) of a completed request."""
+        """(results, result dict) of a completed request: costs of a leaf-cost
+        query, witnesses of a JV recheck."""
         res = _lib.RequestResult()
         status = _lib.lib().ee_server_wait(self._h, request_id, timeout_ms, C.byref(res))
         _, out = self._buffers.pop(request_id, (None, None))
@@ -229,4 +299,5 @@ class KoptServer:
         st = _lib.ServerStats()
         _lib.check(_lib.lib().ee_server_stats_get(self._h, C.byref(st)))
         return {"requests": st.requests, "batches": st.batches, "cells": st.cells,
-                "peak_inflight_bytes": st.peak_inflight_bytes, "device_ms": st.device_ms}
+                "jv_edges": st.jv_edges, "peak_inflight_bytes": st.peak_inflight_bytes,
+                "device_ms": st.device_ms}

@@ -124,10 +124,14 @@ def config3(seed: int = SEEDS[3], n: int = 15000, rows: int = 65536) -> KoptWork
     return KoptWorkload(graph, ks, kopt_tasks(seed ^ 0xA5A5, graph, ks))
 
 
-def config4_instance(index: int, seed: int = SEEDS[4], rows: int = 4096):
-    """One instance of the many-instance batch: (graph, KoptWorkload)."""
+def config4_instance(index: int, seed: int = SEEDS[4], rows: int = 4096,
+                     n_range: tuple[int, int] = (200, 4000)):
+    """One instance of the many-instance batch: (graph, KoptWorkload).  n is
+    log-uniform in n_range; instances alternate uniform / 8-cluster Gaussian
+    coordinates in pairs, and odd indices use CEIL_2D."""
     rng = np.random.Generator(np.random.PCG64(seed + index))
-    n = int(round(math.exp(rng.uniform(math.log(200), math.log(4000)))))
+    lo, hi = n_range
+    n = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
     sub = seed * 1000003 + index
     if (index // 2) % 2 == 0:
         x, y = _points_uniform(sub, n, 0, 1_000_000)

@@ -265,10 +265,50 @@ ee_status ee_jv_session_proof(const ee_jv_session* session, char* buffer, size_t
 /* Active flags after the last run (edge_count entries). */
 ee_status ee_jv_session_active(const ee_jv_session* session, uint8_t* active);
 
+/* ------------------------------------------------------- many JV graphs */
+/* No single reference counterpart: runs RunJvElimination
+ * (src/cpu/jv_elimination.cpp:250-368) for many independent graphs at once.
+ * Every graph keeps its own host run (re-verification, commit, V1 proof
+ * identical to a standalone ee_jv_session run of that graph); per epoch the
+ * device side is one upload of all graphs' active flags and CSR, one JV
+ * launch over all their edge slots and one download, optionally replayed as a
+ * captured HIP graph (use_graph != 0).  Graphs that reached their fixed point
+ * are skipped in later epochs.  Graphs may mix EUC_2D and CEIL_2D. */
+typedef struct ee_jv_fleet ee_jv_fleet;
+
+typedef struct {
+  uint32_t instances;
+  uint32_t epochs;          /* lockstep epochs until every graph stopped */
+  uint64_t edge_slots;      /* edge slots of the union */
+  uint64_t edges_scanned;   /* summed over graphs and epochs */
+  uint64_t proposed;
+  uint64_t committed;
+  double device_ms;         /* per epoch: the launch, or the whole replayed graph */
+  double assemble_ms;       /* host packing of the union's per-epoch arrays */
+  double propose_ms;        /* assemble + device round trip */
+  double host_ms;           /* re-verification, commit and CSR rebuild of all graphs */
+  double total_ms;
+  uint32_t graph_launches;
+  uint32_t reserved;
+} ee_jv_fleet_stats;
+
+ee_status ee_jv_fleet_create(ee_ctx* ctx, const ee_graph_view* graphs, uint32_t count,
+                             ee_jv_fleet** out);
+void ee_jv_fleet_destroy(ee_jv_fleet* fleet);
+ee_status ee_jv_fleet_run(ee_jv_fleet* fleet, uint32_t max_rounds, int use_graph,
+                          ee_jv_fleet_stats* stats);
+/* The last run of graph `index`: its stats (device fields zero), V1 proof
+ * text and final active flags. */
+ee_status ee_jv_fleet_instance_stats(const ee_jv_fleet* fleet, uint32_t index,
+                                     ee_jv_run_stats* stats);
+ee_status ee_jv_fleet_proof(const ee_jv_fleet* fleet, uint32_t index, char* buffer,
+                            size_t capacity, size_t* length);
+ee_status ee_jv_fleet_active(const ee_jv_fleet* fleet, uint32_t index, uint8_t* active);
+
 /* ------------------------------------------------------- serving k-opt */
 /* No reference counterpart.  A request loop over device-resident graph
  * snapshots: callers submit leaf-cost queries (one graph, one k, task rows)
- * from any thread; a worker thread packs whatever has arrived into the next
+ * and JV rechecks (ee_server_submit_jv) from any thread; a worker thread packs whatever has arrived into the next
  * batched launch (continuous batching) as long as the batch stays within the
  * cell and HBM budgets, runs up to `pipeline_depth` batches concurrently on
  * separate streams (upload, kernel and download of one batch overlap with the
@@ -299,7 +339,8 @@ typedef struct {
 typedef struct {
   uint64_t requests;
   uint64_t batches;
-  uint64_t cells;
+  uint64_t cells;       /* leaf-cost cells served */
+  uint64_t jv_edges;    /* edge slots rechecked */
   uint64_t peak_inflight_bytes;
   double device_ms;     /* sum of batch kernel times (HIP events) */
 } ee_server_stats;
@@ -316,6 +357,16 @@ ee_status ee_server_add_graph(ee_server* server, const int64_t* x, const int64_t
 ee_status ee_server_submit(ee_server* server, uint32_t graph, uint32_t k,
                            const ee_kopt_task* tasks, uint64_t count, int64_t* costs,
                            uint64_t* request_id);
+/* Makes a JV graph snapshot (coordinates, edges, active flags) resident for
+ * recheck requests (before or while serving); *jv_graph receives its id. */
+ee_status ee_server_add_jv_graph(ee_server* server, const ee_graph_view* graph, uint32_t* jv_graph);
+/* Queues a JV recheck of edge slots edges[0..count) of a resident JV graph:
+ * witnesses[i] receives the JV witness of slot edges[i] on that snapshot
+ * (the FindJvCandidatesCuda result, src/cuda/jv_cuda.cu:221-288), or -1.
+ * Rechecks share launches with leaf-cost requests; witnesses must stay valid
+ * until the request completes. */
+ee_status ee_server_submit_jv(ee_server* server, uint32_t jv_graph, const int32_t* edges,
+                              uint64_t count, int32_t* witnesses, uint64_t* request_id);
 /* Blocks until the request completes (timeout_ms < 0: no limit). */
 ee_status ee_server_wait(ee_server* server, uint64_t request_id, double timeout_ms,
                          ee_request_result* result);

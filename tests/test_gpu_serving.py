@@ -129,3 +129,50 @@ def test_admission_rejects_bad_requests(ctx, graphs_and_rows):
     costs, _ = server.wait(rid)
     assert np.array_equal(costs, expected(graphs, 0, 3, rows[0][3][:4]))
     server.close()
+
+
+@pytest.mark.parametrize("batching", [True, False])
+def test_jv_rechecks_share_launches_with_leaf_costs(ctx, graphs_and_rows, batching):
+    """JV rechecks of edge-slot subsets on resident snapshots (EUC_2D and
+    CEIL_2D, some edges inactive) mixed with leaf-cost queries: every
+    witness equals the oracle's on that snapshot, every cost row the oracle's."""
+    say(f"serving JV rechecks batching={batching}")
+    graphs, rows = graphs_and_rows
+    ceil = W.config1(seed=0xEE0005C1, n=500)
+    ceil = W.knn_graph(ceil.x, ceil.y, 1, 12)
+    sparse = graphs[1].copy()
+    rng = np.random.default_rng(7)
+    sparse.edge_active[rng.random(sparse.edge_count) < 0.1] = 0
+    sparse.rebuild_csr()
+    jv_graphs = [graphs[0], ceil, sparse]
+    want = [O.OGraph.from_snapshot(g).witnesses() for g in jv_graphs]
+    server = KoptServer(ctx, batching=batching, max_batch_cells=60000)
+    kids = [server.add_graph(g) for g in graphs]
+    jids = [server.add_jv_graph(g) for g in jv_graphs]
+    submitted = []
+    kopt_reqs = make_requests(rows, 60, seed=5)
+    for i in range(120):
+        if i % 2 == 0:
+            j = int(rng.integers(len(jv_graphs)))
+            m = jv_graphs[j].edge_count
+            edges = rng.choice(m, size=int(rng.integers(1, min(m, 700))), replace=False).astype(np.int32)
+            submitted.append(("jv", server.submit_jv(jids[j], edges), j, edges))
+        else:
+            g, k, t = kopt_reqs[i // 2]
+            submitted.append(("kopt", server.submit(kids[g], k, t), g, (k, t)))
+    total_edges = 0
+    for kind, rid, g, payload in submitted:
+        out, _ = server.wait(rid, 60000)
+        if kind == "jv":
+            assert np.array_equal(out, want[g][payload]), (g, len(payload))
+            total_edges += len(payload)
+        else:
+            k, t = payload
+            assert np.array_equal(out, expected(graphs, g, k, t))
+    assert server.stats()["jv_edges"] == total_edges
+    assert any((w >= 0).any() for w in want)
+    with pytest.raises(_lib.InvalidArgument):
+        server.submit_jv(jids[0], np.array([jv_graphs[0].edge_count], np.int32))
+    with pytest.raises(_lib.InvalidArgument):
+        server.submit_jv(len(jv_graphs) + 3, np.array([0], np.int32))
+    server.close()
