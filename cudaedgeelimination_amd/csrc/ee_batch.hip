@@ -55,9 +55,16 @@ struct Batch {
     graph = nullptr;
   }
 
+  bool NearOnly() const {
+    for (const KoptInstance& inst : instances) {
+      if (inst.near == 0) return false;
+    }
+    return true;
+  }
+
   size_t SharedBytes() const {
     KoptLaunch probe{};
-    return KoptPrepareLayout(&probe, plans.data(), k_present, tuning);
+    return KoptPrepareLayout(&probe, plans.data(), k_present, tuning, NearOnly());
   }
 
   // Splits segments into tiles for the current tuning and uploads the maps.
@@ -94,7 +101,7 @@ struct Batch {
 
   void LaunchOnce() {
     KoptLaunch a = Args();
-    const size_t shared = KoptPrepareLayout(&a, plans.data(), k_present, tuning);
+    const size_t shared = KoptPrepareLayout(&a, plans.data(), k_present, tuning, NearOnly());
     LaunchKopt(a, tuning, shared, ctx->stream);
   }
 

@@ -128,8 +128,10 @@ void KoptAssignTiles(KoptSegment* segments, size_t count, const KoptInstance* in
                      KoptLaunch* a);
 // Fills the layout of `a` for the plans present and returns the dynamic LDS
 // bytes of one workgroup.
+// near_only: every tile of the launch is on a near instance, whose staged
+// coordinates take 8 bytes per port instead of 16.
 size_t KoptPrepareLayout(KoptLaunch* a, const KoptPlan plans[3], const bool present[3],
-                         const KoptTuning& tuning);
+                         const KoptTuning& tuning, bool near_only);
 void LaunchKopt(const KoptLaunch& args, const KoptTuning& tuning, size_t shared_bytes,
                 hipStream_t stream);
 

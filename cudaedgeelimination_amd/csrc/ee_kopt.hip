@@ -330,7 +330,7 @@ uint32_t Align16(uint32_t v) { return (v + 15u) & ~15u; }
 }  // namespace
 
 size_t KoptPrepareLayout(KoptLaunch* a, const KoptPlan plans[3], const bool present[3],
-                         const KoptTuning& tuning) {
+                         const KoptTuning& tuning, bool near_only) {
   if (tuning.block == 0 || tuning.block > 256 || tuning.block % kWave != 0) {
     throw std::invalid_argument("k-opt block size must be a multiple of 64 up to 256");
   }
@@ -359,10 +359,9 @@ size_t KoptPrepareLayout(KoptLaunch* a, const KoptPlan plans[3], const bool pres
       L.passes = (L.T + kWave - 1) / kWave;
       L.run = rows;
     }
-    // records per pair: at least rows, and 2 mod 4, so that one pass of 16
-    // lanes reading two adjacent records (4 dwords) of 16 different pairs
-    // covers the 64 LDS banks without conflicts
-    L.pair_stride = rows + ((6u - rows % 4u) % 4u);
+    // records per pair: at least rows, and odd, so that 32 lanes reading one
+    // 8-byte record each of 32 different pairs hit 64 distinct LDS banks
+    L.pair_stride = rows | 1u;
     L.pair_step_r = kWave / L.P;
     L.pair_step_p = kWave - L.pair_step_r * L.P;
     L.pq_off = 0;
@@ -370,7 +369,7 @@ size_t KoptPrepareLayout(KoptLaunch* a, const KoptPlan plans[3], const bool pres
     plan_bytes = std::max(plan_bytes, Align16(L.tpl_off + L.T * 8));
     const uint32_t ports = 2 * k;
     L.xy_off = 0;
-    L.eq_off = Align16(rows * ports * 16);
+    L.eq_off = Align16(rows * ports * (near_only ? 8u : 16u));
     L.row_off = L.eq_off + Align16(rows * ports * 4);
     L.pair_off = L.row_off + Align16(rows * 4);
     // the port node ids of phase A share the pair record area

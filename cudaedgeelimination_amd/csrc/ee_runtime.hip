@@ -604,7 +604,7 @@ void EvalKoptCosts(Context& ctx, const int64_t* x, const int64_t* y, int32_t n, 
   a.costs = c.d_costs.get();
   bool present[3] = {false, false, false};
   present[k - 3] = true;
-  const size_t shared = KoptPrepareLayout(&a, c.host_plans.data(), present, ctx.kopt_tuning);
+  const size_t shared = KoptPrepareLayout(&a, c.host_plans.data(), present, ctx.kopt_tuning, c.near);
   t0 = std::chrono::steady_clock::now();
   LaunchKopt(a, ctx.kopt_tuning, shared, ctx.stream);
   ctx.Sync();

@@ -247,7 +247,9 @@ bool Server::Issue(Slot& slot, std::vector<std::shared_ptr<Request>> batch) {
       a.tasks = slot.d_tasks.get();
       a.plans = d_plans.get();
       a.costs = slot.d_costs.get();
-      const size_t shared = KoptPrepareLayout(&a, plans.data(), present, tuning);
+      bool near_only = true;
+      for (const KoptSegment& sg : slot.segments) near_only = near_only && worker_instances[sg.instance].near != 0;
+      const size_t shared = KoptPrepareLayout(&a, plans.data(), present, tuning, near_only);
       LaunchKopt(a, tuning, shared, slot.stream);
     }
     if (items > 0) {
@@ -469,6 +471,23 @@ ee_status ee_server_create(ee_ctx* handle, const ee_server_config* config, ee_se
     s->d_plans.Upload(s->plans.data(), 3, ctx->stream);
     ctx->Sync();
     s->slots.resize(config->pipeline_depth);
+    // size every slot for a full batch up front, so that serving does not
+    // allocate (device frees synchronise the device; pinned allocations are slow)
+    const uint64_t max_cells =
+        std::max<uint64_t>(1, std::min<uint64_t>(config->max_batch_cells, s->SlotBudget() / sizeof(int64_t)));
+    const uint64_t max_rows = max_cells / 4 + 1;  // k = 3 has 4 cells per row
+    for (ee::Slot& slot : s->slots) {
+      slot.d_tasks.Allocate(max_rows, ctx->device);
+      slot.d_costs.Allocate(max_cells, ctx->device);
+      slot.d_tiles.Allocate(max_rows, ctx->device);
+      slot.d_jv_list.Allocate(max_cells, ctx->device);
+      slot.d_jv_out.Allocate(max_cells, ctx->device);
+      slot.h_tasks.Reserve(max_rows);
+      slot.h_costs.Reserve(max_cells);
+      slot.h_tiles.Reserve(max_rows);
+      slot.h_jv_list.Reserve(max_cells);
+      slot.h_jv_out.Reserve(max_cells);
+    }
     for (ee::Slot& slot : s->slots) {
       ee::CheckHip(hipStreamCreateWithFlags(&slot.stream, hipStreamNonBlocking), "hipStreamCreate");
       ee::CheckHip(hipEventCreate(&slot.ev_begin), "hipEventCreate");
