@@ -51,6 +51,16 @@ def kopt_algorithmic_bytes(wl) -> int:
     return 48 * rows + 8 * cells + 16 * wl.graph.dimension + 15 * sum(TEMPLATES.values())
 
 
+# The process's CPU set at start: the reference protocol's OMP_PROC_BIND makes
+# the OpenMP runtime pin the thread that first runs a parallel region, and
+# threads started later (the serving loop's) would inherit that one core.
+START_AFFINITY = os.sched_getaffinity(0)
+
+
+def restore_affinity() -> None:
+    os.sched_setaffinity(0, START_AFFINITY)
+
+
 def host_info() -> dict:
     """CPU model and the cores this process may use (the GPU host's share)."""
     model = "unknown"
@@ -61,7 +71,7 @@ def host_info() -> dict:
                 break
     except OSError:
         pass
-    return {"cpu_model": model, "usable_cores": len(os.sched_getaffinity(0)),
+    return {"cpu_model": model, "usable_cores": len(START_AFFINITY),
             "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_DYNAMIC",
                                                      "OMP_PROC_BIND", "OMP_PLACES")}}
 
@@ -97,6 +107,7 @@ def cpu_baseline_kopt(wl, budget_s: float):
         return outputs, threads
 
     median, p95, runs, (outputs, threads) = timed(full, budget_s)
+    restore_affinity()
     return {
         "value": wl.cells() / median,
         "unit": "cells/s",
@@ -129,6 +140,7 @@ def cpu_baseline_jv(graph):
             return O.jv_run(O.OGraph.load(tsp, edg))
 
         median, p95, runs, text = timed(run, 2.0)
+    restore_affinity()
     scanned = sum(int(line.split()[2]) for line in text.splitlines() if line.startswith("metrics "))
     return {"edges_per_s": scanned / median, "p95_edges_per_s": scanned / p95,
             "kind": "reference" if O.ref_available() else "port", "cores": 1,
@@ -174,6 +186,7 @@ def many_instance_leg(ctx, cpu: bool, sample: int = 16):
         jv["proofs_equal_cpu"] = f"{sum(O.canonical_proof(fleet.proof(i)) == O.canonical_proof(w) for i, w in enumerate(want))}/{sample}"
         jv["cpu_baseline"] = {"edges_per_s": scanned / cpu_s, "kind": "port", "cores": 1,
                               "sample": f"first {sample} instances, JV to fixed point"}
+        restore_affinity()
     for _, f in runs:
         f.close()
     out["jv"] = jv
@@ -203,6 +216,7 @@ def many_instance_leg(ctx, cpu: bool, sample: int = 16):
             want, _, _ = O.kopt_costs(g.x, g.y, g.distance_type, k, tasks[i])
             ok += bool(np.array_equal(batch.segment_costs(i), want))
         kopt["segments_equal_cpu"] = f"{ok}/{3 * sample}"
+        restore_affinity()
     batch.close()
     out["kopt"] = kopt
     return out
