@@ -222,7 +222,8 @@ def many_instance_leg(ctx, cpu: bool, sample: int = 16):
     return out
 
 
-def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005, jv_share: float = 0.2):
+def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005, jv_share: float = 0.2,
+                burst: int = 20000):
     """Open-loop Poisson queries against four resident graphs (n = 1k, 5k,
     15k, 30k, uniform, 10-NN).  Leaf-cost queries: sizes lognormal (median
     2,896 cells, sigma 1.04) clipped to [128, 65,536] cells, k = 3 / 4 / 5 at
@@ -230,7 +231,8 @@ def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005, jv_share:
     edge slots of the graph (count lognormal, median 256, sigma 1.0, clipped
     to [16, 4,096]).  Latency runs from the scheduled arrival to completion,
     so queueing counts (no coordinated omission).  Same request stream for the
-    batched server and the plain per-query path."""
+    batched server and the plain per-query path.  A rate of 0 is a burst of
+    `burst` requests all due at once: its cells/s is each path's capacity."""
     import threading
 
     graphs = W.config5_graphs()
@@ -244,8 +246,14 @@ def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005, jv_share:
     out = {"graphs": [g.dimension for g in graphs], "points": []}
     for qps in qps_list:
         rng = np.random.default_rng(seed + qps)
-        count = max(int(qps * seconds), 50)
-        arrivals = np.cumsum(rng.exponential(1.0 / qps, count))
+        if qps == 0:
+            # burst: every request is due at once, so the wall time measures
+            # how fast each path drains a full queue (its capacity)
+            count = burst
+            arrivals = np.zeros(count)
+        else:
+            count = max(int(qps * seconds), 50)
+            arrivals = np.cumsum(rng.exponential(1.0 / qps, count))
         cells = np.clip(np.exp(rng.normal(np.log(2896.0), 1.04, count)), 128, 65536)
         kinds = rng.choice([3, 4, 5], size=count, p=[0.5, 0.3, 0.2])
         which = rng.integers(0, len(graphs), count)
@@ -266,7 +274,7 @@ def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005, jv_share:
             start = int(rng.integers(0, len(pool) - rows + 1))
             requests.append((g, k, pool[start:start + rows]))
         kopt_cells = [len(t) * TEMPLATES[k] for _, k, t in requests if k]
-        point = {"qps": qps, "requests": count, "jv_rechecks": int(is_jv.sum()),
+        point = {"qps": qps if qps else "burst", "requests": count, "jv_rechecks": int(is_jv.sum()),
                  "mean_cells": float(np.mean(kopt_cells)),
                  "mean_jv_edges": float(np.mean([len(t) for _, k, t in requests if not k] or [0]))}
         for mode in ("batched", "plain"):
@@ -316,7 +324,9 @@ def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005, jv_share:
                 "launches": stats["batches"],
                 "device_cells_per_s": stats["cells"] / (stats["device_ms"] * 1e-3)
                 if stats["device_ms"] > 0 else None,
+                "wall_s": wall,
             }
+        point["batched_over_plain_cells_per_s"] = point["batched"]["cells_per_s"] / point["plain"]["cells_per_s"]
         out["points"].append(point)
     return out
 
@@ -366,7 +376,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--no-many", action="store_true", help="skip the many-instance workload")
-    ap.add_argument("--serve-qps", default="2000,8000,20000")
+    ap.add_argument("--serve-qps", default="2000,8000,20000,0",
+                    help="Poisson rates of the serving leg; 0 is a burst of --serve-burst requests")
+    ap.add_argument("--serve-burst", type=int, default=20000)
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--tuning", default=None,
                     help="r3,r4,r5,grid,block,nt: skip the autotuner and use this tuning")
@@ -493,7 +505,7 @@ def main() -> int:
     if not args.no_serve:
         log("serving workload (open-loop Poisson queries)")
         result["serving"] = serving_leg(ctx, [int(q) for q in args.serve_qps.split(",")],
-                                        args.serve_seconds)
+                                        args.serve_seconds, burst=args.serve_burst)
 
     batch.close()
     ctx.close()

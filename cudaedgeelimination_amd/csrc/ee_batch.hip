@@ -18,7 +18,7 @@ namespace {
 
 constexpr uint32_t kTemplateCount[3] = {4, 25, 208};
 // Bumped whenever the k-opt kernel changes, so cached tunings are retuned.
-constexpr int kKernelRevision = 3;
+constexpr int kKernelRevision = 4;
 
 struct Batch {
   explicit Batch(Context* c) : ctx(c) {}
@@ -82,9 +82,11 @@ struct Batch {
                     tiles.data(), &ranges);
     tile_count = ranges.tile_count;
     if (SharedBytes() > 65536) throw std::invalid_argument("k-opt tuning needs more than 64 KiB of LDS");
-    d_tiles.Allocate(std::max<size_t>(tiles.size(), 1), ctx->device);
-    d_tiles.Upload(tiles.data(), tiles.size(), ctx->stream);
-    ctx->Sync();
+    if (ranges.inline_count == 0) {  // otherwise the kernel derives tiles from its arguments
+      d_tiles.Allocate(std::max<size_t>(tiles.size(), 1), ctx->device);
+      d_tiles.Upload(tiles.data(), tiles.size(), ctx->stream);
+      ctx->Sync();
+    }
     layout_dirty = false;
     DropGraph();
   }
@@ -386,6 +388,9 @@ ee_status ee_kopt_batch_set_work(ee_kopt_batch* batch, const ee_kopt_segment_des
       if (d.task_count > 0 && tasks == nullptr) throw std::invalid_argument("tasks is NULL");
       ee::ValidateTasks(static_cast<int32_t>(b.instances[d.instance].n), d.k, tasks + rows,
                         d.task_count);
+      // every segment starts on a 128-byte boundary, so whole-line stores of
+      // its rows stay line-aligned
+      cells = (cells + ee::kSegmentAlignCells - 1) / ee::kSegmentAlignCells * ee::kSegmentAlignCells;
       if (cell_offsets) cell_offsets[s] = cells;
       ee::KoptSegment seg{d.instance, d.k, rows, d.task_count, cells, 0, 1, 0};
       segs.push_back(seg);

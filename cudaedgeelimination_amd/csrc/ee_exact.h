@@ -54,29 +54,34 @@ constexpr int64_t kNearExtent = int64_t{1} << 25;
 
 // Same result as ExactDistance for |dx|, |dy| < 2^25, entirely in FP64:
 // s = dx^2 + dy^2 < 2^51, so s, r*r and s - r*r are exact doubles for every
-// r below 2^26.  The root estimate is within one unit of m = floor(sqrt(s)),
-// so floor(estimate) is m - 1, m or m + 1, and one step moves it onto m:
-// s - r*r < 0 means r = m + 1, s - r*r > 2r means r = m - 1 (m itself has
-// 0 <= s - m*m <= 2m).  On the device the estimate is the hardware reciprocal
-// square root (relative error about 2^-23) refined by one Newton step (about
-// 2^-45, i.e. below 2^-19 absolute for roots below 2^26).
+// r below 2^26.  With m = floor(sqrt(s)) and an estimate of sqrt(s) within
+// e < 1/4 of it, r = floor(estimate) is m, or m - 1 when sqrt(s) < m + e, or
+// m + 1 when sqrt(s) > m + 1 - e.  The rounding then needs no step onto m:
+//  - EUC_2D is m + (s - m^2 > m); with rem = s - r^2 it is r + (rem > r):
+//    r = m - 1 gives rem = s - m^2 + 2m - 1 > r and m + 1 is not taken since
+//    s - m^2 < 2me + e^2 <= m; r = m + 1 gives rem < 0 and indeed
+//    s - m^2 > 2m + 1 - 2e(m + 1) > m.
+//  - CEIL_2D is m + (s != m^2); it is r + (rem > 0) + (rem > 2r + 1): for
+//    r = m - 1, rem = s - m^2 + 2r + 1; for r = m + 1, rem < 0 and s > m^2.
+// On the device the estimate is the hardware reciprocal square root
+// (relative error about 2^-23) refined by one Newton step (about 2^-45, i.e.
+// below 2^-19 absolute for roots below 2^26); s = 0 is evaluated as 1/4,
+// whose root floors to 0 as well.
 EE_HD int64_t ExactDistanceNear(int32_t dx, int32_t dy, int type) {
   const double fx = static_cast<double>(dx), fy = static_cast<double>(dy);
   const double s = fma(fx, fx, fy * fy);
 #if defined(__HIP_DEVICE_COMPILE__)
-  const double y = __builtin_amdgcn_rsq(s);
-  const double g = s * y, h = 0.5 * y;
-  const double est = floor(fma(g, fma(-h, g, 0.5), g));
-  double r = s > 0.0 ? est : 0.0;  // rsq(0) is +inf
+  const double s4 = fmax(s, 0.25);  // rsq(0) is +inf
+  const double y = __builtin_amdgcn_rsq(s4);
+  const double g = s4 * y, h = 0.5 * y;
+  const double r = floor(fma(g, fma(-h, g, 0.5), g));
 #else
-  double r = floor(sqrt(s));
+  const double r = floor(sqrt(s));
 #endif
-  const double rem0 = fma(-r, r, s);
-  r += rem0 < 0.0 ? -1.0 : (rem0 > r + r ? 1.0 : 0.0);
   const double rem = fma(-r, r, s);
-  // rem >= 0 now, so for CEIL_2D "rem != 0" is "rem > 0"
-  const bool up = rem > (type == 0 ? r : 0.0);
-  return static_cast<int64_t>(static_cast<uint32_t>(r)) + (up ? 1 : 0);
+  const uint32_t up = type == 0 ? static_cast<uint32_t>(rem > r)
+                                : static_cast<uint32_t>(rem > 0.0) + static_cast<uint32_t>(rem > fma(2.0, r, 1.0));
+  return static_cast<int64_t>(static_cast<uint32_t>(r) + up);
 }
 
 }  // namespace ee

@@ -78,35 +78,60 @@ struct KoptInstance {
 // LDS layout of one k inside a launch: the block-shared plan (pair ports and
 // packed template pair indices) and the per-wave tile arrays.  Cells are
 // walked template-stationary: when T <= 64 the wave holds `group` = 64 / T
-// copies of the templates, copy g walking rows [g * run, (g + 1) * run);
+// copies of the templates, copy g walking rows g, g + group, g + 2 group, ...;
 // otherwise `passes` blocks of 64 templates each walk all rows.  Pair records
 // are stored pair-major with `pair_stride` records per pair.
 struct KoptKLayout {
   uint32_t present;
   uint32_t P, T, rows;
-  uint32_t group, passes, run, pair_stride;
+  uint32_t group, passes, pair_stride;
   uint32_t pair_step_r, pair_step_p;  // 64 = pair_step_r * P + pair_step_p
   uint32_t pq_off, tpl_off;
   uint32_t xy_off, eq_off, row_off, pair_off;
 };
 
+// A segment carried in the kernel arguments: for launches of at most
+// kInlineSegments segments the kernel derives each tile from these instead of
+// reading a tile descriptor, which takes one dependent memory access off the
+// start of every tile.
+struct KoptInlineSegment {
+  uint64_t task_offset;
+  uint64_t cost_offset;
+  uint32_t tile_begin;  // global tile index of the segment's first tile
+  uint32_t task_count;
+  uint32_t coord_base;
+  uint16_t distance_type;
+  uint16_t near;
+};
+constexpr uint32_t kInlineSegments = 8;
+// Cost pools start every segment on a multiple of this many int64 cells (128
+// bytes, one L2 line): partial-line stores cost the memory system a
+// read-modify-write.
+constexpr uint64_t kSegmentAlignCells = 16;
+
 // Tiles are numbered k by k: tiles [k_tile_begin[i], k_tile_begin[i] +
-// k_tile_count[i]) belong to segments with k = 3 + i, and workgroups
-// [k_block_begin[i], k_block_begin[i + 1]) process exactly those tiles, so
-// every workgroup runs code for one k only.
+// k_tile_count[i]) belong to segments with k = 3 + i, and the k_block_count[i]
+// workgroups from k_block_first[i] process exactly those tiles, so every
+// workgroup runs code for one k only, in the order k = 3, 4, 5.
 struct KoptLaunch {
   const longlong2* xy;         // coordinate pool
-  const KoptTile* tiles;       // [tile_count]
+  const KoptTile* tiles;       // [tile_count]; unused when inline_count > 0
   uint64_t tile_count;
   uint64_t k_tile_begin[3];
   uint64_t k_tile_count[3];
-  uint32_t k_block_begin[4];   // filled by LaunchKopt
+  uint32_t k_block_first[3];   // filled by LaunchKopt
+  uint32_t k_block_count[3];
   const ee_kopt_task* tasks;   // task pool
   const KoptPlan* plans;       // [3] for k = 3, 4, 5
   int64_t* costs;              // cost pool
   KoptKLayout layout[3];
   uint32_t wave_base;   // LDS offset of the first wave's tile arrays
   uint32_t wave_bytes;  // LDS bytes per wave
+  // inline_count > 0: the launch's segments, k by k in tile order;
+  // inline_seg[k_inline_begin[i] .. k_inline_begin[i + 1]) have k = 3 + i
+  uint32_t inline_count;
+  uint32_t k_inline_begin[4];
+  KoptInlineSegment inline_seg[kInlineSegments];
 };
 
 // Rows per wavefront tile for each k (at most 64); `grid` = 0 launches
@@ -122,7 +147,10 @@ struct KoptTuning {
 uint64_t KoptCountTiles(const KoptSegment* segments, size_t count, const KoptTuning& tuning);
 // Sets rows_per_tile and tile_begin of every segment, numbering tiles k by k,
 // writes the descriptor of every tile to `tiles` (KoptCountTiles entries; the
-// segments' instances index `instances`) and the per-k tile ranges to `a`.
+// segments' instances index `instances`; may be null) and the per-k tile
+// ranges to `a`.  Launches of at most kInlineSegments segments also get the
+// inline segment table (a->inline_count > 0): their kernels ignore `tiles`,
+// so callers need not upload the descriptors.
 void KoptAssignTiles(KoptSegment* segments, size_t count, const KoptInstance* instances,
                      const KoptPlan plans[3], const KoptTuning& tuning, KoptTile* tiles,
                      KoptLaunch* a);

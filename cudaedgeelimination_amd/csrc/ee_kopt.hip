@@ -31,6 +31,7 @@
 //    hence a counter never exceeds 3 for k <= 5, and at most five nodes hold
 //    two or more of the 2k <= 10 ports (ten counters).
 #include <algorithm>
+#include <type_traits>
 
 #include "ee_exact.h"
 #include "ee_internal.h"
@@ -44,12 +45,18 @@ constexpr int kCounterBase = 35;
 constexpr uint64_t kCostMask = (uint64_t{1} << kCounterBase) - 1;
 constexpr uint64_t kSentinel = uint64_t{1} << 61;
 constexpr uint64_t kBadMask = (uint64_t{0xAAAAA} << kCounterBase) | (uint64_t{7} << 61);
-constexpr uint32_t kUnroll = 2;  // rows per step of the cell loop (one ds_read2_b64 per pair)
 
 __device__ __forceinline__ void WaveSync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A value every lane holds alike, moved to scalar registers.
+__device__ __forceinline__ uint64_t Uniform64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
 // q = c / d for c < 2^24 via a float reciprocal plus one correction step.
@@ -123,6 +130,68 @@ __device__ __forceinline__ void StagePorts(const KoptLaunch& a, const KoptKLayou
   WaveSync();
 }
 
+// Counter bit and sentinel of pair (p, q) from the port info words of one
+// row, as the high word of a pair record (bits 32..63).
+__device__ __forceinline__ uint32_t PairFlags(uint32_t ip, uint32_t iq, uint32_t q) {
+  // both nodes held by two or more ports: count the edge in the counter of
+  // their ranks lo < hi < 5, bits 33 + lo * (7 - lo) + 2 * hi and one up
+  const uint32_t ra = ip >> 16, rb = iq >> 16;
+  const uint32_t lo = ra < rb ? ra : rb, hi = ra < rb ? rb : ra;
+  const uint32_t pos = lo * (7u - lo) + 2u * hi + 1u;  // bit of the high word
+  const uint32_t bit = hi < 5u ? 1u << pos : 0u;
+  // self loop, or the node pair of deleted edge e = (2e, 2e+1) in either order
+  const uint32_t readd = ((ip & (iq >> 1)) | ((ip >> 1) & iq)) & 0x5555u;
+  const bool sentinel = (((ip >> q) & 1u) | readd) != 0;
+  return sentinel ? static_cast<uint32_t>(kSentinel >> 32) : bit;
+}
+
+// Phase B: pair records, pair-major (record of pair pi and row r at
+// pi * stride + r).  Every lane computes the distance, sentinel or not, and
+// the duplicate-edge counter bit without branches, so the loop body does not
+// diverge; the next iteration's pair ports are read at the top of the current
+// one.  kType is the distance type (0 EUC_2D, 1 CEIL_2D).
+template <int K, bool kNear, int kType>
+__device__ __forceinline__ void PairRecords(const KoptKLayout& L, uint32_t rows, const uint16_t* s_pq,
+                                            const char* wave, uint64_t* w_pair, int lane) {
+  constexpr uint32_t kPorts = 2 * K;
+  const longlong2* w_xy = reinterpret_cast<const longlong2*>(wave + L.xy_off);
+  const uint2* w_xy2 = reinterpret_cast<const uint2*>(wave + L.xy_off);  // near tiles: low words
+  const uint32_t* w_info = reinterpret_cast<const uint32_t*>(wave + L.eq_off);
+  const uint32_t P = L.P, stride = L.pair_stride;
+  uint32_t r = DivSmall(static_cast<uint32_t>(lane), P);
+  uint32_t pi = static_cast<uint32_t>(lane) - r * P;
+  uint32_t pq = s_pq[pi];
+  while (r < rows) {
+    const uint32_t p = pq & 0xffu, q = pq >> 8;
+    const uint32_t row = r, slot = pi;
+    r += L.pair_step_r;
+    pi += L.pair_step_p;
+    if (pi >= P) {
+      pi -= P;
+      ++r;
+    }
+    pq = s_pq[pi];  // pi < P always, so this read is in range even past the last row
+    const uint32_t base = __umul24(row, kPorts);
+    const uint32_t ip = w_info[base + p], iq = w_info[base + q];
+    uint32_t lo_word, hi_word;
+    if (kNear) {
+      const uint2 pa = w_xy2[base + p], pb = w_xy2[base + q];
+      lo_word = static_cast<uint32_t>(ExactDistanceNear(static_cast<int32_t>(pa.x - pb.x),
+                                                        static_cast<int32_t>(pa.y - pb.y), kType));
+      hi_word = 0;  // near distances are below 2^27
+    } else {
+      const longlong2 pa = w_xy[base + p], pb = w_xy[base + q];
+      const uint64_t dist = static_cast<uint64_t>(ExactDistance(pa.x, pa.y, pb.x, pb.y, kType));
+      lo_word = static_cast<uint32_t>(dist);
+      hi_word = static_cast<uint32_t>(dist >> 32);
+    }
+    const uint32_t flags = PairFlags(ip, iq, q);
+    const bool sentinel = flags == static_cast<uint32_t>(kSentinel >> 32);
+    const uint64_t rec = (static_cast<uint64_t>(hi_word | flags) << 32) | lo_word;
+    w_pair[__umul24(slot, stride) + row] = sentinel ? kSentinel : rec;
+  }
+}
+
 // Phases A2, B and C of one staged tile.
 template <int K, bool kNear, bool kNonTemporal>
 __device__ void TileCells(const KoptLaunch& a, const KoptKLayout& L, const KoptTile& d,
@@ -130,13 +199,11 @@ __device__ void TileCells(const KoptLaunch& a, const KoptKLayout& L, const KoptT
   constexpr uint32_t kPorts = 2 * K;
   const uint16_t* s_pq = reinterpret_cast<const uint16_t*>(smem + L.pq_off);
   const uint64_t* s_tpl = reinterpret_cast<const uint64_t*>(smem + L.tpl_off);
-  const longlong2* w_xy = reinterpret_cast<const longlong2*>(wave + L.xy_off);
-  const uint2* w_xy2 = reinterpret_cast<const uint2*>(wave + L.xy_off);  // near tiles: low words
   uint32_t* w_eq = reinterpret_cast<uint32_t*>(wave + L.eq_off);
   uint32_t* w_row = reinterpret_cast<uint32_t*>(wave + L.row_off);
   uint64_t* w_pair = reinterpret_cast<uint64_t*>(wave + L.pair_off);
   const int32_t* w_node = reinterpret_cast<const int32_t*>(wave + L.pair_off);
-  const uint32_t P = L.P, T = L.T, stride = L.pair_stride;
+  const uint32_t T = L.T, stride = L.pair_stride;
   const uint32_t rows = d.rows;
 
   // A2. same-node masks; per row, the first port of every node held by two
@@ -152,106 +219,96 @@ __device__ void TileCells(const KoptLaunch& a, const KoptKLayout& L, const KoptT
     if (eq != (1u << p) && __builtin_ctz(eq) == p) atomicOr(&w_row[r], 1u << p);
   }
   WaveSync();
-
-  // B. pair records, pair-major: record (pair pi, row r) at pi * stride + r.
-  // Every lane computes the distance, sentinel or not, and the duplicate-edge
-  // counter bit without branches, so the loop body does not diverge.  The
-  // next iteration's pair ports are read at the top of the current one.
-  const int type = d.distance_type;
-  uint32_t r = DivSmall(static_cast<uint32_t>(lane), P);
-  uint32_t pi = static_cast<uint32_t>(lane) - r * P;
-  uint32_t pq = s_pq[pi];
-  while (r < rows) {
-    const uint32_t p = pq & 0xffu, q = pq >> 8;
-    const uint32_t row = r, slot = pi;
-    r += L.pair_step_r;
-    pi += L.pair_step_p;
-    if (pi >= P) {
-      pi -= P;
-      ++r;
-    }
-    pq = s_pq[pi];  // pi < P always, so this read is in range even past the last row
-    const uint32_t base = __umul24(row, kPorts);
-    const uint32_t eqp = w_eq[base + p], eqq = w_eq[base + q];
-    uint64_t rec;
-    if (kNear) {
-      const uint2 pa = w_xy2[base + p], pb = w_xy2[base + q];
-      rec = static_cast<uint64_t>(ExactDistanceNear(static_cast<int32_t>(pa.x - pb.x),
-                                                    static_cast<int32_t>(pa.y - pb.y), type));
-    } else {
-      const longlong2 pa = w_xy[base + p], pb = w_xy[base + q];
-      rec = static_cast<uint64_t>(ExactDistance(pa.x, pa.y, pb.x, pb.y, type));
-    }
-    // both nodes hold two or more ports: count the edge in the counter of
-    // their ranks lo < hi < 5 among such nodes
-    const uint32_t multi = w_row[row];
-    const uint32_t ia = __builtin_popcount(multi & (eqp ^ (eqp - 1)) & ~eqp);
-    const uint32_t ib = __builtin_popcount(multi & (eqq ^ (eqq - 1)) & ~eqq);
-    const uint32_t lo = ia < ib ? ia : ib, hi = ia < ib ? ib : ia;
-    const bool both_multi = (eqp & (eqp - 1)) != 0 && (eqq & (eqq - 1)) != 0;
-    const uint32_t counter = kCounterBase + 2 * (lo * (7 - lo) / 2 + hi - 1);
-    rec |= static_cast<uint64_t>(both_multi ? 1u : 0u) << (both_multi ? counter : 0u);
-    // self loop, or the node pair of deleted edge e = (2e, 2e+1) in either order
-    const uint32_t readd = ((eqp & (eqq >> 1)) | ((eqp >> 1) & eqq)) & 0x5555u;
-    const bool sentinel = (((eqp >> q) & 1u) | readd) != 0;
-    w_pair[__umul24(slot, stride) + row] = sentinel ? kSentinel : rec;
+  // port info: the same-node mask in bits 0..9 and, in bits 16..18, the rank
+  // of the port's node among the row's nodes held by two or more ports (by
+  // first port), or 7 for a node held by this port alone
+  for (uint32_t s = lane; s < rows * kPorts; s += kWave) {
+    const uint32_t r = s / kPorts;
+    const uint32_t eq = w_eq[s];
+    const uint32_t below = w_row[r] & ((eq & (0u - eq)) - 1u);
+    const uint32_t rank = (eq & (eq - 1u)) != 0 ? static_cast<uint32_t>(__builtin_popcount(below)) : 7u;
+    w_eq[s] = eq | (rank << 16);
   }
   WaveSync();
 
-  // C. cells: each lane keeps template t and walks rows [rb, rb + run)
-  int64_t* const out = a.costs + d.cost_index;  // uniform
+  // B. pair records
+  if (d.distance_type == 0) {
+    PairRecords<K, kNear, 0>(L, rows, s_pq, wave, w_pair, lane);
+  } else {
+    PairRecords<K, kNear, 1>(L, rows, s_pq, wave, w_pair, lane);
+  }
+  WaveSync();
+
+  // C. cells: each lane keeps template t.  With T <= 64 the wave holds G =
+  // 64 / T rows at a time (lane group g takes rows g, g + G, g + 2G, ...), so
+  // one store writes G consecutive rows, G * T contiguous cells; with T > 64
+  // it walks every row once per pass of 64 templates.
+  const uint64_t cost_index = Uniform64(d.cost_index);
+  int64_t* const out = a.costs + cost_index;
   const char* const pairs = reinterpret_cast<const char*>(w_pair);
-  // rows every lane of the pass has: all of them unless this is the last,
-  // partial tile of a segment with T <= 64
-  const uint32_t n = __builtin_amdgcn_readfirstlane(
-      T > static_cast<uint32_t>(kWave) ? rows : (rows == L.group * L.run ? L.run : 0u));
+  const uint32_t G = L.group;
+  const uint32_t n = __builtin_amdgcn_readfirstlane(rows / G);  // rows every group has
+  const uint32_t extra = rows - n * G;                          // groups g < extra have one more
+  const uint32_t step_bytes = G * T * 8;
   for (uint32_t pass = 0; pass < L.passes; ++pass) {
     uint32_t g = 0, t = pass * kWave + static_cast<uint32_t>(lane);
     if (T <= static_cast<uint32_t>(kWave)) {
       g = DivSmall(static_cast<uint32_t>(lane), T);
       t = static_cast<uint32_t>(lane) - g * T;
     }
-    const bool active = g < L.group && t < T;
-    const uint32_t rb = active ? g * L.run : 0u;
+    const bool active = g < G && t < T;
     const uint64_t tp = s_tpl[active ? t : 0u];
-    uint32_t off[K];  // byte offset of record (pair j of the template, row rb)
+    uint32_t off[K];  // byte offset of record (pair j of the template, row g)
 #pragma unroll
-    for (int j = 0; j < K; ++j) off[j] = (static_cast<uint32_t>((tp >> (8 * j)) & 0xffu) * stride + rb) * 8;
+    for (int j = 0; j < K; ++j) off[j] = (static_cast<uint32_t>((tp >> (8 * j)) & 0xffu) * stride + g) * 8;
     // byte offset of the lane's next cell; a tile's cells stay below 2^24
-    uint32_t at = (rb * T + t) * 8;
-    const uint32_t row_bytes = T * 8;
+    uint32_t at = (g * T + t) * 8;
     if (!active) continue;
+    // U steps from one base address per pair (immediate offsets u * G)
+    auto steps_at = [&](auto unroll, uint32_t i) {
+      constexpr uint32_t U = decltype(unroll)::value;
+      uint64_t sum[U];
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) sum[u] = 0;
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        const uint64_t* rec = reinterpret_cast<const uint64_t*>(pairs + off[j] + i * G * 8);
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) sum[u] += rec[u * G];
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        StoreCell<kNonTemporal>(out, at, sum[u]);
+        at += step_bytes;
+      }
+    };
     uint32_t i = 0;
-    for (; i + kUnroll <= n; i += kUnroll) {
-#pragma unroll
-      for (uint32_t u = 0; u < kUnroll; ++u) {
-        uint64_t sum = 0;
-#pragma unroll
-        for (int j = 0; j < K; ++j) sum += *reinterpret_cast<const uint64_t*>(pairs + off[j] + (i + u) * 8);
-        StoreCell<kNonTemporal>(out, at, sum);
-        at += row_bytes;
-      }
-    }
-    for (; i < n; ++i) {
-      uint64_t sum = 0;
-#pragma unroll
-      for (int j = 0; j < K; ++j) sum += *reinterpret_cast<const uint64_t*>(pairs + off[j] + i * 8);
-      StoreCell<kNonTemporal>(out, at, sum);
-      at += row_bytes;
-    }
-    if (n == 0) {
-      // last tile of a segment: lanes hold runs of different lengths
-      const uint32_t re = rows < rb + L.run ? rows : rb + L.run;
-      for (uint32_t rr = rb; rr < re; ++rr) {
-        uint64_t sum = 0;
-#pragma unroll
-        for (int j = 0; j < K; ++j) sum += *reinterpret_cast<const uint64_t*>(pairs + off[j] + (rr - rb) * 8);
-        StoreCell<kNonTemporal>(out, at, sum);
-        at += row_bytes;
-      }
-    }
+    for (; i + 2 <= n; i += 2) steps_at(std::integral_constant<uint32_t, 2>{}, i);
+    if (i < n) steps_at(std::integral_constant<uint32_t, 1>{}, i);
+    if (g < extra) steps_at(std::integral_constant<uint32_t, 1>{}, n);  // row g + n * G < rows
   }
   WaveSync();
+}
+
+// Descriptor of `tile` (a tile of this k): from the inline segment table
+// when the launch has one, else from the descriptor array.
+template <int K>
+__device__ __forceinline__ KoptTile TileAt(const KoptLaunch& a, const KoptKLayout& L, uint64_t tile) {
+  if (a.inline_count == 0) return a.tiles[tile];
+  constexpr int ki = K - 3;
+  uint32_t s = a.k_inline_begin[ki];
+  const uint32_t e = a.k_inline_begin[ki + 1];
+  while (s + 1 < e && tile >= a.inline_seg[s + 1].tile_begin) ++s;
+  const KoptInlineSegment& g = a.inline_seg[s];
+  const uint32_t row0 = static_cast<uint32_t>(tile - g.tile_begin) * L.rows;
+  KoptTile d;
+  d.task_index = g.task_offset + row0;
+  d.cost_index = g.cost_offset + static_cast<uint64_t>(row0) * L.T;
+  d.coord_base = g.coord_base;
+  d.rows = g.task_count - row0 < L.rows ? g.task_count - row0 : L.rows;
+  d.distance_type = g.distance_type;
+  d.near = g.near;
+  return d;
 }
 
 // The tiles of one k handed to this workgroup, one per wavefront at a time.
@@ -276,17 +333,18 @@ __device__ void TileLoop(const KoptLaunch& a, char* smem) {
   const uint32_t wib = tid / kWave;
   const uint32_t waves_per_block = bd / kWave;
   char* wave = smem + a.wave_base + wib * a.wave_bytes;
-  const uint32_t block = blockIdx.x - a.k_block_begin[ki];
-  const uint32_t blocks = a.k_block_begin[ki + 1] - a.k_block_begin[ki];
+  const uint32_t block = blockIdx.x - a.k_block_first[ki];
+  const uint32_t blocks = a.k_block_count[ki];
   const uint64_t stride = static_cast<uint64_t>(blocks) * waves_per_block;
   const uint64_t end = a.k_tile_begin[ki] + a.k_tile_count[ki];
   uint64_t tile = a.k_tile_begin[ki] + __builtin_amdgcn_readfirstlane(block * waves_per_block + wib);
+  KoptTile d{};
   if (tile < end) {
-    const KoptTile d0 = a.tiles[tile];
-    if (d0.near != 0) {
-      StagePorts<K, true>(a, L, d0, wave, lane);
+    d = TileAt<K>(a, L, tile);
+    if (d.near != 0) {
+      StagePorts<K, true>(a, L, d, wave, lane);
     } else {
-      StagePorts<K, false>(a, L, d0, wave, lane);
+      StagePorts<K, false>(a, L, d, wave, lane);
     }
   }
 
@@ -302,7 +360,7 @@ __device__ void TileLoop(const KoptLaunch& a, char* smem) {
   __syncthreads();
 
   for (bool first = true; tile < end; tile += stride, first = false) {
-    const KoptTile d = a.tiles[tile];
+    if (!first) d = TileAt<K>(a, L, tile);
     if (d.near != 0) {
       if (!first) StagePorts<K, true>(a, L, d, wave, lane);
       TileCells<K, true, kNonTemporal>(a, L, d, smem, wave, lane);
@@ -316,9 +374,9 @@ __device__ void TileLoop(const KoptLaunch& a, char* smem) {
 template <bool kNonTemporal>
 __global__ void __launch_bounds__(256) KoptKernel(KoptLaunch a) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
-  if (blockIdx.x < a.k_block_begin[1]) {
+  if (blockIdx.x < a.k_block_first[1]) {
     TileLoop<3, kNonTemporal>(a, smem);
-  } else if (blockIdx.x < a.k_block_begin[2]) {
+  } else if (blockIdx.x < a.k_block_first[2]) {
     TileLoop<4, kNonTemporal>(a, smem);
   } else {
     TileLoop<5, kNonTemporal>(a, smem);
@@ -353,11 +411,9 @@ size_t KoptPrepareLayout(KoptLaunch* a, const KoptPlan plans[3], const bool pres
     if (L.T <= static_cast<uint32_t>(kWave)) {
       L.group = kWave / L.T;
       L.passes = 1;
-      L.run = (rows + L.group - 1) / L.group;
     } else {
       L.group = 1;
       L.passes = (L.T + kWave - 1) / kWave;
-      L.run = rows;
     }
     // records per pair: at least rows, and odd, so that 32 lanes reading one
     // 8-byte record each of 32 different pairs hit 64 distinct LDS banks
@@ -394,16 +450,31 @@ void KoptAssignTiles(KoptSegment* segments, size_t count, const KoptInstance* in
                      const KoptPlan plans[3], const KoptTuning& tuning, KoptTile* tiles,
                      KoptLaunch* a) {
   uint64_t tile = 0;
+  bool inline_ok = count <= kInlineSegments;
+  uint32_t n_inline = 0;
   for (int i = 0; i < 3; ++i) {
     a->k_tile_begin[i] = tile;
+    a->k_inline_begin[i] = n_inline;
     for (size_t s = 0; s < count; ++s) {
       KoptSegment& seg = segments[s];
       if (seg.k != static_cast<uint32_t>(3 + i)) continue;
       seg.rows_per_tile = tuning.rows_per_tile[i];
       seg.tile_begin = tile;
       const uint64_t n_tiles = (seg.task_count + seg.rows_per_tile - 1) / seg.rows_per_tile;
+      const KoptInstance& inst = instances[seg.instance];
+      if (inline_ok && tile <= UINT32_MAX && seg.task_count < (uint64_t{1} << 31) && inst.coord_base <= UINT32_MAX) {
+        KoptInlineSegment& g = a->inline_seg[n_inline++];
+        g.task_offset = seg.task_offset;
+        g.cost_offset = seg.cost_offset;
+        g.tile_begin = static_cast<uint32_t>(tile);
+        g.task_count = static_cast<uint32_t>(seg.task_count);
+        g.coord_base = static_cast<uint32_t>(inst.coord_base);
+        g.distance_type = inst.distance_type;
+        g.near = inst.near;
+      } else {
+        inline_ok = false;
+      }
       if (tiles != nullptr) {
-        const KoptInstance& inst = instances[seg.instance];
         const uint64_t T = plans[i].templates;
         for (uint64_t j = 0; j < n_tiles; ++j) {
           const uint64_t row0 = j * seg.rows_per_tile;
@@ -420,7 +491,9 @@ void KoptAssignTiles(KoptSegment* segments, size_t count, const KoptInstance* in
     }
     a->k_tile_count[i] = tile - a->k_tile_begin[i];
   }
+  a->k_inline_begin[3] = n_inline;
   a->tile_count = tile;
+  a->inline_count = inline_ok && tile <= UINT32_MAX ? n_inline : 0;
 }
 
 void LaunchKopt(const KoptLaunch& args, const KoptTuning& tuning, size_t shared_bytes,
@@ -456,9 +529,13 @@ void LaunchKopt(const KoptLaunch& args, const KoptTuning& tuning, size_t shared_
     throw std::invalid_argument("k-opt launch needs too many workgroups");
   }
   KoptLaunch a = args;
-  a.k_block_begin[0] = 0;
-  for (int i = 0; i < 3; ++i) a.k_block_begin[i + 1] = a.k_block_begin[i] + static_cast<uint32_t>(blocks[i]);
-  const dim3 g(a.k_block_begin[3]), b(tuning.block);
+  uint32_t first = 0;
+  for (int i = 0; i < 3; ++i) {
+    a.k_block_first[i] = first;
+    a.k_block_count[i] = static_cast<uint32_t>(blocks[i]);
+    first += a.k_block_count[i];
+  }
+  const dim3 g(first), b(tuning.block);
   if (tuning.non_temporal) {
     hipLaunchKernelGGL(KoptKernel<true>, g, b, shared_bytes, stream, a);
   } else {

@@ -590,10 +590,12 @@ void EvalKoptCosts(Context& ctx, const int64_t* x, const int64_t* y, int32_t n, 
   std::vector<KoptTile> tiles(n_tiles);
   KoptLaunch a{};
   KoptAssignTiles(&segment, 1, &inst, c.host_plans.data(), ctx.kopt_tuning, tiles.data(), &a);
-  if (c.d_tiles.count() < n_tiles) c.d_tiles.Allocate(Grow(c.d_tiles.count(), n_tiles), ctx.device);
+  // a one-segment launch derives its tiles from the kernel arguments
+  const bool descriptors = a.inline_count == 0;
+  if (descriptors && c.d_tiles.count() < n_tiles) c.d_tiles.Allocate(Grow(c.d_tiles.count(), n_tiles), ctx.device);
   auto t0 = std::chrono::steady_clock::now();
   c.d_tasks.Upload(tasks, task_count, ctx.stream);
-  c.d_tiles.Upload(tiles.data(), n_tiles, ctx.stream);
+  if (descriptors) c.d_tiles.Upload(tiles.data(), n_tiles, ctx.stream);
   ctx.Sync();
   u.h2d_ms = MsSince(t0);
 

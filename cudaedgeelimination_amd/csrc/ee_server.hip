@@ -211,6 +211,8 @@ bool Server::Issue(Slot& slot, std::vector<std::shared_ptr<Request>> batch) {
         items += r->edges.size();
         continue;
       }
+      // requests start on 128-byte boundaries (whole-line stores stay aligned)
+      cells = (cells + kSegmentAlignCells - 1) / kSegmentAlignCells * kSegmentAlignCells;
       slot.offsets.push_back(cells);
       rows += r->tasks.size();
       cells += r->cells;
@@ -241,7 +243,7 @@ bool Server::Issue(Slot& slot, std::vector<std::shared_ptr<Request>> batch) {
                       slot.h_tiles.get(), &a);
       if (a.tile_count != tiles) throw std::logic_error("server tile count mismatch");
       slot.d_tasks.Upload(slot.h_tasks.get(), rows, slot.stream);
-      slot.d_tiles.Upload(slot.h_tiles.get(), tiles, slot.stream);
+      if (a.inline_count == 0) slot.d_tiles.Upload(slot.h_tiles.get(), tiles, slot.stream);
       a.xy = d_xy.get();
       a.tiles = slot.d_tiles.get();
       a.tasks = slot.d_tasks.get();
@@ -476,14 +478,16 @@ ee_status ee_server_create(ee_ctx* handle, const ee_server_config* config, ee_se
     const uint64_t max_cells =
         std::max<uint64_t>(1, std::min<uint64_t>(config->max_batch_cells, s->SlotBudget() / sizeof(int64_t)));
     const uint64_t max_rows = max_cells / 4 + 1;  // k = 3 has 4 cells per row
+    // plus the alignment padding between the requests of a batch
+    const uint64_t max_pool = max_cells + ee::kSegmentAlignCells * config->max_batch_requests;
     for (ee::Slot& slot : s->slots) {
       slot.d_tasks.Allocate(max_rows, ctx->device);
-      slot.d_costs.Allocate(max_cells, ctx->device);
+      slot.d_costs.Allocate(max_pool, ctx->device);
       slot.d_tiles.Allocate(max_rows, ctx->device);
       slot.d_jv_list.Allocate(max_cells, ctx->device);
       slot.d_jv_out.Allocate(max_cells, ctx->device);
       slot.h_tasks.Reserve(max_rows);
-      slot.h_costs.Reserve(max_cells);
+      slot.h_costs.Reserve(max_pool);
       slot.h_tiles.Reserve(max_rows);
       slot.h_jv_list.Reserve(max_cells);
       slot.h_jv_out.Reserve(max_cells);

@@ -48,7 +48,8 @@ class KoptBatch:
 
     def set_work(self, segments, tasks: np.ndarray) -> np.ndarray:
         """segments: [(instance, k, task_count)], tasks concatenated in
-        segment order.  Returns the first cell of every segment (+ total)."""
+        segment order.  Returns the first cell of every segment in the cost
+        pool (segments start on 16-cell boundaries) and the pool size."""
         descs = (_lib.SegmentDesc * max(len(segments), 1))()
         for i, (inst, k, count) in enumerate(segments):
             descs[i].instance, descs[i].k, descs[i].task_count = inst, k, count
@@ -63,6 +64,11 @@ class KoptBatch:
 
     @property
     def cells(self) -> int:
+        """Cells of all segments (the pool may hold padding between them)."""
+        return sum(count * TEMPLATES_PER_K[k] for _, k, count in getattr(self, "segments", []))
+
+    @property
+    def pool_cells(self) -> int:
         return int(self.cell_offsets[-1])
 
     def tuning(self) -> dict:
@@ -92,15 +98,15 @@ class KoptBatch:
         return ms.value
 
     def download(self, first: int = 0, count: int | None = None) -> np.ndarray:
-        count = self.cells - first if count is None else count
+        count = self.pool_cells - first if count is None else count
         out = np.empty(count, np.int64)
         _lib.check(_lib.lib().ee_kopt_batch_download(self._h, first, count,
                                                      _lib.ptr(out) if count else None))
         return out
 
     def segment_costs(self, index: int) -> np.ndarray:
-        lo, hi = int(self.cell_offsets[index]), int(self.cell_offsets[index + 1])
-        return self.download(lo, hi - lo)
+        _, k, count = self.segments[index]
+        return self.download(int(self.cell_offsets[index]), count * TEMPLATES_PER_K[k])
 
     def resident_bytes(self) -> int:
         return int(_lib.lib().ee_kopt_batch_resident_bytes(self._h))

@@ -205,7 +205,9 @@ ee_status ee_kopt_batch_add_instance(ee_kopt_batch* batch, const int64_t* x, con
                                      uint32_t* instance);
 /* Validates and uploads the task rows of every segment (segment order) and
  * sizes the resident cost output.  cell_offsets (nullable, count + 1
- * entries) receives the first cell of every segment and the total. */
+ * entries) receives the first cell of every segment and the size of the cost
+ * pool; segment s holds task_count * T(k) cells from cell_offsets[s], and
+ * every segment starts on a multiple of 16 cells (128 bytes). */
 ee_status ee_kopt_batch_set_work(ee_kopt_batch* batch, const ee_kopt_segment_desc* segments,
                                  uint32_t segment_count, const ee_kopt_task* tasks,
                                  uint64_t* cell_offsets);

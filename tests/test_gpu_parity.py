@@ -21,6 +21,7 @@ import pytest
 import _oracle as O
 from cudaedgeelimination_amd import _lib, backend, workloads as W
 from cudaedgeelimination_amd.graph import CEIL_2D, EUC_2D, GraphSnapshot, exact_distances
+from cudaedgeelimination_amd.resident import KoptBatch
 
 pytestmark = pytest.mark.gpu
 
@@ -93,6 +94,13 @@ def test_device_near_distances_are_exact(ctx, dtype):
     x = np.concatenate([[0], m, np.zeros_like(m), rng.integers(0, lim + 1, 20000)]).astype(np.int64)
     y = np.concatenate([[0], np.zeros_like(m), m, rng.integers(0, lim + 1, 20000)]).astype(np.int64)
     y[1:len(m) + 1:3] = 1  # s = m^2 + 1
+    # s = (2t^2 + 1)^2 - 1 just below a square, s = u^4 + u^2 on the EUC_2D
+    # half-way point (m = u^2, s - m^2 = m) and s = u^4 - u^2 + 1 just past it
+    t = np.arange(3000, 4096, dtype=np.int64)
+    u = np.arange(4000, 5793, dtype=np.int64)
+    x = np.concatenate([x, 2 * t * t, u * u, u * u - 1])
+    y = np.concatenate([y, 2 * t, u, u])
+    assert x.max() < (1 << 25) and y.max() < (1 << 25)
     n = len(x)
     a = np.concatenate([np.zeros(n - 1, np.int32), rng.integers(0, n, 200000).astype(np.int32)])
     b = np.concatenate([np.arange(1, n, dtype=np.int32), rng.integers(0, n, 200000).astype(np.int32)])
@@ -184,6 +192,63 @@ def test_kopt_random_rows_match_oracle(ctx, k, dtype, half):
     got, _, _ = backend.evaluate_kopt_template_costs_cuda(g, table, tasks, ctx)
     want, _, _ = O.kopt_costs(x, y, dtype, k, tasks)
     assert np.array_equal(got, want)
+
+
+def random_partition_row(rng, k):
+    """Port labels 0..2k-1 of one valid task (k distinct deleted edges, no self
+    loop): all distinct, a chain, or heavy repetition over k + 1 labels."""
+    mode = rng.random()
+    if mode < 0.3:
+        return list(range(2 * k))
+    if mode < 0.6:
+        return [v for e in range(k) for v in (e, e + 1)]
+    labels = []
+    for _ in range(k):
+        while True:
+            a, b = rng.sample(range(k + 1), 2)
+            if {a, b} not in [{labels[i], labels[i + 1]} for i in range(0, len(labels), 2)]:
+                break
+        labels += [a, b]
+    return labels
+
+
+@pytest.mark.parametrize("dtype", [EUC_2D, CEIL_2D])
+@pytest.mark.parametrize("k", [3, 4, 5])
+def test_kopt_shared_partition_tiles_match_oracle(ctx, k, dtype):
+    """Blocks of 64 rows that all have one port partition (the same labels
+    under random relabelings: all distinct, a chain, or heavy repetition),
+    some with one row of another partition; several tile sizes move the block
+    edges inside tiles."""
+    say(f"k-opt shared-partition tiles k={k} dtype={dtype}")
+    rng = random.Random(1000 * k + dtype)
+    n = 96
+    x = np.array([rng.randrange(0, 1 << 24) for _ in range(n)], np.int64)
+    y = np.array([rng.randrange(0, 1 << 24) for _ in range(n)], np.int64)
+    g = GraphSnapshot.from_edges(x, y, dtype, np.zeros(0, np.int32), np.zeros(0, np.int32))
+    tasks = np.zeros(64 * 12, _lib.KOPT_TASK_DTYPE)
+    for blk in range(12):
+        labels = random_partition_row(rng, k)
+        odd = rng.randrange(64) if blk % 3 == 1 else -1
+        for i in range(64):
+            row = labels if i != odd else random_partition_row(rng, k)
+            relabel = rng.sample(range(n), 2 * k)
+            t = tasks[blk * 64 + i]
+            t["port_nodes"][:2 * k] = [relabel[v] for v in row]
+            t["port_nodes"][2 * k:] = rng.randrange(n)
+    want, _, _ = O.kopt_costs(x, y, dtype, k, tasks)
+    table = backend.build_kopt_reconnect_table(k)
+    got, _, _ = backend.evaluate_kopt_template_costs_cuda(g, table, tasks, ctx)
+    assert np.array_equal(got, want)
+    b = KoptBatch(ctx)
+    inst = b.add_instance(g)
+    b.set_work([(inst, k, len(tasks))], tasks)
+    for rows in (64, 24, 7, 1):
+        tuning = [32, 16, 8]
+        tuning[k - 3] = rows
+        b.set_tuning(tuning, 0, 64, 0)
+        b.run(1)
+        assert np.array_equal(b.segment_costs(0), want), rows
+    b.close()
 
 
 def test_kopt_arbitrary_template_tables_match_edge_list_oracle(ctx):

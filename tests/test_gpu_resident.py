@@ -16,6 +16,8 @@ from cudaedgeelimination_amd import backend, workloads as W
 from cudaedgeelimination_amd.graph import CEIL_2D, EUC_2D, GraphSnapshot
 from cudaedgeelimination_amd.resident import JvSession, KoptBatch
 
+T_OF = {3: 4, 4: 25, 5: 208}
+
 pytestmark = pytest.mark.gpu
 
 ROOT = pathlib.Path(__file__).resolve().parent.parent
@@ -118,10 +120,13 @@ def test_batch_many_instances_match_oracle(ctx):
     b.autotune(2)
     b.run(1)
     out = b.download()
+    assert len(out) == int(offsets[-1]) and b.cells == sum(len(t) * T_OF[k] for t, (_, k, _) in zip(tasks, segs))
     for s, (inst, k, count) in enumerate(segs):
         g = graphs[inst]
         want, _, _ = O.kopt_costs(g.x, g.y, g.distance_type, k, tasks[s])
-        assert np.array_equal(out[int(offsets[s]):int(offsets[s + 1])], want), (inst, k)
+        assert int(offsets[s]) % 16 == 0  # 128-byte aligned segments
+        assert np.array_equal(out[int(offsets[s]):int(offsets[s]) + len(want)], want), (inst, k)
+        assert np.array_equal(b.segment_costs(s), want)
     b.close()
 
 
