@@ -192,6 +192,69 @@ __device__ __forceinline__ void PairRecords(const KoptKLayout& L, uint32_t rows,
   }
 }
 
+// The cells of one lane's template: n steps of G rows (rows g, g + G, ...),
+// plus row g + n * G when `extra_row`.  `off` holds the byte offsets of the
+// template's k records for row g; `at` the byte offset of its first cell.
+// Near tiles sum the two 32-bit halves of the records separately: their
+// distances are below 2^26, so the low halves sum to the cost without a carry
+// and the high halves to the counters and sentinels.  kG is G when it is a
+// compile-time constant, else 0.
+template <int K, bool kNear, bool kNonTemporal, uint32_t kG>
+__device__ __forceinline__ void CellSteps(const char* pairs, const uint32_t (&off)[K], int64_t* out,
+                                          uint32_t at, uint32_t step_bytes, uint32_t g_rows, uint32_t n,
+                                          bool extra_row) {
+  const uint32_t G = kG != 0 ? kG : g_rows;
+  auto steps = [&](auto unroll, uint32_t i) {
+    constexpr uint32_t U = decltype(unroll)::value;
+    if (kNear) {
+      uint32_t lo[U], hi[U];
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) lo[u] = hi[u] = 0;
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        const uint2* rec = reinterpret_cast<const uint2*>(pairs + off[j] + i * G * 8);
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+          const uint2 v = rec[u * G];
+          lo[u] += v.x;
+          hi[u] += v.y;
+        }
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        const bool bad = (hi[u] & static_cast<uint32_t>(kBadMask >> 32)) != 0;
+        const int64_t value = bad ? kInvalid : static_cast<int64_t>(lo[u]);
+        int64_t* dst = reinterpret_cast<int64_t*>(reinterpret_cast<char*>(out) + at);
+        if (kNonTemporal) {
+          __builtin_nontemporal_store(value, dst);
+        } else {
+          *dst = value;
+        }
+        at += step_bytes;
+      }
+    } else {
+      uint64_t sum[U];
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) sum[u] = 0;
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        const uint64_t* rec = reinterpret_cast<const uint64_t*>(pairs + off[j] + i * G * 8);
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) sum[u] += rec[u * G];
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        StoreCell<kNonTemporal>(out, at, sum[u]);
+        at += step_bytes;
+      }
+    }
+  };
+  uint32_t i = 0;
+  for (; i + 2 <= n; i += 2) steps(std::integral_constant<uint32_t, 2>{}, i);
+  if (i < n) steps(std::integral_constant<uint32_t, 1>{}, i);
+  if (extra_row) steps(std::integral_constant<uint32_t, 1>{}, n);  // row g + n * G < rows
+}
+
 // Phases A2, B and C of one staged tile.
 template <int K, bool kNear, bool kNonTemporal>
 __device__ void TileCells(const KoptLaunch& a, const KoptKLayout& L, const KoptTile& d,
@@ -262,30 +325,16 @@ __device__ void TileCells(const KoptLaunch& a, const KoptKLayout& L, const KoptT
 #pragma unroll
     for (int j = 0; j < K; ++j) off[j] = (static_cast<uint32_t>((tp >> (8 * j)) & 0xffu) * stride + g) * 8;
     // byte offset of the lane's next cell; a tile's cells stay below 2^24
-    uint32_t at = (g * T + t) * 8;
+    const uint32_t at = (g * T + t) * 8;
     if (!active) continue;
-    // U steps from one base address per pair (immediate offsets u * G)
-    auto steps_at = [&](auto unroll, uint32_t i) {
-      constexpr uint32_t U = decltype(unroll)::value;
-      uint64_t sum[U];
-#pragma unroll
-      for (uint32_t u = 0; u < U; ++u) sum[u] = 0;
-#pragma unroll
-      for (int j = 0; j < K; ++j) {
-        const uint64_t* rec = reinterpret_cast<const uint64_t*>(pairs + off[j] + i * G * 8);
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u) sum[u] += rec[u * G];
-      }
-#pragma unroll
-      for (uint32_t u = 0; u < U; ++u) {
-        StoreCell<kNonTemporal>(out, at, sum[u]);
-        at += step_bytes;
-      }
-    };
-    uint32_t i = 0;
-    for (; i + 2 <= n; i += 2) steps_at(std::integral_constant<uint32_t, 2>{}, i);
-    if (i < n) steps_at(std::integral_constant<uint32_t, 1>{}, i);
-    if (g < extra) steps_at(std::integral_constant<uint32_t, 1>{}, n);  // row g + n * G < rows
+    // the standard templates give T = 4 / 25 / 208, so G = 16 / 2 / 1: with G
+    // a constant the rows of a step are immediate offsets of one address
+    constexpr uint32_t kStdG = K == 3 ? 16u : (K == 4 ? 2u : 1u);
+    if (G == kStdG) {
+      CellSteps<K, kNear, kNonTemporal, kStdG>(pairs, off, out, at, step_bytes, G, n, g < extra);
+    } else {
+      CellSteps<K, kNear, kNonTemporal, 0>(pairs, off, out, at, step_bytes, G, n, g < extra);
+    }
   }
   WaveSync();
 }
