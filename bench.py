@@ -350,16 +350,22 @@ def copy_calibration(launches: int) -> int:
     return 0
 
 
-def measured_traffic():
-    """HBM bytes per KoptKernel launch from the committed counter summary
-    (FETCH_SIZE + WRITE_SIZE, each scaled by the copy calibration), or None."""
-    path = ROOT / "profiles" / "kopt_kernel_config3_traffic.md"
+def profile_figure(name: str, key: str):
+    """A figure `key: value` from a committed rocprofv3 summary, or None."""
+    path = ROOT / "profiles" / name
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
-            return int(line.split(":", 1)[1].strip())
+        if line.startswith(key + ":"):
+            return float(line.split(":", 1)[1].strip())
     return None
+
+
+def measured_traffic():
+    """HBM bytes per KoptKernel launch from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the copy calibration), or None."""
+    v = profile_figure("kopt_kernel_config3_traffic.md", "traffic_bytes_per_launch")
+    return None if v is None else int(v)
 
 
 def main() -> int:
@@ -455,6 +461,8 @@ def main() -> int:
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+            # SQ_INSTS_VALU x 4 cycles over every SIMD's cycles of the traced launch
+            "valu_utilisation": profile_figure("kopt_kernel_config3_valu.md", "valu_utilisation"),
             "kernel": "KoptKernel", "algorithmic_bytes_per_launch": algo_bytes,
             "kernel_ms": device_ms, "graph_replay_kernel_ms": graph_ms,
         },
