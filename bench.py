@@ -252,7 +252,8 @@ def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005, jv_share:
             count = burst
             arrivals = np.zeros(count)
         else:
-            count = max(int(qps * seconds), 50)
+            # at least 5,000 requests, so that p99 rests on 50 of them
+            count = max(int(qps * seconds), 5000)
             arrivals = np.cumsum(rng.exponential(1.0 / qps, count))
         cells = np.clip(np.exp(rng.normal(np.log(2896.0), 1.04, count)), 128, 65536)
         kinds = rng.choice([3, 4, 5], size=count, p=[0.5, 0.3, 0.2])

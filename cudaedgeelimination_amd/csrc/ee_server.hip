@@ -12,9 +12,11 @@
 // is in flight the worker keeps admitting and issuing into a free slot, so
 // requests that arrive meanwhile overlap on the device.  A second thread
 // blocks on the oldest batch in flight, copies each request's results into
-// the caller's buffer, wakes waiters and frees the slot.  With one slot there
-// is no second thread: the worker issues, waits and completes each batch in
-// turn (the plain per-query path).
+// the caller's buffer, wakes waiters and frees the slot; a batch issued while
+// the server is otherwise idle is completed by the worker itself, which polls
+// it until it finishes or another request arrives.  With one slot there is no
+// second thread: the worker issues, waits and completes each batch in turn
+// (the plain per-query path).
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -387,9 +389,28 @@ void Server::Loop() {
         Finish(*free_slot);
         continue;
       }
-      std::lock_guard<std::mutex> lk(mu);
-      inflight.push_back(free_slot);
-      cv_inflight.notify_one();
+      // With nothing else in flight and nothing waiting, completing here
+      // saves the hand-over to the completing thread (the lone request's
+      // latency is then the plain path's); the batch goes to that thread as
+      // soon as another request arrives, so the issuer is free to overlap it.
+      bool inline_finish = false;
+      for (;;) {
+        {
+          std::lock_guard<std::mutex> lk(mu);
+          if (!inflight.empty() || !pending.empty() || stopping) {
+            inflight.push_back(free_slot);
+            cv_inflight.notify_one();
+            break;
+          }
+        }
+        const hipError_t q = hipEventQuery(free_slot->ev_done);
+        if (q != hipErrorNotReady) {  // done, or failed: Finish reports it
+          inline_finish = true;
+          break;
+        }
+        std::this_thread::yield();
+      }
+      if (inline_finish) Finish(*free_slot);
     }
   }
   std::lock_guard<std::mutex> lk(mu);
