@@ -57,9 +57,9 @@ struct Fleet {
     node_base.assign(count + 1, 0);
     edge_base.assign(count + 1, 0);
     size_t adj = 0;
+    for (uint32_t i = 0; i < count; ++i) CheckGraphView(&graphs[i]);
+    ParallelFor(count, [&](size_t i) { hosts[i].Load(&graphs[i]); });
     for (uint32_t i = 0; i < count; ++i) {
-      CheckGraphView(&graphs[i]);
-      hosts[i].Load(&graphs[i]);
       node_base[i + 1] = node_base[i] + static_cast<size_t>(hosts[i].n);
       edge_base[i + 1] = edge_base[i] + static_cast<size_t>(hosts[i].m);
       adj += hosts[i].neighbors.size();
@@ -74,7 +74,7 @@ struct Fleet {
     std::vector<int64_t> weight(std::max<size_t>(slots, 1));
     std::vector<uint8_t> mode(std::max<size_t>(slots, 1));
     std::vector<longlong2> xy(std::max<size_t>(nodes, 1));
-    for (uint32_t i = 0; i < count; ++i) {
+    ParallelFor(count, [&](size_t i) {
       const JvHost& h = hosts[i];
       const int32_t nb = static_cast<int32_t>(node_base[i]);
       const uint8_t md = static_cast<uint8_t>((h.type & 1u) | (h.near ? 2u : 0u));
@@ -85,7 +85,7 @@ struct Fleet {
         mode[s] = md;
       }
       for (int32_t vtx = 0; vtx < h.n; ++vtx) xy[node_base[i] + vtx] = longlong2{h.x[vtx], h.y[vtx]};
-    }
+    });
     const int dev = ctx->device;
     d_uv.Allocate(uv.size(), dev);
     d_weight.Allocate(weight.size(), dev);
@@ -113,21 +113,22 @@ struct Fleet {
   // Writes the union's active flags and CSR into the pinned staging; graphs
   // whose run is done contribute inactive slots.
   void Assemble() {
-    size_t adj = 0;
-    for (size_t i = 0; i < hosts.size(); ++i) {
+    std::vector<size_t> adj_base(hosts.size() + 1, 0);
+    for (size_t i = 0; i < hosts.size(); ++i) adj_base[i + 1] = adj_base[i] + hosts[i].neighbors.size();
+    const size_t adj = adj_base[hosts.size()];
+    if (adj > adjacency) throw std::logic_error("JV fleet adjacency grew");
+    ParallelFor(hosts.size(), [&](size_t i) {
       const JvHost& h = hosts[i];
-      const size_t eb = edge_base[i], nb = node_base[i];
+      const size_t eb = edge_base[i], nb = node_base[i], ab = adj_base[i];
       for (int32_t e = 0; e < h.m; ++e) h_active.get()[eb + e] = (!h.done && h.active[e]) ? 1 : 0;
       for (int32_t vtx = 0; vtx < h.n; ++vtx) {
-        h_row_offsets.get()[nb + vtx] = static_cast<int32_t>(adj) + h.row_offsets[vtx];
+        h_row_offsets.get()[nb + vtx] = static_cast<int32_t>(ab) + h.row_offsets[vtx];
       }
       for (size_t s = 0; s < h.neighbors.size(); ++s) {
-        h_neighbors.get()[adj + s] = static_cast<int32_t>(nb) + h.neighbors[s];
-        h_edge_ids.get()[adj + s] = static_cast<int32_t>(eb) + h.edge_ids[s];
+        h_neighbors.get()[ab + s] = static_cast<int32_t>(nb) + h.neighbors[s];
+        h_edge_ids.get()[ab + s] = static_cast<int32_t>(eb) + h.edge_ids[s];
       }
-      adj += h.neighbors.size();
-    }
-    if (adj > adjacency) throw std::logic_error("JV fleet adjacency grew");
+    });
     h_row_offsets.get()[nodes] = static_cast<int32_t>(adj);
   }
 
@@ -174,15 +175,15 @@ struct Fleet {
     ee_jv_fleet_stats st{};
     st.instances = static_cast<uint32_t>(hosts.size());
     st.edge_slots = slots;
-    for (JvHost& h : hosts) h.StartRun();
+    // the graphs' host runs are independent: each phase runs over the host cores
+    ParallelFor(hosts.size(), [&](size_t i) { hosts[i].StartRun(); });
     for (uint32_t epoch = 0; epoch < max_rounds; ++epoch) {
       bool live = false;
-      for (JvHost& h : hosts) {
-        if (h.done) continue;
-        h.BeginEpoch(epoch);
-        live = true;
-      }
+      for (const JvHost& h : hosts) live = live || !h.done;
       if (!live) break;
+      ParallelFor(hosts.size(), [&](size_t i) {
+        if (!hosts[i].done) hosts[i].BeginEpoch(epoch);
+      });
       const auto t0 = std::chrono::steady_clock::now();
       Assemble();
       st.assemble_ms += MsSince(t0);
@@ -209,9 +210,9 @@ struct Fleet {
       const double propose_ms = MsSince(t0);
       st.propose_ms += propose_ms;
       const auto t1 = std::chrono::steady_clock::now();
-      for (size_t i = 0; i < hosts.size(); ++i) {
+      ParallelFor(hosts.size(), [&](size_t i) {
         JvHost& h = hosts[i];
-        if (h.done) continue;
+        if (h.done) return;
         // witnesses are union node ids: back to the graph's own ids
         int32_t* w = h_witnesses.get() + edge_base[i];
         const int32_t nb = static_cast<int32_t>(node_base[i]);
@@ -219,13 +220,13 @@ struct Fleet {
           if (w[e] >= 0) w[e] -= nb;
         }
         h.FinishEpoch(w, propose_ms);
-      }
+      });
       st.host_ms += MsSince(t1);
       ++st.epochs;
     }
     const double total = MsSince(t_total);
-    for (JvHost& h : hosts) {
-      h.EndRun(total);
+    ParallelFor(hosts.size(), [&](size_t i) { hosts[i].EndRun(total); });
+    for (const JvHost& h : hosts) {
       st.edges_scanned += h.stats.edges_scanned;
       st.proposed += h.stats.proposed;
       st.committed += h.stats.committed;

@@ -165,7 +165,7 @@ __global__ void __launch_bounds__(kBlock) JvKernel(JvLaunch g) {
           ls = s_score[wib][cur][lane];
           ln = s_node[wib][cur][lane];
         }
-        {
+        if (cnt > 0) {  // uniform: the first pass has no running list
           const int c = CountLessInWave(vmask, lanes, sc, x, ls, ln);
           if (lane < cnt) lrank = lane + c;
         }

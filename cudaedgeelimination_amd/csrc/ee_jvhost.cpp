@@ -1,8 +1,15 @@
 // Host side of a resident JV run; see ee_jvhost.h.
 #include "ee_jvhost.h"
 
+#include <sched.h>
+
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <exception>
+#include <mutex>
+#include <thread>
+#include <vector>
 #include <cstdio>
 #include <sstream>
 #include <stdexcept>
@@ -27,6 +34,39 @@ std::string Hex(uint64_t h) {
 }
 
 }  // namespace
+
+void ParallelFor(size_t n, const std::function<void(size_t)>& body, unsigned threads) {
+  if (threads == 0) {
+    cpu_set_t set;
+    CPU_ZERO(&set);
+    threads = sched_getaffinity(0, sizeof(set), &set) == 0 ? static_cast<unsigned>(CPU_COUNT(&set)) : 1u;
+    threads = std::min(threads, 32u);
+  }
+  const size_t workers = std::min<size_t>(std::max(threads, 1u), n);
+  if (workers <= 1) {
+    for (size_t i = 0; i < n; ++i) body(i);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  std::mutex mu;
+  std::exception_ptr first;
+  auto work = [&] {
+    for (size_t i = next.fetch_add(1); i < n; i = next.fetch_add(1)) {
+      try {
+        body(i);
+      } catch (...) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!first) first = std::current_exception();
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  pool.reserve(workers - 1);
+  for (size_t t = 1; t < workers; ++t) pool.emplace_back(work);
+  work();
+  for (std::thread& t : pool) t.join();
+  if (first) std::rethrow_exception(first);
+}
 
 void JvHost::Load(const ee_graph_view* g) {
   n = g->dimension;

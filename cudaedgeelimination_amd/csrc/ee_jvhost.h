@@ -6,12 +6,18 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
 #include "ee_hip.h"
 
 namespace ee {
+
+// Runs body(i) for every i in [0, n) on up to `threads` host threads (0: the
+// cores this process may run on, at most 32); each index runs exactly once.
+// The first exception a body throws is rethrown once every thread is done.
+void ParallelFor(size_t n, const std::function<void(size_t)>& body, unsigned threads = 0);
 
 struct JvHost {
   int32_t n = 0, m = 0;
