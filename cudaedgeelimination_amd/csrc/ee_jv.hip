@@ -12,12 +12,14 @@
 //   2. top ten by (score, node): every valid lane computes its rank by
 //      counting smaller keys with readlane; ranks < 10 land in LDS in
 //      ascending order.  Rows longer than one pass merge into the running list.
-//   3. witness test (jv_elimination.cpp:86-117): candidates in rank order; for
-//      candidate c, lane j tests the j-th neighbour d of c; a ballot tells
-//      whether any d is compatible (w_ab + w_cd <= D(a,c) + D(d,b) or
-//      w_ab + w_cd <= D(a,d) + D(b,c)).  The first c with an empty ballot is
-//      the witness.  Testing every d at once gives the same verdict as the
-//      reference's early exit at the first compatible d.
+//   3. witness test (jv_elimination.cpp:86-117): candidates in rank order,
+//      as many consecutive candidates per pass as their rows fit 64 lanes;
+//      lane j tests one neighbour d of its candidate c and a ballot tells,
+//      per candidate, whether any d is compatible (w_ab + w_cd <= D(a,c) +
+//      D(d,b) or w_ab + w_cd <= D(a,d) + D(b,c)).  The first c with no
+//      compatible d is the witness.  Testing every d at once gives the same
+//      verdict as the reference's early exit at the first compatible d, and
+//      testing later candidates of a pass does not change which is first.
 // Sums stay exact in uint64: the host guarantees weights <= 2^62 and
 // distances <= 2^32, so no sum wraps and unsigned comparison equals the
 // reference CPU's 128-bit comparison.
@@ -79,8 +81,6 @@ __device__ __forceinline__ int64_t Dist(const longlong2 p, const longlong2 q, in
 __global__ void __launch_bounds__(kBlock) JvKernel(JvLaunch g) {
   __shared__ uint64_t s_score[kWavesPerBlock][2][kCandidates];
   __shared__ int32_t s_node[kWavesPerBlock][2][kCandidates];
-  __shared__ int64_t s_cac[kWavesPerBlock][kCandidates];
-  __shared__ int64_t s_cbc[kWavesPerBlock][kCandidates];
 
   const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
   const int wib = static_cast<int>(threadIdx.x) / kWave;
@@ -183,38 +183,79 @@ __global__ void __launch_bounds__(kBlock) JvKernel(JvLaunch g) {
         cur = nxt;
       }
 
-      // ---- 3: witness test in candidate order
+      // ---- 3: witness test in candidate order.  Lane q < cnt holds
+      // candidate q (node, row, D(a,c), D(b,c)); consecutive candidates whose
+      // rows fit 64 lanes together are tested in one pass, lane j testing the
+      // j-th neighbour of its candidate, and the first candidate whose lanes
+      // have no compatible neighbour is the witness.
+      int32_t c_node = -1, c_r0 = 0, c_deg = 0;
+      uint64_t c_ac = 0, c_bc = 0;
       if (lane < cnt) {
-        const longlong2 pc = g.xy[s_node[wib][cur][lane]];
-        s_cac[wib][lane] = Dist(pa, pc, type, near);
-        s_cbc[wib][lane] = Dist(pb, pc, type, near);
+        c_node = s_node[wib][cur][lane];
+        const longlong2 pc = g.xy[c_node];
+        c_ac = static_cast<uint64_t>(Dist(pa, pc, type, near));
+        c_bc = static_cast<uint64_t>(Dist(pb, pc, type, near));
+        c_r0 = g.row_offsets[c_node];
+        c_deg = g.row_offsets[c_node + 1] - c_r0;
       }
-      WaveSync();
-      for (int i = 0; i < cnt; ++i) {
-        const int32_t c = s_node[wib][cur][i];
-        const uint64_t cac = static_cast<uint64_t>(s_cac[wib][i]);
-        const uint64_t cbc = static_cast<uint64_t>(s_cbc[wib][i]);
-        const int32_t rc0 = g.row_offsets[c], rc1 = g.row_offsets[c + 1];
-        bool compatible = false;
-        for (int32_t base = rc0; base < rc1 && !compatible; base += kWave) {
-          const int32_t slot = base + lane;
-          bool hit = false;
-          if (slot < rc1) {
-            const int32_t d = g.neighbors[slot];
-            if (d != a && d != b) {
-              const longlong2 pd = g.xy[d];
-              const uint64_t lhs = wab + static_cast<uint64_t>(g.edge_weight[g.csr_edge_ids[slot]]);
-              const uint64_t first = cac + static_cast<uint64_t>(Dist(pd, pb, type, near));
-              const uint64_t second = static_cast<uint64_t>(Dist(pa, pd, type, near)) + cbc;
-              hit = lhs <= first || lhs <= second;
-            }
+      auto compatible_at = [&](int32_t slot, uint64_t cac, uint64_t cbc) -> bool {
+        const int32_t d = g.neighbors[slot];
+        if (d == a || d == b) return false;
+        const longlong2 pd = g.xy[d];
+        const uint64_t lhs = wab + static_cast<uint64_t>(g.edge_weight[g.csr_edge_ids[slot]]);
+        return lhs <= cac + static_cast<uint64_t>(Dist(pd, pb, type, near)) ||
+               lhs <= static_cast<uint64_t>(Dist(pa, pd, type, near)) + cbc;
+      };
+      for (int i = 0; i < cnt;) {  // uniform
+        const int deg_i = ReadLane(c_deg, i);
+        if (deg_i > kWave) {
+          // a row longer than a wavefront: this candidate alone, pass by pass
+          const int32_t r0 = ReadLane(c_r0, i), r1 = r0 + deg_i;
+          const uint64_t cac = ReadLane64(c_ac, i), cbc = ReadLane64(c_bc, i);
+          bool compatible = false;
+          for (int32_t base = r0; base < r1 && !compatible; base += kWave) {
+            const int32_t slot = base + lane;
+            compatible = __ballot(slot < r1 && compatible_at(slot, cac, cbc)) != 0;
           }
-          compatible = __ballot(hit) != 0;
+          if (!compatible) {
+            result = ReadLane(c_node, i);
+            break;
+          }
+          ++i;
+          continue;
         }
-        if (!compatible) {
-          result = c;
-          break;
+        // candidates i .. j-1: lanes [start_q, start_q + deg_q) test candidate q
+        int j = i, used = 0, my_q = -1, my_k = 0;
+        while (j < cnt) {
+          const int dj = ReadLane(c_deg, j);
+          if (used + dj > kWave) break;
+          if (lane >= used && lane < used + dj) {
+            my_q = j;
+            my_k = lane - used;
+          }
+          used += dj;
+          ++j;
         }
+        const int src = my_q < 0 ? 0 : my_q;
+        const int32_t r0 = __shfl(c_r0, src);
+        const uint64_t cac = (static_cast<uint64_t>(static_cast<uint32_t>(__shfl(static_cast<int>(c_ac >> 32), src))) << 32) |
+                             static_cast<uint32_t>(__shfl(static_cast<int>(c_ac), src));
+        const uint64_t cbc = (static_cast<uint64_t>(static_cast<uint32_t>(__shfl(static_cast<int>(c_bc >> 32), src))) << 32) |
+                             static_cast<uint32_t>(__shfl(static_cast<int>(c_bc), src));
+        const uint64_t hits = __ballot(my_q >= 0 && compatible_at(r0 + my_k, cac, cbc));
+        bool found = false;
+        for (int q = i, start = 0; q < j; ++q) {  // uniform
+          const int dq = ReadLane(c_deg, q);
+          const uint64_t span = dq >= kWave ? ~0ull : ((1ull << dq) - 1ull);
+          if ((hits & (span << start)) == 0) {
+            result = ReadLane(c_node, q);
+            found = true;
+            break;
+          }
+          start += dq;
+        }
+        if (found) break;
+        i = j;
       }
       WaveSync();
     }
