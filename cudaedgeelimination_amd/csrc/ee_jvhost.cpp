@@ -69,6 +69,8 @@ void ParallelFor(size_t n, const std::function<void(size_t)>& body, unsigned thr
 }
 
 void JvHost::Load(const ee_graph_view* g) {
+  if (pending_hash_.valid()) pending_hash_.wait();  // it reads the arrays replaced here
+  pending_hash_ = std::future<uint64_t>();
   n = g->dimension;
   m = g->edge_count;
   type = g->distance_type;
@@ -115,12 +117,33 @@ void JvHost::RebuildCsr() {
                                  neighbors.data(), edge_ids.data()));
 }
 
+uint64_t JvHost::ComputeHash() const {
+  return ContentHashEdges(prefix_hash_, m, u.data(), v.data(), weight.data(), active.data());
+}
+
 uint64_t JvHost::Hash() const {
+  if (pending_hash_.valid()) {
+    const uint64_t h = pending_hash_.get();
+    if (pending_generation_ == generation_) {
+      hash_ = h;
+      hash_generation_ = generation_;
+    }
+  }
   if (hash_generation_ != generation_) {
-    hash_ = ContentHashEdges(prefix_hash_, m, u.data(), v.data(), weight.data(), active.data());
+    hash_ = ComputeHash();
     hash_generation_ = generation_;
   }
   return hash_;
+}
+
+void JvHost::HashAsync() {
+  if (!async_hash || hash_generation_ == generation_) return;
+  if (pending_hash_.valid()) {
+    if (pending_generation_ == generation_) return;
+    pending_hash_.wait();
+  }
+  pending_generation_ = generation_;
+  pending_hash_ = std::async(std::launch::async, [this] { return ComputeHash(); });
 }
 
 bool JvHost::Verify(int32_t e, int32_t c) const {
@@ -142,7 +165,8 @@ void JvHost::StartRun() {
   records_.clear();
   metrics_.clear();
   stats = ee_jv_run_stats{};
-  stats.initial_hash = Hash();
+  initial_pending_ = true;  // the snapshot hash of the first commit or the end
+  HashAsync();
   done = false;
   proof.clear();
 }
@@ -151,8 +175,8 @@ void JvHost::BeginEpoch(uint32_t epoch) {
   cur_ = Metrics{};
   cur_.epoch = epoch;
   for (int32_t e = 0; e < m; ++e) cur_.before += active[e] != 0;
-  snap_ = Hash();
   snap_generation_ = generation_;
+  HashAsync();
 }
 
 size_t JvHost::FinishEpoch(const int32_t* witnesses, double propose_ms) {
@@ -187,8 +211,11 @@ size_t JvHost::FinishEpoch(const int32_t* witnesses, double propose_ms) {
   t0 = std::chrono::steady_clock::now();
   // the snapshot must be the one proposed on (elimination_commit.cpp:16-18);
   // it changes only through commits, which bump the generation
-  if (generation_ != snap_generation_ || Hash() != snap_) {
-    throw std::runtime_error("snapshot changed before commit");
+  if (generation_ != snap_generation_) throw std::runtime_error("snapshot changed before commit");
+  const uint64_t snap = Hash();
+  if (initial_pending_) {
+    stats.initial_hash = snap;
+    initial_pending_ = false;
   }
   std::sort(verified.begin(), verified.end(), [&](const auto& l, const auto& r) {
     return std::make_tuple(u[l.first], v[l.first], l.second, l.first) <
@@ -216,7 +243,7 @@ size_t JvHost::FinishEpoch(const int32_t* witnesses, double propose_ms) {
   stats.commit_ms += MsSince(t0);
   mt.committed = committed.size();
   for (const auto& c : committed) {
-    records_.push_back({mt.epoch, snap_, c.first, u[c.first], v[c.first], c.second});
+    records_.push_back({mt.epoch, snap, c.first, u[c.first], v[c.first], c.second});
   }
   metrics_.push_back(mt);
   stats.edges_scanned += mt.before;
@@ -229,6 +256,10 @@ size_t JvHost::FinishEpoch(const int32_t* witnesses, double propose_ms) {
 
 void JvHost::EndRun(double total_ms) {
   done = true;
+  if (initial_pending_) {
+    stats.initial_hash = Hash();
+    initial_pending_ = false;
+  }
   stats.final_hash = Hash();
   stats.total_ms = total_ms;
   // V1 proof text (src/cpu/jv_elimination.cpp:324-368)

@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <functional>
+#include <future>
 #include <string>
 #include <vector>
 
@@ -38,12 +39,17 @@ struct JvHost {
   // ContentHash of the current snapshot; recomputed only after a commit
   // changed it (the point prefix is hashed once, at Load).
   uint64_t Hash() const;
+  // With async_hash set, a stale snapshot hash is computed on a helper thread
+  // from the start of a run or epoch until the commit needs it, overlapping
+  // the device propose step (FNV-1a is byte-serial).
+  bool async_hash = false;
   // IsJvWitness (jv_elimination.cpp:86-117) with 128-bit sums.
   bool Verify(int32_t e, int32_t c) const;
 
   // ---- one run
   void StartRun();
-  // Snapshot hash and active edge count at the start of `epoch`.
+  // Active edge count at the start of `epoch`; the snapshot hash is taken
+  // when the epoch commits (the snapshot does not change in between).
   void BeginEpoch(uint32_t epoch);
   // Re-verifies every proposal of witnesses[0..m) (witness or -1), throws if
   // any is rejected, commits and records; returns the committed count.  The
@@ -67,15 +73,21 @@ struct JvHost {
     size_t before, proposed, verified, rejected, committed;
     double propose_ms, verify_ms;
   };
+  void HashAsync();
+  uint64_t ComputeHash() const;
+
   std::vector<Record> records_;
   std::vector<Metrics> metrics_;
   Metrics cur_{};
-  uint64_t snap_ = 0;
   uint64_t snap_generation_ = 0;
+  bool initial_pending_ = false;
   // every change of the snapshot bumps the generation
   uint64_t generation_ = 1;
   uint64_t prefix_hash_ = 0;
   mutable uint64_t hash_ = 0, hash_generation_ = 0;
+  // the hash being computed for snapshot generation pending_generation_
+  mutable std::future<uint64_t> pending_hash_;
+  mutable uint64_t pending_generation_ = 0;
 };
 
 }  // namespace ee

@@ -119,6 +119,7 @@ ee_status ee_jv_session_create(ee_ctx* handle, const ee_graph_view* g, ee_jv_ses
     auto* ctx = reinterpret_cast<ee::Context*>(handle);
     ctx->Activate();
     auto s = std::make_unique<ee::Session>(ctx);
+    s->host.async_hash = true;  // one graph: hash beside the device step
     s->host.Load(g);
     const ee::JvHost& h = s->host;
     const size_t n = static_cast<size_t>(h.n), m = static_cast<size_t>(h.m);
