@@ -497,6 +497,9 @@ def main() -> int:
             "propose_edges_per_s": best.edges_scanned / (best.propose_ms * 1e-3),
             "algorithm_edges_per_s": best.edges_scanned / (best.total_ms * 1e-3),
             "kernel_ms": best.device_ms, "total_ms": best.total_ms,
+            # where the rest of the run goes (host side of each epoch)
+            "h2d_ms": best.h2d_ms, "d2h_ms": best.d2h_ms, "propose_ms": best.propose_ms,
+            "verify_ms": best.verify_ms, "commit_ms": best.commit_ms,
             "final_hash": f"{best.final_hash:016x}",
         }
         if not args.no_cpu:
