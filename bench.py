@@ -332,6 +332,15 @@ def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005, jv_share:
     return out
 
 
+def pack_improving(costs: np.ndarray, tasks: np.ndarray, T: int, words: int) -> np.ndarray:
+    """Per row, bit j of word j / 32: cell (row, j) < the row's deleted_cost."""
+    improving = costs.reshape(len(tasks), T) < tasks["deleted_cost"][:, None]
+    out = np.zeros((len(tasks), words), np.uint32)
+    for j in range(T):
+        out[:, j // 32] |= improving[:, j].astype(np.uint32) << np.uint32(j % 32)
+    return out
+
+
 COPY_CELLS = 3348853  # int64 cells of the k-opt workload
 
 
@@ -429,6 +438,21 @@ def main() -> int:
     gpu_out = {k: batch.segment_costs(i) for i, k in enumerate((3, 4, 5))}
     algo_bytes = kopt_algorithmic_bytes(wl)
     achieved = algo_bytes / (device_ms * 1e-3) / 1e9
+    # consumer-side output: per row the mask of improving templates
+    # (cell < deleted_cost) instead of the cells, evaluated and downloaded
+    batch.set_output("masks")
+    batch.run(args.warmup)
+    mask_ms = batch.run(args.steps)
+    t_dl = time.perf_counter()
+    gpu_masks = batch.download_masks()
+    mask_dl_s = time.perf_counter() - t_dl
+    batch.set_output("costs")
+    masks = {
+        "words_per_row": int(gpu_masks.shape[1]), "kernel_ms": mask_ms,
+        "download_ms": mask_dl_s * 1e3, "bytes": int(gpu_masks.nbytes),
+        "improving_cells": int(np.unpackbits(gpu_masks.view(np.uint8)).sum()),
+        "cells_download_ms": None,
+    }
 
     log("plain per-k path (synchronous upload, launch, download per k)")
     plain_samples = []
@@ -469,12 +493,20 @@ def main() -> int:
         },
         "plain_path": {"cells_per_s": cells / plain_s, "ms": plain_s * 1e3,
                        "batched_speedup": value / (cells / plain_s)},
+        "improving_masks": masks,
     }
+    # download time of the full matrix for comparison (all three segments)
+    t_dl = time.perf_counter()
+    batch.download()
+    masks["cells_download_ms"] = (time.perf_counter() - t_dl) * 1e3
 
     if not args.no_cpu:
         log(f"CPU baseline ({args.cpu_seconds:.0f} s budget)")
         cpu, cpu_out = cpu_baseline_kopt(wl, args.cpu_seconds)
         result["cpu_baseline"] = cpu
+        want_masks = np.concatenate([pack_improving(cpu_out[k], wl.bucket(k), TEMPLATES[k], masks["words_per_row"])
+                                     for k in (3, 4, 5)])
+        masks["equal_cpu"] = bool(np.array_equal(gpu_masks, want_masks))
         result["parity"] = {"kopt_cells_equal_cpu": all(
             np.array_equal(gpu_out[k], cpu_out[k]) for k in (3, 4, 5))}
     else:

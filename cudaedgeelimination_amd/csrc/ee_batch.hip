@@ -34,6 +34,9 @@ struct Batch {
   std::vector<KoptSegment> segments;
   DeviceArray<ee_kopt_task> d_tasks;
   DeviceArray<int64_t> d_costs;
+  // improving-template masks instead of cells (ee_kopt_batch_set_output)
+  bool mask_output = false;
+  DeviceArray<uint32_t> d_masks;
   std::vector<KoptTile> tiles;
   DeviceArray<KoptTile> d_tiles;
   KoptLaunch ranges{};  // per-k tile ranges of the current tiles
@@ -91,6 +94,15 @@ struct Batch {
     DropGraph();
   }
 
+  // words per task row of the mask output: the widest present k
+  uint32_t MaskWords() const {
+    uint32_t w = 1;
+    for (int i = 0; i < 3; ++i) {
+      if (k_present[i]) w = std::max(w, (plans[i].templates + 31u) / 32u);
+    }
+    return w;
+  }
+
   KoptLaunch Args() {
     KoptLaunch a = ranges;
     a.xy = d_xy.get();
@@ -98,6 +110,10 @@ struct Batch {
     a.tasks = d_tasks.get();
     a.plans = d_plans.get();
     a.costs = d_costs.get();
+    if (mask_output) {
+      a.masks = d_masks.get();
+      a.mask_words = MaskWords();
+    }
     return a;
   }
 
@@ -133,7 +149,8 @@ struct Batch {
   }
 
   uint64_t ResidentBytes() const {
-    return d_xy.bytes() + d_tasks.bytes() + d_costs.bytes() + d_tiles.bytes() + d_plans.bytes();
+    return d_xy.bytes() + d_tasks.bytes() + d_costs.bytes() + d_masks.bytes() + d_tiles.bytes() +
+           d_plans.bytes();
   }
 };
 
@@ -404,11 +421,12 @@ ee_status ee_kopt_batch_set_work(ee_kopt_batch* batch, const ee_kopt_segment_des
     b.d_tasks.Allocate(std::max<uint64_t>(rows, 1), b.ctx->device);
     b.d_tasks.Upload(tasks, rows, b.ctx->stream);
     b.d_costs.Allocate(std::max<uint64_t>(cells, 1), b.ctx->device);
-    b.ctx->Sync();
     b.segments = std::move(segs);
     b.task_count = rows;
     b.cell_count = cells;
     for (int i = 0; i < 3; ++i) b.k_present[i] = present[i];
+    if (b.mask_output) b.d_masks.Allocate(std::max<uint64_t>(rows * b.MaskWords(), 1), b.ctx->device);
+    b.ctx->Sync();
     b.layout_dirty = true;
   });
 }
@@ -461,6 +479,42 @@ ee_status ee_kopt_batch_download(ee_kopt_batch* batch, uint64_t first_cell, uint
     if (cell_count > 0 && costs == nullptr) throw std::invalid_argument("costs is NULL");
     b.ctx->Activate();
     b.d_costs.DownloadAt(first_cell, costs, cell_count, b.ctx->stream);
+    b.ctx->Sync();
+  });
+}
+
+ee_status ee_kopt_batch_set_output(ee_kopt_batch* batch, uint32_t mode) {
+  return Guarded([&] {
+    ee::Batch& b = ee::AsBatch(batch);
+    if (mode > 1) throw std::invalid_argument("output mode must be 0 (costs) or 1 (improving masks)");
+    b.ctx->Activate();
+    b.mask_output = mode == 1;
+    if (b.mask_output) {
+      b.d_masks.Allocate(std::max<uint64_t>(b.task_count * b.MaskWords(), 1), b.ctx->device);
+    }
+    b.DropGraph();  // the launch arguments change
+  });
+}
+
+ee_status ee_kopt_batch_mask_words(const ee_kopt_batch* batch, uint32_t* words) {
+  return Guarded([&] {
+    if (batch == nullptr || words == nullptr) throw std::invalid_argument("NULL argument");
+    *words = reinterpret_cast<const ee::Batch*>(batch)->MaskWords();
+  });
+}
+
+ee_status ee_kopt_batch_download_masks(ee_kopt_batch* batch, uint64_t first_row, uint64_t row_count,
+                                       uint32_t* masks) {
+  return Guarded([&] {
+    ee::Batch& b = ee::AsBatch(batch);
+    if (!b.mask_output) throw std::logic_error("the batch writes cost cells, not masks");
+    if (first_row > b.task_count || row_count > b.task_count - first_row) {
+      throw std::invalid_argument("row range outside the batch");
+    }
+    if (row_count > 0 && masks == nullptr) throw std::invalid_argument("masks is NULL");
+    const uint64_t w = b.MaskWords();
+    b.ctx->Activate();
+    b.d_masks.DownloadAt(first_row * w, masks, row_count * w, b.ctx->stream);
     b.ctx->Sync();
   });
 }

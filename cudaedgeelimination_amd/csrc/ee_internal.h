@@ -87,7 +87,7 @@ struct KoptKLayout {
   uint32_t group, passes, pair_stride;
   uint32_t pair_step_r, pair_step_p;  // 64 = pair_step_r * P + pair_step_p
   uint32_t pq_off, tpl_off;
-  uint32_t xy_off, eq_off, row_off, pair_off;
+  uint32_t xy_off, eq_off, row_off, dcost_off, pair_off;
 };
 
 // A segment carried in the kernel arguments: for launches of at most
@@ -124,6 +124,12 @@ struct KoptLaunch {
   const ee_kopt_task* tasks;   // task pool
   const KoptPlan* plans;       // [3] for k = 3, 4, 5
   int64_t* costs;              // cost pool
+  // non-null: instead of the cost cells, write per task row (global row r of
+  // the task pool) the bit mask of its improving templates, bit j of word
+  // masks[r * mask_words + j / 32] set iff cell (r, j) < the row's
+  // deleted_cost (the cells TryReconnectFromCostRow rechecks)
+  uint32_t* masks;
+  uint32_t mask_words;
   KoptKLayout layout[3];
   uint32_t wave_base;   // LDS offset of the first wave's tile arrays
   uint32_t wave_bytes;  // LDS bytes per wave

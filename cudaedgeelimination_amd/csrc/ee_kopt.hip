@@ -83,7 +83,7 @@ __device__ __forceinline__ void StoreCell(int64_t* out, uint32_t at, uint64_t su
 
 // Phase A: the tile's port node ids (into the pair record area, which is
 // free until phase B) and their coordinates, two gathers in flight per lane.
-template <int K, bool kNear>
+template <int K, bool kNear, bool kMask>
 __device__ __forceinline__ void StagePorts(const KoptLaunch& a, const KoptKLayout& L,
                                            const KoptTile& d, char* wave, int lane) {
   constexpr uint32_t kPorts = 2 * K;
@@ -123,7 +123,14 @@ __device__ __forceinline__ void StagePorts(const KoptLaunch& a, const KoptKLayou
         } else {
           w_xy[s] = longlong2{cx[i], cy[i]};
         }
-        if (p == 0) w_row[r] = 0;
+        if (p == 0) {
+          w_row[r] = 0;
+          // mask output: the row's deleted_cost (bytes 40..47 of the task row)
+          if (kMask) {
+            reinterpret_cast<int64_t*>(wave + L.dcost_off)[r] =
+                reinterpret_cast<const int64_t*>(ports + r * 12)[5];
+          }
+        }
       }
     }
   }
@@ -199,14 +206,65 @@ __device__ __forceinline__ void PairRecords(const KoptKLayout& L, uint32_t rows,
 // distances are below 2^26, so the low halves sum to the cost without a carry
 // and the high halves to the counters and sentinels.  kG is G when it is a
 // compile-time constant, else 0.
-template <int K, bool kNear, bool kNonTemporal, uint32_t kG>
+//
+// Mask mode (m.words > 0): no cells are stored; a ballot per row gathers the
+// lanes whose cell is below the row's deleted cost, and the first lane of each
+// lane group writes that row's bits of its templates (m.tbase .. m.tbase +
+// templates of the pass) into the row's mask words.
+struct MaskOut {
+  uint32_t* masks;       // mask of tile row 0 (the tile's first task row)
+  uint32_t words;        // words per row; 0 = cost output
+  const int64_t* dcost;  // LDS: deleted cost of each tile row
+  uint32_t tbase;        // first template of the pass
+  uint32_t tcount;       // templates per lane group in this pass (<= 64)
+  uint32_t g;            // lane group (its rows are g, g + G, ...)
+  bool writer;           // first lane of its group
+  bool single_pass;      // T <= 64: one pass writes every word of a row
+};
+
+__device__ __forceinline__ void WriteMaskRow(const MaskOut& m, uint32_t row, bool improving) {
+  const uint64_t ballot = __ballot(improving);
+  if (!m.writer) return;
+  const uint64_t bits = (ballot >> (m.g * m.tcount)) & (m.tcount >= 64 ? ~0ull : ((1ull << m.tcount) - 1ull));
+  uint32_t* dst = m.masks + static_cast<uint64_t>(row) * m.words + m.tbase / 32;
+  if (m.single_pass) {
+    // all templates in one pass: this write covers every word of the row
+    for (uint32_t w = 0; w < m.words; ++w) {
+      dst[w] = w == 0 ? static_cast<uint32_t>(bits) : (w == 1 ? static_cast<uint32_t>(bits >> 32) : 0u);
+    }
+    return;
+  }
+  // passes of 64 templates: words 2p and 2p + 1 of the row
+  dst[0] = static_cast<uint32_t>(bits);
+  if (m.tbase / 32 + 1 < m.words) dst[1] = static_cast<uint32_t>(bits >> 32);
+}
+
+template <int K, bool kNear, bool kNonTemporal, uint32_t kG, bool kMask>
 __device__ __forceinline__ void CellSteps(const char* pairs, const uint32_t (&off)[K], int64_t* out,
                                           uint32_t at, uint32_t step_bytes, uint32_t g_rows, uint32_t n,
-                                          bool extra_row) {
+                                          bool extra_row, const MaskOut& m) {
   const uint32_t G = kG != 0 ? kG : g_rows;
   auto steps = [&](auto unroll, uint32_t i) {
     constexpr uint32_t U = decltype(unroll)::value;
-    if (kNear) {
+    if (kMask) {
+      uint64_t sum[U];
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) sum[u] = 0;
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        const uint64_t* rec = reinterpret_cast<const uint64_t*>(pairs + off[j] + i * G * 8);
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) sum[u] += rec[u * G];
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        const uint32_t row = m.g + (i + u) * G;
+        const uint32_t hi = static_cast<uint32_t>(sum[u] >> 32);
+        const bool bad = (hi & static_cast<uint32_t>(kBadMask >> 32)) != 0;
+        const int64_t value = bad ? kInvalid : static_cast<int64_t>(sum[u] & kCostMask);
+        WriteMaskRow(m, row, value < m.dcost[row]);
+      }
+    } else if (kNear) {
       uint32_t lo[U], hi[U];
 #pragma unroll
       for (uint32_t u = 0; u < U; ++u) lo[u] = hi[u] = 0;
@@ -256,7 +314,7 @@ __device__ __forceinline__ void CellSteps(const char* pairs, const uint32_t (&of
 }
 
 // Phases A2, B and C of one staged tile.
-template <int K, bool kNear, bool kNonTemporal>
+template <int K, bool kNear, bool kNonTemporal, bool kMask>
 __device__ void TileCells(const KoptLaunch& a, const KoptKLayout& L, const KoptTile& d,
                           const char* smem, char* wave, int lane) {
   constexpr uint32_t kPorts = 2 * K;
@@ -330,10 +388,21 @@ __device__ void TileCells(const KoptLaunch& a, const KoptKLayout& L, const KoptT
     // the standard templates give T = 4 / 25 / 208, so G = 16 / 2 / 1: with G
     // a constant the rows of a step are immediate offsets of one address
     constexpr uint32_t kStdG = K == 3 ? 16u : (K == 4 ? 2u : 1u);
-    if (G == kStdG) {
-      CellSteps<K, kNear, kNonTemporal, kStdG>(pairs, off, out, at, step_bytes, G, n, g < extra);
+    if (kMask) {
+      MaskOut m;
+      m.masks = a.masks + (d.task_index * a.mask_words);
+      m.words = a.mask_words;
+      m.dcost = reinterpret_cast<const int64_t*>(wave + L.dcost_off);
+      m.tbase = pass * kWave;
+      m.tcount = T <= static_cast<uint32_t>(kWave) ? T : (T - m.tbase < kWave ? T - m.tbase : kWave);
+      m.g = g;
+      m.writer = (T <= static_cast<uint32_t>(kWave) ? t : t - m.tbase) == 0;
+      m.single_pass = T <= static_cast<uint32_t>(kWave);
+      CellSteps<K, kNear, kNonTemporal, 0, true>(pairs, off, out, at, step_bytes, G, n, g < extra, m);
+    } else if (G == kStdG) {
+      CellSteps<K, kNear, kNonTemporal, kStdG, false>(pairs, off, out, at, step_bytes, G, n, g < extra, MaskOut{});
     } else {
-      CellSteps<K, kNear, kNonTemporal, 0>(pairs, off, out, at, step_bytes, G, n, g < extra);
+      CellSteps<K, kNear, kNonTemporal, 0, false>(pairs, off, out, at, step_bytes, G, n, g < extra, MaskOut{});
     }
   }
   WaveSync();
@@ -363,7 +432,7 @@ __device__ __forceinline__ KoptTile TileAt(const KoptLaunch& a, const KoptKLayou
 // The tiles of one k handed to this workgroup, one per wavefront at a time.
 // The plan is fetched into registers first, so its latency overlaps the
 // first tile's gathers; it reaches LDS before the only block barrier.
-template <int K, bool kNonTemporal>
+template <int K, bool kNonTemporal, bool kMask>
 __device__ void TileLoop(const KoptLaunch& a, char* smem) {
   constexpr int ki = K - 3;
   constexpr uint32_t kPlanRegs = 4;
@@ -391,9 +460,9 @@ __device__ void TileLoop(const KoptLaunch& a, char* smem) {
   if (tile < end) {
     d = TileAt<K>(a, L, tile);
     if (d.near != 0) {
-      StagePorts<K, true>(a, L, d, wave, lane);
+      StagePorts<K, true, kMask>(a, L, d, wave, lane);
     } else {
-      StagePorts<K, false>(a, L, d, wave, lane);
+      StagePorts<K, false, kMask>(a, L, d, wave, lane);
     }
   }
 
@@ -411,24 +480,24 @@ __device__ void TileLoop(const KoptLaunch& a, char* smem) {
   for (bool first = true; tile < end; tile += stride, first = false) {
     if (!first) d = TileAt<K>(a, L, tile);
     if (d.near != 0) {
-      if (!first) StagePorts<K, true>(a, L, d, wave, lane);
-      TileCells<K, true, kNonTemporal>(a, L, d, smem, wave, lane);
+      if (!first) StagePorts<K, true, kMask>(a, L, d, wave, lane);
+      TileCells<K, true, kNonTemporal, kMask>(a, L, d, smem, wave, lane);
     } else {
-      if (!first) StagePorts<K, false>(a, L, d, wave, lane);
-      TileCells<K, false, kNonTemporal>(a, L, d, smem, wave, lane);
+      if (!first) StagePorts<K, false, kMask>(a, L, d, wave, lane);
+      TileCells<K, false, kNonTemporal, kMask>(a, L, d, smem, wave, lane);
     }
   }
 }
 
-template <bool kNonTemporal>
+template <bool kNonTemporal, bool kMask>
 __global__ void __launch_bounds__(256) KoptKernel(KoptLaunch a) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   if (blockIdx.x < a.k_block_first[1]) {
-    TileLoop<3, kNonTemporal>(a, smem);
+    TileLoop<3, kNonTemporal, kMask>(a, smem);
   } else if (blockIdx.x < a.k_block_first[2]) {
-    TileLoop<4, kNonTemporal>(a, smem);
+    TileLoop<4, kNonTemporal, kMask>(a, smem);
   } else {
-    TileLoop<5, kNonTemporal>(a, smem);
+    TileLoop<5, kNonTemporal, kMask>(a, smem);
   }
 }
 
@@ -476,7 +545,8 @@ size_t KoptPrepareLayout(KoptLaunch* a, const KoptPlan plans[3], const bool pres
     L.xy_off = 0;
     L.eq_off = Align16(rows * ports * (near_only ? 8u : 16u));
     L.row_off = L.eq_off + Align16(rows * ports * 4);
-    L.pair_off = L.row_off + Align16(rows * 4);
+    L.dcost_off = L.row_off + Align16(rows * 4);
+    L.pair_off = L.dcost_off + Align16(rows * 8);
     // the port node ids of phase A share the pair record area
     const uint32_t pair_bytes = std::max(L.P * L.pair_stride * 8, rows * ports * 4);
     wave_bytes = std::max<uint32_t>(wave_bytes, L.pair_off + Align16(pair_bytes));
@@ -585,10 +655,13 @@ void LaunchKopt(const KoptLaunch& args, const KoptTuning& tuning, size_t shared_
     first += a.k_block_count[i];
   }
   const dim3 g(first), b(tuning.block);
-  if (tuning.non_temporal) {
-    hipLaunchKernelGGL(KoptKernel<true>, g, b, shared_bytes, stream, a);
+  if (a.masks != nullptr) {
+    if (a.mask_words == 0) throw std::invalid_argument("mask output needs mask_words > 0");
+    hipLaunchKernelGGL((KoptKernel<false, true>), g, b, shared_bytes, stream, a);
+  } else if (tuning.non_temporal) {
+    hipLaunchKernelGGL((KoptKernel<true, false>), g, b, shared_bytes, stream, a);
   } else {
-    hipLaunchKernelGGL(KoptKernel<false>, g, b, shared_bytes, stream, a);
+    hipLaunchKernelGGL((KoptKernel<false, false>), g, b, shared_bytes, stream, a);
   }
   CheckHip(hipGetLastError(), "k-opt kernel launch");
 }

@@ -104,6 +104,26 @@ class KoptBatch:
                                                      _lib.ptr(out) if count else None))
         return out
 
+    def set_output(self, mode: str) -> None:
+        """"costs" (int64 cells) or "masks" (per task row, the bit mask of the
+        templates whose cell is below the row's deleted_cost)."""
+        _lib.check(_lib.lib().ee_kopt_batch_set_output(self._h, {"costs": 0, "masks": 1}[mode]))
+
+    def mask_words(self) -> int:
+        w = C.c_uint32()
+        _lib.check(_lib.lib().ee_kopt_batch_mask_words(self._h, C.byref(w)))
+        return w.value
+
+    def download_masks(self, first_row: int = 0, count: int | None = None) -> np.ndarray:
+        """[count, mask_words] uint32; rows in segment order."""
+        rows = sum(c for _, _, c in getattr(self, "segments", []))
+        count = rows - first_row if count is None else count
+        w = self.mask_words()
+        out = np.empty((count, w), np.uint32)
+        _lib.check(_lib.lib().ee_kopt_batch_download_masks(self._h, first_row, count,
+                                                           _lib.ptr(out) if count else None))
+        return out
+
     def segment_costs(self, index: int) -> np.ndarray:
         _, k, count = self.segments[index]
         return self.download(int(self.cell_offsets[index]), count * TEMPLATES_PER_K[k])

@@ -227,6 +227,18 @@ ee_status ee_kopt_batch_run(ee_kopt_batch* batch, uint32_t launches, int use_gra
                             double* device_ms);
 ee_status ee_kopt_batch_download(ee_kopt_batch* batch, uint64_t first_cell, uint64_t cell_count,
                                  int64_t* costs);
+/* Output mode of later runs: 0 = the int64 cost cells (default); 1 = per
+ * task row (rows numbered in segment order) the bit mask of its improving
+ * templates, bit j of word j / 32 set iff cell (row, j) < the row's
+ * deleted_cost, i.e. exactly the templates TryReconnectFromCostRow rechecks
+ * (src/cpu/kopt_search.cpp:693-697), in template order.  No reference
+ * counterpart: it replaces the 8-byte-per-cell download by T / 8 bytes per
+ * row for callers that only consume improving cells. */
+ee_status ee_kopt_batch_set_output(ee_kopt_batch* batch, uint32_t mode);
+/* 32-bit words per row of the mask output (ceil(T / 32) of the widest k). */
+ee_status ee_kopt_batch_mask_words(const ee_kopt_batch* batch, uint32_t* words);
+ee_status ee_kopt_batch_download_masks(ee_kopt_batch* batch, uint64_t first_row, uint64_t row_count,
+                                       uint32_t* masks);
 /* Device bytes held by the batch. */
 uint64_t ee_kopt_batch_resident_bytes(const ee_kopt_batch* batch);
 

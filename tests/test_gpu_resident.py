@@ -206,3 +206,84 @@ def test_session_random_graphs(ctx, seed):
     run = s.run(100)
     assert O.canonical_proof(run.proof) == O.canonical_proof(O.jv_run(O.OGraph.from_snapshot(g)))
     s.close()
+
+
+def pack_masks(costs: np.ndarray, tasks: np.ndarray, T: int, words: int) -> np.ndarray:
+    """Bit j of word j / 32 of each row: cell (row, j) < the row's deleted_cost."""
+    improving = costs.reshape(len(tasks), T) < tasks["deleted_cost"][:, None]
+    out = np.zeros((len(tasks), words), np.uint32)
+    for j in range(T):
+        out[:, j // 32] |= improving[:, j].astype(np.uint32) << np.uint32(j % 32)
+    return out
+
+
+@pytest.mark.parametrize("rows", [(32, 16, 8), (48, 24, 6), (64, 64, 64), (1, 1, 1)])
+def test_batch_improving_masks_match_oracle(ctx, config3_small, rows):
+    """Mask output: exactly the templates TryReconnectFromCostRow rechecks
+    (cell < deleted_cost), per row, for every tile shape; switching back to
+    cost output restores the cells."""
+    say(f"batch improving masks rows={rows}")
+    wl = config3_small
+    g = wl.graph
+    b = KoptBatch(ctx)
+    inst = b.add_instance(g)
+    segs = [(inst, k, len(wl.bucket(k))) for k in (3, 4, 5)]
+    tasks = np.concatenate([wl.bucket(k) for k in (3, 4, 5)])
+    b.set_work(segs, tasks)
+    b.set_tuning(list(rows), 0, 64, 0)
+    b.set_output("masks")
+    assert b.mask_words() == 7
+    b.run(1)
+    got = b.download_masks()
+    row = 0
+    total_bits = 0
+    for k in (3, 4, 5):
+        t = wl.bucket(k)
+        want_costs, _, _ = O.kopt_costs(g.x, g.y, g.distance_type, k, t)
+        want = pack_masks(want_costs, t, T_OF[k], 7)
+        assert np.array_equal(got[row:row + len(t)], want), k
+        total_bits += int(np.unpackbits(want.view(np.uint8)).sum())
+        row += len(t)
+    assert total_bits > 0  # the workload has improving cells
+    b.run(2, use_graph=True)
+    assert np.array_equal(b.download_masks(), got)
+    b.set_output("costs")
+    b.run(1)
+    for i, k in enumerate((3, 4, 5)):
+        assert sha(b.segment_costs(i)) == G["config3_small"][f"k{k}"]["sha256"]
+    b.close()
+
+
+def test_batch_masks_single_k_and_many_instances(ctx):
+    """One k only (one mask word per row) and many instances with mixed k."""
+    say("batch masks single k / many instances")
+    g, wl = W.config4_instance(5, rows=300)
+    b = KoptBatch(ctx)
+    inst = b.add_instance(g)
+    t3 = wl.bucket(3)
+    b.set_work([(inst, 3, len(t3))], t3)
+    b.set_output("masks")
+    assert b.mask_words() == 1
+    b.run(1)
+    want_costs, _, _ = O.kopt_costs(g.x, g.y, g.distance_type, 3, t3)
+    assert np.array_equal(b.download_masks(), pack_masks(want_costs, t3, 4, 1))
+    b.close()
+    graphs = [W.config4_instance(i, rows=200) for i in range(4)]
+    b = KoptBatch(ctx)
+    ids = [b.add_instance(gg) for gg, _ in graphs]
+    segs, tasks = [], []
+    for i, (gg, w) in enumerate(graphs):
+        for k in (5, 4, 3):
+            segs.append((ids[i], k, len(w.bucket(k))))
+            tasks.append(w.bucket(k))
+    b.set_work(segs, np.concatenate(tasks))
+    b.set_output("masks")
+    b.run(1)
+    got = b.download_masks()
+    row = 0
+    for (inst, k, count), t in zip(segs, tasks):
+        gg = graphs[inst][0]
+        want_costs, _, _ = O.kopt_costs(gg.x, gg.y, gg.distance_type, k, t)
+        assert np.array_equal(got[row:row + count], pack_masks(want_costs, t, T_OF[k], 7)), (inst, k)
+        row += count
+    b.close()
