@@ -51,6 +51,19 @@ def kopt_algorithmic_bytes(wl) -> int:
     return 48 * rows + 8 * cells + 16 * wl.graph.dimension + 15 * sum(TEMPLATES.values())
 
 
+# distinct port pairs the canonical templates use per row (one exact distance
+# each), against k distances per cell in a per-cell evaluation
+PAIRS = {3: 9, 4: 20, 5: 35}
+
+
+def kopt_distance_evals(wl) -> tuple[int, int]:
+    """(distances the kernel evaluates per launch, distances a per-cell
+    evaluation would: k per cell)."""
+    rows = {k: int((wl.ks == k).sum()) for k in (3, 4, 5)}
+    return (sum(rows[k] * PAIRS[k] for k in rows),
+            sum(rows[k] * TEMPLATES[k] * k for k in rows))
+
+
 # The process's CPU set at start: the reference protocol's OMP_PROC_BIND makes
 # the OpenMP runtime pin the thread that first runs a parallel region, and
 # threads started later (the serving loop's) would inherit that one core.
@@ -438,6 +451,7 @@ def main() -> int:
     gpu_out = {k: batch.segment_costs(i) for i, k in enumerate((3, 4, 5))}
     algo_bytes = kopt_algorithmic_bytes(wl)
     achieved = algo_bytes / (device_ms * 1e-3) / 1e9
+    dist_evals, per_cell_evals = kopt_distance_evals(wl)
     # consumer-side output: per row the mask of improving templates
     # (cell < deleted_cost) instead of the cells, evaluated and downloaded
     batch.set_output("masks")
@@ -490,6 +504,10 @@ def main() -> int:
             "valu_utilisation": profile_figure("kopt_kernel_config3_valu.md", "valu_utilisation"),
             "kernel": "KoptKernel", "algorithmic_bytes_per_launch": algo_bytes,
             "kernel_ms": device_ms, "graph_replay_kernel_ms": graph_ms,
+            # exact distances per second: evaluated once per (row, port pair),
+            # and as a per-cell evaluation (k per cell) would count them
+            "distance_evals_per_s": dist_evals / (device_ms * 1e-3),
+            "per_cell_equivalent_distance_evals_per_s": per_cell_evals / (device_ms * 1e-3),
         },
         "plain_path": {"cells_per_s": cells / plain_s, "ms": plain_s * 1e3,
                        "batched_speedup": value / (cells / plain_s)},
