@@ -148,6 +148,12 @@ class RequestResult(C.Structure):
                 ("latency_ms", C.c_double)]
 
 
+class ServerRequest(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("graph", C.c_uint32), ("k", C.c_uint32),
+                ("reserved", C.c_uint32), ("input", C.c_void_p), ("count", C.c_uint64),
+                ("output", C.c_void_p)]
+
+
 class ServerStats(C.Structure):
     _fields_ = [("requests", C.c_uint64), ("batches", C.c_uint64), ("cells", C.c_uint64),
                 ("jv_edges", C.c_uint64), ("peak_inflight_bytes", C.c_uint64),
@@ -199,6 +205,8 @@ PROTOTYPES = {
     "ee_server_add_jv_graph": (C.c_int, [P, C.POINTER(GraphView), C.POINTER(U32)]),
     "ee_server_submit_jv": (C.c_int, [P, U32, P, U64, P, C.POINTER(U64)]),
     "ee_server_wait": (C.c_int, [P, U64, DBL, C.POINTER(RequestResult)]),
+    "ee_server_submit_many": (C.c_int, [P, U32, P, P]),
+    "ee_server_wait_many": (C.c_int, [P, U32, P, DBL, P]),
     "ee_server_stats_get": (C.c_int, [P, C.POINTER(ServerStats)]),
     "ee_jv_find_candidates": (C.c_int, [P, C.POINTER(GraphView), P, C.POINTER(JvUsage)]),
     "ee_jv_clear_cache": (C.c_int, [P]),

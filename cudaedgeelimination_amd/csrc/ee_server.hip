@@ -50,7 +50,9 @@ struct Request {
   uint64_t id = 0;
   bool jv = false;  // a JV recheck rather than a leaf-cost query
   uint32_t graph = 0, k = 0;
-  std::vector<ee_kopt_task> tasks;
+  std::vector<ee_kopt_task> tasks;  // owned copy (ee_server_submit)
+  const ee_kopt_task* task_data = nullptr;  // tasks.data() or the caller's rows
+  uint64_t task_rows = 0;
   int64_t* out = nullptr;
   std::vector<int32_t> edges;  // JV: union edge slots
   int32_t* witnesses = nullptr;
@@ -216,9 +218,9 @@ bool Server::Issue(Slot& slot, std::vector<std::shared_ptr<Request>> batch) {
       // requests start on 128-byte boundaries (whole-line stores stay aligned)
       cells = (cells + kSegmentAlignCells - 1) / kSegmentAlignCells * kSegmentAlignCells;
       slot.offsets.push_back(cells);
-      rows += r->tasks.size();
+      rows += r->task_rows;
       cells += r->cells;
-      tiles += (r->tasks.size() + tuning.rows_per_tile[r->k - 3] - 1) / tuning.rows_per_tile[r->k - 3];
+      tiles += (r->task_rows + tuning.rows_per_tile[r->k - 3] - 1) / tuning.rows_per_tile[r->k - 3];
       present[r->k - 3] = true;
       ++kopt_requests;
     }
@@ -236,9 +238,9 @@ bool Server::Issue(Slot& slot, std::vector<std::shared_ptr<Request>> batch) {
       for (size_t i = 0; i < batch.size(); ++i) {
         const Request& r = *batch[i];
         if (r.jv) continue;
-        std::memcpy(slot.h_tasks.get() + row, r.tasks.data(), r.tasks.size() * sizeof(ee_kopt_task));
-        slot.segments[seg++] = KoptSegment{r.graph, r.k, row, r.tasks.size(), slot.offsets[i], 0, 0, 0};
-        row += r.tasks.size();
+        std::memcpy(slot.h_tasks.get() + row, r.task_data, r.task_rows * sizeof(ee_kopt_task));
+        slot.segments[seg++] = KoptSegment{r.graph, r.k, row, r.task_rows, slot.offsets[i], 0, 0, 0};
+        row += r.task_rows;
       }
       KoptLaunch a{};
       KoptAssignTiles(slot.segments.data(), kopt_requests, worker_instances.data(), plans.data(), tuning,
@@ -470,6 +472,91 @@ template <typename F> ee_status Guarded(F&& body) {
 
 using ee::Guarded;
 
+namespace ee {
+namespace {
+
+// A validated leaf-cost request; `borrow` keeps the caller's rows instead of
+// copying them (they must then stay valid until the request completes).
+std::shared_ptr<Request> MakeKoptRequest(Server& s, uint32_t graph, uint32_t k, const ee_kopt_task* tasks,
+                                         uint64_t count, int64_t* costs, bool borrow) {
+  if (k < 3 || k > 5) throw std::invalid_argument("request k must be in [3,5]");
+  if (count == 0 || tasks == nullptr || costs == nullptr) {
+    throw std::invalid_argument("request needs at least one task and an output buffer");
+  }
+  auto r = std::make_shared<Request>();
+  r->graph = graph;
+  r->k = k;
+  r->cells = count * kTemplates[k - 3];
+  r->bytes = r->cells * sizeof(int64_t) + count * sizeof(ee_kopt_task);
+  r->out = costs;
+  if (r->bytes > s.SlotBudget()) throw std::invalid_argument("request exceeds the HBM budget");
+  uint32_t n = 0;
+  {
+    std::lock_guard<std::mutex> lk(s.mu);
+    if (graph >= s.instances.size()) throw std::invalid_argument("unknown graph id");
+    n = s.instances[graph].n;
+  }
+  ValidateTasks(static_cast<int32_t>(n), k, tasks, count);
+  if (borrow) {
+    r->task_data = tasks;
+  } else {
+    r->tasks.assign(tasks, tasks + count);
+    r->task_data = r->tasks.data();
+  }
+  r->task_rows = count;
+  return r;
+}
+
+// A validated JV recheck of edge slots of a resident JV graph.
+std::shared_ptr<Request> MakeJvRequest(Server& s, uint32_t jv_graph, const int32_t* edges, uint64_t count,
+                                       int32_t* witnesses) {
+  if (count == 0 || edges == nullptr || witnesses == nullptr) {
+    throw std::invalid_argument("a JV recheck needs at least one edge slot and an output buffer");
+  }
+  if (count >= (uint64_t{1} << 31)) throw std::invalid_argument("JV recheck too large");
+  auto r = std::make_shared<Request>();
+  r->jv = true;
+  r->graph = jv_graph;
+  r->cells = count;  // counts against the batch limit like cells
+  r->bytes = count * 2 * sizeof(int32_t);
+  r->witnesses = witnesses;
+  if (r->bytes > s.SlotBudget()) throw std::invalid_argument("request exceeds the HBM budget");
+  int32_t eb = 0, m = 0;
+  {
+    std::lock_guard<std::mutex> lk(s.mu);
+    if (jv_graph >= s.jv.node_base.size()) throw std::invalid_argument("unknown JV graph id");
+    eb = s.jv.edge_base[jv_graph];
+    m = s.jv.edge_count[jv_graph];
+    r->node_base = s.jv.node_base[jv_graph];
+  }
+  r->edges.resize(count);
+  for (uint64_t i = 0; i < count; ++i) {
+    if (edges[i] < 0 || edges[i] >= m) throw std::invalid_argument("JV recheck edge slot out of range");
+    r->edges[i] = eb + edges[i];
+  }
+  return r;
+}
+
+// Queues validated requests in order under one lock and wakes the worker.
+void Enqueue(Server& s, std::vector<std::shared_ptr<Request>>& rs, uint64_t* ids) {
+  {
+    std::lock_guard<std::mutex> lk(s.mu);
+    if (s.stopping) throw std::runtime_error("server is stopping");
+    const Clock::time_point now = Clock::now();
+    for (size_t i = 0; i < rs.size(); ++i) {
+      rs[i]->id = s.next_id++;
+      rs[i]->submit = now;
+      s.live[rs[i]->id] = rs[i];
+      s.pending.push_back(rs[i]);
+      if (ids) ids[i] = rs[i]->id;
+    }
+  }
+  s.cv_work.notify_one();
+}
+
+}  // namespace
+}  // namespace ee
+
 extern "C" {
 
 ee_status ee_server_create(ee_ctx* handle, const ee_server_config* config, ee_server** out) {
@@ -572,40 +659,37 @@ ee_status ee_server_add_graph(ee_server* server, const int64_t* x, const int64_t
   });
 }
 
+
 ee_status ee_server_submit(ee_server* server, uint32_t graph, uint32_t k,
                            const ee_kopt_task* tasks, uint64_t count, int64_t* costs,
                            uint64_t* request_id) {
   return Guarded([&] {
     ee::Server& s = ee::AsServer(server);
-    if (k < 3 || k > 5) throw std::invalid_argument("request k must be in [3,5]");
-    if (count == 0 || tasks == nullptr || costs == nullptr) {
-      throw std::invalid_argument("request needs at least one task and an output buffer");
-    }
-    auto r = std::make_shared<ee::Request>();
-    r->graph = graph;
-    r->k = k;
-    r->cells = count * ee::kTemplates[k - 3];
-    r->bytes = r->cells * sizeof(int64_t) + count * sizeof(ee_kopt_task);
-    r->out = costs;
-    if (r->bytes > s.SlotBudget()) throw std::invalid_argument("request exceeds the HBM budget");
-    uint32_t n = 0;
-    {
-      std::lock_guard<std::mutex> lk(s.mu);
-      if (graph >= s.instances.size()) throw std::invalid_argument("unknown graph id");
-      n = s.instances[graph].n;
-    }
-    ee::ValidateTasks(static_cast<int32_t>(n), k, tasks, count);
-    r->tasks.assign(tasks, tasks + count);
-    {
-      std::lock_guard<std::mutex> lk(s.mu);
-      if (s.stopping) throw std::runtime_error("server is stopping");
-      r->id = s.next_id++;
-      r->submit = ee::Clock::now();
-      s.live[r->id] = r;
-      s.pending.push_back(r);
-    }
-    s.cv_work.notify_one();
-    if (request_id) *request_id = r->id;
+    std::vector<std::shared_ptr<ee::Request>> rs{ee::MakeKoptRequest(s, graph, k, tasks, count, costs, false)};
+    ee::Enqueue(s, rs, request_id);
+  });
+}
+
+ee_status ee_server_submit_many(ee_server* server, uint32_t count, const ee_server_request* requests,
+                                uint64_t* request_ids) {
+  return Guarded([&] {
+    ee::Server& s = ee::AsServer(server);
+    if (count > 0 && (requests == nullptr || request_ids == nullptr)) throw std::invalid_argument("NULL argument");
+    std::vector<std::shared_ptr<ee::Request>> rs(count);
+    // validation is independent per request: spread it over the host cores
+    ee::ParallelFor(count, [&](size_t i) {
+      const ee_server_request& q = requests[i];
+      if (q.kind == 0) {
+        rs[i] = ee::MakeKoptRequest(s, q.graph, q.k, static_cast<const ee_kopt_task*>(q.input), q.count,
+                                    static_cast<int64_t*>(q.output), true);
+      } else if (q.kind == 1) {
+        rs[i] = ee::MakeJvRequest(s, q.graph, static_cast<const int32_t*>(q.input), q.count,
+                                  static_cast<int32_t*>(q.output));
+      } else {
+        throw std::invalid_argument("request kind must be 0 (leaf costs) or 1 (JV recheck)");
+      }
+    });
+    ee::Enqueue(s, rs, request_ids);
   });
 }
 
@@ -650,35 +734,8 @@ ee_status ee_server_submit_jv(ee_server* server, uint32_t jv_graph, const int32_
                               uint64_t count, int32_t* witnesses, uint64_t* request_id) {
   return Guarded([&] {
     ee::Server& s = ee::AsServer(server);
-    if (count == 0 || edges == nullptr || witnesses == nullptr) {
-      throw std::invalid_argument("a JV recheck needs at least one edge slot and an output buffer");
-    }
-    if (count >= (uint64_t{1} << 31)) throw std::invalid_argument("JV recheck too large");
-    auto r = std::make_shared<ee::Request>();
-    r->jv = true;
-    r->graph = jv_graph;
-    r->cells = count;  // counts against the batch limit like cells
-    r->bytes = count * 2 * sizeof(int32_t);
-    r->witnesses = witnesses;
-    if (r->bytes > s.SlotBudget()) throw std::invalid_argument("request exceeds the HBM budget");
-    {
-      std::lock_guard<std::mutex> lk(s.mu);
-      if (jv_graph >= s.jv.node_base.size()) throw std::invalid_argument("unknown JV graph id");
-      const int32_t eb = s.jv.edge_base[jv_graph], m = s.jv.edge_count[jv_graph];
-      r->node_base = s.jv.node_base[jv_graph];
-      r->edges.resize(count);
-      for (uint64_t i = 0; i < count; ++i) {
-        if (edges[i] < 0 || edges[i] >= m) throw std::invalid_argument("JV recheck edge slot out of range");
-        r->edges[i] = eb + edges[i];
-      }
-      if (s.stopping) throw std::runtime_error("server is stopping");
-      r->id = s.next_id++;
-      r->submit = ee::Clock::now();
-      s.live[r->id] = r;
-      s.pending.push_back(r);
-    }
-    s.cv_work.notify_one();
-    if (request_id) *request_id = r->id;
+    std::vector<std::shared_ptr<ee::Request>> rs{ee::MakeJvRequest(s, jv_graph, edges, count, witnesses)};
+    ee::Enqueue(s, rs, request_id);
   });
 }
 
@@ -707,6 +764,25 @@ ee_status ee_server_wait(ee_server* server, uint64_t request_id, double timeout_
       result->latency_ms = ee::Ms(r->submit, r->done);
     }
     if (r->status != EE_OK) throw std::runtime_error(r->error);
+  });
+}
+
+ee_status ee_server_wait_many(ee_server* server, uint32_t count, const uint64_t* request_ids,
+                              double timeout_ms, ee_request_result* results) {
+  return Guarded([&] {
+    if (count > 0 && request_ids == nullptr) throw std::invalid_argument("request_ids is NULL");
+    const auto deadline = std::chrono::steady_clock::now() +
+                          std::chrono::duration<double, std::milli>(timeout_ms < 0 ? 0.0 : timeout_ms);
+    std::string first_error;
+    for (uint32_t i = 0; i < count; ++i) {
+      double left = -1.0;
+      if (timeout_ms >= 0) {
+        left = std::max(0.0, std::chrono::duration<double, std::milli>(deadline - std::chrono::steady_clock::now()).count());
+      }
+      const ee_status st = ee_server_wait(server, request_ids[i], left, results ? &results[i] : nullptr);
+      if (st != EE_OK && first_error.empty()) first_error = ee::LastError();
+    }
+    if (!first_error.empty()) throw std::runtime_error(first_error);
   });
 }
 

@@ -321,6 +321,39 @@ class KoptServer:
                      "queue_ms": res.queue_ms, "service_ms": res.service_ms,
                      "latency_ms": res.latency_ms}
 
+    def submit_many(self, requests) -> list[int]:
+        """Queues many requests at once: each is (graph, k, task rows) for a
+        leaf-cost query or (jv_graph, 0, int32 edge slots) for a JV recheck.
+        The inputs are not copied; this object keeps them (and the outputs)
+        alive until wait_many() collects the requests."""
+        descs = (_lib.ServerRequest * max(len(requests), 1))()
+        keep = []
+        for i, (graph, k, data) in enumerate(requests):
+            if k:
+                data = np.ascontiguousarray(data, dtype=_lib.KOPT_TASK_DTYPE)
+                out = np.empty(len(data) * TEMPLATES_PER_K[k], np.int64)
+            else:
+                data = np.ascontiguousarray(data, dtype=np.int32)
+                out = np.empty(len(data), np.int32)
+            descs[i].kind, descs[i].graph, descs[i].k = (0 if k else 1), graph, k
+            descs[i].input, descs[i].count, descs[i].output = _lib.ptr(data), len(data), _lib.ptr(out)
+            keep.append((data, out))
+        ids = np.zeros(max(len(requests), 1), np.uint64)
+        _lib.check(_lib.lib().ee_server_submit_many(self._h, len(requests), descs, _lib.ptr(ids)))
+        ids = [int(v) for v in ids[:len(requests)]]
+        for rid, bufs in zip(ids, keep):
+            self._buffers[rid] = bufs
+        return ids
+
+    def wait_many(self, request_ids, timeout_ms: float = -1.0):
+        """[(results, latency_ms)] of the requests, in the given order."""
+        ids = np.ascontiguousarray(request_ids, dtype=np.uint64)
+        res = (_lib.RequestResult * max(len(ids), 1))()
+        status = _lib.lib().ee_server_wait_many(self._h, len(ids), _lib.ptr(ids), timeout_ms, res)
+        outs = [self._buffers.pop(int(rid), (None, None))[1] for rid in ids]
+        _lib.check(status)
+        return [(outs[i], res[i].latency_ms) for i in range(len(ids))]
+
     def stats(self) -> dict:
         st = _lib.ServerStats()
         _lib.check(_lib.lib().ee_server_stats_get(self._h, C.byref(st)))

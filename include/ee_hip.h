@@ -384,6 +384,30 @@ ee_status ee_server_submit_jv(ee_server* server, uint32_t jv_graph, const int32_
 /* Blocks until the request completes (timeout_ms < 0: no limit). */
 ee_status ee_server_wait(ee_server* server, uint64_t request_id, double timeout_ms,
                          ee_request_result* result);
+
+/* One request of ee_server_submit_many. */
+typedef struct {
+  uint32_t kind;     /* 0: leaf-cost query; 1: JV recheck */
+  uint32_t graph;    /* graph id (kind 0) or JV graph id (kind 1) */
+  uint32_t k;        /* 3, 4 or 5 (kind 0) */
+  uint32_t reserved;
+  const void* input; /* ee_kopt_task rows (kind 0) or int32 edge slots (kind 1) */
+  uint64_t count;    /* rows or edge slots */
+  void* output;      /* count x T(k) int64 cells (kind 0) or count int32 witnesses (kind 1) */
+} ee_server_request;
+/* Queues `count` requests at once, in order, as a front end receiving many
+ * queries in one message would: every request is validated first (in
+ * parallel; on an error none is queued), then all are queued under one lock.
+ * Unlike ee_server_submit the task rows are not copied: inputs and outputs
+ * must stay valid until each request completes.  request_ids[i] receives the
+ * id of requests[i]. */
+ee_status ee_server_submit_many(ee_server* server, uint32_t count, const ee_server_request* requests,
+                                uint64_t* request_ids);
+/* ee_server_wait for each id in turn within one overall timeout; results
+ * (nullable) has count entries.  Returns the first failure after waiting for
+ * every request. */
+ee_status ee_server_wait_many(ee_server* server, uint32_t count, const uint64_t* request_ids,
+                              double timeout_ms, ee_request_result* results);
 ee_status ee_server_stats_get(const ee_server* server, ee_server_stats* stats);
 
 #ifdef __cplusplus

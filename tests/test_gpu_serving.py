@@ -176,3 +176,41 @@ def test_jv_rechecks_share_launches_with_leaf_costs(ctx, graphs_and_rows, batchi
     with pytest.raises(_lib.InvalidArgument):
         server.submit_jv(len(jv_graphs) + 3, np.array([0], np.int32))
     server.close()
+
+
+@pytest.mark.parametrize("batching", [True, False])
+def test_submit_many_equals_oracle(ctx, graphs_and_rows, batching):
+    """Many leaf-cost queries and JV rechecks queued in one call (inputs
+    borrowed, not copied) give the oracle's cells and witnesses; a bad
+    request anywhere in the list queues none of them."""
+    say(f"serving submit_many batching={batching}")
+    graphs, rows = graphs_and_rows
+    server = KoptServer(ctx, batching=batching, max_batch_cells=50000)
+    kids = [server.add_graph(g) for g in graphs]
+    jids = [server.add_jv_graph(g) for g in graphs]
+    want_w = [O.OGraph.from_snapshot(g).witnesses() for g in graphs]
+    rng = np.random.default_rng(11)
+    reqs, meta = [], []
+    for i, (g, k, t) in enumerate(make_requests(rows, 200, seed=9)):
+        if i % 4 == 3:
+            m = graphs[g].edge_count
+            edges = rng.choice(m, size=int(rng.integers(1, min(m, 400))), replace=False).astype(np.int32)
+            reqs.append((jids[g], 0, edges))
+            meta.append(("jv", g, edges))
+        else:
+            reqs.append((kids[g], k, t))
+            meta.append(("kopt", g, (k, t)))
+    ids = server.submit_many(reqs)
+    assert len(ids) == len(reqs) and len(set(ids)) == len(ids)
+    for (out, latency), (kind, g, payload) in zip(server.wait_many(ids, 60000), meta):
+        assert latency >= 0
+        if kind == "jv":
+            assert np.array_equal(out, want_w[g][payload])
+        else:
+            assert np.array_equal(out, expected(graphs, g, *payload))
+    before = server.stats()["requests"]
+    bad = [(kids[0], 3, rows[0][3][:5]), (99, 3, rows[0][3][:5])]  # graph 99 does not exist
+    with pytest.raises(_lib.InvalidArgument):
+        server.submit_many(bad)
+    assert server.stats()["requests"] == before
+    server.close()
