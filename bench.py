@@ -423,8 +423,11 @@ def main() -> int:
         os.environ.setdefault(key, value)
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    # a step is one pass over the batch (about 13 us of GPU time): enough of
+    # them that the timed call's fixed cost (event wait, host call) stays
+    # below one percent of the timed region
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-jv", action="store_true", help="skip the JV workload")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
