@@ -50,6 +50,13 @@ struct Batch {
   // graph replay of one launch
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // the finalized launch arguments and their device copy, which the kernel
+  // reads instead of the per-launch argument segment
+  KoptLaunch host_args{};
+  DeviceArray<KoptLaunch> d_args;
+  bool args_valid = false;
+  uint32_t grid = 0;
+  size_t shared = 0;
 
   void DropGraph() {
     if (exec != nullptr) static_cast<void>(hipGraphExecDestroy(exec));
@@ -117,16 +124,34 @@ struct Batch {
     return a;
   }
 
+  // Brings the device copy of the launch arguments up to date; a change
+  // (new work, tuning or output mode) drops the captured graph, whose launch
+  // shape belongs to the old arguments.
+  void SyncArgs() {
+    KoptLaunch a;
+    std::memset(&a, 0, sizeof(a));  // padding included, for the comparison below
+    a = Args();
+    shared = KoptPrepareLayout(&a, plans.data(), k_present, tuning, NearOnly());
+    grid = KoptFinalize(&a, tuning, shared);
+    if (args_valid && std::memcmp(&a, &host_args, sizeof(a)) == 0) return;
+    DropGraph();
+    if (d_args.count() == 0) d_args.Allocate(1, ctx->device);
+    host_args = a;
+    d_args.Upload(&host_args, 1, ctx->stream);
+    ctx->Sync();
+    args_valid = true;
+  }
+
   void LaunchOnce() {
-    KoptLaunch a = Args();
-    const size_t shared = KoptPrepareLayout(&a, plans.data(), k_present, tuning, NearOnly());
-    LaunchKopt(a, tuning, shared, ctx->stream);
+    SyncArgs();
+    LaunchKoptResident(host_args, d_args.get(), grid, tuning, shared, ctx->stream);
   }
 
   // Mean device time of one launch over `launches` back-to-back launches.
   double Time(uint32_t launches, bool use_graph) {
     Prepare();
     if (tile_count == 0 || launches == 0) return 0.0;
+    SyncArgs();  // any upload happens here, never inside the capture
     if (use_graph && exec == nullptr) {
       CheckHip(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
       LaunchOnce();

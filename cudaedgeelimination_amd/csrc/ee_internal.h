@@ -168,6 +168,14 @@ size_t KoptPrepareLayout(KoptLaunch* a, const KoptPlan plans[3], const bool pres
                          const KoptTuning& tuning, bool near_only);
 void LaunchKopt(const KoptLaunch& args, const KoptTuning& tuning, size_t shared_bytes,
                 hipStream_t stream);
+// Fills the workgroup ranges of every k (k_block_first, k_block_count) and
+// returns the grid; 0 for an empty launch.
+uint32_t KoptFinalize(KoptLaunch* args, const KoptTuning& tuning, size_t shared_bytes);
+// Launches with the arguments read from `device_args`, a device copy of
+// `finalized` (a KoptFinalize result) that stays unchanged until the launch
+// has run.
+void LaunchKoptResident(const KoptLaunch& finalized, const KoptLaunch* device_args, uint32_t grid,
+                        const KoptTuning& tuning, size_t shared_bytes, hipStream_t stream);
 
 // ------------------------------------------------------------------- JV
 // Edge slots edge_list[0..edge_count) (or 0..edge_count when edge_list is

@@ -46,6 +46,16 @@ constexpr uint64_t kCostMask = (uint64_t{1} << kCounterBase) - 1;
 constexpr uint64_t kSentinel = uint64_t{1} << 61;
 constexpr uint64_t kBadMask = (uint64_t{0xAAAAA} << kCounterBase) | (uint64_t{7} << 61);
 
+// The launch's buffers, taken from the kernel arguments.
+struct KoptPointers {
+  const longlong2* xy;
+  const KoptTile* tiles;
+  const ee_kopt_task* tasks;
+  const KoptPlan* plans;
+  int64_t* costs;
+  uint32_t* masks;
+};
+
 __device__ __forceinline__ void WaveSync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -84,7 +94,7 @@ __device__ __forceinline__ void StoreCell(int64_t* out, uint32_t at, uint64_t su
 // Phase A: the tile's port node ids (into the pair record area, which is
 // free until phase B) and their coordinates, two gathers in flight per lane.
 template <int K, bool kNear, bool kMask>
-__device__ __forceinline__ void StagePorts(const KoptLaunch& a, const KoptKLayout& L,
+__device__ __forceinline__ void StagePorts(const KoptLaunch& a, const KoptPointers& bufs, const KoptKLayout& L,
                                            const KoptTile& d, char* wave, int lane) {
   constexpr uint32_t kPorts = 2 * K;
   constexpr uint32_t kBatch = 2;
@@ -93,8 +103,8 @@ __device__ __forceinline__ void StagePorts(const KoptLaunch& a, const KoptKLayou
   uint32_t* w_row = reinterpret_cast<uint32_t*>(wave + L.row_off);
   int32_t* w_node = reinterpret_cast<int32_t*>(wave + L.pair_off);
   const uint32_t total = d.rows * kPorts;
-  const longlong2* xy = a.xy + d.coord_base;
-  const int32_t* ports = reinterpret_cast<const int32_t*>(a.tasks + d.task_index);
+  const longlong2* xy = bufs.xy + d.coord_base;
+  const int32_t* ports = reinterpret_cast<const int32_t*>(bufs.tasks + d.task_index);
   for (uint32_t base = 0; base < total; base += kBatch * kWave) {
     // lanes past the end re-read port 0 of the tile, so every load is in range
     int32_t node[kBatch];
@@ -315,7 +325,7 @@ __device__ __forceinline__ void CellSteps(const char* pairs, const uint32_t (&of
 
 // Phases A2, B and C of one staged tile.
 template <int K, bool kNear, bool kNonTemporal, bool kMask>
-__device__ void TileCells(const KoptLaunch& a, const KoptKLayout& L, const KoptTile& d,
+__device__ void TileCells(const KoptLaunch& a, const KoptPointers& bufs, const KoptKLayout& L, const KoptTile& d,
                           const char* smem, char* wave, int lane) {
   constexpr uint32_t kPorts = 2 * K;
   const uint16_t* s_pq = reinterpret_cast<const uint16_t*>(smem + L.pq_off);
@@ -365,7 +375,7 @@ __device__ void TileCells(const KoptLaunch& a, const KoptKLayout& L, const KoptT
   // one store writes G consecutive rows, G * T contiguous cells; with T > 64
   // it walks every row once per pass of 64 templates.
   const uint64_t cost_index = Uniform64(d.cost_index);
-  int64_t* const out = a.costs + cost_index;
+  int64_t* const out = bufs.costs + cost_index;
   const char* const pairs = reinterpret_cast<const char*>(w_pair);
   const uint32_t G = L.group;
   const uint32_t n = __builtin_amdgcn_readfirstlane(rows / G);  // rows every group has
@@ -390,7 +400,7 @@ __device__ void TileCells(const KoptLaunch& a, const KoptKLayout& L, const KoptT
     constexpr uint32_t kStdG = K == 3 ? 16u : (K == 4 ? 2u : 1u);
     if (kMask) {
       MaskOut m;
-      m.masks = a.masks + (d.task_index * a.mask_words);
+      m.masks = bufs.masks + (d.task_index * a.mask_words);
       m.words = a.mask_words;
       m.dcost = reinterpret_cast<const int64_t*>(wave + L.dcost_off);
       m.tbase = pass * kWave;
@@ -411,8 +421,9 @@ __device__ void TileCells(const KoptLaunch& a, const KoptKLayout& L, const KoptT
 // Descriptor of `tile` (a tile of this k): from the inline segment table
 // when the launch has one, else from the descriptor array.
 template <int K>
-__device__ __forceinline__ KoptTile TileAt(const KoptLaunch& a, const KoptKLayout& L, uint64_t tile) {
-  if (a.inline_count == 0) return a.tiles[tile];
+__device__ __forceinline__ KoptTile TileAt(const KoptLaunch& a, const KoptPointers& bufs, const KoptKLayout& L,
+                                           uint64_t tile) {
+  if (a.inline_count == 0) return bufs.tiles[tile];
   constexpr int ki = K - 3;
   uint32_t s = a.k_inline_begin[ki];
   const uint32_t e = a.k_inline_begin[ki + 1];
@@ -433,11 +444,11 @@ __device__ __forceinline__ KoptTile TileAt(const KoptLaunch& a, const KoptKLayou
 // The plan is fetched into registers first, so its latency overlaps the
 // first tile's gathers; it reaches LDS before the only block barrier.
 template <int K, bool kNonTemporal, bool kMask>
-__device__ void TileLoop(const KoptLaunch& a, char* smem) {
+__device__ void TileLoop(const KoptLaunch& a, const KoptPointers& bufs, char* smem) {
   constexpr int ki = K - 3;
   constexpr uint32_t kPlanRegs = 4;
   const KoptKLayout& L = a.layout[ki];
-  const KoptPlan& plan = a.plans[ki];
+  const KoptPlan& plan = bufs.plans[ki];
   const uint32_t tid = threadIdx.x, bd = blockDim.x;
   uint64_t tv[kPlanRegs];
 #pragma unroll
@@ -458,11 +469,11 @@ __device__ void TileLoop(const KoptLaunch& a, char* smem) {
   uint64_t tile = a.k_tile_begin[ki] + __builtin_amdgcn_readfirstlane(block * waves_per_block + wib);
   KoptTile d{};
   if (tile < end) {
-    d = TileAt<K>(a, L, tile);
+    d = TileAt<K>(a, bufs, L, tile);
     if (d.near != 0) {
-      StagePorts<K, true, kMask>(a, L, d, wave, lane);
+      StagePorts<K, true, kMask>(a, bufs, L, d, wave, lane);
     } else {
-      StagePorts<K, false, kMask>(a, L, d, wave, lane);
+      StagePorts<K, false, kMask>(a, bufs, L, d, wave, lane);
     }
   }
 
@@ -478,27 +489,47 @@ __device__ void TileLoop(const KoptLaunch& a, char* smem) {
   __syncthreads();
 
   for (bool first = true; tile < end; tile += stride, first = false) {
-    if (!first) d = TileAt<K>(a, L, tile);
+    if (!first) d = TileAt<K>(a, bufs, L, tile);
     if (d.near != 0) {
-      if (!first) StagePorts<K, true, kMask>(a, L, d, wave, lane);
-      TileCells<K, true, kNonTemporal, kMask>(a, L, d, smem, wave, lane);
+      if (!first) StagePorts<K, true, kMask>(a, bufs, L, d, wave, lane);
+      TileCells<K, true, kNonTemporal, kMask>(a, bufs, L, d, smem, wave, lane);
     } else {
-      if (!first) StagePorts<K, false, kMask>(a, L, d, wave, lane);
-      TileCells<K, false, kNonTemporal, kMask>(a, L, d, smem, wave, lane);
+      if (!first) StagePorts<K, false, kMask>(a, bufs, L, d, wave, lane);
+      TileCells<K, false, kNonTemporal, kMask>(a, bufs, L, d, smem, wave, lane);
     }
   }
 }
 
 template <bool kNonTemporal, bool kMask>
+__device__ __forceinline__ void KoptBody(const KoptLaunch& a, const KoptPointers& bufs, char* smem) {
+  if (blockIdx.x < a.k_block_first[1]) {
+    TileLoop<3, kNonTemporal, kMask>(a, bufs, smem);
+  } else if (blockIdx.x < a.k_block_first[2]) {
+    TileLoop<4, kNonTemporal, kMask>(a, bufs, smem);
+  } else {
+    TileLoop<5, kNonTemporal, kMask>(a, bufs, smem);
+  }
+}
+
+// The launch arguments by value, in the kernel argument segment.
+template <bool kNonTemporal, bool kMask>
 __global__ void __launch_bounds__(256) KoptKernel(KoptLaunch a) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
-  if (blockIdx.x < a.k_block_first[1]) {
-    TileLoop<3, kNonTemporal, kMask>(a, smem);
-  } else if (blockIdx.x < a.k_block_first[2]) {
-    TileLoop<4, kNonTemporal, kMask>(a, smem);
-  } else {
-    TileLoop<5, kNonTemporal, kMask>(a, smem);
-  }
+  const KoptPointers bufs{a.xy, a.tiles, a.tasks, a.plans, a.costs, a.masks};
+  KoptBody<kNonTemporal, kMask>(a, bufs, smem);
+}
+
+// The launch arguments in ordinary device memory, for callers that launch
+// the same arguments many times.  The argument segment is rewritten by the
+// host for every launch and read uncached, and every wavefront's first
+// memory accesses depend on a chain of reads from it; a resident copy is
+// served from the caches after the first wavefront.  The buffer pointers
+// still come as arguments (one batch of independent reads), so the compiler
+// knows them as global memory.
+template <bool kNonTemporal, bool kMask>
+__global__ void __launch_bounds__(256) KoptKernelResident(const KoptLaunch* __restrict__ ap, KoptPointers bufs) {
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  KoptBody<kNonTemporal, kMask>(*ap, bufs, smem);
 }
 
 uint32_t Align16(uint32_t v) { return (v + 15u) & ~15u; }
@@ -615,9 +646,9 @@ void KoptAssignTiles(KoptSegment* segments, size_t count, const KoptInstance* in
   a->inline_count = inline_ok && tile <= UINT32_MAX ? n_inline : 0;
 }
 
-void LaunchKopt(const KoptLaunch& args, const KoptTuning& tuning, size_t shared_bytes,
-                hipStream_t stream) {
-  if (args.tile_count == 0) return;
+uint32_t KoptFinalize(KoptLaunch* out, const KoptTuning& tuning, size_t shared_bytes) {
+  const KoptLaunch& args = *out;
+  if (args.tile_count == 0) return 0;
   if (shared_bytes > 65536) throw std::invalid_argument("k-opt launch needs more than 64 KiB of LDS");
   uint64_t tiles = 0;
   for (int i = 0; i < 3; ++i) tiles += args.k_tile_count[i];
@@ -647,21 +678,47 @@ void LaunchKopt(const KoptLaunch& args, const KoptTuning& tuning, size_t shared_
   if (blocks[0] + blocks[1] + blocks[2] > 0x7fffffffull) {
     throw std::invalid_argument("k-opt launch needs too many workgroups");
   }
-  KoptLaunch a = args;
+  if (args.masks != nullptr && args.mask_words == 0) {
+    throw std::invalid_argument("mask output needs mask_words > 0");
+  }
   uint32_t first = 0;
   for (int i = 0; i < 3; ++i) {
-    a.k_block_first[i] = first;
-    a.k_block_count[i] = static_cast<uint32_t>(blocks[i]);
-    first += a.k_block_count[i];
+    out->k_block_first[i] = first;
+    out->k_block_count[i] = static_cast<uint32_t>(blocks[i]);
+    first += out->k_block_count[i];
   }
-  const dim3 g(first), b(tuning.block);
+  return first;
+}
+
+void LaunchKopt(const KoptLaunch& args, const KoptTuning& tuning, size_t shared_bytes,
+                hipStream_t stream) {
+  KoptLaunch a = args;
+  const uint32_t grid = KoptFinalize(&a, tuning, shared_bytes);
+  if (grid == 0) return;
+  const dim3 g(grid), b(tuning.block);
   if (a.masks != nullptr) {
-    if (a.mask_words == 0) throw std::invalid_argument("mask output needs mask_words > 0");
     hipLaunchKernelGGL((KoptKernel<false, true>), g, b, shared_bytes, stream, a);
   } else if (tuning.non_temporal) {
     hipLaunchKernelGGL((KoptKernel<true, false>), g, b, shared_bytes, stream, a);
   } else {
     hipLaunchKernelGGL((KoptKernel<false, false>), g, b, shared_bytes, stream, a);
+  }
+  CheckHip(hipGetLastError(), "k-opt kernel launch");
+}
+
+void LaunchKoptResident(const KoptLaunch& finalized, const KoptLaunch* device_args, uint32_t grid,
+                        const KoptTuning& tuning, size_t shared_bytes, hipStream_t stream) {
+  if (grid == 0) return;
+  if (device_args == nullptr) throw std::invalid_argument("resident k-opt arguments are NULL");
+  const dim3 gd(grid), b(tuning.block);
+  const KoptPointers ptrs{finalized.xy, finalized.tiles, finalized.tasks, finalized.plans, finalized.costs,
+                          finalized.masks};
+  if (finalized.masks != nullptr) {
+    hipLaunchKernelGGL((KoptKernelResident<false, true>), gd, b, shared_bytes, stream, device_args, ptrs);
+  } else if (tuning.non_temporal) {
+    hipLaunchKernelGGL((KoptKernelResident<true, false>), gd, b, shared_bytes, stream, device_args, ptrs);
+  } else {
+    hipLaunchKernelGGL((KoptKernelResident<false, false>), gd, b, shared_bytes, stream, device_args, ptrs);
   }
   CheckHip(hipGetLastError(), "k-opt kernel launch");
 }
