@@ -45,6 +45,10 @@ constexpr int kCounterBase = 35;
 constexpr uint64_t kCostMask = (uint64_t{1} << kCounterBase) - 1;
 constexpr uint64_t kSentinel = uint64_t{1} << 61;
 constexpr uint64_t kBadMask = (uint64_t{0xAAAAA} << kCounterBase) | (uint64_t{7} << 61);
+// Bytes of one port's record in the port area: coordinates (the low words on
+// near tiles), then the two info words of phase A2.
+constexpr uint32_t kNearPortBytes = 16;
+constexpr uint32_t kFarPortBytes = 32;
 
 // The launch's buffers, taken from the kernel arguments.
 struct KoptPointers {
@@ -98,8 +102,7 @@ __device__ __forceinline__ void StagePorts(const KoptLaunch& a, const KoptPointe
                                            const KoptTile& d, char* wave, int lane) {
   constexpr uint32_t kPorts = 2 * K;
   constexpr uint32_t kBatch = 2;
-  longlong2* w_xy = reinterpret_cast<longlong2*>(wave + L.xy_off);
-  uint2* w_xy2 = reinterpret_cast<uint2*>(wave + L.xy_off);
+  char* w_port = wave + L.xy_off;
   uint32_t* w_row = reinterpret_cast<uint32_t*>(wave + L.row_off);
   int32_t* w_node = reinterpret_cast<int32_t*>(wave + L.pair_off);
   const uint32_t total = d.rows * kPorts;
@@ -129,9 +132,10 @@ __device__ __forceinline__ void StagePorts(const KoptLaunch& a, const KoptPointe
         w_node[s] = node[i];
         if (kNear) {
           // coordinate extents below 2^25: differences of the low words are exact
-          w_xy2[s] = uint2{static_cast<uint32_t>(cx[i]), static_cast<uint32_t>(cy[i])};
+          *reinterpret_cast<uint2*>(w_port + s * kNearPortBytes) =
+              uint2{static_cast<uint32_t>(cx[i]), static_cast<uint32_t>(cy[i])};
         } else {
-          w_xy[s] = longlong2{cx[i], cy[i]};
+          *reinterpret_cast<longlong2*>(w_port + s * kFarPortBytes) = longlong2{cx[i], cy[i]};
         }
         if (p == 0) {
           w_row[r] = 0;
@@ -147,33 +151,49 @@ __device__ __forceinline__ void StagePorts(const KoptLaunch& a, const KoptPointe
   WaveSync();
 }
 
-// Counter bit and sentinel of pair (p, q) from the port info words of one
-// row, as the high word of a pair record (bits 32..63).
-__device__ __forceinline__ uint32_t PairFlags(uint32_t ip, uint32_t iq, uint32_t q) {
-  // both nodes held by two or more ports: count the edge in the counter of
-  // their ranks lo < hi < 5, bits 33 + lo * (7 - lo) + 2 * hi and one up
-  const uint32_t ra = ip >> 16, rb = iq >> 16;
-  const uint32_t lo = ra < rb ? ra : rb, hi = ra < rb ? rb : ra;
-  const uint32_t pos = lo * (7u - lo) + 2u * hi + 1u;  // bit of the high word
-  const uint32_t bit = hi < 5u ? 1u << pos : 0u;
-  // self loop, or the node pair of deleted edge e = (2e, 2e+1) in either order
-  const uint32_t readd = ((ip & (iq >> 1)) | ((ip >> 1) & iq)) & 0x5555u;
-  const bool sentinel = (((ip >> q) & 1u) | readd) != 0;
-  return sentinel ? static_cast<uint32_t>(kSentinel >> 32) : bit;
+// Duplicate-edge counters: the high word of a pair record holds, for every
+// two nodes of ranks lo < hi < 5 (the row's nodes held by two or more ports,
+// ranked by first port), a 2-bit counter at bits lo * (7 - lo) + 2 * hi + 1
+// and one up, so a template that adds the same node pair twice carries into
+// the upper bit.  RankMaskOf(a) has the low counter bit of every pair of
+// nodes that includes rank a: for ranks a != b, RankMaskOf(a) &
+// RankMaskOf(b) is exactly the counter bit of pair (a, b).
+constexpr uint32_t CounterBit(uint32_t lo, uint32_t hi) { return 1u << (lo * (7u - lo) + 2u * hi + 1u); }
+constexpr uint32_t RankMaskOf(uint32_t a) {
+  uint32_t m = 0;
+  for (uint32_t x = 0; x < 5; ++x) {
+    if (x != a) m |= x < a ? CounterBit(x, a) : CounterBit(a, x);
+  }
+  return m;
+}
+static_assert((RankMaskOf(1) & RankMaskOf(3)) == CounterBit(1, 3), "one common bit");
+static_assert((RankMaskOf(0) & RankMaskOf(4)) == CounterBit(0, 4), "one common bit");
+
+__device__ __forceinline__ uint32_t RankMask(uint32_t rank) {
+  return rank == 0 ? RankMaskOf(0)
+       : rank == 1 ? RankMaskOf(1)
+       : rank == 2 ? RankMaskOf(2)
+       : rank == 3 ? RankMaskOf(3)
+                   : RankMaskOf(4);
 }
 
 // Phase B: pair records, pair-major (record of pair pi and row r at
-// pi * stride + r).  Every lane computes the distance, sentinel or not, and
-// the duplicate-edge counter bit without branches, so the loop body does not
-// diverge; the next iteration's pair ports are read at the top of the current
-// one.  kType is the distance type (0 EUC_2D, 1 CEIL_2D).
+// pi * stride + r), from the port records of p and q:
+//  - sentinel: q's node is p's node, or the two nodes are the end nodes of
+//    one deleted edge (info word 0: the same-node mask of p in bits 0..9, the
+//    mask together with the deleted-edge partners of its ports in bits
+//    16..25);
+//  - duplicate-edge counter bit: info word 1 of p AND that of q (RankMask of
+//    the node's rank, 0 for a node held by one port).
+// Every lane computes the distance, sentinel or not, so the loop body does
+// not diverge; the next iteration's pair ports are read at the top of the
+// current one.  kType is the distance type (0 EUC_2D, 1 CEIL_2D).
 template <int K, bool kNear, int kType>
 __device__ __forceinline__ void PairRecords(const KoptKLayout& L, uint32_t rows, const uint16_t* s_pq,
                                             const char* wave, uint64_t* w_pair, int lane) {
   constexpr uint32_t kPorts = 2 * K;
-  const longlong2* w_xy = reinterpret_cast<const longlong2*>(wave + L.xy_off);
-  const uint2* w_xy2 = reinterpret_cast<const uint2*>(wave + L.xy_off);  // near tiles: low words
-  const uint32_t* w_info = reinterpret_cast<const uint32_t*>(wave + L.eq_off);
+  constexpr uint32_t kPortBytes = kNear ? kNearPortBytes : kFarPortBytes;
+  const char* w_port = wave + L.xy_off;
   const uint32_t P = L.P, stride = L.pair_stride;
   uint32_t r = DivSmall(static_cast<uint32_t>(lane), P);
   uint32_t pi = static_cast<uint32_t>(lane) - r * P;
@@ -189,22 +209,29 @@ __device__ __forceinline__ void PairRecords(const KoptKLayout& L, uint32_t rows,
     }
     pq = s_pq[pi];  // pi < P always, so this read is in range even past the last row
     const uint32_t base = __umul24(row, kPorts);
-    const uint32_t ip = w_info[base + p], iq = w_info[base + q];
+    const char* ra = w_port + (base + p) * kPortBytes;
+    const char* rb = w_port + (base + q) * kPortBytes;
     uint32_t lo_word, hi_word;
+    uint2 ia, ib;
     if (kNear) {
-      const uint2 pa = w_xy2[base + p], pb = w_xy2[base + q];
+      const uint4 pa = *reinterpret_cast<const uint4*>(ra);
+      const uint4 pb = *reinterpret_cast<const uint4*>(rb);
       lo_word = static_cast<uint32_t>(ExactDistanceNear(static_cast<int32_t>(pa.x - pb.x),
                                                         static_cast<int32_t>(pa.y - pb.y), kType));
       hi_word = 0;  // near distances are below 2^27
+      ia = uint2{pa.z, pa.w};
+      ib = uint2{pb.z, pb.w};
     } else {
-      const longlong2 pa = w_xy[base + p], pb = w_xy[base + q];
+      const longlong2 pa = *reinterpret_cast<const longlong2*>(ra);
+      const longlong2 pb = *reinterpret_cast<const longlong2*>(rb);
       const uint64_t dist = static_cast<uint64_t>(ExactDistance(pa.x, pa.y, pb.x, pb.y, kType));
       lo_word = static_cast<uint32_t>(dist);
       hi_word = static_cast<uint32_t>(dist >> 32);
+      ia = *reinterpret_cast<const uint2*>(ra + 16);
+      ib = *reinterpret_cast<const uint2*>(rb + 16);
     }
-    const uint32_t flags = PairFlags(ip, iq, q);
-    const bool sentinel = flags == static_cast<uint32_t>(kSentinel >> 32);
-    const uint64_t rec = (static_cast<uint64_t>(hi_word | flags) << 32) | lo_word;
+    const bool sentinel = ((ia.x >> 16) & ib.x) != 0;
+    const uint64_t rec = (static_cast<uint64_t>(hi_word | (ia.y & ib.y)) << 32) | lo_word;
     w_pair[__umul24(slot, stride) + row] = sentinel ? kSentinel : rec;
   }
 }
@@ -350,15 +377,21 @@ __device__ void TileCells(const KoptLaunch& a, const KoptPointers& bufs, const K
     if (eq != (1u << p) && __builtin_ctz(eq) == p) atomicOr(&w_row[r], 1u << p);
   }
   WaveSync();
-  // port info: the same-node mask in bits 0..9 and, in bits 16..18, the rank
-  // of the port's node among the row's nodes held by two or more ports (by
-  // first port), or 7 for a node held by this port alone
+  // port info words: word 0 the same-node mask eq in bits 0..9 and, in bits
+  // 16..25, eq together with the deleted-edge partners of its ports (port
+  // 2e <-> 2e + 1); word 1 RankMask of the rank of the port's node among
+  // the row's nodes held by two or more ports (by first port), or 0 for a
+  // node held by this port alone
+  char* w_port = wave + L.xy_off;
   for (uint32_t s = lane; s < rows * kPorts; s += kWave) {
     const uint32_t r = s / kPorts;
     const uint32_t eq = w_eq[s];
     const uint32_t below = w_row[r] & ((eq & (0u - eq)) - 1u);
-    const uint32_t rank = (eq & (eq - 1u)) != 0 ? static_cast<uint32_t>(__builtin_popcount(below)) : 7u;
-    w_eq[s] = eq | (rank << 16);
+    const uint32_t mask =
+        (eq & (eq - 1u)) != 0 ? RankMask(static_cast<uint32_t>(__builtin_popcount(below))) : 0u;
+    const uint32_t partners = eq | ((eq & 0x155u) << 1) | ((eq >> 1) & 0x155u);
+    *reinterpret_cast<uint2*>(w_port + s * (kNear ? kNearPortBytes : kFarPortBytes) + (kNear ? 8 : 16)) =
+        uint2{eq | (partners << 16), mask};
   }
   WaveSync();
 
@@ -574,7 +607,7 @@ size_t KoptPrepareLayout(KoptLaunch* a, const KoptPlan plans[3], const bool pres
     plan_bytes = std::max(plan_bytes, Align16(L.tpl_off + L.T * 8));
     const uint32_t ports = 2 * k;
     L.xy_off = 0;
-    L.eq_off = Align16(rows * ports * (near_only ? 8u : 16u));
+    L.eq_off = Align16(rows * ports * (near_only ? kNearPortBytes : kFarPortBytes));
     L.row_off = L.eq_off + Align16(rows * ports * 4);
     L.dcost_off = L.row_off + Align16(rows * 4);
     L.pair_off = L.dcost_off + Align16(rows * 8);
