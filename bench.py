@@ -410,7 +410,7 @@ def profile_figure(name: str, key: str):
 
 
 def measured_traffic():
-    """HBM bytes per KoptKernel launch from the committed counter summary
+    """HBM bytes per k-opt kernel launch from the committed counter summary
     (FETCH_SIZE + WRITE_SIZE, each scaled by the copy calibration), or None."""
     v = profile_figure("kopt_kernel_config3_traffic.md", "traffic_bytes_per_launch")
     return None if v is None else int(v)
@@ -530,7 +530,7 @@ def main() -> int:
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
             # SQ_INSTS_VALU x 4 cycles over every SIMD's cycles of the traced launch
             "valu_utilisation": profile_figure("kopt_kernel_config3_valu.md", "valu_utilisation"),
-            "kernel": "KoptKernel", "algorithmic_bytes_per_launch": algo_bytes,
+            "kernel": "KoptKernelResident", "algorithmic_bytes_per_launch": algo_bytes,
             "kernel_ms": device_ms, "graph_replay_kernel_ms": graph_ms,
             # exact distances per second: evaluated once per (row, port pair),
             # and as a per-cell evaluation (k per cell) would count them
