@@ -18,7 +18,7 @@ namespace {
 
 constexpr uint32_t kTemplateCount[3] = {4, 25, 208};
 // Bumped whenever the k-opt kernel changes, so cached tunings are retuned.
-constexpr int kKernelRevision = 5;
+constexpr int kKernelRevision = 6;
 
 struct Batch {
   explicit Batch(Context* c) : ctx(c) {}
@@ -74,6 +74,7 @@ struct Batch {
 
   size_t SharedBytes() const {
     KoptLaunch probe{};
+    probe.masks = mask_output ? const_cast<uint32_t*>(d_masks.get()) : nullptr;  // mask mode stages deleted costs
     return KoptPrepareLayout(&probe, plans.data(), k_present, tuning, NearOnly());
   }
 
@@ -299,8 +300,10 @@ ee_kopt_tuning Autotune(Batch& b, uint32_t launches) {
   };
   double best_ms = measure(best);
   const uint32_t cu = static_cast<uint32_t>(std::max(b.ctx->compute_units, 1));
+  // residency cliffs (every block reserves the LDS of the launch's largest
+  // k) make the time a bumpy function of the rows, hence the fine options
   const std::vector<uint32_t> rows_options[3] = {
-      {16, 32, 48, 64}, {8, 16, 24, 32, 48, 64}, {4, 6, 8, 12, 16, 24, 32}};
+      {16, 24, 28, 32, 36, 40, 48, 64}, {8, 16, 20, 22, 24, 26, 28, 32, 48, 64}, {4, 5, 6, 7, 8, 12, 16, 24, 32}};
   for (int pass = 0; pass < 2; ++pass) {
     for (int k = 2; k >= 0; --k) {
       if (!b.k_present[k]) continue;
@@ -341,6 +344,30 @@ ee_kopt_tuning Autotune(Batch& b, uint32_t launches) {
         best = t;
       }
     }
+  }
+  // local search: one row option up or down for one k at a time, until no
+  // neighbour is faster
+  for (int round = 0; round < 8; ++round) {
+    bool improved = false;
+    for (int k = 0; k < 3; ++k) {
+      if (!b.k_present[k]) continue;
+      const std::vector<uint32_t>& opts = rows_options[k];
+      const auto at = std::find(opts.begin(), opts.end(), best.rows_per_tile[k]);
+      if (at == opts.end()) continue;
+      for (int step : {-1, 1}) {
+        const auto pos = (at - opts.begin()) + step;
+        if (pos < 0 || pos >= static_cast<long>(opts.size())) continue;
+        ee_kopt_tuning t = best;
+        t.rows_per_tile[k] = opts[static_cast<size_t>(pos)];
+        const double ms = measure(t);
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = t;
+          improved = true;
+        }
+      }
+    }
+    if (!improved) break;
   }
   int top = -1;
   for (int k = 2; k >= 0 && top < 0; --k) {

@@ -87,7 +87,7 @@ struct KoptKLayout {
   uint32_t group, passes, pair_stride;
   uint32_t pair_step_r, pair_step_p;  // 64 = pair_step_r * P + pair_step_p
   uint32_t pq_off, tpl_off;
-  uint32_t xy_off, eq_off, row_off, dcost_off, pair_off;
+  uint32_t xy_off, row_off, dcost_off, pair_off;
 };
 
 // A segment carried in the kernel arguments: for launches of at most
@@ -161,9 +161,10 @@ void KoptAssignTiles(KoptSegment* segments, size_t count, const KoptInstance* in
                      const KoptPlan plans[3], const KoptTuning& tuning, KoptTile* tiles,
                      KoptLaunch* a);
 // Fills the layout of `a` for the plans present and returns the dynamic LDS
-// bytes of one workgroup.
-// near_only: every tile of the launch is on a near instance, whose staged
-// coordinates take 8 bytes per port instead of 16.
+// bytes of one workgroup.  `a->masks` must already be set: mask output stages
+// each row's deleted cost in LDS.
+// near_only: every tile of the launch is on a near instance, whose port
+// records take 16 bytes instead of 32.
 size_t KoptPrepareLayout(KoptLaunch* a, const KoptPlan plans[3], const bool present[3],
                          const KoptTuning& tuning, bool near_only);
 void LaunchKopt(const KoptLaunch& args, const KoptTuning& tuning, size_t shared_bytes,

@@ -357,12 +357,16 @@ __device__ void TileCells(const KoptLaunch& a, const KoptPointers& bufs, const K
   constexpr uint32_t kPorts = 2 * K;
   const uint16_t* s_pq = reinterpret_cast<const uint16_t*>(smem + L.pq_off);
   const uint64_t* s_tpl = reinterpret_cast<const uint64_t*>(smem + L.tpl_off);
-  uint32_t* w_eq = reinterpret_cast<uint32_t*>(wave + L.eq_off);
   uint32_t* w_row = reinterpret_cast<uint32_t*>(wave + L.row_off);
   uint64_t* w_pair = reinterpret_cast<uint64_t*>(wave + L.pair_off);
   const int32_t* w_node = reinterpret_cast<const int32_t*>(wave + L.pair_off);
   const uint32_t T = L.T, stride = L.pair_stride;
   const uint32_t rows = d.rows;
+  // a port's info words in its port record; the first pass parks its
+  // same-node mask in word 0, where only its own lane reads it back
+  char* const w_port = wave + L.xy_off;
+  constexpr uint32_t kPortBytes = kNear ? kNearPortBytes : kFarPortBytes;
+  constexpr uint32_t kInfoOffset = kNear ? 8 : 16;
 
   // A2. same-node masks; per row, the first port of every node held by two
   // or more ports
@@ -373,7 +377,7 @@ __device__ void TileCells(const KoptLaunch& a, const KoptPointers& bufs, const K
     uint32_t eq = 0;
 #pragma unroll
     for (uint32_t j = 0; j < kPorts; ++j) eq |= static_cast<uint32_t>(nd[j] == me) << j;
-    w_eq[s] = eq;
+    *reinterpret_cast<uint32_t*>(w_port + s * kPortBytes + kInfoOffset) = eq;
     if (eq != (1u << p) && __builtin_ctz(eq) == p) atomicOr(&w_row[r], 1u << p);
   }
   WaveSync();
@@ -382,16 +386,14 @@ __device__ void TileCells(const KoptLaunch& a, const KoptPointers& bufs, const K
   // 2e <-> 2e + 1); word 1 RankMask of the rank of the port's node among
   // the row's nodes held by two or more ports (by first port), or 0 for a
   // node held by this port alone
-  char* w_port = wave + L.xy_off;
   for (uint32_t s = lane; s < rows * kPorts; s += kWave) {
     const uint32_t r = s / kPorts;
-    const uint32_t eq = w_eq[s];
+    const uint32_t eq = *reinterpret_cast<const uint32_t*>(w_port + s * kPortBytes + kInfoOffset);
     const uint32_t below = w_row[r] & ((eq & (0u - eq)) - 1u);
     const uint32_t mask =
         (eq & (eq - 1u)) != 0 ? RankMask(static_cast<uint32_t>(__builtin_popcount(below))) : 0u;
     const uint32_t partners = eq | ((eq & 0x155u) << 1) | ((eq >> 1) & 0x155u);
-    *reinterpret_cast<uint2*>(w_port + s * (kNear ? kNearPortBytes : kFarPortBytes) + (kNear ? 8 : 16)) =
-        uint2{eq | (partners << 16), mask};
+    *reinterpret_cast<uint2*>(w_port + s * kPortBytes + kInfoOffset) = uint2{eq | (partners << 16), mask};
   }
   WaveSync();
 
@@ -606,11 +608,13 @@ size_t KoptPrepareLayout(KoptLaunch* a, const KoptPlan plans[3], const bool pres
     L.tpl_off = Align16(2 * kMaxPairs);
     plan_bytes = std::max(plan_bytes, Align16(L.tpl_off + L.T * 8));
     const uint32_t ports = 2 * k;
+    // every block of a launch reserves the LDS of its largest k, so the
+    // per-wave arrays are kept to what each phase needs: the same-node masks
+    // of A2 live in the port records, the deleted costs only in mask mode
     L.xy_off = 0;
-    L.eq_off = Align16(rows * ports * (near_only ? kNearPortBytes : kFarPortBytes));
-    L.row_off = L.eq_off + Align16(rows * ports * 4);
+    L.row_off = Align16(rows * ports * (near_only ? kNearPortBytes : kFarPortBytes));
     L.dcost_off = L.row_off + Align16(rows * 4);
-    L.pair_off = L.dcost_off + Align16(rows * 8);
+    L.pair_off = L.dcost_off + (a->masks != nullptr ? Align16(rows * 8) : 0u);
     // the port node ids of phase A share the pair record area
     const uint32_t pair_bytes = std::max(L.P * L.pair_stride * 8, rows * ports * 4);
     wave_bytes = std::max<uint32_t>(wave_bytes, L.pair_off + Align16(pair_bytes));
