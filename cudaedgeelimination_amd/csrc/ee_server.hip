@@ -18,6 +18,7 @@
 // second thread: the worker issues, waits and completes each batch in turn
 // (the plain per-query path).
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -132,10 +133,32 @@ struct Server {
 
   uint64_t SlotBudget() const { return cfg.hbm_budget_bytes / std::max<uint32_t>(cfg.pipeline_depth, 1); }
 
+  // Batched mode: how long the issuing and completing threads poll (yielding
+  // the core) for their next piece of work before they block.  A blocked
+  // thread is woken by the operating system, tens of microseconds later and
+  // more in the tail; at serving rates the next request or batch usually
+  // comes within this window.
+  static constexpr double kSpinMicros = 200.0;
+  // bumped after every change the two threads wait for, so that a polling
+  // thread looks at the queue (under the lock) only when something changed
+  std::atomic<uint64_t> signal_seq{0};
+  void Signal(std::condition_variable& cv, bool all = false) {
+    signal_seq.fetch_add(1, std::memory_order_release);
+    if (all) {
+      cv.notify_all();
+    } else {
+      cv.notify_one();
+    }
+  }
+  template <typename Pred>
+  void SpinThenWait(std::unique_lock<std::mutex>& lk, std::condition_variable& cv, Pred ready);
+
   void Loop();
   void CompleteLoop();
   // Waits for a slot's batch and completes (or fails) its requests.
-  void Finish(Slot& slot);
+  // `poll`: wait for the batch by polling its event (yielding) rather than
+  // blocking on it
+  void Finish(Slot& slot, bool poll = false);
   bool Issue(Slot& slot, std::vector<std::shared_ptr<Request>> batch);
   void Complete(Slot& slot);
   void Fail(std::vector<std::shared_ptr<Request>>& batch, const std::string& why);
@@ -349,6 +372,27 @@ void Server::Complete(Slot& slot) {
   cv_done.notify_all();
 }
 
+template <typename Pred>
+void Server::SpinThenWait(std::unique_lock<std::mutex>& lk, std::condition_variable& cv, Pred ready) {
+  if (ready()) return;
+  if (slots.size() > 1) {
+    const auto until = std::chrono::steady_clock::now() + std::chrono::duration<double, std::micro>(kSpinMicros);
+    uint64_t seen = signal_seq.load(std::memory_order_acquire);
+    lk.unlock();
+    while (std::chrono::steady_clock::now() < until) {
+      if (signal_seq.load(std::memory_order_acquire) != seen) {
+        lk.lock();
+        if (ready()) return;
+        seen = signal_seq.load(std::memory_order_acquire);
+        lk.unlock();
+      }
+      std::this_thread::yield();
+    }
+    lk.lock();
+  }
+  cv.wait(lk, ready);
+}
+
 // Issuing side: whenever a slot is free and requests are pending, packs the
 // next batch in arrival order and issues it.
 void Server::Loop() {
@@ -358,7 +402,7 @@ void Server::Loop() {
     std::vector<std::shared_ptr<Request>> batch;
     {
       std::unique_lock<std::mutex> lk(mu);
-      cv_work.wait(lk, [&] {
+      SpinThenWait(lk, cv_work, [&] {
         if (stopping && pending.empty()) return true;
         if (pending.empty()) return false;
         for (Slot& s : slots) {
@@ -401,7 +445,7 @@ void Server::Loop() {
           std::lock_guard<std::mutex> lk(mu);
           if (!inflight.empty() || !pending.empty() || stopping) {
             inflight.push_back(free_slot);
-            cv_inflight.notify_one();
+            Signal(cv_inflight);
             break;
           }
         }
@@ -417,7 +461,7 @@ void Server::Loop() {
   }
   std::lock_guard<std::mutex> lk(mu);
   issuer_done = true;
-  cv_inflight.notify_one();
+  Signal(cv_inflight);
 }
 
 // Completing side: blocks on the oldest issued batch, hands its results to
@@ -425,20 +469,29 @@ void Server::Loop() {
 void Server::CompleteLoop() {
   for (;;) {
     Slot* slot = nullptr;
+    bool poll = false;
     {
       std::unique_lock<std::mutex> lk(mu);
-      cv_inflight.wait(lk, [&] { return !inflight.empty() || issuer_done; });
+      SpinThenWait(lk, cv_inflight, [&] { return !inflight.empty() || issuer_done; });
       if (inflight.empty()) return;  // the issuer stopped and everything completed
       slot = inflight.front();
       inflight.pop_front();
+      // with requests queued the issuer is busy in the HIP runtime, and
+      // polling the event would contend with its calls
+      poll = pending.empty();
     }
-    Finish(*slot);
-    cv_work.notify_one();
+    Finish(*slot, poll);
+    Signal(cv_work);
   }
 }
 
-void Server::Finish(Slot& slot) {
-  const hipError_t st = hipEventSynchronize(slot.ev_done);
+void Server::Finish(Slot& slot, bool poll) {
+  hipError_t st;
+  if (poll) {
+    while ((st = hipEventQuery(slot.ev_done)) == hipErrorNotReady) std::this_thread::yield();
+  } else {
+    st = hipEventSynchronize(slot.ev_done);
+  }
   if (st == hipSuccess) {
     Complete(slot);
     return;
@@ -551,7 +604,7 @@ void Enqueue(Server& s, std::vector<std::shared_ptr<Request>>& rs, uint64_t* ids
       if (ids) ids[i] = rs[i]->id;
     }
   }
-  s.cv_work.notify_one();
+  s.Signal(s.cv_work);
 }
 
 }  // namespace
@@ -625,7 +678,7 @@ void ee_server_destroy(ee_server* server) {
     std::lock_guard<std::mutex> lk(s->mu);
     s->stopping = true;
   }
-  s->cv_work.notify_all();
+  s->Signal(s->cv_work, true);
   if (s->worker.joinable()) s->worker.join();
   if (s->completer.joinable()) s->completer.join();
   static_cast<void>(hipSetDevice(s->ctx->device));
