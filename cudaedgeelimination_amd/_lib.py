@@ -16,6 +16,9 @@ import numpy as np
 LIBRARY_PATH = pathlib.Path(__file__).resolve().with_name("libee_hip.so")
 
 EE_OK = 0
+# ee_kopt_batch_run flags (EE_KOPT_RUN_* in include/ee_hip.h)
+KOPT_RUN_GRAPH = 1
+KOPT_RUN_PIPELINED = 2
 STATUS_NAMES = {
     1: "EE_ERR_INVALID_ARGUMENT",
     2: "EE_ERR_OVERFLOW",

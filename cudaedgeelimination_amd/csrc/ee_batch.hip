@@ -22,7 +22,11 @@ constexpr int kKernelRevision = 6;
 
 struct Batch {
   explicit Batch(Context* c) : ctx(c) {}
-  ~Batch() { DropGraph(); }
+  ~Batch() {
+    DropGraph();
+    if (ev_join != nullptr) static_cast<void>(hipEventDestroy(ev_join));
+    if (stream2 != nullptr) static_cast<void>(hipStreamDestroy(stream2));
+  }
 
   Context* ctx;
   // instances
@@ -57,6 +61,12 @@ struct Batch {
   bool args_valid = false;
   uint32_t grid = 0;
   size_t shared = 0;
+  // pipelined runs: launches alternate between the context's stream and a
+  // second one, and between the cost pool and a second pool of its size, so
+  // the next batch's launch can start while the previous one drains
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_join = nullptr;
+  DeviceArray<int64_t> d_costs2;
 
   void DropGraph() {
     if (exec != nullptr) static_cast<void>(hipGraphExecDestroy(exec));
@@ -148,11 +158,14 @@ struct Batch {
     LaunchKoptResident(host_args, d_args.get(), grid, tuning, shared, ctx->stream);
   }
 
-  // Mean device time of one launch over `launches` back-to-back launches.
-  double Time(uint32_t launches, bool use_graph) {
+  // Mean device time of one launch over `launches` back-to-back launches,
+  // pipelined over two streams when asked.
+  double Time(uint32_t launches, bool use_graph, bool pipelined = false) {
+    if (use_graph && pipelined) throw std::invalid_argument("graph replay and pipelined runs do not combine");
     Prepare();
     if (tile_count == 0 || launches == 0) return 0.0;
     SyncArgs();  // any upload happens here, never inside the capture
+    if (pipelined) return TimePipelined(launches);
     if (use_graph && exec == nullptr) {
       CheckHip(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
       LaunchOnce();
@@ -174,9 +187,36 @@ struct Batch {
     return static_cast<double>(ms) / launches;
   }
 
+  double TimePipelined(uint32_t launches) {
+    if (mask_output) throw std::invalid_argument("pipelined runs write cost cells only");
+    if (stream2 == nullptr) {
+      CheckHip(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking), "second stream");
+      CheckHip(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming), "join event");
+    }
+    if (d_costs2.count() != d_costs.count()) d_costs2.Allocate(d_costs.count(), ctx->device);
+    // the kernel writes through the pointer arguments, so the device copy of
+    // the launch description serves both pools
+    KoptLaunch second = host_args;
+    second.costs = d_costs2.get();
+    CheckHip(hipEventRecord(ctx->ev_begin, ctx->stream), "event record");
+    CheckHip(hipStreamWaitEvent(stream2, ctx->ev_begin, 0), "stream wait");
+    for (uint32_t i = 0; i < launches; ++i) {
+      const bool odd = (i & 1u) != 0;
+      LaunchKoptResident(odd ? second : host_args, d_args.get(), grid, tuning, shared,
+                         odd ? stream2 : ctx->stream);
+    }
+    CheckHip(hipEventRecord(ev_join, stream2), "event record");
+    CheckHip(hipStreamWaitEvent(ctx->stream, ev_join, 0), "stream wait");
+    CheckHip(hipEventRecord(ctx->ev_end, ctx->stream), "event record");
+    CheckHip(hipEventSynchronize(ctx->ev_end), "event synchronize");
+    float ms = 0.0f;
+    CheckHip(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end), "event elapsed");
+    return static_cast<double>(ms) / launches;
+  }
+
   uint64_t ResidentBytes() const {
-    return d_xy.bytes() + d_tasks.bytes() + d_costs.bytes() + d_masks.bytes() + d_tiles.bytes() +
-           d_plans.bytes();
+    return d_xy.bytes() + d_tasks.bytes() + d_costs.bytes() + d_costs2.bytes() + d_masks.bytes() +
+           d_tiles.bytes() + d_plans.bytes();
   }
 };
 
@@ -536,10 +576,13 @@ ee_status ee_kopt_batch_autotune(ee_kopt_batch* batch, uint32_t launches_per_can
   });
 }
 
-ee_status ee_kopt_batch_run(ee_kopt_batch* batch, uint32_t launches, int use_graph,
-                            double* device_ms) {
+ee_status ee_kopt_batch_run(ee_kopt_batch* batch, uint32_t launches, int flags, double* device_ms) {
   return Guarded([&] {
-    const double ms = ee::AsBatch(batch).Time(launches, use_graph != 0);
+    if ((flags & ~(EE_KOPT_RUN_GRAPH | EE_KOPT_RUN_PIPELINED)) != 0) {
+      throw std::invalid_argument("unknown k-opt run flags");
+    }
+    const double ms = ee::AsBatch(batch).Time(launches, (flags & EE_KOPT_RUN_GRAPH) != 0,
+                                               (flags & EE_KOPT_RUN_PIPELINED) != 0);
     if (device_ms) *device_ms = ms;
   });
 }

@@ -136,9 +136,11 @@ struct Server {
   // Batched mode: how long the issuing and completing threads poll (yielding
   // the core) for their next piece of work before they block.  A blocked
   // thread is woken by the operating system, tens of microseconds later and
-  // more in the tail; at serving rates the next request or batch usually
-  // comes within this window.
-  static constexpr double kSpinMicros = 200.0;
+  // more in the tail.  The issuer's window covers the gaps between requests
+  // of a steady stream down to a few hundred requests per second; the
+  // completer's the time between a batch's issue and its hand-over.
+  static constexpr double kIssuerSpinMicros = 5000.0;
+  static constexpr double kCompleterSpinMicros = 200.0;
   // bumped after every change the two threads wait for, so that a polling
   // thread looks at the queue (under the lock) only when something changed
   std::atomic<uint64_t> signal_seq{0};
@@ -151,7 +153,8 @@ struct Server {
     }
   }
   template <typename Pred>
-  void SpinThenWait(std::unique_lock<std::mutex>& lk, std::condition_variable& cv, Pred ready);
+  void SpinThenWait(std::unique_lock<std::mutex>& lk, std::condition_variable& cv, Pred ready,
+                    double spin_micros);
 
   void Loop();
   void CompleteLoop();
@@ -373,10 +376,11 @@ void Server::Complete(Slot& slot) {
 }
 
 template <typename Pred>
-void Server::SpinThenWait(std::unique_lock<std::mutex>& lk, std::condition_variable& cv, Pred ready) {
+void Server::SpinThenWait(std::unique_lock<std::mutex>& lk, std::condition_variable& cv, Pred ready,
+                          double spin_micros) {
   if (ready()) return;
   if (slots.size() > 1) {
-    const auto until = std::chrono::steady_clock::now() + std::chrono::duration<double, std::micro>(kSpinMicros);
+    const auto until = std::chrono::steady_clock::now() + std::chrono::duration<double, std::micro>(spin_micros);
     uint64_t seen = signal_seq.load(std::memory_order_acquire);
     lk.unlock();
     while (std::chrono::steady_clock::now() < until) {
@@ -409,7 +413,7 @@ void Server::Loop() {
           if (!s.busy) return true;
         }
         return false;
-      });
+      }, kIssuerSpinMicros);
       if (stopping && pending.empty()) break;
       for (Slot& s : slots) {
         if (!s.busy) {
@@ -472,7 +476,7 @@ void Server::CompleteLoop() {
     bool poll = false;
     {
       std::unique_lock<std::mutex> lk(mu);
-      SpinThenWait(lk, cv_inflight, [&] { return !inflight.empty() || issuer_done; });
+      SpinThenWait(lk, cv_inflight, [&] { return !inflight.empty() || issuer_done; }, kCompleterSpinMicros);
       if (inflight.empty()) return;  // the issuer stopped and everything completed
       slot = inflight.front();
       inflight.pop_front();

@@ -90,11 +90,13 @@ class KoptBatch:
             C.byref(t)))
         return t.as_dict()
 
-    def run(self, launches: int = 1, use_graph: bool = False) -> float:
-        """Mean device milliseconds of one launch (HIP events)."""
+    def run(self, launches: int = 1, use_graph: bool = False, pipelined: bool = False) -> float:
+        """Mean device milliseconds of one launch (HIP events).  pipelined:
+        the launches alternate between two streams and two cost pools, so a
+        launch starts while the previous one drains (cost output only)."""
         ms = C.c_double()
-        _lib.check(_lib.lib().ee_kopt_batch_run(self._h, launches, 1 if use_graph else 0,
-                                                C.byref(ms)))
+        flags = (_lib.KOPT_RUN_GRAPH if use_graph else 0) | (_lib.KOPT_RUN_PIPELINED if pipelined else 0)
+        _lib.check(_lib.lib().ee_kopt_batch_run(self._h, launches, flags, C.byref(ms)))
         return ms.value
 
     def download(self, first: int = 0, count: int | None = None) -> np.ndarray:

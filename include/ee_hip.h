@@ -219,12 +219,20 @@ ee_status ee_kopt_batch_get_tuning(const ee_kopt_batch* batch, ee_kopt_tuning* t
  * it to that file under a key made of the device name and the batch shape. */
 ee_status ee_kopt_batch_autotune(ee_kopt_batch* batch, uint32_t launches_per_candidate,
                                  const char* cache_path, ee_kopt_tuning* chosen);
-/* Enqueues `launches` back-to-back evaluations of the whole batch on the
- * context's stream and waits; *device_ms receives the mean device time of one
- * launch measured with HIP events on that stream.  use_graph = 1 replays a
- * captured HIP graph of one launch instead of launching directly. */
-ee_status ee_kopt_batch_run(ee_kopt_batch* batch, uint32_t launches, int use_graph,
-                            double* device_ms);
+/* Enqueues `launches` back-to-back evaluations of the whole batch and waits;
+ * *device_ms receives the mean device time of one launch, measured with HIP
+ * events from the first launch's start to the last one's end.  flags:
+ * EE_KOPT_RUN_GRAPH replays a captured HIP graph of one launch instead of
+ * launching directly; EE_KOPT_RUN_PIPELINED alternates the launches between
+ * the context's stream and a second stream and between the cost pool and a
+ * second pool of its size (cost output only), so that a launch starts while
+ * the previous one drains; every launch evaluates the whole batch, and the
+ * cost pool that ee_kopt_batch_download reads holds the full result
+ * afterwards.  0 = plain launches on the context's stream.  The flags do not
+ * combine. */
+#define EE_KOPT_RUN_GRAPH 1
+#define EE_KOPT_RUN_PIPELINED 2
+ee_status ee_kopt_batch_run(ee_kopt_batch* batch, uint32_t launches, int flags, double* device_ms);
 ee_status ee_kopt_batch_download(ee_kopt_batch* batch, uint64_t first_cell, uint64_t cell_count,
                                  int64_t* costs);
 /* Output mode of later runs: 0 = the int64 cost cells (default); 1 = per

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import _oracle as O
-from cudaedgeelimination_amd import backend, workloads as W
+from cudaedgeelimination_amd import _lib, backend, workloads as W
 from cudaedgeelimination_amd.graph import CEIL_2D, EUC_2D, GraphSnapshot
 from cudaedgeelimination_amd.resident import JvSession, KoptBatch
 
@@ -81,6 +81,26 @@ def test_batch_full_config3_graph_replay(ctx):
         for i, k in enumerate((3, 4, 5)):
             assert sha(b.segment_costs(i)) == G["config3"][f"k{k}"]["sha256"], f"replay {rep}"
         b.run(1, use_graph=False)
+    b.close()
+
+
+def test_batch_pipelined_runs_equal_reference(ctx):
+    """Launches alternating over two streams and two cost pools: the pool that
+    downloads read holds the whole result after 1, 2 and 5 launches, also
+    between ordinary launches; mask output and graph replay refuse it."""
+    say("batch config3 pipelined runs")
+    wl = W.config3()
+    b = single_instance_batch(ctx, wl)
+    for launches in (1, 2, 5):
+        assert b.run(launches, pipelined=True) > 0
+        for i, k in enumerate((3, 4, 5)):
+            assert sha(b.segment_costs(i)) == G["config3"][f"k{k}"]["sha256"], f"{launches} launches k={k}"
+        b.run(1)
+    with pytest.raises(_lib.InvalidArgument):
+        b.run(2, use_graph=True, pipelined=True)
+    b.set_output("masks")
+    with pytest.raises(_lib.InvalidArgument):
+        b.run(2, pipelined=True)
     b.close()
 
 
