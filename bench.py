@@ -31,6 +31,8 @@ from cudaedgeelimination_amd import backend, workloads as W  # noqa: E402
 from cudaedgeelimination_amd.resident import JvSession, KoptBatch, KoptServer  # noqa: E402
 
 METRIC = "exact k-opt cost cells/s + JV edges/s; p50/p99 query latency vs plain GPU path"
+# streams (and cost pools) the timed k-opt launches are dealt over
+PIPELINE_DEPTH = 4
 HBM_PEAK_GBS = 8000.0
 TEMPLATES = {3: 4, 4: 25, 5: 208}
 
@@ -474,9 +476,10 @@ def main() -> int:
     ctx_sync(ctx.handle)
     wall = time.perf_counter() - t0
     graph_ms = batch.run(args.steps, use_graph=True)
-    # the headline: launches alternating over two streams and two cost pools,
+    # the headline: launches dealt over four streams and four cost pools,
     # each still a full pass over the batch, so a launch's start (dependent
-    # gathers) overlaps the previous one's drain
+    # gathers) overlaps the previous ones' drain
+    batch.set_pipeline_depth(PIPELINE_DEPTH)
     batch.run(max(args.warmup, 1), pipelined=True)
     ctx_sync(ctx.handle)
     t0 = time.perf_counter()
@@ -534,7 +537,7 @@ def main() -> int:
                         "k3/k4/k5=32768/19661/13107, 3348853 int64 cells, EUC_2D",
             "rows": int(len(wl.tasks)), "cells": cells, "nodes": wl.graph.dimension,
             "tuning": tuning, "launches_per_step": 1,
-            "streams": 2, "output_pools": 2,
+            "streams": PIPELINE_DEPTH, "output_pools": PIPELINE_DEPTH,
         },
         # the same steps launched back to back on one stream
         "serial": {"value": cells / (wall / args.steps), "ms_per_step": wall * 1e3 / args.steps,

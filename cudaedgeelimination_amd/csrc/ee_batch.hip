@@ -24,8 +24,10 @@ struct Batch {
   explicit Batch(Context* c) : ctx(c) {}
   ~Batch() {
     DropGraph();
-    if (ev_join != nullptr) static_cast<void>(hipEventDestroy(ev_join));
-    if (stream2 != nullptr) static_cast<void>(hipStreamDestroy(stream2));
+    for (PipeLane& l : lanes) {
+      if (l.ev_join != nullptr) static_cast<void>(hipEventDestroy(l.ev_join));
+      if (l.stream != nullptr) static_cast<void>(hipStreamDestroy(l.stream));
+    }
   }
 
   Context* ctx;
@@ -61,12 +63,16 @@ struct Batch {
   bool args_valid = false;
   uint32_t grid = 0;
   size_t shared = 0;
-  // pipelined runs: launches alternate between the context's stream and a
-  // second one, and between the cost pool and a second pool of its size, so
-  // the next batch's launch can start while the previous one drains
-  hipStream_t stream2 = nullptr;
-  hipEvent_t ev_join = nullptr;
-  DeviceArray<int64_t> d_costs2;
+  // pipelined runs: launch i goes to lane i % depth, lane 0 being the
+  // context's stream and cost pool and every other lane a stream and a pool
+  // of its own, so a batch's launch can start while the previous one drains
+  struct PipeLane {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_join = nullptr;
+    DeviceArray<int64_t> costs;
+  };
+  uint32_t pipeline_depth = 2;
+  std::vector<PipeLane> lanes;  // lanes 1 .. depth - 1
 
   void DropGraph() {
     if (exec != nullptr) static_cast<void>(hipGraphExecDestroy(exec));
@@ -189,24 +195,32 @@ struct Batch {
 
   double TimePipelined(uint32_t launches) {
     if (mask_output) throw std::invalid_argument("pipelined runs write cost cells only");
-    if (stream2 == nullptr) {
-      CheckHip(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking), "second stream");
-      CheckHip(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming), "join event");
+    if (lanes.size() + 1 < pipeline_depth) lanes.resize(pipeline_depth - 1);
+    std::vector<KoptLaunch> lane_args(pipeline_depth, host_args);
+    for (uint32_t j = 1; j < pipeline_depth; ++j) {
+      PipeLane& l = lanes[j - 1];
+      if (l.stream == nullptr) {
+        CheckHip(hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking), "pipeline stream");
+        CheckHip(hipEventCreateWithFlags(&l.ev_join, hipEventDisableTiming), "join event");
+      }
+      if (l.costs.count() != d_costs.count()) l.costs.Allocate(d_costs.count(), ctx->device);
+      // the kernel writes through the pointer arguments, so the device copy
+      // of the launch description serves every pool
+      lane_args[j].costs = l.costs.get();
     }
-    if (d_costs2.count() != d_costs.count()) d_costs2.Allocate(d_costs.count(), ctx->device);
-    // the kernel writes through the pointer arguments, so the device copy of
-    // the launch description serves both pools
-    KoptLaunch second = host_args;
-    second.costs = d_costs2.get();
     CheckHip(hipEventRecord(ctx->ev_begin, ctx->stream), "event record");
-    CheckHip(hipStreamWaitEvent(stream2, ctx->ev_begin, 0), "stream wait");
-    for (uint32_t i = 0; i < launches; ++i) {
-      const bool odd = (i & 1u) != 0;
-      LaunchKoptResident(odd ? second : host_args, d_args.get(), grid, tuning, shared,
-                         odd ? stream2 : ctx->stream);
+    for (uint32_t j = 1; j < pipeline_depth; ++j) {
+      CheckHip(hipStreamWaitEvent(lanes[j - 1].stream, ctx->ev_begin, 0), "stream wait");
     }
-    CheckHip(hipEventRecord(ev_join, stream2), "event record");
-    CheckHip(hipStreamWaitEvent(ctx->stream, ev_join, 0), "stream wait");
+    for (uint32_t i = 0; i < launches; ++i) {
+      const uint32_t j = i % pipeline_depth;
+      LaunchKoptResident(lane_args[j], d_args.get(), grid, tuning, shared,
+                         j == 0 ? ctx->stream : lanes[j - 1].stream);
+    }
+    for (uint32_t j = 1; j < pipeline_depth; ++j) {
+      CheckHip(hipEventRecord(lanes[j - 1].ev_join, lanes[j - 1].stream), "event record");
+      CheckHip(hipStreamWaitEvent(ctx->stream, lanes[j - 1].ev_join, 0), "stream wait");
+    }
     CheckHip(hipEventRecord(ctx->ev_end, ctx->stream), "event record");
     CheckHip(hipEventSynchronize(ctx->ev_end), "event synchronize");
     float ms = 0.0f;
@@ -215,8 +229,10 @@ struct Batch {
   }
 
   uint64_t ResidentBytes() const {
-    return d_xy.bytes() + d_tasks.bytes() + d_costs.bytes() + d_costs2.bytes() + d_masks.bytes() +
-           d_tiles.bytes() + d_plans.bytes();
+    uint64_t pools = 0;
+    for (const PipeLane& l : lanes) pools += l.costs.bytes();
+    return d_xy.bytes() + d_tasks.bytes() + d_costs.bytes() + pools + d_masks.bytes() + d_tiles.bytes() +
+           d_plans.bytes();
   }
 };
 
@@ -584,6 +600,13 @@ ee_status ee_kopt_batch_run(ee_kopt_batch* batch, uint32_t launches, int flags, 
     const double ms = ee::AsBatch(batch).Time(launches, (flags & EE_KOPT_RUN_GRAPH) != 0,
                                                (flags & EE_KOPT_RUN_PIPELINED) != 0);
     if (device_ms) *device_ms = ms;
+  });
+}
+
+ee_status ee_kopt_batch_set_pipeline_depth(ee_kopt_batch* batch, uint32_t depth) {
+  return Guarded([&] {
+    if (depth < 2 || depth > 4) throw std::invalid_argument("pipeline depth must be in [2, 4]");
+    ee::AsBatch(batch).pipeline_depth = depth;
   });
 }
 

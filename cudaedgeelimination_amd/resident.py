@@ -99,6 +99,10 @@ class KoptBatch:
         _lib.check(_lib.lib().ee_kopt_batch_run(self._h, launches, flags, C.byref(ms)))
         return ms.value
 
+    def set_pipeline_depth(self, depth: int) -> None:
+        """Streams and cost pools of pipelined runs (2 to 4)."""
+        _lib.check(_lib.lib().ee_kopt_batch_set_pipeline_depth(self._h, depth))
+
     def download(self, first: int = 0, count: int | None = None) -> np.ndarray:
         count = self.pool_cells - first if count is None else count
         out = np.empty(count, np.int64)

@@ -223,16 +223,20 @@ ee_status ee_kopt_batch_autotune(ee_kopt_batch* batch, uint32_t launches_per_can
  * *device_ms receives the mean device time of one launch, measured with HIP
  * events from the first launch's start to the last one's end.  flags:
  * EE_KOPT_RUN_GRAPH replays a captured HIP graph of one launch instead of
- * launching directly; EE_KOPT_RUN_PIPELINED alternates the launches between
- * the context's stream and a second stream and between the cost pool and a
- * second pool of its size (cost output only), so that a launch starts while
- * the previous one drains; every launch evaluates the whole batch, and the
- * cost pool that ee_kopt_batch_download reads holds the full result
- * afterwards.  0 = plain launches on the context's stream.  The flags do not
- * combine. */
+ * launching directly; EE_KOPT_RUN_PIPELINED deals the launches round-robin
+ * over the context's stream and further streams, each writing its own cost
+ * pool of the same size (cost output only; ee_kopt_batch_set_pipeline_depth),
+ * so that a launch starts while the previous one drains; every launch
+ * evaluates the whole batch, and the cost pool that ee_kopt_batch_download
+ * reads holds the full result afterwards.  0 = plain launches on the
+ * context's stream.  The flags do not combine. */
 #define EE_KOPT_RUN_GRAPH 1
 #define EE_KOPT_RUN_PIPELINED 2
 ee_status ee_kopt_batch_run(ee_kopt_batch* batch, uint32_t launches, int flags, double* device_ms);
+/* Streams (and cost pools) of pipelined runs, 2 to 4 (default 2); each pool
+ * beyond the first is allocated on the first pipelined run that needs it.
+ * No reference counterpart. */
+ee_status ee_kopt_batch_set_pipeline_depth(ee_kopt_batch* batch, uint32_t depth);
 ee_status ee_kopt_batch_download(ee_kopt_batch* batch, uint64_t first_cell, uint64_t cell_count,
                                  int64_t* costs);
 /* Output mode of later runs: 0 = the int64 cost cells (default); 1 = per

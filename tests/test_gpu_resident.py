@@ -85,17 +85,23 @@ def test_batch_full_config3_graph_replay(ctx):
 
 
 def test_batch_pipelined_runs_equal_reference(ctx):
-    """Launches alternating over two streams and two cost pools: the pool that
+    """Launches dealt over 2 to 4 streams and cost pools: the pool that
     downloads read holds the whole result after 1, 2 and 5 launches, also
     between ordinary launches; mask output and graph replay refuse it."""
     say("batch config3 pipelined runs")
     wl = W.config3()
     b = single_instance_batch(ctx, wl)
-    for launches in (1, 2, 5):
-        assert b.run(launches, pipelined=True) > 0
-        for i, k in enumerate((3, 4, 5)):
-            assert sha(b.segment_costs(i)) == G["config3"][f"k{k}"]["sha256"], f"{launches} launches k={k}"
-        b.run(1)
+    for depth in (2, 3, 4):
+        b.set_pipeline_depth(depth)
+        for launches in (1, 2, 5):
+            assert b.run(launches, pipelined=True) > 0
+            for i, k in enumerate((3, 4, 5)):
+                assert sha(b.segment_costs(i)) == G["config3"][f"k{k}"]["sha256"], \
+                    f"depth {depth}, {launches} launches, k={k}"
+            b.run(1)
+    for bad in (1, 5):
+        with pytest.raises(_lib.InvalidArgument):
+            b.set_pipeline_depth(bad)
     with pytest.raises(_lib.InvalidArgument):
         b.run(2, use_graph=True, pipelined=True)
     b.set_output("masks")
