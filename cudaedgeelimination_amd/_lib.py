@@ -184,6 +184,7 @@ PROTOTYPES = {
     "ee_kopt_batch_set_tuning": (C.c_int, [P, C.POINTER(KoptTuning)]),
     "ee_kopt_batch_get_tuning": (C.c_int, [P, C.POINTER(KoptTuning)]),
     "ee_kopt_batch_autotune": (C.c_int, [P, U32, C.c_char_p, C.POINTER(KoptTuning)]),
+    "ee_kopt_batch_autotune_runs": (C.c_int, [P, U32, C.c_int, C.c_char_p, C.POINTER(KoptTuning)]),
     "ee_kopt_batch_run": (C.c_int, [P, U32, C.c_int, C.POINTER(DBL)]),
     "ee_kopt_batch_set_pipeline_depth": (C.c_int, [P, U32]),
     "ee_kopt_batch_download": (C.c_int, [P, U64, U64, P]),

@@ -281,7 +281,7 @@ ee_kopt_tuning Export(const KoptTuning& t) {
   return o;
 }
 
-std::string TuningKey(const Batch& b) {
+std::string TuningKey(const Batch& b, bool pipelined) {
   hipDeviceProp_t prop{};
   static_cast<void>(hipGetDeviceProperties(&prop, b.ctx->device));
   uint64_t rows[3] = {0, 0, 0};
@@ -294,6 +294,7 @@ std::string TuningKey(const Batch& b) {
     while ((uint64_t{1} << bucket) < r) ++bucket;
     key << '-' << (r == 0 ? -1 : bucket);
   }
+  if (pipelined) key << "/pipelined" << b.pipeline_depth;
   std::string s = key.str();
   std::replace(s.begin(), s.end(), ' ', '_');
   return s;
@@ -341,7 +342,7 @@ void StoreCached(const char* path, const std::string& key, const ee_kopt_tuning&
 // rows of the largest k present and the grid (small tiles pay only with a
 // wavefront per tile, so the two move together); every candidate is timed on
 // the batch itself, best of three runs of `launches` launches.
-ee_kopt_tuning Autotune(Batch& b, uint32_t launches) {
+ee_kopt_tuning Autotune(Batch& b, uint32_t launches, bool pipelined) {
   ee_kopt_tuning best = Export(b.tuning);
   const auto measure = [&](const ee_kopt_tuning& t) -> double {
     try {
@@ -350,8 +351,8 @@ ee_kopt_tuning Autotune(Batch& b, uint32_t launches) {
       return 1e30;  // does not fit LDS
     }
     b.Time(1, false);  // warm
-    double ms = b.Time(launches, false);
-    for (int rep = 0; rep < 2; ++rep) ms = std::min(ms, b.Time(launches, false));
+    double ms = b.Time(launches, false, pipelined);
+    for (int rep = 0; rep < 2; ++rep) ms = std::min(ms, b.Time(launches, false, pipelined));
     return ms;
   };
   double best_ms = measure(best);
@@ -576,20 +577,27 @@ ee_status ee_kopt_batch_get_tuning(const ee_kopt_batch* batch, ee_kopt_tuning* t
   });
 }
 
-ee_status ee_kopt_batch_autotune(ee_kopt_batch* batch, uint32_t launches_per_candidate,
-                                 const char* cache_path, ee_kopt_tuning* chosen) {
+ee_status ee_kopt_batch_autotune_runs(ee_kopt_batch* batch, uint32_t launches_per_candidate, int run_flags,
+                                      const char* cache_path, ee_kopt_tuning* chosen) {
   return Guarded([&] {
+    if ((run_flags & ~EE_KOPT_RUN_PIPELINED) != 0) throw std::invalid_argument("autotune times plain or pipelined runs");
     ee::Batch& b = ee::AsBatch(batch);
-    const std::string key = ee::TuningKey(b);
+    const bool pipelined = (run_flags & EE_KOPT_RUN_PIPELINED) != 0;
+    const std::string key = ee::TuningKey(b, pipelined);
     ee_kopt_tuning t{};
     if (cache_path != nullptr && ee::LoadCached(cache_path, key, &t)) {
       ee::ApplyTuning(b, t);
     } else {
-      t = ee::Autotune(b, std::max<uint32_t>(launches_per_candidate, 1));
+      t = ee::Autotune(b, std::max<uint32_t>(launches_per_candidate, pipelined ? 2 * b.pipeline_depth : 1), pipelined);
       if (cache_path != nullptr) ee::StoreCached(cache_path, key, t);
     }
     if (chosen) *chosen = t;
   });
+}
+
+ee_status ee_kopt_batch_autotune(ee_kopt_batch* batch, uint32_t launches_per_candidate,
+                                 const char* cache_path, ee_kopt_tuning* chosen) {
+  return ee_kopt_batch_autotune_runs(batch, launches_per_candidate, 0, cache_path, chosen);
 }
 
 ee_status ee_kopt_batch_run(ee_kopt_batch* batch, uint32_t launches, int flags, double* device_ms) {

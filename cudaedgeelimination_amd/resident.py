@@ -83,17 +83,21 @@ class KoptBatch:
         t.grid, t.block, t.non_temporal = grid, block, non_temporal
         _lib.check(_lib.lib().ee_kopt_batch_set_tuning(self._h, C.byref(t)))
 
-    def autotune(self, launches_per_candidate: int = 10, cache_path: str | None = None) -> dict:
+    def autotune(self, launches_per_candidate: int = 10, cache_path: str | None = None,
+                 pipelined: bool = False) -> dict:
+        """Times tuning candidates on the batch and keeps the fastest;
+        pipelined: in pipelined runs at the batch's pipeline depth."""
         t = _lib.KoptTuning()
-        _lib.check(_lib.lib().ee_kopt_batch_autotune(
-            self._h, launches_per_candidate, cache_path.encode() if cache_path else None,
-            C.byref(t)))
+        _lib.check(_lib.lib().ee_kopt_batch_autotune_runs(
+            self._h, launches_per_candidate, _lib.KOPT_RUN_PIPELINED if pipelined else 0,
+            cache_path.encode() if cache_path else None, C.byref(t)))
         return t.as_dict()
 
     def run(self, launches: int = 1, use_graph: bool = False, pipelined: bool = False) -> float:
         """Mean device milliseconds of one launch (HIP events).  pipelined:
-        the launches alternate between two streams and two cost pools, so a
-        launch starts while the previous one drains (cost output only)."""
+        the launches are dealt over the pipeline depth's streams and cost
+        pools, so a launch starts while the previous one drains (cost output
+        only)."""
         ms = C.c_double()
         flags = (_lib.KOPT_RUN_GRAPH if use_graph else 0) | (_lib.KOPT_RUN_PIPELINED if pipelined else 0)
         _lib.check(_lib.lib().ee_kopt_batch_run(self._h, launches, flags, C.byref(ms)))

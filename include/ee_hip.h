@@ -219,6 +219,12 @@ ee_status ee_kopt_batch_get_tuning(const ee_kopt_batch* batch, ee_kopt_tuning* t
  * it to that file under a key made of the device name and the batch shape. */
 ee_status ee_kopt_batch_autotune(ee_kopt_batch* batch, uint32_t launches_per_candidate,
                                  const char* cache_path, ee_kopt_tuning* chosen);
+/* The same, timing each candidate in the run mode of `run_flags`: 0 (plain
+ * launches, as ee_kopt_batch_autotune) or EE_KOPT_RUN_PIPELINED (at the
+ * batch's pipeline depth, which is part of the cache key).  No reference
+ * counterpart. */
+ee_status ee_kopt_batch_autotune_runs(ee_kopt_batch* batch, uint32_t launches_per_candidate, int run_flags,
+                                      const char* cache_path, ee_kopt_tuning* chosen);
 /* Enqueues `launches` back-to-back evaluations of the whole batch and waits;
  * *device_ms receives the mean device time of one launch, measured with HIP
  * events from the first launch's start to the last one's end.  flags:

@@ -121,6 +121,15 @@ def test_batch_autotune_and_cache(ctx, config3_small, tmp_path):
         assert sha(b.segment_costs(i)) == G["config3_small"][f"k{k}"]["sha256"]
     b2 = single_instance_batch(ctx, config3_small)
     assert b2.autotune(3, str(cache)) == chosen  # read back, no re-tuning
+    # pipelined runs are tuned and cached under their own key
+    b.set_pipeline_depth(3)
+    piped = b.autotune(6, str(cache), pipelined=True)
+    assert len(cache.read_text().strip().splitlines()) == 2
+    b.run(3, pipelined=True)
+    for i, k in enumerate((3, 4, 5)):
+        assert sha(b.segment_costs(i)) == G["config3_small"][f"k{k}"]["sha256"]
+    b2.set_pipeline_depth(3)
+    assert b2.autotune(6, str(cache), pipelined=True) == piped
     b.close()
     b2.close()
 
