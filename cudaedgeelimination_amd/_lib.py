@@ -104,6 +104,16 @@ class KoptUsage(C.Structure):
     ]
 
 
+class TourView(C.Structure):
+    _fields_ = [
+        ("tour", C.c_void_p),
+        ("edge_costs", C.c_void_p),
+        ("selectable", C.c_void_p),
+        ("tour_length", C.c_uint32),
+        ("selectable_count", C.c_uint32),
+    ]
+
+
 class SegmentDesc(C.Structure):
     _fields_ = [("instance", C.c_uint32), ("k", C.c_uint32), ("task_count", C.c_uint64)]
 
@@ -216,6 +226,9 @@ PROTOTYPES = {
     "ee_jv_find_candidates": (C.c_int, [P, C.POINTER(GraphView), P, C.POINTER(JvUsage)]),
     "ee_jv_clear_cache": (C.c_int, [P]),
     "ee_kopt_build_templates": (C.c_int, [U32, P, U32, C.POINTER(U32), C.POINTER(U64)]),
+    "ee_kopt_cursor_count": (C.c_int, [C.POINTER(TourView), U32, C.POINTER(U64)]),
+    "ee_kopt_cursor_eval": (C.c_int, [P, P, P, I32, U8, C.POINTER(TourView), U32, P, U32, U64, U64,
+                                      U32, P, P, P, P, C.POINTER(U32), C.POINTER(KoptUsage)]),
     "ee_kopt_eval_costs": (C.c_int, [P, P, P, I32, U8, U32, P, U32, U64, P, SZ, P,
                                      C.POINTER(KoptUsage)]),
     "ee_kopt_clear_cache": (C.c_int, [P]),

@@ -190,5 +190,129 @@ def evaluate_kopt_template_costs_cuda(graph: GraphSnapshot, table: KOptReconnect
         int(usage.resident_bytes), usage.h2d_ms, usage.kernel_ms, usage.d2h_ms)
 
 
+@dataclasses.dataclass
+class TourContext:
+    """The TourContext fields the reference's k-opt cost cursor reads
+    (src/cpu/kopt_search.cpp:125-134): the tour, the exact cost of the edge
+    leaving each position (-1 off the paths) and the selectable positions
+    (path-edge positions, the required edge's first)."""
+    tour: np.ndarray        # int32 [L]
+    edge_costs: np.ndarray  # int64 [L]
+    selectable: np.ndarray  # uint64 [S]
+
+    def __post_init__(self):
+        self.tour = np.ascontiguousarray(self.tour, dtype=np.int32)
+        self.edge_costs = np.ascontiguousarray(self.edge_costs, dtype=np.int64)
+        self.selectable = np.ascontiguousarray(self.selectable, dtype=np.uint64)
+
+    def view(self) -> _lib.TourView:
+        return _lib.TourView(_lib.ptr(self.tour), _lib.ptr(self.edge_costs),
+                             _lib.ptr(self.selectable) if len(self.selectable) else None,
+                             len(self.tour), len(self.selectable))
+
+
+def kopt_cursor_count(tour: TourContext, k: int) -> int:
+    """Deletion sets of size k the reference cursor enumerates:
+    C(|selectable| - 1, k - 1) for 3 <= k <= 5 and k < |selectable|."""
+    out = C.c_uint64()
+    view = tour.view()
+    _lib.check(_lib.lib().ee_kopt_cursor_count(C.byref(view), k, C.byref(out)))
+    return int(out.value)
+
+
+@dataclasses.dataclass
+class KOptCursorBlock:
+    """One block of KOptSearchCursor::PrepareNextBlock
+    (src/cpu/kopt_search.cpp:2123-2156) with its evaluated costs."""
+    k: int
+    first: int               # rank of the block's first deletion set within k
+    tasks: np.ndarray        # KOPT_TASK_DTYPE [rows]
+    positions: np.ndarray    # uint32 [rows, k], each row's sorted deleted positions
+    costs: np.ndarray | None  # int64 [rows * T] (output "costs")
+    masks: np.ndarray | None  # uint32 [rows, ceil(T / 32)] (output "masks")
+    usage: KOptCudaCacheUsage
+
+
+def evaluate_kopt_cursor_cuda(graph: GraphSnapshot | None, table: KOptReconnectTable | None,
+                              tour: TourContext, k: int, first: int, count: int,
+                              output: str | None = "costs",
+                              ctx: Context | None = None) -> KOptCursorBlock:
+    """Deletion sets [first, first + count) of size k built and evaluated on
+    the device (ee_kopt_cursor_eval).  output: "costs", "masks" or None (the
+    rows only; graph may then be a node count holder and table None)."""
+    ctx = _ctx(ctx)
+    view = tour.view()
+    total = kopt_cursor_count(tour, k)
+    rows = max(0, min(count, total - first))
+    T = len(table.templates) if table is not None else 0
+    tasks = np.zeros(max(rows, 1), dtype=_lib.KOPT_TASK_DTYPE)
+    positions = np.zeros((max(rows, 1), k), dtype=np.uint32)
+    costs = masks = None
+    if output == "costs":
+        costs = np.empty(max(rows * T, 1), dtype=np.int64)
+    elif output == "masks":
+        masks = np.zeros((max(rows, 1), (T + 31) // 32), dtype=np.uint32)
+    elif output is not None:
+        raise ValueError("output must be 'costs', 'masks' or None")
+    templates = (np.ascontiguousarray(table.templates, dtype=_lib.MATCHING_DTYPE)
+                 if table is not None else None)
+    produced = C.c_uint32()
+    usage = _lib.KoptUsage()
+    _lib.check(_lib.lib().ee_kopt_cursor_eval(
+        ctx.handle, _lib.ptr(graph.x) if output else None, _lib.ptr(graph.y) if output else None,
+        graph.dimension, graph.distance_type if output else 0, C.byref(view), k,
+        _lib.ptr(templates) if templates is not None else None, T,
+        table.generator_hash if table is not None else 0, first, count,
+        _lib.ptr(tasks), _lib.ptr(positions),
+        _lib.ptr(costs) if costs is not None else None,
+        _lib.ptr(masks) if masks is not None else None, C.byref(produced), C.byref(usage)))
+    n = produced.value
+    return KOptCursorBlock(
+        k, first, tasks[:n], positions[:n], costs[:n * T] if costs is not None else None,
+        masks[:n] if masks is not None else None,
+        KOptCudaCacheUsage(bool(usage.snapshot_hit), bool(usage.template_hit),
+                           bool(usage.workspace_hit), int(usage.resident_bytes), usage.h2d_ms,
+                           usage.kernel_ms, usage.d2h_ms))
+
+
+class KOptCostCursor:
+    """The block sequence of the reference's k-opt cost cursor
+    (KOptSearchCursor, src/cpu/kopt_search.cpp:2098-2156, 2258-2271) with each
+    block's task rows built and evaluated on the device: for k = 3 .. max_k
+    (k < |selectable|), blocks of at most cost_batch_size deletion sets, a
+    block never spanning two k; max_deletion_sets > 0 caps the sets over all
+    blocks, as the cursor's budget does."""
+
+    def __init__(self, graph: GraphSnapshot, tour: TourContext, max_k: int = 5,
+                 cost_batch_size: int = 4096, max_deletion_sets: int = 0,
+                 output: str = "costs", ctx: Context | None = None):
+        if not 3 <= max_k <= 5 or cost_batch_size <= 0:
+            raise ValueError("max_k must lie in [3, 5] and cost_batch_size must be positive")
+        self.graph, self.tour, self.max_k = graph, tour, max_k
+        self.batch, self.budget, self.output, self.ctx = (cost_batch_size, max_deletion_sets,
+                                                          output, ctx)
+        self.tables = {k: build_kopt_reconnect_table(k) for k in range(3, max_k + 1)}
+        self.deletion_sets = 0
+        self.budget_exhausted = False
+
+    def __iter__(self):
+        for k in range(3, self.max_k + 1):
+            total = kopt_cursor_count(self.tour, k)
+            first = 0
+            while first < total:
+                rows = min(self.batch, total - first)
+                if self.budget:
+                    left = self.budget - self.deletion_sets
+                    if left <= 0:
+                        self.budget_exhausted = True
+                        return
+                    rows = min(rows, left)
+                block = evaluate_kopt_cursor_cuda(self.graph, self.tables[k], self.tour, k, first,
+                                                  rows, self.output, self.ctx)
+                self.deletion_sets += len(block.tasks)
+                first += len(block.tasks)
+                yield block
+
+
 def clear_kopt_cost_cuda_cache(ctx: Context | None = None) -> None:
     _lib.check(_lib.lib().ee_kopt_clear_cache(_ctx(ctx).handle))

@@ -7,6 +7,7 @@
 #include <array>
 #include <chrono>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -195,6 +196,20 @@ struct KoptCache {
   DeviceArray<KoptTile> d_tiles;
 };
 
+// Device copy of one tour context for the deletion-set producer, kept while
+// the caller passes the same tour, edge costs and selectable positions.
+struct KoptCursorCache {
+  bool valid = false;
+  std::vector<int32_t> tour;
+  std::vector<int64_t> edge_costs;
+  std::vector<uint64_t> selectable;
+  DeviceArray<int32_t> d_tour;
+  DeviceArray<int64_t> d_edge_costs;
+  DeviceArray<uint64_t> d_selectable;
+  DeviceArray<uint32_t> d_positions;
+  DeviceArray<uint32_t> d_status;  // bit 0: some deleted cost overflowed int64
+};
+
 struct Context {
   explicit Context(int requested_device);
   ~Context();
@@ -212,11 +227,26 @@ struct Context {
   KoptTuning kopt_tuning;
   JvCache jv;
   KoptCache kopt;
+  KoptCursorCache cursor;
 };
 
 std::vector<ee_endpoint_matching> CanonicalTemplates(uint32_t k, uint64_t* hash);
 KoptPlan BuildPlan(uint32_t k, const ee_endpoint_matching* templates, uint32_t count);
 void ValidateTasks(int32_t n, uint32_t k, const ee_kopt_task* tasks, size_t count);
 
+// Writes the task rows of one plain evaluation into the device task pool
+// (`d_tasks`, room for the evaluation's rows), enqueued on the context's
+// stream.  The rows must pass ValidateTasks.
+using KoptTaskSource = std::function<void(ee_kopt_task* d_tasks)>;
+
+// The plain synchronous k-opt evaluation (snapshot, template and workspace
+// caches, one launch, download) with its task rows taken from `fill_tasks`.
+// Exactly one of `costs` (task_count x template_count cells) and `masks`
+// (task_count x ceil(template_count / 32) improving-template words) is
+// non-null.
+void EvalKoptCostsFrom(Context& ctx, const int64_t* x, const int64_t* y, int32_t n, uint8_t type,
+                       uint32_t k, const ee_endpoint_matching* templates, uint32_t template_count,
+                       uint64_t generator_hash, size_t task_count, const KoptTaskSource& fill_tasks,
+                       int64_t* costs, uint32_t* masks, ee_kopt_usage* usage);
 
 }  // namespace ee

@@ -520,6 +520,26 @@ void EvalKoptCosts(Context& ctx, const int64_t* x, const int64_t* y, int32_t n, 
                    uint32_t k, const ee_endpoint_matching* templates, uint32_t template_count,
                    uint64_t generator_hash, const ee_kopt_task* tasks, size_t task_count,
                    int64_t* costs, ee_kopt_usage* usage) {
+  ctx.Activate();
+  if (task_count == 0) {
+    if (usage) *usage = ee_kopt_usage{};
+    return;
+  }
+  if (tasks == nullptr || costs == nullptr) throw std::invalid_argument("HIP k-opt NULL buffer");
+  ValidateTasks(n, k, tasks, task_count);
+  const KoptTaskSource upload = [&](ee_kopt_task* d_tasks) {
+    CheckHip(hipMemcpyAsync(d_tasks, tasks, task_count * sizeof(ee_kopt_task), hipMemcpyHostToDevice,
+                            ctx.stream),
+             "hipMemcpyAsync tasks");
+  };
+  EvalKoptCostsFrom(ctx, x, y, n, type, k, templates, template_count, generator_hash, task_count,
+                    upload, costs, nullptr, usage);
+}
+
+void EvalKoptCostsFrom(Context& ctx, const int64_t* x, const int64_t* y, int32_t n, uint8_t type,
+                       uint32_t k, const ee_endpoint_matching* templates, uint32_t template_count,
+                       uint64_t generator_hash, size_t task_count, const KoptTaskSource& fill_tasks,
+                       int64_t* costs, uint32_t* masks, ee_kopt_usage* usage) {
   ee_kopt_usage u{};
   ctx.Activate();
   if (task_count == 0) {
@@ -532,10 +552,12 @@ void EvalKoptCosts(Context& ctx, const int64_t* x, const int64_t* y, int32_t n, 
   if (x == nullptr || y == nullptr || n <= 0 || (type != EE_EUC_2D && type != EE_CEIL_2D)) {
     throw std::invalid_argument("HIP k-opt coordinates are invalid");
   }
-  if (tasks == nullptr || costs == nullptr) throw std::invalid_argument("HIP k-opt NULL buffer");
+  if ((costs == nullptr) == (masks == nullptr)) {
+    throw std::invalid_argument("HIP k-opt needs exactly one of costs and masks");
+  }
   if (task_count > SIZE_MAX / template_count) throw std::overflow_error("k-opt cell count overflows");
   const size_t cells = task_count * template_count;
-  ValidateTasks(n, k, tasks, task_count);
+  const uint32_t mask_words = (template_count + 31) / 32;
   KoptCache& c = ctx.kopt;
 
   // snapshot
@@ -577,9 +599,11 @@ void EvalKoptCosts(Context& ctx, const int64_t* x, const int64_t* y, int32_t n, 
   }
 
   // workspace
-  const bool workspace_hit = c.d_tasks.count() >= task_count && c.d_costs.count() >= cells;
+  // mask output needs task_count x mask_words words, held in the cost pool
+  const size_t out_cells = masks ? (task_count * mask_words + 1) / 2 : cells;
+  const bool workspace_hit = c.d_tasks.count() >= task_count && c.d_costs.count() >= out_cells;
   if (c.d_tasks.count() < task_count) c.d_tasks.Allocate(Grow(c.d_tasks.count(), task_count), ctx.device);
-  if (c.d_costs.count() < cells) c.d_costs.Allocate(Grow(c.d_costs.count(), cells), ctx.device);
+  if (c.d_costs.count() < out_cells) c.d_costs.Allocate(Grow(c.d_costs.count(), out_cells), ctx.device);
 
   KoptInstance inst{};
   inst.n = static_cast<uint32_t>(n);
@@ -594,7 +618,7 @@ void EvalKoptCosts(Context& ctx, const int64_t* x, const int64_t* y, int32_t n, 
   const bool descriptors = a.inline_count == 0;
   if (descriptors && c.d_tiles.count() < n_tiles) c.d_tiles.Allocate(Grow(c.d_tiles.count(), n_tiles), ctx.device);
   auto t0 = std::chrono::steady_clock::now();
-  c.d_tasks.Upload(tasks, task_count, ctx.stream);
+  fill_tasks(c.d_tasks.get());
   if (descriptors) c.d_tiles.Upload(tiles.data(), n_tiles, ctx.stream);
   ctx.Sync();
   u.h2d_ms = MsSince(t0);
@@ -604,6 +628,10 @@ void EvalKoptCosts(Context& ctx, const int64_t* x, const int64_t* y, int32_t n, 
   a.tasks = c.d_tasks.get();
   a.plans = c.d_plans.get();
   a.costs = c.d_costs.get();
+  if (masks) {
+    a.masks = reinterpret_cast<uint32_t*>(c.d_costs.get());
+    a.mask_words = mask_words;
+  }
   bool present[3] = {false, false, false};
   present[k - 3] = true;
   const size_t shared = KoptPrepareLayout(&a, c.host_plans.data(), present, ctx.kopt_tuning, c.near);
@@ -613,7 +641,13 @@ void EvalKoptCosts(Context& ctx, const int64_t* x, const int64_t* y, int32_t n, 
   u.kernel_ms = MsSince(t0);
 
   t0 = std::chrono::steady_clock::now();
-  c.d_costs.Download(costs, cells, ctx.stream);
+  if (masks) {
+    CheckHip(hipMemcpyAsync(masks, c.d_costs.get(), task_count * mask_words * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, ctx.stream),
+             "hipMemcpyAsync masks");
+  } else {
+    c.d_costs.Download(costs, cells, ctx.stream);
+  }
   ctx.Sync();
   u.d2h_ms = MsSince(t0);
   u.snapshot_hit = snapshot_hit ? 1 : 0;

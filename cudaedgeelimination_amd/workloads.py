@@ -22,7 +22,7 @@ import math
 import numpy as np
 
 from . import _lib
-from .graph import CEIL_2D, EUC_2D, GraphSnapshot
+from .graph import CEIL_2D, EUC_2D, GraphSnapshot, exact_distances
 
 SEEDS = {1: 0xEE000001, 2: 0xEE000002, 3: 0xEE000003, 4: 0xEE000004, 5: 0xEE000005}
 KOPT_QUOTA_FRACTIONS = (0.5, 0.3, 0.2)
@@ -90,6 +90,31 @@ def kopt_tasks(seed: int, graph: GraphSnapshot, ks) -> np.ndarray:
         _lib.ptr(graph.row_offsets), _lib.ptr(graph.neighbors), 0, _lib.ptr(ks), len(ks),
         _lib.ptr(tasks)))
     return tasks
+
+
+def tour_context(seed: int, graph: GraphSnapshot, length: int, selectable: int,
+                 required: bool = True):
+    """A tour context of the shape BuildTourContext makes
+    (src/cpu/kopt_search.cpp:270-350): a cycle over `length` distinct nodes
+    of the snapshot whose `selectable` path-edge positions carry their exact
+    cost (-1 on the other positions, the outside-matching edges), listed in
+    tour order and, with `required`, rotated so that a random one (the
+    required edge) comes first.  numpy's PCG64 stream of `seed` picks the
+    nodes and positions."""
+    from .backend import TourContext
+    if not 3 <= length <= graph.dimension or not 0 <= selectable <= length:
+        raise ValueError("tour_context: 3 <= length <= n and selectable <= length")
+    rng = np.random.default_rng(seed)
+    tour = rng.permutation(graph.dimension)[:length].astype(np.int32)
+    picked = np.sort(rng.permutation(length)[:selectable]).astype(np.uint64)
+    if required and selectable:
+        picked = np.roll(picked, -int(rng.integers(selectable)))
+    nxt = np.roll(tour, -1)
+    costs = np.full(length, -1, dtype=np.int64)
+    at = picked.astype(np.int64)
+    costs[at] = exact_distances(graph.x[tour[at]], graph.y[tour[at]], graph.x[nxt[at]],
+                                graph.y[nxt[at]], graph.distance_type)
+    return TourContext(tour, costs, picked)
 
 
 @dataclasses.dataclass

@@ -173,6 +173,51 @@ ee_status ee_kopt_eval_costs(ee_ctx* ctx, const int64_t* x, const int64_t* y, in
 /* Replaces cudaee::detail::ClearKOptCostCudaCache (src/cuda/kopt_cost.cu:472). */
 ee_status ee_kopt_clear_cache(ee_ctx* ctx);
 
+/* ------------------------------------------- device deletion-set producer */
+/* The TourContext fields the reference's k-opt cost cursor reads
+ * (src/cpu/kopt_search.cpp:125-134): the tour (distinct snapshot nodes),
+ * the exact cost of the edge leaving each tour position (-1 where that
+ * edge is not a path edge), and the selectable positions (the path-edge
+ * positions, with the required edge's first, kopt_search.cpp:318-349). */
+typedef struct {
+  const int32_t* tour;        /* [tour_length] */
+  const int64_t* edge_costs;  /* [tour_length] */
+  const uint64_t* selectable; /* [selectable_count], at most 65536 */
+  uint32_t tour_length;
+  uint32_t selectable_count;
+} ee_tour_view;
+
+/* Number of deletion sets of size k the reference cursor enumerates
+ * (KOptSearchCursor::PrepareNextK, kopt_search.cpp:2258-2271, with
+ * AdvanceCombination, kopt_search.cpp:736-752, which never moves index 0):
+ * C(selectable_count - 1, k - 1) for 3 <= k <= 5 and k < selectable_count,
+ * else 0. */
+ee_status ee_kopt_cursor_count(const ee_tour_view* tour, uint32_t k, uint64_t* count);
+
+/* Replaces the host loop of KOptSearchCursor::PrepareNextBlock
+ * (src/cpu/kopt_search.cpp:2123-2156) and the task upload of
+ * EvaluateKOptTemplateCostsCuda (src/cuda/kopt_cost.cu:449): the task rows
+ * of deletion sets [first, first + count) of size k, in the cursor's order,
+ * are built on the device by BuildKOptCostTask's rule
+ * (kopt_search.cpp:561-584) and evaluated there like ee_kopt_eval_costs.
+ * count is clamped to the sets left; *produced (may be NULL) receives the
+ * rows made.  Outputs (each may be NULL): tasks (the rows), positions (each
+ * row's k sorted tour positions, the cursor's deleted_positions), and at
+ * most one of costs (rows x template_count cells) and masks (rows x
+ * ceil(template_count / 32) words, bit j set iff cell j < the row's
+ * deleted_cost).  With neither costs nor masks only the rows are made, and
+ * x, y, distance_type and the templates are not read.  Fails with
+ * EE_ERR_RUNTIME where a deleted cost sum overflows int64 or a selectable
+ * position's cost is negative or out of the tour, as BuildKOptCostTask
+ * throws; the positions are checked up front, not as the sets reach them.
+ * usage->h2d_ms covers building the rows on the device. */
+ee_status ee_kopt_cursor_eval(ee_ctx* ctx, const int64_t* x, const int64_t* y, int32_t dimension,
+                              uint8_t distance_type, const ee_tour_view* tour, uint32_t k,
+                              const ee_endpoint_matching* templates, uint32_t template_count,
+                              uint64_t generator_hash, uint64_t first, uint32_t count,
+                              ee_kopt_task* tasks, uint32_t* positions, int64_t* costs,
+                              uint32_t* masks, uint32_t* produced, ee_kopt_usage* usage);
+
 /* --------------------------------------------------- resident k-opt batch */
 /* No reference counterpart.  The reference evaluates one (graph, k) matrix
  * per synchronous call with a full upload and download

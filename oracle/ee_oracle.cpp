@@ -1204,4 +1204,66 @@ int orc_kopt_costs(const int64_t* x, const int64_t* y, int32_t dimension, int di
   });
 }
 
+
+int64_t orc_kopt_cursor_tasks(const int32_t* tour, const int64_t* edge_costs, uint32_t tour_length,
+                              const uint64_t* selectable, uint32_t selectable_count, uint32_t k,
+                              uint64_t first, uint64_t count, orc_kopt_task* tasks,
+                              uint64_t* positions) {
+  int64_t written = 0;
+  const int status = Guard([&] {
+    // PrepareNextK (kopt_search.cpp:2258-2271): k in [3, max_k] and k < |selectable|
+    if (k < 3 || k > 5 || k >= selectable_count) return;
+    std::vector<std::size_t> combination(k);
+    for (std::size_t i = 0; i < k; ++i) combination[i] = i;
+    // AdvanceCombination (kopt_search.cpp:736-752), restated: the last index
+    // that can still move moves up by one and the ones after it follow; index 0
+    // never moves.
+    const auto advance = [&]() -> bool {
+      std::size_t pos = combination.size() - 1;
+      while (pos >= 1 && combination[pos] == selectable_count - (combination.size() - pos)) {
+        if (pos == 1) return false;
+        --pos;
+      }
+      ++combination[pos];
+      for (std::size_t next = pos + 1; next < combination.size(); ++next) {
+        combination[next] = combination[next - 1] + 1;
+      }
+      return true;
+    };
+    uint64_t index = 0;
+    bool more = true;
+    while (more && static_cast<uint64_t>(written) < count) {
+      if (index >= first) {
+        std::size_t deleted[5];
+        for (uint32_t e = 0; e < k; ++e) deleted[e] = selectable[combination[e]];
+        std::sort(deleted, deleted + k);
+        // BuildKOptCostTask (kopt_search.cpp:561-584)
+        orc_kopt_task t{};
+        __int128 cost = 0;
+        for (uint32_t e = 0; e < k; ++e) {
+          const std::size_t at = deleted[e];
+          if (at >= tour_length || edge_costs[at] < 0) {
+            throw std::runtime_error("deletion position or cached edge cost is invalid");
+          }
+          t.port_nodes[2 * e] = tour[at];
+          t.port_nodes[2 * e + 1] = tour[(at + 1) % tour_length];
+          cost += edge_costs[at];
+        }
+        if (cost > std::numeric_limits<int64_t>::max()) {
+          throw std::runtime_error("k-opt deleted cost overflows int64_t");
+        }
+        t.deleted_cost = static_cast<int64_t>(cost);
+        tasks[written] = t;
+        if (positions) {
+          for (uint32_t e = 0; e < k; ++e) positions[written * k + e] = deleted[e];
+        }
+        ++written;
+      }
+      ++index;
+      more = advance();
+    }
+  });
+  return status == 0 ? written : status;
+}
+
 }  // extern "C"

@@ -113,6 +113,20 @@ int orc_kopt_costs(const int64_t* x, const int64_t* y, int32_t dimension, int di
                    uint32_t k, const orc_kopt_task* tasks, size_t task_count, int64_t* costs,
                    int max_threads, uint32_t* threads_used, uint32_t* distance_cache_nodes);
 
+/* The deletion sets of KOptSearchCursor (src/cpu/kopt_search.cpp:2123-2156,
+ * 2258-2271): combinations of k selectable positions, advanced by
+ * AdvanceCombination (kopt_search.cpp:736-752), which never moves the first
+ * index, so every set holds selectable[0]; each set's positions sorted and
+ * turned into a task by BuildKOptCostTask (kopt_search.cpp:561-584).
+ * Writes the tasks of sets [first, first + count) and, when positions is not
+ * NULL, their sorted tour positions (count x k).  Returns the number of sets
+ * written (fewer than count where the enumeration ends; 0 when k < 3, k > 5
+ * or k >= selectable_count, as PrepareNextK), negative on failure. */
+int64_t orc_kopt_cursor_tasks(const int32_t* tour, const int64_t* edge_costs, uint32_t tour_length,
+                              const uint64_t* selectable, uint32_t selectable_count, uint32_t k,
+                              uint64_t first, uint64_t count, orc_kopt_task* tasks,
+                              uint64_t* positions);
+
 #ifdef __cplusplus
 }
 #endif

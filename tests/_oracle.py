@@ -45,6 +45,7 @@ _ORACLE = {
     "orc_kopt_templates": (C.c_int, [U32, P, U32, C.POINTER(U32), C.POINTER(U64)]),
     "orc_kopt_costs": (C.c_int, [P, P, I32, C.c_int, U32, P, SZ, P, C.c_int, C.POINTER(U32),
                                  C.POINTER(U32)]),
+    "orc_kopt_cursor_tasks": (I64, [P, P, U32, P, U32, U32, U64, U64, P, P]),
 }
 _REF = {
     "ref_last_error": (C.c_char_p, []),
@@ -245,6 +246,22 @@ def kopt_costs(x, y, dtype, k, tasks, max_threads=8):
                           C.byref(threads), C.byref(nodes)) != 0:
         _raise(lib.orc_last_error)
     return out, threads.value, nodes.value
+
+
+def cursor_tasks(tour, edge_costs, selectable, k, first, count):
+    """Oracle deletion-set cursor: (tasks, positions[rows, k])."""
+    tour = np.ascontiguousarray(tour, np.int32)
+    edge_costs = np.ascontiguousarray(edge_costs, np.int64)
+    selectable = np.ascontiguousarray(selectable, np.uint64)
+    tasks = np.zeros(max(count, 1), TASK_DTYPE)
+    positions = np.zeros((max(count, 1), k), np.uint64)
+    lib = oracle()
+    n = lib.orc_kopt_cursor_tasks(p(tour), p(edge_costs), len(tour),
+                                  p(selectable) if len(selectable) else None, len(selectable), k,
+                                  first, count, p(tasks), p(positions))
+    if n < 0:
+        _raise(lib.orc_last_error)
+    return tasks[:n], positions[:n]
 
 
 def ref_kopt_costs(x, y, dtype, k, tasks):
