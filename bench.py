@@ -12,6 +12,7 @@ Prints one JSON line; see the keys below.
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import hashlib
 import json
 import os
@@ -418,6 +419,72 @@ def measured_traffic():
     return None if v is None else int(v)
 
 
+def cursor_leg(ctx, graph, blocks: int = 40) -> dict:
+    """The reference's leaf search walks a tour context's deletion sets in
+    blocks of cost_batch_size (4096) rows, one synchronous evaluation per
+    block.  Times k = 5 blocks three ways: rows uploaded to the plain path
+    (ee_kopt_eval_costs, the reference-shaped call; the rows are made
+    beforehand and not timed), rows built on the device with the cells
+    downloaded, and rows built on the device with only the improving-template
+    masks downloaded."""
+    tour = W.tour_context(0xEE0000C3, graph, 400, 120)
+    table = backend.build_kopt_reconnect_table(5)
+    total = backend.kopt_cursor_count(tour, 5)
+    rows = 4096
+    firsts = [i * rows for i in range(blocks)]
+    pre = [backend.evaluate_kopt_cursor_cuda(graph, None, tour, 5, f, rows, output=None, ctx=ctx).tasks
+           for f in firsts]
+    cells = rows * len(table.templates)
+
+    # C-ABI calls on preallocated buffers, so the figures are the library's
+    from cudaedgeelimination_amd import _lib
+    lib = _lib.lib()
+    T = len(table.templates)
+    templates = np.ascontiguousarray(table.templates, dtype=_lib.MATCHING_DTYPE)
+    costs = np.empty(rows * T, dtype=np.int64)
+    masks = np.empty(rows * ((T + 31) // 32), dtype=np.uint32)
+    view = tour.view()
+    usage, produced = _lib.KoptUsage(), C.c_uint32()
+    gx, gy = _lib.ptr(graph.x), _lib.ptr(graph.y)
+
+    def plain_call(f, t):
+        _lib.check(lib.ee_kopt_eval_costs(ctx.handle, gx, gy, graph.dimension, graph.distance_type, 5,
+                                          _lib.ptr(templates), T, table.generator_hash, _lib.ptr(t),
+                                          len(t), _lib.ptr(costs), C.byref(usage)))
+
+    def cursor_call(f, t, out_costs, out_masks):
+        _lib.check(lib.ee_kopt_cursor_eval(ctx.handle, gx, gy, graph.dimension, graph.distance_type,
+                                           C.byref(view), 5, _lib.ptr(templates), T,
+                                           table.generator_hash, f, rows, None, None, out_costs,
+                                           out_masks, C.byref(produced), C.byref(usage)))
+
+    def timed(call):
+        call(firsts[0], pre[0])  # warm the caches and workspaces
+        parts = np.zeros(3)
+        t0 = time.perf_counter()
+        for f, tasks in zip(firsts, pre):
+            call(f, tasks)
+            parts += (usage.h2d_ms, usage.kernel_ms, usage.d2h_ms)
+        wall = time.perf_counter() - t0
+        return {"ms_per_block": wall * 1e3 / blocks, "cells_per_s": cells * blocks / wall,
+                "sets_per_s": rows * blocks / wall, "rows_ms": parts[0] / blocks,
+                "kernel_ms": parts[1] / blocks, "download_ms": parts[2] / blocks}
+
+    plain = timed(plain_call)
+    dev_costs = timed(lambda f, t: cursor_call(f, t, _lib.ptr(costs), None))
+    dev_masks = timed(lambda f, t: cursor_call(f, t, None, _lib.ptr(masks)))
+    check = backend.evaluate_kopt_cursor_cuda(graph, table, tour, 5, firsts[-1], rows, "costs", ctx)
+    same = bool(np.array_equal(
+        check.costs, backend.evaluate_kopt_template_costs_cuda(graph, table, pre[-1], ctx)[0]))
+    return {"workload": "tour of 400 nodes of the n=15000 graph, 120 selectable positions, "
+                        f"k=5: {total} deletion sets, {blocks} blocks of {rows} rows",
+            # rows_ms: the rows' upload (plain) or enqueueing their kernel;
+            # kernel_ms: launch to finish (the row kernel included)
+            "plain_uploaded_rows": plain, "device_rows_costs": dev_costs,
+            "device_rows_masks": dev_masks, "cells_equal_plain": same,
+            "masks_speedup_over_plain": plain["ms_per_block"] / dev_masks["ms_per_block"]}
+
+
 def main() -> int:
     # the reference protocol for its OpenMP CPU path (set before it loads)
     for key, value in (("OMP_DYNAMIC", "FALSE"), ("OMP_PROC_BIND", "spread"),
@@ -435,6 +502,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--no-many", action="store_true", help="skip the many-instance workload")
+    ap.add_argument("--no-cursor", action="store_true", help="skip the deletion-set cursor leg")
     ap.add_argument("--serve-qps", default="2000,8000,20000,0",
                     help="Poisson rates of the serving leg; 0 is a burst of --serve-burst requests")
     ap.add_argument("--serve-burst", type=int, default=20000)
@@ -613,6 +681,10 @@ def main() -> int:
             jv["cpu_baseline"] = cpu_jv
             jv["proof_equals_cpu"] = O.canonical_proof(text) == O.canonical_proof(best.proof)
         result["jv"] = jv
+
+    if not args.no_cursor:
+        log("deletion-set cursor blocks (device-built rows)")
+        result["cursor"] = cursor_leg(ctx, wl.graph)
 
     if not args.no_many:
         log("many-instance workload (256 instances: JV fleet + k-opt batch)")

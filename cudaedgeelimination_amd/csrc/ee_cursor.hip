@@ -155,7 +155,12 @@ bool PrepareTour(Context& ctx, const ee_tour_view& t) {
   if (c.d_selectable.count() < std::max<size_t>(S, 1)) {
     c.d_selectable.Allocate(Grow(c.d_selectable.count(), std::max<size_t>(S, 1)), ctx.device);
   }
-  if (c.d_status.count() < 1) c.d_status.Allocate(1, ctx.device);
+  if (c.d_status.count() < 1) {
+    // the flag is cleared here and after a reported overflow, so a launch
+    // finds it clear without a clearing launch of its own
+    c.d_status.Allocate(1, ctx.device);
+    CheckHip(hipMemsetAsync(c.d_status.get(), 0, sizeof(uint32_t), ctx.stream), "hipMemsetAsync status");
+  }
   c.d_tour.Upload(t.tour, L, ctx.stream);
   c.d_edge_costs.Upload(t.edge_costs, L, ctx.stream);
   c.d_selectable.Upload(t.selectable, S, ctx.stream);
@@ -173,7 +178,6 @@ void LaunchCursor(Context& ctx, const ee_tour_view& t, uint32_t k, uint64_t firs
   if (want_positions && c.d_positions.count() < static_cast<size_t>(count) * k) {
     c.d_positions.Allocate(Grow(c.d_positions.count(), static_cast<size_t>(count) * k), ctx.device);
   }
-  CheckHip(hipMemsetAsync(c.d_status.get(), 0, sizeof(uint32_t), ctx.stream), "hipMemsetAsync status");
   const uint32_t blocks = (count + kCursorBlock - 1) / kCursorBlock;
   hipLaunchKernelGGL(KoptCursorKernel, dim3(blocks), dim3(kCursorBlock), 0, ctx.stream,
                      c.d_tour.get(), c.d_edge_costs.get(), c.d_selectable.get(), t.tour_length,
@@ -190,7 +194,11 @@ void FinishCursor(Context& ctx, uint32_t k, uint32_t count, ee_kopt_task* tasks,
   if (tasks != nullptr) ctx.kopt.d_tasks.Download(tasks, count, ctx.stream);
   if (positions != nullptr) c.d_positions.Download(positions, static_cast<size_t>(count) * k, ctx.stream);
   ctx.Sync();
-  if (status & 1u) throw std::runtime_error("k-opt cursor: deleted cost sum overflows int64_t");
+  if (status & 1u) {
+    CheckHip(hipMemsetAsync(c.d_status.get(), 0, sizeof(uint32_t), ctx.stream), "hipMemsetAsync status");
+    ctx.Sync();
+    throw std::runtime_error("k-opt cursor: deleted cost sum overflows int64_t");
+  }
 }
 
 void CursorEval(Context& ctx, const int64_t* x, const int64_t* y, int32_t n, uint8_t type,
@@ -211,16 +219,24 @@ void CursorEval(Context& ctx, const int64_t* x, const int64_t* y, int32_t n, uin
   if (rows == 0) return;
   PrepareTour(ctx, *tour);
   const bool want_positions = positions != nullptr;
-  if (costs == nullptr && masks == nullptr) {  // producer only
-    KoptCache& kc = ctx.kopt;
-    if (kc.d_tasks.count() < rows) kc.d_tasks.Allocate(Grow(kc.d_tasks.count(), rows), ctx.device);
-    LaunchCursor(ctx, *tour, k, first, rows, kc.d_tasks.get(), want_positions);
-  } else {
-    const KoptTaskSource produce = [&](ee_kopt_task* d_tasks) {
-      LaunchCursor(ctx, *tour, k, first, rows, d_tasks, want_positions);
-    };
-    EvalKoptCostsFrom(ctx, x, y, n, type, k, templates, template_count, generator_hash, rows,
-                      produce, costs, masks, usage);
+  try {
+    if (costs == nullptr && masks == nullptr) {  // producer only
+      KoptCache& kc = ctx.kopt;
+      if (kc.d_tasks.count() < rows) kc.d_tasks.Allocate(Grow(kc.d_tasks.count(), rows), ctx.device);
+      LaunchCursor(ctx, *tour, k, first, rows, kc.d_tasks.get(), want_positions);
+    } else {
+      const KoptTaskSource produce = [&](ee_kopt_task* d_tasks) {
+        LaunchCursor(ctx, *tour, k, first, rows, d_tasks, want_positions);
+      };
+      EvalKoptCostsFrom(ctx, x, y, n, type, k, templates, template_count, generator_hash, rows,
+                        produce, false, costs, masks, usage);
+    }
+  } catch (...) {
+    // a failed evaluation may leave the overflow flag set; clear it (best
+    // effort) so that the next call does not report this one's overflow
+    static_cast<void>(hipMemsetAsync(ctx.cursor.d_status.get(), 0, sizeof(uint32_t), ctx.stream));
+    static_cast<void>(hipStreamSynchronize(ctx.stream));
+    throw;
   }
   FinishCursor(ctx, k, rows, tasks, positions);
   if (produced) *produced = rows;

@@ -243,10 +243,12 @@ using KoptTaskSource = std::function<void(ee_kopt_task* d_tasks)>;
 // caches, one launch, download) with its task rows taken from `fill_tasks`.
 // Exactly one of `costs` (task_count x template_count cells) and `masks`
 // (task_count x ceil(template_count / 32) improving-template words) is
-// non-null.
+// non-null.  sync_after_fill: wait for the rows before the launch (usage's
+// h2d_ms is then their transfer); rows made by a kernel on the context's
+// stream need no wait, and their time falls into kernel_ms.
 void EvalKoptCostsFrom(Context& ctx, const int64_t* x, const int64_t* y, int32_t n, uint8_t type,
                        uint32_t k, const ee_endpoint_matching* templates, uint32_t template_count,
                        uint64_t generator_hash, size_t task_count, const KoptTaskSource& fill_tasks,
-                       int64_t* costs, uint32_t* masks, ee_kopt_usage* usage);
+                       bool sync_after_fill, int64_t* costs, uint32_t* masks, ee_kopt_usage* usage);
 
 }  // namespace ee

@@ -533,13 +533,13 @@ void EvalKoptCosts(Context& ctx, const int64_t* x, const int64_t* y, int32_t n, 
              "hipMemcpyAsync tasks");
   };
   EvalKoptCostsFrom(ctx, x, y, n, type, k, templates, template_count, generator_hash, task_count,
-                    upload, costs, nullptr, usage);
+                    upload, true, costs, nullptr, usage);
 }
 
 void EvalKoptCostsFrom(Context& ctx, const int64_t* x, const int64_t* y, int32_t n, uint8_t type,
                        uint32_t k, const ee_endpoint_matching* templates, uint32_t template_count,
                        uint64_t generator_hash, size_t task_count, const KoptTaskSource& fill_tasks,
-                       int64_t* costs, uint32_t* masks, ee_kopt_usage* usage) {
+                       bool sync_after_fill, int64_t* costs, uint32_t* masks, ee_kopt_usage* usage) {
   ee_kopt_usage u{};
   ctx.Activate();
   if (task_count == 0) {
@@ -620,7 +620,7 @@ void EvalKoptCostsFrom(Context& ctx, const int64_t* x, const int64_t* y, int32_t
   auto t0 = std::chrono::steady_clock::now();
   fill_tasks(c.d_tasks.get());
   if (descriptors) c.d_tiles.Upload(tiles.data(), n_tiles, ctx.stream);
-  ctx.Sync();
+  if (sync_after_fill) ctx.Sync();
   u.h2d_ms = MsSince(t0);
 
   a.xy = c.d_xy.get();
