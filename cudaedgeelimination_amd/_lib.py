@@ -104,6 +104,17 @@ class KoptUsage(C.Structure):
     ]
 
 
+class HtUsage(C.Structure):
+    _fields_ = [
+        ("graph_hit", C.c_int32),
+        ("workspace_hit", C.c_int32),
+        ("resident_bytes", C.c_uint64),
+        ("h2d_ms", C.c_double),
+        ("kernel_ms", C.c_double),
+        ("d2h_ms", C.c_double),
+    ]
+
+
 class TourView(C.Structure):
     _fields_ = [
         ("tour", C.c_void_p),
@@ -226,6 +237,10 @@ PROTOTYPES = {
     "ee_jv_find_candidates": (C.c_int, [P, C.POINTER(GraphView), P, C.POINTER(JvUsage)]),
     "ee_jv_clear_cache": (C.c_int, [P]),
     "ee_kopt_build_templates": (C.c_int, [U32, P, U32, C.POINTER(U32), C.POINTER(U64)]),
+    "ee_ht_hamilton_replies": (C.c_int, [P, P, I32, I32, P, SZ, P, P, U64, C.POINTER(U64),
+                                         C.POINTER(HtUsage)]),
+    "ee_ht_end_replies": (C.c_int, [P, P, P, SZ, P, P, U64, C.POINTER(U64), C.POINTER(HtUsage)]),
+    "ee_ht_clear_cache": (C.c_int, [P]),
     "ee_kopt_cursor_count": (C.c_int, [C.POINTER(TourView), U32, C.POINTER(U64)]),
     "ee_kopt_cursor_eval": (C.c_int, [P, P, P, I32, U8, C.POINTER(TourView), U32, P, U32, U64, U64,
                                       U32, P, P, P, P, C.POINTER(U32), C.POINTER(KoptUsage)]),

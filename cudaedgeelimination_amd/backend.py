@@ -191,6 +191,63 @@ def evaluate_kopt_template_costs_cuda(graph: GraphSnapshot, table: KOptReconnect
 
 
 @dataclasses.dataclass
+class HtReplyCudaCacheUsage:
+    graph_hit: bool = False
+    workspace_hit: bool = False
+    resident_bytes: int = 0
+    h2d_ms: float = 0.0
+    kernel_ms: float = 0.0
+    d2h_ms: float = 0.0
+
+
+def _ht_usage(u) -> HtReplyCudaCacheUsage:
+    return HtReplyCudaCacheUsage(bool(u.graph_hit), bool(u.workspace_hit), int(u.resident_bytes),
+                                 u.h2d_ms, u.kernel_ms, u.d2h_ms)
+
+
+def evaluate_ht_hamilton_replies_cuda(graph: GraphSnapshot, target, centers,
+                                      ctx: Context | None = None):
+    """EvaluateHtHamiltonRepliesCuda (src/cuda/ht_replies.cu:585-663):
+    (offsets uint64[len(centers) + 1], replies int32[total, 3] as
+    (center, first, second), usage)."""
+    ctx = _ctx(ctx)
+    centers = np.ascontiguousarray(centers, dtype=np.int32)
+    deg = np.diff(graph.row_offsets).astype(np.int64)[centers] if len(centers) else np.zeros(0)
+    bound = int((deg * (deg - 1) // 2).sum()) if len(centers) else 0
+    offsets = np.zeros(len(centers) + 1, dtype=np.uint64)
+    replies = np.zeros((max(bound, 1), 3), dtype=np.int32)
+    total, usage = C.c_uint64(), _lib.HtUsage()
+    view = graph.view()
+    _lib.check(_lib.lib().ee_ht_hamilton_replies(
+        ctx.handle, C.byref(view), int(target[0]), int(target[1]),
+        _lib.ptr(centers) if len(centers) else None, len(centers), _lib.ptr(offsets),
+        _lib.ptr(replies), bound, C.byref(total), C.byref(usage)))
+    return offsets, replies[:total.value].copy(), _ht_usage(usage)
+
+
+def evaluate_ht_end_replies_cuda(graph: GraphSnapshot, tasks, ctx: Context | None = None):
+    """EvaluateHtEndRepliesCuda (src/cuda/ht_replies.cu:667-751): tasks as
+    (endpoint, internal_neighbor) rows; (offsets uint64[len + 1], replies
+    int32[total, 2] canonical edges, usage)."""
+    ctx = _ctx(ctx)
+    tasks = np.ascontiguousarray(np.asarray(tasks, dtype=np.int32).reshape(-1, 2))
+    deg = np.diff(graph.row_offsets).astype(np.int64)
+    bound = int((deg[tasks[:, 0]] - 1).clip(min=0).sum()) if len(tasks) else 0
+    offsets = np.zeros(len(tasks) + 1, dtype=np.uint64)
+    replies = np.zeros((max(bound, 1), 2), dtype=np.int32)
+    total, usage = C.c_uint64(), _lib.HtUsage()
+    view = graph.view()
+    _lib.check(_lib.lib().ee_ht_end_replies(
+        ctx.handle, C.byref(view), _lib.ptr(tasks) if len(tasks) else None, len(tasks),
+        _lib.ptr(offsets), _lib.ptr(replies), bound, C.byref(total), C.byref(usage)))
+    return offsets, replies[:total.value].copy(), _ht_usage(usage)
+
+
+def clear_ht_reply_cuda_cache(ctx: Context | None = None) -> None:
+    _lib.check(_lib.lib().ee_ht_clear_cache(_ctx(ctx).handle))
+
+
+@dataclasses.dataclass
 class TourContext:
     """The TourContext fields the reference's k-opt cost cursor reads
     (src/cpu/kopt_search.cpp:125-134): the tour, the exact cost of the edge

@@ -210,6 +210,23 @@ struct KoptCursorCache {
   DeviceArray<uint32_t> d_status;  // bit 0: some deleted cost overflowed int64
 };
 
+// Device copy of one snapshot's coordinates and active CSR for the HT reply
+// enumerations, kept while the caller passes the same arrays, plus their
+// growing workspaces.
+struct HtCache {
+  bool valid = false;
+  int32_t dimension = 0;
+  uint8_t distance_type = 0;
+  std::vector<int64_t> x, y;
+  std::vector<int32_t> row_offsets, neighbors;
+  DeviceArray<longlong2> d_xy;
+  DeviceArray<int32_t> d_row_offsets, d_neighbors;
+  DeviceArray<int32_t> d_items;      // centers, or (endpoint, internal) task pairs
+  DeviceArray<uint64_t> d_counts;    // per item
+  DeviceArray<uint64_t> d_offsets;   // per item + 1
+  DeviceArray<int32_t> d_replies;    // triples or pairs
+};
+
 struct Context {
   explicit Context(int requested_device);
   ~Context();
@@ -228,11 +245,14 @@ struct Context {
   JvCache jv;
   KoptCache kopt;
   KoptCursorCache cursor;
+  HtCache ht;
 };
 
 std::vector<ee_endpoint_matching> CanonicalTemplates(uint32_t k, uint64_t* hash);
 KoptPlan BuildPlan(uint32_t k, const ee_endpoint_matching* templates, uint32_t count);
 void ValidateTasks(int32_t n, uint32_t k, const ee_kopt_task* tasks, size_t count);
+// Null arrays, dimensions, distance type and integer coordinates of a view.
+void CheckGraphView(const ee_graph_view* g);
 
 // Writes the task rows of one plain evaluation into the device task pool
 // (`d_tasks`, room for the evaluation's rows), enqueued on the context's

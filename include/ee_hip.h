@@ -173,6 +173,58 @@ ee_status ee_kopt_eval_costs(ee_ctx* ctx, const int64_t* x, const int64_t* y, in
 /* Replaces cudaee::detail::ClearKOptCostCudaCache (src/cuda/kopt_cost.cu:472). */
 ee_status ee_kopt_clear_cache(ee_ctx* ctx);
 
+/* -------------------------------------------------------- HT replies */
+/* Reference HtNeighborPair, NodeEdge, HtEndReplyTask
+ * (include/cuda_edge_elimination/hamilton_tutte.hpp:26-32, 76-81) and
+ * HtReplyCudaCacheUsage (:515-519) plus timings. */
+typedef struct {
+  int32_t center;
+  int32_t first;
+  int32_t second;
+} ee_ht_neighbor_pair;
+typedef struct {
+  int32_t u;
+  int32_t v;
+} ee_node_edge;
+typedef struct {
+  int32_t endpoint;
+  int32_t internal_neighbor;
+} ee_ht_end_task;
+typedef struct {
+  int32_t graph_hit;
+  int32_t workspace_hit;
+  uint64_t resident_bytes;
+  double h2d_ms;
+  double kernel_ms;
+  double d2h_ms;
+} ee_ht_usage;
+
+/* Replaces cudaee::EvaluateHtHamiltonRepliesCuda (src/cuda/ht_replies.cu:585-663):
+ * for each center in order, every pair of its CSR neighbours (first before
+ * second) that is not the target edge, whose two center edges survive the
+ * 2-opt test against the target, and whose target + center edges cost at
+ * most the pair's edge + the center-to-target-end edges
+ * (src/cpu/hamilton_tutte.cpp:230-262).  The target may be given in either
+ * order.  offsets receives center_count + 1 entries (center i's replies are
+ * [offsets[i], offsets[i+1])), replies the first min(*total, capacity)
+ * replies, *total their count (at most the sum over centers of
+ * C(degree, 2)).  The snapshot must pass the reference's HT graph checks
+ * (ht_replies.cu:521-553): a symmetric CSR of exactly the active edges,
+ * rows strictly ascending. */
+ee_status ee_ht_hamilton_replies(ee_ctx* ctx, const ee_graph_view* graph, int32_t target_u,
+                                 int32_t target_v, const int32_t* centers, size_t center_count,
+                                 uint64_t* offsets, ee_ht_neighbor_pair* replies, uint64_t capacity,
+                                 uint64_t* total, ee_ht_usage* usage);
+/* Replaces cudaee::EvaluateHtEndRepliesCuda (src/cuda/ht_replies.cu:667-751):
+ * per task every active edge of the endpoint except the one to its internal
+ * neighbour, canonical (u < v), in CSR order.  Outputs as
+ * ee_ht_hamilton_replies (*total = the sum of degree - 1). */
+ee_status ee_ht_end_replies(ee_ctx* ctx, const ee_graph_view* graph, const ee_ht_end_task* tasks,
+                            size_t task_count, uint64_t* offsets, ee_node_edge* replies,
+                            uint64_t capacity, uint64_t* total, ee_ht_usage* usage);
+/* Replaces cudaee::ClearHtReplyCudaCache (src/cuda/ht_replies.cu:753). */
+ee_status ee_ht_clear_cache(ee_ctx* ctx);
+
 /* ------------------------------------------- device deletion-set producer */
 /* The TourContext fields the reference's k-opt cost cursor reads
  * (src/cpu/kopt_search.cpp:125-134): the tour (distinct snapshot nodes),

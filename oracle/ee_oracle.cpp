@@ -1266,4 +1266,87 @@ int64_t orc_kopt_cursor_tasks(const int32_t* tour, const int64_t* edge_costs, ui
   return status == 0 ? written : status;
 }
 
+
+int64_t orc_ht_hamilton_replies(const orc_graph* graph, int32_t target_u, int32_t target_v,
+                                const int32_t* centers, size_t count, uint64_t* offsets,
+                                int32_t* replies, uint64_t capacity) {
+  uint64_t total = 0;
+  const int status = Guard([&] {
+    const orc_graph& g = *graph;
+    const int32_t tu = std::min(target_u, target_v), tv = std::max(target_u, target_v);
+    if (tu < 0 || tv >= g.dimension || tu == tv || !HasActiveEdge(g, tu, tv)) {
+      throw std::invalid_argument("HT reply target is not an active edge");
+    }
+    for (size_t i = 0; i < count; ++i) {
+      if (centers[i] < 0 || centers[i] >= g.dimension || centers[i] == tu || centers[i] == tv) {
+        throw std::invalid_argument("HT reply center is invalid");
+      }
+    }
+    // 2-opt survival of edge (a, b) against the target (hamilton_tutte.cpp:79-87)
+    const auto survives = [&](int32_t a, int32_t b) {
+      const int32_t lo = std::min(a, b), hi = std::max(a, b);
+      const __int128 original = static_cast<__int128>(Distance(g, tu, tv)) + Distance(g, lo, hi);
+      const __int128 first = static_cast<__int128>(Distance(g, tu, hi)) + Distance(g, tv, lo);
+      const __int128 second = static_cast<__int128>(Distance(g, tu, lo)) + Distance(g, tv, hi);
+      return first >= original || second >= original;
+    };
+    offsets[0] = 0;
+    for (size_t i = 0; i < count; ++i) {
+      const int32_t c = centers[i];
+      const int32_t begin = g.row_offsets[c], end = g.row_offsets[c + 1];
+      const __int128 center_target = static_cast<__int128>(Distance(g, tu, c)) + Distance(g, tv, c);
+      for (int32_t a = begin; a < end; ++a) {
+        for (int32_t b = a + 1; b < end; ++b) {
+          const int32_t f = g.neighbors[a], s = g.neighbors[b];
+          if (std::min(f, s) == tu && std::max(f, s) == tv) continue;
+          if (!survives(c, f) || !survives(c, s)) continue;
+          const __int128 original =
+              static_cast<__int128>(Distance(g, tu, tv)) + Distance(g, c, f) + Distance(g, c, s);
+          const __int128 replacement = static_cast<__int128>(Distance(g, f, s)) + center_target;
+          if (original <= replacement) {
+            if (total < capacity) {
+              replies[3 * total] = c;
+              replies[3 * total + 1] = f;
+              replies[3 * total + 2] = s;
+            }
+            ++total;
+          }
+        }
+      }
+      offsets[i + 1] = total;
+    }
+  });
+  return status == 0 ? static_cast<int64_t>(total) : status;
+}
+
+int64_t orc_ht_end_replies(const orc_graph* graph, const int32_t* tasks, size_t count,
+                           uint64_t* offsets, int32_t* replies, uint64_t capacity) {
+  uint64_t total = 0;
+  const int status = Guard([&] {
+    const orc_graph& g = *graph;
+    for (size_t i = 0; i < count; ++i) {
+      const int32_t e = tasks[2 * i], in = tasks[2 * i + 1];
+      if (e < 0 || e >= g.dimension || in < 0 || in >= g.dimension || e == in ||
+          !HasActiveEdge(g, e, in)) {
+        throw std::invalid_argument("HT end reply task must name an active path-internal edge");
+      }
+    }
+    offsets[0] = 0;
+    for (size_t i = 0; i < count; ++i) {
+      const int32_t e = tasks[2 * i], in = tasks[2 * i + 1];
+      for (int32_t a = g.row_offsets[e]; a < g.row_offsets[e + 1]; ++a) {
+        const int32_t nb = g.neighbors[a];
+        if (nb == in) continue;
+        if (total < capacity) {
+          replies[2 * total] = std::min(e, nb);
+          replies[2 * total + 1] = std::max(e, nb);
+        }
+        ++total;
+      }
+      offsets[i + 1] = total;
+    }
+  });
+  return status == 0 ? static_cast<int64_t>(total) : status;
+}
+
 }  // extern "C"

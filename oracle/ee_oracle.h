@@ -127,6 +127,27 @@ int64_t orc_kopt_cursor_tasks(const int32_t* tour, const int64_t* edge_costs, ui
                               uint64_t first, uint64_t count, orc_kopt_task* tasks,
                               uint64_t* positions);
 
+/* HT Hamilton replies of the reference (src/cpu/hamilton_tutte.cpp:230-262,
+ * 79-87, 386-440 without the center deduplication, which the device entry
+ * EvaluateHtHamiltonRepliesCuda does not do either): for each center in
+ * order, every pair of its CSR neighbours (first before second) that is not
+ * the target edge, whose two center edges each survive the 2-opt test
+ * against the target, and with target + center edges <= the pair's edge +
+ * the center-to-target-ends edges.  offsets receives count + 1 entries;
+ * replies (triples center, first, second) receives min(total, capacity)
+ * entries.  Returns the total, negative on failure (invalid target or
+ * center, as ValidateTarget / the center checks). */
+int64_t orc_ht_hamilton_replies(const orc_graph* graph, int32_t target_u, int32_t target_v,
+                                const int32_t* centers, size_t count, uint64_t* offsets,
+                                int32_t* replies, uint64_t capacity);
+/* HT end replies (src/cpu/hamilton_tutte.cpp:264-277, 497-560 without the
+ * task deduplication): per task (endpoint, internal neighbour) every CSR
+ * neighbour of the endpoint except the internal one, as canonical (u < v)
+ * edges.  tasks: count pairs; replies: pairs.  Returns the total, negative
+ * on failure (a task that is not an active edge). */
+int64_t orc_ht_end_replies(const orc_graph* graph, const int32_t* tasks, size_t count,
+                           uint64_t* offsets, int32_t* replies, uint64_t capacity);
+
 #ifdef __cplusplus
 }
 #endif

@@ -204,4 +204,51 @@ int ref_kopt_costs(const int64_t* x, const int64_t* y, int32_t n, int type, uint
   });
 }
 
+
+// The reference's CPU HT replies (EvaluateHtHamiltonReplies /
+// EvaluateHtEndReplies with PathCompatibilityBackend::kCpu, include/
+// cuda_edge_elimination/hamilton_tutte.hpp:409-420), flattened like the
+// oracle's outputs.  Returns the total reply count, negative on failure.
+int64_t ref_ht_hamilton_replies(int32_t n, int type, const int64_t* x, const int64_t* y, int32_t m,
+                                const int32_t* u, const int32_t* v, const int64_t* w,
+                                const uint8_t* active, int32_t tu, int32_t tv,
+                                const int32_t* centers, size_t count, uint64_t* offsets,
+                                int32_t* replies, uint64_t capacity) {
+  int64_t total = 0;
+  const int status = Guard([&] {
+    const cudaee::GraphSnapshot g = FromArrays(n, type, x, y, m, u, v, w, active);
+    const std::vector<int32_t> c(centers, centers + count);
+    const auto r = cudaee::EvaluateHtHamiltonReplies(g, cudaee::NodeEdge{tu, tv}, c,
+                                                     cudaee::PathCompatibilityBackend::kCpu);
+    std::copy(r.offsets.begin(), r.offsets.end(), offsets);
+    for (size_t i = 0; i < r.replies.size() && i < capacity; ++i) {
+      replies[3 * i] = r.replies[i].center;
+      replies[3 * i + 1] = r.replies[i].first;
+      replies[3 * i + 2] = r.replies[i].second;
+    }
+    total = static_cast<int64_t>(r.replies.size());
+  });
+  return status == 0 ? total : status;
+}
+
+int64_t ref_ht_end_replies(int32_t n, int type, const int64_t* x, const int64_t* y, int32_t m,
+                           const int32_t* u, const int32_t* v, const int64_t* w,
+                           const uint8_t* active, const int32_t* tasks, size_t count,
+                           uint64_t* offsets, int32_t* replies, uint64_t capacity) {
+  int64_t total = 0;
+  const int status = Guard([&] {
+    const cudaee::GraphSnapshot g = FromArrays(n, type, x, y, m, u, v, w, active);
+    std::vector<cudaee::HtEndReplyTask> t(count);
+    for (size_t i = 0; i < count; ++i) t[i] = {tasks[2 * i], tasks[2 * i + 1]};
+    const auto r = cudaee::EvaluateHtEndReplies(g, t, cudaee::PathCompatibilityBackend::kCpu);
+    std::copy(r.offsets.begin(), r.offsets.end(), offsets);
+    for (size_t i = 0; i < r.replies.size() && i < capacity; ++i) {
+      replies[2 * i] = r.replies[i].u;
+      replies[2 * i + 1] = r.replies[i].v;
+    }
+    total = static_cast<int64_t>(r.replies.size());
+  });
+  return status == 0 ? total : status;
+}
+
 }  // extern "C"

@@ -46,6 +46,8 @@ _ORACLE = {
     "orc_kopt_costs": (C.c_int, [P, P, I32, C.c_int, U32, P, SZ, P, C.c_int, C.POINTER(U32),
                                  C.POINTER(U32)]),
     "orc_kopt_cursor_tasks": (I64, [P, P, U32, P, U32, U32, U64, U64, P, P]),
+    "orc_ht_hamilton_replies": (I64, [P, I32, I32, P, SZ, P, P, U64]),
+    "orc_ht_end_replies": (I64, [P, P, SZ, P, P, U64]),
 }
 _REF = {
     "ref_last_error": (C.c_char_p, []),
@@ -58,6 +60,9 @@ _REF = {
     "ref_jv_replay_files": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p]),
     "ref_kopt_templates": (C.c_int, [U32, P, U32, C.POINTER(U32), C.POINTER(U64)]),
     "ref_kopt_costs": (C.c_int, [P, P, I32, C.c_int, U32, P, SZ, P, C.POINTER(U32), C.POINTER(U32)]),
+    "ref_ht_hamilton_replies": (I64, [I32, C.c_int, P, P, I32, P, P, P, P, I32, I32, P, SZ, P, P,
+                                      U64]),
+    "ref_ht_end_replies": (I64, [I32, C.c_int, P, P, I32, P, P, P, P, P, SZ, P, P, U64]),
 }
 
 
@@ -262,6 +267,52 @@ def cursor_tasks(tour, edge_costs, selectable, k, first, count):
     if n < 0:
         _raise(lib.orc_last_error)
     return tasks[:n], positions[:n]
+
+
+def _ht_call(fn, err, lead, items, width, bound):
+    items = np.ascontiguousarray(items, np.int32)
+    count = len(items) if items.ndim == 1 else len(items)
+    offsets = np.zeros(count + 1, np.uint64)
+    replies = np.zeros((max(bound, 1), width), np.int32)
+    total = fn(*lead, p(items) if count else None, count, p(offsets), p(replies), bound)
+    if total < 0:
+        _raise(err)
+    assert total <= bound
+    return offsets, replies[:total].copy()
+
+
+def ht_hamilton_replies(graph: OGraph, target, centers, bound):
+    lib = oracle()
+    return _ht_call(lib.orc_ht_hamilton_replies, lib.orc_last_error,
+                    (graph.h, int(target[0]), int(target[1])), centers, 3, bound)
+
+
+def ht_end_replies(graph: OGraph, tasks, bound):
+    lib = oracle()
+    return _ht_call(lib.orc_ht_end_replies, lib.orc_last_error, (graph.h,),
+                    np.asarray(tasks, np.int32).reshape(-1, 2), 2, bound)
+
+
+def _ref_graph_args(x, y, dtype, u, v, w, active):
+    arrs = [np.ascontiguousarray(a, t) for a, t in ((x, np.int64), (y, np.int64), (u, np.int32),
+                                                     (v, np.int32), (w, np.int64),
+                                                     (active, np.uint8))]
+    return arrs, (len(arrs[0]), dtype, p(arrs[0]), p(arrs[1]), len(arrs[2]), p(arrs[2]),
+                  p(arrs[3]), p(arrs[4]), p(arrs[5]))
+
+
+def ref_ht_hamilton_replies(x, y, dtype, u, v, w, active, target, centers, bound):
+    lib = ref()
+    keep, lead = _ref_graph_args(x, y, dtype, u, v, w, active)
+    return _ht_call(lib.ref_ht_hamilton_replies, lib.ref_last_error,
+                    lead + (int(target[0]), int(target[1])), centers, 3, bound)
+
+
+def ref_ht_end_replies(x, y, dtype, u, v, w, active, tasks, bound):
+    lib = ref()
+    keep, lead = _ref_graph_args(x, y, dtype, u, v, w, active)
+    return _ht_call(lib.ref_ht_end_replies, lib.ref_last_error, lead,
+                    np.asarray(tasks, np.int32).reshape(-1, 2), 2, bound)
 
 
 def ref_kopt_costs(x, y, dtype, k, tasks):

@@ -1,0 +1,125 @@
+"""The oracle's HT reply enumerations against the reference's own CPU path.
+
+Reference: EvaluateHtHamiltonReplies / EvaluateHtEndReplies with the CPU
+backend (src/cpu/hamilton_tutte.cpp:386-640, over
+AppendHtHamiltonRepliesUnchecked / AppendHtEndRepliesUnchecked, :230-277),
+built from the reference sources into oracle/_ref and called through
+oracle/ref_harness.cpp.  Graphs: the 300-node uniform instance, the
+drilling-grid lattice (many equal distances, so the >= and <= ties of the
+2-opt and cost tests matter), and CEIL_2D.  Closed forms beside it: end
+replies number sum(degree - 1) and never hold the internal edge; every
+Hamilton reply is a pair of the center's neighbours in CSR order.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import _oracle as O
+from conftest import require_ref
+from cudaedgeelimination_amd import workloads as W
+from cudaedgeelimination_amd.graph import CEIL_2D, EUC_2D
+
+
+def graphs():
+    g1 = W.config1()
+    g2 = W.config2()
+    x, y = W._points_uniform(0xEE0000D1, 400, 0, 3000)
+    g3 = W.knn_graph(x, y, CEIL_2D, 9)
+    return {"uniform300": g1, "lattice": g2, "ceil400": g3}
+
+
+@pytest.fixture(scope="module")
+def snapshots():
+    return graphs()
+
+
+def ht_cases(g, seed, targets=6, centers=300):
+    """Seeded targets (active edges) and center lists (repeats included)."""
+    rng = np.random.default_rng(seed)
+    active = np.flatnonzero(g.edge_active)
+    out = []
+    for e in rng.choice(active, size=targets, replace=False):
+        u, v = int(g.edge_u[e]), int(g.edge_v[e])
+        pool = np.setdiff1d(np.arange(g.dimension), [u, v])
+        # half the centers near the target (its ends' neighbours and theirs),
+        # where the 2-opt and cost tests reject pairs, half anywhere
+        near = set()
+        for a in (u, v):
+            for b in g.neighbors[g.row_offsets[a]:g.row_offsets[a + 1]]:
+                near.update(g.neighbors[g.row_offsets[b]:g.row_offsets[b + 1]].tolist())
+                near.add(int(b))
+        near = np.setdiff1d(np.array(sorted(near)), [u, v])
+        c = np.concatenate([rng.choice(near, size=centers // 2, replace=True),
+                            rng.choice(pool, size=centers - centers // 2, replace=True)])
+        c = rng.permutation(c).astype(np.int32)
+        out.append(((u, v) if rng.integers(2) else (v, u), c))
+    return out
+
+
+def end_tasks(g, seed, count=400):
+    rng = np.random.default_rng(seed)
+    active = np.flatnonzero(g.edge_active)
+    e = rng.choice(active, size=count, replace=True)
+    flip = rng.integers(2, size=count).astype(bool)
+    a = np.where(flip, g.edge_v[e], g.edge_u[e])
+    b = np.where(flip, g.edge_u[e], g.edge_v[e])
+    return np.stack([a, b], axis=1).astype(np.int32)
+
+
+def pair_bound(g, centers):
+    d = np.diff(g.row_offsets).astype(np.int64)[centers]
+    return int((d * (d - 1) // 2).sum())
+
+
+def ref_args(g):
+    return (g.x, g.y, g.distance_type, g.edge_u, g.edge_v, g.edge_weight, g.edge_active)
+
+
+@pytest.mark.parametrize("name", ["uniform300", "lattice", "ceil400"])
+def test_hamilton_replies_equal_reference(snapshots, name):
+    require_ref()
+    g = snapshots[name]
+    og = O.OGraph.from_snapshot(g)
+    for target, centers in ht_cases(g, 0xA1):
+        bound = pair_bound(g, centers)
+        off, rep = O.ht_hamilton_replies(og, target, centers, bound)
+        roff, rrep = O.ref_ht_hamilton_replies(*ref_args(g), target, centers, bound)
+        assert np.array_equal(off, roff)
+        assert np.array_equal(rep, rrep)
+        # each reply: a pair of its center's neighbours in CSR order
+        for i in range(0, len(centers), 37):
+            c = centers[i]
+            row = list(g.neighbors[g.row_offsets[c]:g.row_offsets[c + 1]])
+            for r in rep[off[i]:off[i + 1]]:
+                assert r[0] == c and row.index(r[1]) < row.index(r[2])
+
+
+@pytest.mark.parametrize("name", ["uniform300", "lattice", "ceil400"])
+def test_end_replies_equal_reference(snapshots, name):
+    require_ref()
+    g = snapshots[name]
+    og = O.OGraph.from_snapshot(g)
+    tasks = end_tasks(g, 0xB2)
+    deg = np.diff(g.row_offsets).astype(np.int64)
+    bound = int((deg[tasks[:, 0]] - 1).sum())
+    off, rep = O.ht_end_replies(og, tasks, bound)
+    roff, rrep = O.ref_ht_end_replies(*ref_args(g), tasks, bound)
+    assert np.array_equal(off, roff) and np.array_equal(rep, rrep)
+    assert int(off[-1]) == bound
+    internal = {(min(a, b), max(a, b)) for a, b in tasks.tolist()}
+    for i, (a, b) in enumerate(tasks.tolist()):
+        got = {tuple(r) for r in rep[off[i]:off[i + 1]].tolist()}
+        assert (min(a, b), max(a, b)) not in got
+        assert all(a in r for r in got)
+    assert internal  # the tasks name active edges
+
+
+def test_ht_rejects_invalid_inputs(snapshots):
+    g = snapshots["uniform300"]
+    og = O.OGraph.from_snapshot(g)
+    u, v = int(g.edge_u[0]), int(g.edge_v[0])
+    with pytest.raises(RuntimeError):
+        O.ht_hamilton_replies(og, (u, v), np.array([u], np.int32), 100)
+    with pytest.raises(RuntimeError):
+        O.ht_end_replies(og, np.array([[u, u]], np.int32), 100)
