@@ -485,6 +485,61 @@ def cursor_leg(ctx, graph, blocks: int = 40) -> dict:
             "masks_speedup_over_plain": plain["ms_per_block"] / dev_masks["ms_per_block"]}
 
 
+def ht_leg(ctx, cpu: bool, reps: int = 20) -> dict:
+    """HT Hamilton replies of one target edge against every other node of the
+    30,000-node served graph (10-NN), and end replies of 30,000 tasks: device
+    time (count, scan and write kernels) and whole calls, with the reference
+    CPU path (EvaluateHtHamiltonReplies, CPU backend, oracle/_ref) beside it."""
+    g = W.config5_graphs(sizes=(30000,))[0]
+    e = int(np.flatnonzero(g.edge_active)[12345])
+    target = (int(g.edge_u[e]), int(g.edge_v[e]))
+    centers = np.setdiff1d(np.arange(g.dimension), target).astype(np.int32)
+    deg = np.diff(g.row_offsets).astype(np.int64)
+    pairs = int((deg[centers] * (deg[centers] - 1) // 2).sum())
+    # exact distances: 5 per neighbour (center edge, two 2-opt orientations),
+    # 1 per surviving-candidate pair, 3 per center
+    dists = int(5 * deg[centers].sum() + pairs + 3 * len(centers))
+    backend.evaluate_ht_hamilton_replies_cuda(g, target, centers, ctx)  # warm caches
+    kernel, wall = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        off, rep, u = backend.evaluate_ht_hamilton_replies_cuda(g, target, centers, ctx)
+        wall.append(time.perf_counter() - t0)
+        kernel.append(u.kernel_ms)
+    rng = np.random.default_rng(0xEE0000E1)
+    act = np.flatnonzero(g.edge_active)
+    pick = rng.choice(act, size=30000)
+    tasks = np.stack([g.edge_u[pick], g.edge_v[pick]], axis=1).astype(np.int32)
+    backend.evaluate_ht_end_replies_cuda(g, tasks, ctx)
+    end_kernel, end_wall = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        eoff, erep, eu = backend.evaluate_ht_end_replies_cuda(g, tasks, ctx)
+        end_wall.append(time.perf_counter() - t0)
+        end_kernel.append(eu.kernel_ms)
+    k_ms = statistics.median(kernel)
+    out = {"workload": f"target edge {target} of the n=30000 10-NN graph, {len(centers)} centers, "
+                       f"{pairs} neighbour pairs; 30000 end tasks",
+           "replies": int(len(rep)), "kernel_ms": k_ms, "call_ms": statistics.median(wall) * 1e3,
+           "pairs_per_s": pairs / (k_ms * 1e-3), "distance_evals_per_s": dists / (k_ms * 1e-3),
+           "end_replies": int(len(erep)), "end_kernel_ms": statistics.median(end_kernel),
+           "end_call_ms": statistics.median(end_wall) * 1e3}
+    if cpu:
+        import _oracle as O
+        if O.ref_available():
+            ref_args = (g.x, g.y, g.distance_type, g.edge_u, g.edge_v, g.edge_weight, g.edge_active)
+            times = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                roff, rrep = O.ref_ht_hamilton_replies(*ref_args, target, centers, pairs)
+                times.append(time.perf_counter() - t0)
+            out["cpu_baseline"] = {"ms": statistics.median(times) * 1e3, "kind": "reference",
+                                   "cores": 1, "sample": "the same call, five runs (median)",
+                                   "replies_equal": bool(np.array_equal(roff, off)
+                                                         and np.array_equal(rrep, rep))}
+    return out
+
+
 def main() -> int:
     # the reference protocol for its OpenMP CPU path (set before it loads)
     for key, value in (("OMP_DYNAMIC", "FALSE"), ("OMP_PROC_BIND", "spread"),
@@ -503,6 +558,7 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--no-many", action="store_true", help="skip the many-instance workload")
     ap.add_argument("--no-cursor", action="store_true", help="skip the deletion-set cursor leg")
+    ap.add_argument("--no-ht", action="store_true", help="skip the HT reply leg")
     ap.add_argument("--serve-qps", default="2000,8000,20000,0",
                     help="Poisson rates of the serving leg; 0 is a burst of --serve-burst requests")
     ap.add_argument("--serve-burst", type=int, default=20000)
@@ -681,6 +737,10 @@ def main() -> int:
             jv["cpu_baseline"] = cpu_jv
             jv["proof_equals_cpu"] = O.canonical_proof(text) == O.canonical_proof(best.proof)
         result["jv"] = jv
+
+    if not args.no_ht:
+        log("HT replies (30,000 centers of one target)")
+        result["ht_replies"] = ht_leg(ctx, cpu=not args.no_cpu)
 
     if not args.no_cursor:
         log("deletion-set cursor blocks (device-built rows)")

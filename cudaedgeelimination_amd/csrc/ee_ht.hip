@@ -224,16 +224,24 @@ void ValidateHtGraph(const ee_graph_view& g) {
   }
 }
 
+// Keeps the device copy of the snapshot.  A snapshot equal to the resident
+// one (coordinates, CSR and active flags) passed the HT checks when it was
+// uploaded, so only a new one is checked (ValidateHtGraph) and uploaded.
 bool PrepareHtGraph(Context& ctx, const ee_graph_view& g) {
   HtCache& c = ctx.ht;
   const size_t n = static_cast<size_t>(g.dimension);
-  const size_t a = static_cast<size_t>(g.row_offsets[g.dimension]);
+  const size_t m = static_cast<size_t>(g.edge_count);
   const bool hit = c.valid && c.dimension == g.dimension && c.distance_type == g.distance_type &&
-                   c.neighbors.size() == a && std::memcmp(c.x.data(), g.x, n * 8) == 0 &&
-                   std::memcmp(c.y.data(), g.y, n * 8) == 0 &&
+                   c.active.size() == m && c.row_offsets.size() == n + 1 &&
                    std::memcmp(c.row_offsets.data(), g.row_offsets, (n + 1) * 4) == 0 &&
-                   (a == 0 || std::memcmp(c.neighbors.data(), g.neighbors, a * 4) == 0);
+                   std::memcmp(c.x.data(), g.x, n * 8) == 0 &&
+                   std::memcmp(c.y.data(), g.y, n * 8) == 0 &&
+                   (c.neighbors.empty() ||
+                    std::memcmp(c.neighbors.data(), g.neighbors, c.neighbors.size() * 4) == 0) &&
+                   (m == 0 || std::memcmp(c.active.data(), g.edge_active, m) == 0);
   if (hit) return true;
+  ValidateHtGraph(g);
+  const size_t a = static_cast<size_t>(g.row_offsets[g.dimension]);
   c.valid = false;
   std::vector<longlong2> xy(n);
   for (size_t i = 0; i < n; ++i) xy[i] = make_longlong2(g.x[i], g.y[i]);
@@ -248,6 +256,7 @@ bool PrepareHtGraph(Context& ctx, const ee_graph_view& g) {
   c.y.assign(g.y, g.y + n);
   c.row_offsets.assign(g.row_offsets, g.row_offsets + n + 1);
   c.neighbors.assign(g.neighbors, g.neighbors + a);
+  c.active.assign(g.edge_active, g.edge_active + m);
   c.dimension = g.dimension;
   c.distance_type = g.distance_type;
   c.valid = true;
@@ -277,7 +286,8 @@ void HamiltonReplies(Context& ctx, const ee_graph_view* graph, int32_t target_u,
       (capacity > 0 && replies == nullptr)) {
     throw std::invalid_argument("HT Hamilton replies: NULL argument");
   }
-  ValidateHtGraph(g);
+  auto t0 = std::chrono::steady_clock::now();
+  u.graph_hit = PrepareHtGraph(ctx, g) ? 1 : 0;  // checks a new snapshot
   const int32_t tu = std::min(target_u, target_v), tv = std::max(target_u, target_v);
   if (tu < 0 || tv >= g.dimension || tu == tv || !RowHas(g, tu, tv)) {
     throw std::invalid_argument("HT Hamilton replies: the target is not an active edge");
@@ -294,8 +304,6 @@ void HamiltonReplies(Context& ctx, const ee_graph_view* graph, int32_t target_u,
   offsets[0] = 0;
   *total = 0;
   HtCache& c = ctx.ht;
-  auto t0 = std::chrono::steady_clock::now();
-  u.graph_hit = PrepareHtGraph(ctx, g) ? 1 : 0;
   if (count == 0) {
     if (usage) *usage = u;
     return;
@@ -355,7 +363,8 @@ void EndReplies(Context& ctx, const ee_graph_view* graph, const ee_ht_end_task* 
       (capacity > 0 && replies == nullptr)) {
     throw std::invalid_argument("HT end replies: NULL argument");
   }
-  ValidateHtGraph(g);
+  auto t0 = std::chrono::steady_clock::now();
+  u.graph_hit = PrepareHtGraph(ctx, g) ? 1 : 0;  // checks a new snapshot
   offsets[0] = 0;
   for (size_t i = 0; i < count; ++i) {
     const int32_t e = tasks[i].endpoint, in = tasks[i].internal_neighbor;
@@ -367,8 +376,6 @@ void EndReplies(Context& ctx, const ee_graph_view* graph, const ee_ht_end_task* 
   }
   *total = offsets[count];
   HtCache& c = ctx.ht;
-  auto t0 = std::chrono::steady_clock::now();
-  u.graph_hit = PrepareHtGraph(ctx, g) ? 1 : 0;
   if (count == 0) {
     if (usage) *usage = u;
     return;

@@ -219,6 +219,7 @@ struct HtCache {
   uint8_t distance_type = 0;
   std::vector<int64_t> x, y;
   std::vector<int32_t> row_offsets, neighbors;
+  std::vector<uint8_t> active;  // the snapshot passed the HT checks with these flags
   DeviceArray<longlong2> d_xy;
   DeviceArray<int32_t> d_row_offsets, d_neighbors;
   DeviceArray<int32_t> d_items;      // centers, or (endpoint, internal) task pairs

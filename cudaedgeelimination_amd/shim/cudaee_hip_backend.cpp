@@ -11,6 +11,9 @@
 //     (include/cuda_edge_elimination/elimination.hpp:333-338)
 //   cudaee::detail::KOptCostCudaAvailable / EvaluateKOptTemplateCostsCuda /
 //     ClearKOptCostCudaCache (include/cuda_edge_elimination/local_search.hpp:235-242)
+//   cudaee::detail::HtHamiltonReplyCudaAvailable / EvaluateHtHamiltonRepliesCuda /
+//     HtEndReplyCudaAvailable / EvaluateHtEndRepliesCuda / ClearHtReplyCudaCache
+//     (include/cuda_edge_elimination/hamilton_tutte.hpp:533-545, src/cuda/ht_replies.cu)
 //
 // It is compiled against the reference headers, so it builds only where the
 // reference tree is present.  Device caches live in one ee_ctx per host
@@ -21,6 +24,7 @@
 // Status codes become the exception types the reference throws.
 #include "cuda_edge_elimination/cuda_device_affinity.hpp"
 #include "cuda_edge_elimination/elimination.hpp"
+#include "cuda_edge_elimination/hamilton_tutte.hpp"
 #include "cuda_edge_elimination/local_search.hpp"
 
 #include <cstddef>
@@ -200,6 +204,137 @@ std::vector<std::int64_t> EvaluateKOptTemplateCostsCuda(const GraphSnapshot& gra
 }
 
 void ClearKOptCostCudaCache() { t_kopt.Reset(); }
+
+// HT reply enumerations (include/cuda_edge_elimination/hamilton_tutte.hpp:529-545)
+namespace {
+
+thread_local ThreadContext t_ht;
+
+// The snapshot's edge arrays and a view over them and its CSR.
+struct HtGraphView {
+  explicit HtGraphView(const GraphSnapshot& graph) : xy(graph) {
+    const std::size_t m = graph.edges.size();
+    if (m > static_cast<std::size_t>(std::numeric_limits<std::int32_t>::max())) {
+      throw std::overflow_error("HIP HT reply edge count exceeds int32 indexing");
+    }
+    u.resize(m);
+    v.resize(m);
+    w.resize(m);
+    active.resize(m);
+    for (std::size_t e = 0; e < m; ++e) {
+      u[e] = graph.edges[e].u;
+      v[e] = graph.edges[e].v;
+      w[e] = graph.edges[e].weight;
+      active[e] = graph.edges[e].active ? 1 : 0;
+    }
+    view.dimension = graph.dimension;
+    view.distance_type = static_cast<std::uint8_t>(graph.distance_type);
+    view.integer_coordinates = graph.integer_coordinates && graph.integer_distance_safe ? 1 : 0;
+    view.x = xy.x.data();
+    view.y = xy.y.data();
+    view.edge_count = static_cast<std::int32_t>(m);
+    view.edge_u = u.data();
+    view.edge_v = v.data();
+    view.edge_weight = w.data();
+    view.edge_active = active.data();
+    view.row_offsets = graph.row_offsets.data();
+    view.neighbors = graph.neighbors.data();
+    view.csr_edge_ids = graph.csr_edge_ids.data();
+  }
+  Coordinates xy;
+  std::vector<std::int32_t> u, v;
+  std::vector<std::int64_t> w;
+  std::vector<std::uint8_t> active;
+  ee_graph_view view{};
+};
+
+void FillUsage(const ee_ht_usage& usage, HtReplyCudaCacheUsage* const cache_usage) {
+  if (cache_usage == nullptr) return;
+  cache_usage->graph_hit = usage.graph_hit != 0;
+  cache_usage->workspace_hit = usage.workspace_hit != 0;
+  cache_usage->resident_bytes = usage.resident_bytes;
+}
+
+ee_ctx* HtContext(int* const selected_device) {
+  std::string reason;
+  ee_ctx* ctx = Acquire(&t_ht, &reason);
+  if (ctx == nullptr) throw std::runtime_error("HIP HT reply backend unavailable: " + reason);
+  if (selected_device != nullptr) *selected_device = t_ht.device;
+  return ctx;
+}
+
+}  // namespace
+
+static_assert(sizeof(HtNeighborPair) == sizeof(ee_ht_neighbor_pair), "reply layout");
+static_assert(sizeof(NodeEdge) == sizeof(ee_node_edge), "edge layout");
+static_assert(sizeof(HtEndReplyTask) == sizeof(ee_ht_end_task), "task layout");
+
+bool HtHamiltonReplyCudaAvailable(std::string* const reason) {
+  return Acquire(&t_ht, reason) != nullptr;
+}
+
+HtHamiltonReplyDeviceBatch EvaluateHtHamiltonRepliesCuda(const GraphSnapshot& graph,
+                                                         const NodeEdge target_edge,
+                                                         const std::vector<std::int32_t>& centers,
+                                                         int* const selected_device,
+                                                         HtReplyCudaCacheUsage* const cache_usage) {
+  if (cache_usage != nullptr) *cache_usage = {};
+  ee_ctx* ctx = HtContext(selected_device);
+  const HtGraphView g(graph);
+  std::uint64_t bound = 0;  // every neighbour pair of every center
+  for (const std::int32_t c : centers) {
+    if (c >= 0 && c < graph.dimension) {
+      const auto d = static_cast<std::uint64_t>(graph.Degree(c));
+      bound += d < 2 ? 0 : d * (d - 1) / 2;
+    }
+  }
+  HtHamiltonReplyDeviceBatch out;
+  out.offsets.assign(centers.size() + 1, 0);
+  std::vector<ee_ht_neighbor_pair> replies(bound);
+  std::uint64_t total = 0;
+  ee_ht_usage usage{};
+  Throw(ee_ht_hamilton_replies(ctx, &g.view, target_edge.u, target_edge.v, centers.data(),
+                               centers.size(), out.offsets.data(), replies.data(), bound, &total,
+                               &usage),
+        "HIP HT Hamilton replies");
+  out.replies.resize(total);
+  for (std::uint64_t i = 0; i < total; ++i) {
+    out.replies[i] = {replies[i].center, replies[i].first, replies[i].second};
+  }
+  FillUsage(usage, cache_usage);
+  return out;
+}
+
+bool HtEndReplyCudaAvailable(std::string* const reason) { return Acquire(&t_ht, reason) != nullptr; }
+
+HtEndReplyDeviceBatch EvaluateHtEndRepliesCuda(const GraphSnapshot& graph,
+                                               const std::vector<HtEndReplyTask>& tasks,
+                                               int* const selected_device,
+                                               HtReplyCudaCacheUsage* const cache_usage) {
+  if (cache_usage != nullptr) *cache_usage = {};
+  ee_ctx* ctx = HtContext(selected_device);
+  const HtGraphView g(graph);
+  std::uint64_t bound = 0;
+  for (const HtEndReplyTask& t : tasks) {
+    if (t.endpoint >= 0 && t.endpoint < graph.dimension) {
+      bound += static_cast<std::uint64_t>(std::max(graph.Degree(t.endpoint) - 1, 0));
+    }
+  }
+  HtEndReplyDeviceBatch out;
+  out.offsets.assign(tasks.size() + 1, 0);
+  std::vector<ee_node_edge> replies(bound);
+  std::uint64_t total = 0;
+  ee_ht_usage usage{};
+  Throw(ee_ht_end_replies(ctx, &g.view, reinterpret_cast<const ee_ht_end_task*>(tasks.data()),
+                          tasks.size(), out.offsets.data(), replies.data(), bound, &total, &usage),
+        "HIP HT end replies");
+  out.replies.resize(total);
+  for (std::uint64_t i = 0; i < total; ++i) out.replies[i] = {replies[i].u, replies[i].v};
+  FillUsage(usage, cache_usage);
+  return out;
+}
+
+void ClearHtReplyCudaCache() { t_ht.Reset(); }
 
 }  // namespace detail
 }  // namespace cudaee

@@ -26,6 +26,7 @@ pytestmark = pytest.mark.gpu
 ROOT = pathlib.Path(__file__).resolve().parent.parent
 CLI = ROOT / "oracle" / "_ref" / "cudaee_hip"
 TEST_CORE = ROOT / "oracle" / "_ref" / "test_core_hip"
+TEST_HT = ROOT / "oracle" / "_ref" / "test_ht_hip"
 RP = ROOT / "tests" / "golden" / "recursive-point"
 
 
@@ -46,6 +47,17 @@ def test_reference_unit_test_core_passes_on_device(tmp_path):
     need(TEST_CORE)
     done = run([TEST_CORE], tmp_path)
     assert done.returncode == 0 and "unit tests passed" in done.stdout
+
+
+def test_reference_unit_test_hamilton_tutte_passes_on_device(tmp_path):
+    # the reference's HT unit test with its device sections on: its Hamilton
+    # and end reply checks (tests/unit/test_hamilton_tutte.cpp:233-265,
+    # 345-380) compare the shim's EvaluateHt*RepliesCuda with the CPU batch,
+    # including the cache-hit pattern; its other device paths are absent and
+    # report themselves skipped
+    need(TEST_HT)
+    done = run([TEST_HT], tmp_path)
+    assert done.returncode == 0 and "Hamilton-Tutte tests passed" in done.stdout
 
 
 @pytest.mark.parametrize("name", ["config1", "config2", "recursive"])
