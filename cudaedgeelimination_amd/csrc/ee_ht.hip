@@ -130,30 +130,57 @@ __global__ __launch_bounds__(kHtBlock) void HtHamiltonKernel(
 }
 
 // One workgroup: offsets[0] = 0, offsets[i + 1] = counts[0] + ... + counts[i].
-// Each thread sums a contiguous segment, the block scans the segment sums,
-// and each thread then writes its segment's running prefix.
-__global__ __launch_bounds__(1024) void HtScanKernel(const uint64_t* __restrict__ counts, uint64_t n,
-                                                     uint64_t* __restrict__ offsets) {
-  __shared__ uint64_t sums[1024];
-  const uint32_t t = threadIdx.x;
-  const uint64_t per = (n + 1023) / 1024;
-  const uint64_t lo = t * per < n ? t * per : n;
-  const uint64_t hi = lo + per < n ? lo + per : n;
-  uint64_t s = 0;
-  for (uint64_t i = lo; i < hi; ++i) s += counts[i];
-  sums[t] = s;
-  __syncthreads();
-  for (uint32_t step = 1; step < 1024; step <<= 1) {
-    const uint64_t add = t >= step ? sums[t - step] : 0;
-    __syncthreads();
-    sums[t] += add;
-    __syncthreads();
+// Chunks of 8,192 counts: each thread reads 8 consecutive counts (a
+// wavefront reads 4 KB in one sweep), the threads' sums are scanned across
+// the wavefront with shuffles and across the 16 wavefronts in LDS, and a
+// running carry joins the chunks.
+constexpr uint32_t kScanThreads = 1024;
+constexpr uint32_t kScanPerThread = 8;
+
+__global__ __launch_bounds__(kScanThreads) void HtScanKernel(const uint64_t* __restrict__ counts,
+                                                              uint64_t n,
+                                                              uint64_t* __restrict__ offsets) {
+  __shared__ unsigned long long wave_sums[kScanThreads / 64];
+  __shared__ unsigned long long carry_s;
+  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+  if (t == 0) {
+    offsets[0] = 0;
+    carry_s = 0;
   }
-  uint64_t run = sums[t] - s;  // exclusive prefix of this segment
-  if (t == 0) offsets[0] = 0;
-  for (uint64_t i = lo; i < hi; ++i) {
-    run += counts[i];
-    offsets[i + 1] = run;
+  __syncthreads();
+  for (uint64_t base = 0; base < n; base += kScanThreads * kScanPerThread) {
+    const uint64_t first = base + static_cast<uint64_t>(t) * kScanPerThread;
+    unsigned long long v[kScanPerThread];
+    unsigned long long own = 0;
+    for (uint32_t e = 0; e < kScanPerThread; ++e) {
+      v[e] = first + e < n ? counts[first + e] : 0;
+      own += v[e];
+    }
+    unsigned long long inc = own;  // inclusive scan of the threads' sums in the wavefront
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+      const unsigned long long o = __shfl_up(inc, d, 64);
+      if (lane >= d) inc += o;
+    }
+    if (lane == 63) wave_sums[wave] = inc;
+    __syncthreads();
+    if (wave == 0) {
+      unsigned long long w = lane < kScanThreads / 64 ? wave_sums[lane] : 0;
+      for (uint32_t d = 1; d < kScanThreads / 64; d <<= 1) {
+        const unsigned long long o = __shfl_up(w, d, 64);
+        if (lane >= d) w += o;
+      }
+      if (lane < kScanThreads / 64) wave_sums[lane] = w;
+    }
+    __syncthreads();
+    const unsigned long long carry = carry_s;
+    unsigned long long run = carry + (wave > 0 ? wave_sums[wave - 1] : 0) + inc - own;
+    for (uint32_t e = 0; e < kScanPerThread; ++e) {
+      run += v[e];
+      if (first + e < n) offsets[first + e + 1] = run;
+    }
+    __syncthreads();
+    if (t == 0) carry_s = carry + wave_sums[kScanThreads / 64 - 1];
+    __syncthreads();
   }
 }
 
@@ -324,7 +351,7 @@ void HamiltonReplies(Context& ctx, const ee_graph_view* graph, int32_t target_u,
                      static_cast<int>(g.distance_type), tu, tv, c.d_items.get(),
                      static_cast<uint64_t>(count), c.d_counts.get(), nullptr, nullptr);
   CheckHip(hipGetLastError(), "HtHamiltonKernel count launch");
-  hipLaunchKernelGGL(HtScanKernel, dim3(1), dim3(1024), 0, ctx.stream, c.d_counts.get(),
+  hipLaunchKernelGGL(HtScanKernel, dim3(1), dim3(kScanThreads), 0, ctx.stream, c.d_counts.get(),
                      static_cast<uint64_t>(count), c.d_offsets.get());
   CheckHip(hipGetLastError(), "HtScanKernel launch");
   hipLaunchKernelGGL(HtHamiltonKernel<true>, dim3(static_cast<uint32_t>(blocks)), dim3(kHtBlock), 0,
