@@ -37,27 +37,36 @@ struct HtNeighbor {
   int32_t survives;
 };
 
+// kNear: both coordinate extents of the snapshot are below 2^25, so the
+// differences fit int32 and the FP64 root with one exact remainder applies
+// (ExactDistanceNear); otherwise the 64-bit integer path.
+template <bool kNear>
 __device__ inline int64_t Dist(const longlong2* xy, int32_t a, int32_t b, int type) {
   const longlong2 p = xy[a], q = xy[b];
-  return ExactDistance(p.x, p.y, q.x, q.y, type);
+  if constexpr (kNear) {
+    return ExactDistanceNear(static_cast<int32_t>(p.x - q.x), static_cast<int32_t>(p.y - q.y), type);
+  } else {
+    return ExactDistance(p.x, p.y, q.x, q.y, type);
+  }
 }
 
 // EdgesSurviveTwoOpt for the edge (a, b) against the target (tu, tv): at
 // least one 2-opt orientation is not a strict improvement.  Distances of a
 // distance-safe snapshot are below 2^32, so the 64-bit sums are exact.
+template <bool kNear>
 __device__ inline bool Survives(const longlong2* xy, int32_t tu, int32_t tv, int64_t target_cost,
                                 int32_t a, int32_t b, int64_t ab, int type) {
   const int32_t lo = min(a, b), hi = max(a, b);
   const int64_t original = target_cost + ab;
-  const int64_t first = Dist(xy, tu, hi, type) + Dist(xy, tv, lo, type);
-  const int64_t second = Dist(xy, tu, lo, type) + Dist(xy, tv, hi, type);
+  const int64_t first = Dist<kNear>(xy, tu, hi, type) + Dist<kNear>(xy, tv, lo, type);
+  const int64_t second = Dist<kNear>(xy, tu, lo, type) + Dist<kNear>(xy, tv, hi, type);
   return first >= original || second >= original;
 }
 
 // First index of row i of the pairs (i, j > i) of d items, first-major.
 __device__ inline uint64_t PairRowStart(uint64_t i, uint64_t d) { return i * (2 * d - i - 1) / 2; }
 
-template <bool kWrite>
+template <bool kWrite, bool kNear>
 __global__ __launch_bounds__(kHtBlock) void HtHamiltonKernel(
     const longlong2* __restrict__ xy, const int32_t* __restrict__ row_offsets,
     const int32_t* __restrict__ neighbors, int type, int32_t tu, int32_t tv,
@@ -70,14 +79,14 @@ __global__ __launch_bounds__(kHtBlock) void HtHamiltonKernel(
   const int32_t c = centers[item];
   const int32_t begin = row_offsets[c];
   const uint64_t d = static_cast<uint64_t>(row_offsets[c + 1] - begin);
-  const int64_t target_cost = Dist(xy, tu, tv, type);
-  const int64_t center_target = Dist(xy, tu, c, type) + Dist(xy, tv, c, type);
+  const int64_t target_cost = Dist<kNear>(xy, tu, tv, type);
+  const int64_t center_target = Dist<kNear>(xy, tu, c, type) + Dist<kNear>(xy, tv, c, type);
   const bool cached = d <= kHtLdsNeighbors;
   HtNeighbor* mine = cache[wave];
   if (cached && lane < d) {
     const int32_t nb = neighbors[begin + lane];
-    const int64_t cost = Dist(xy, c, nb, type);
-    mine[lane] = {cost, nb, Survives(xy, tu, tv, target_cost, c, nb, cost, type) ? 1 : 0};
+    const int64_t cost = Dist<kNear>(xy, c, nb, type);
+    mine[lane] = {cost, nb, Survives<kNear>(xy, tu, tv, target_cost, c, nb, cost, type) ? 1 : 0};
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -103,15 +112,15 @@ __global__ __launch_bounds__(kHtBlock) void HtHamiltonKernel(
       } else {
         f.node = neighbors[begin + i];
         s.node = neighbors[begin + j];
-        f.center_cost = Dist(xy, c, f.node, type);
-        s.center_cost = Dist(xy, c, s.node, type);
-        f.survives = Survives(xy, tu, tv, target_cost, c, f.node, f.center_cost, type);
-        s.survives = Survives(xy, tu, tv, target_cost, c, s.node, s.center_cost, type);
+        f.center_cost = Dist<kNear>(xy, c, f.node, type);
+        s.center_cost = Dist<kNear>(xy, c, s.node, type);
+        f.survives = Survives<kNear>(xy, tu, tv, target_cost, c, f.node, f.center_cost, type);
+        s.survives = Survives<kNear>(xy, tu, tv, target_cost, c, s.node, s.center_cost, type);
       }
       const bool is_target = min(f.node, s.node) == tu && max(f.node, s.node) == tv;
       if (!is_target && f.survives && s.survives) {
         const int64_t original = target_cost + f.center_cost + s.center_cost;
-        const int64_t replacement = Dist(xy, f.node, s.node, type) + center_target;
+        const int64_t replacement = Dist<kNear>(xy, f.node, s.node, type) + center_target;
         keep = original <= replacement;
       }
       if (kWrite && keep) {
@@ -284,6 +293,7 @@ bool PrepareHtGraph(Context& ctx, const ee_graph_view& g) {
   c.row_offsets.assign(g.row_offsets, g.row_offsets + n + 1);
   c.neighbors.assign(g.neighbors, g.neighbors + a);
   c.active.assign(g.edge_active, g.edge_active + m);
+  c.near = NearExtent(g.x, g.y, g.dimension);
   c.dimension = g.dimension;
   c.distance_type = g.distance_type;
   c.valid = true;
@@ -346,7 +356,9 @@ void HamiltonReplies(Context& ctx, const ee_graph_view* graph, int32_t target_u,
 
   t0 = std::chrono::steady_clock::now();
   const uint64_t blocks = (count + kHtWaves - 1) / kHtWaves;
-  hipLaunchKernelGGL(HtHamiltonKernel<false>, dim3(static_cast<uint32_t>(blocks)), dim3(kHtBlock), 0,
+  const auto count_kernel = c.near ? HtHamiltonKernel<false, true> : HtHamiltonKernel<false, false>;
+  const auto write_kernel = c.near ? HtHamiltonKernel<true, true> : HtHamiltonKernel<true, false>;
+  hipLaunchKernelGGL(count_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(kHtBlock), 0,
                      ctx.stream, c.d_xy.get(), c.d_row_offsets.get(), c.d_neighbors.get(),
                      static_cast<int>(g.distance_type), tu, tv, c.d_items.get(),
                      static_cast<uint64_t>(count), c.d_counts.get(), nullptr, nullptr);
@@ -354,7 +366,7 @@ void HamiltonReplies(Context& ctx, const ee_graph_view* graph, int32_t target_u,
   hipLaunchKernelGGL(HtScanKernel, dim3(1), dim3(kScanThreads), 0, ctx.stream, c.d_counts.get(),
                      static_cast<uint64_t>(count), c.d_offsets.get());
   CheckHip(hipGetLastError(), "HtScanKernel launch");
-  hipLaunchKernelGGL(HtHamiltonKernel<true>, dim3(static_cast<uint32_t>(blocks)), dim3(kHtBlock), 0,
+  hipLaunchKernelGGL(write_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(kHtBlock), 0,
                      ctx.stream, c.d_xy.get(), c.d_row_offsets.get(), c.d_neighbors.get(),
                      static_cast<int>(g.distance_type), tu, tv, c.d_items.get(),
                      static_cast<uint64_t>(count), nullptr, c.d_offsets.get(), c.d_replies.get());

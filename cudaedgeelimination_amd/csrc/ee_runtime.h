@@ -220,6 +220,7 @@ struct HtCache {
   std::vector<int64_t> x, y;
   std::vector<int32_t> row_offsets, neighbors;
   std::vector<uint8_t> active;  // the snapshot passed the HT checks with these flags
+  bool near = false;            // NearExtent: the FP64 distance path applies
   DeviceArray<longlong2> d_xy;
   DeviceArray<int32_t> d_row_offsets, d_neighbors;
   DeviceArray<int32_t> d_items;      // centers, or (endpoint, internal) task pairs

@@ -4,7 +4,8 @@ The oracle's enumerations equal the reference CPU path's
 (tests/test_oracle_ht.py).  Here the device's offsets and replies must
 equal the oracle's on the same seeded cases: three graphs (uniform,
 lattice with tied distances, CEIL_2D), a dense graph whose rows exceed the
-64 neighbours a wavefront caches in LDS, all 30,000 nodes of a served-size
+64 neighbours a wavefront caches in LDS, a graph with coordinate extents
+beyond 2^25 (the 64-bit distance path; the others take the FP64 one), all 30,000 nodes of a served-size
 graph as centers, end replies, the cache-hit pattern of the reference's
 HtReplyCudaCacheUsage, and the input checks of its CUDA path
 (src/cuda/ht_replies.cu:521-580).
@@ -33,10 +34,13 @@ def snapshots():
     gs = graphs()
     x, y = W._points_uniform(0xEE0000D2, 300, 0, 10**5)
     gs["dense90"] = W.knn_graph(x, y, EUC_2D, 90)  # rows of 90+ neighbours
+    # coordinate extents beyond 2^25: the 64-bit integer distance path
+    x, y = W._points_uniform(0xEE0000D3, 400, 0, 1 << 30)
+    gs["wide"] = W.knn_graph(x, y, EUC_2D, 10)
     return gs
 
 
-@pytest.mark.parametrize("name", ["uniform300", "lattice", "ceil400", "dense90"])
+@pytest.mark.parametrize("name", ["uniform300", "lattice", "ceil400", "dense90", "wide"])
 def test_hamilton_replies_match_oracle(ctx, snapshots, name):
     g = snapshots[name]
     say(f"ht hamilton {name} max degree {int(np.diff(g.row_offsets).max())}")
@@ -48,7 +52,7 @@ def test_hamilton_replies_match_oracle(ctx, snapshots, name):
         assert np.array_equal(rep, wrep)
 
 
-@pytest.mark.parametrize("name", ["uniform300", "lattice", "ceil400", "dense90"])
+@pytest.mark.parametrize("name", ["uniform300", "lattice", "ceil400", "dense90", "wide"])
 def test_end_replies_match_oracle(ctx, snapshots, name):
     g = snapshots[name]
     say(f"ht end {name}")
