@@ -1,8 +1,8 @@
 // Drop-in C++ backend for cudaEdgeElimination over the gfx950 C ABI.
 //
 // This translation unit takes the place of the reference's CUDA units for
-// the two candidate-generation paths (src/cuda/jv_cuda.cu and
-// src/cuda/kopt_cost.cu): it defines the same C++ symbols, declared in the
+// the candidate-generation paths (src/cuda/jv_cuda.cu, src/cuda/kopt_cost.cu
+// and src/cuda/ht_replies.cu): it defines the same C++ symbols, declared in the
 // reference headers, so the reference's callers (RunJvElimination,
 // EvaluateKOptTemplateCosts, the HT leaf batches, the CLI, the unit tests)
 // link unchanged against libee_hip.so:
