@@ -253,13 +253,7 @@ def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005, jv_share:
     import threading
 
     graphs = W.config5_graphs()
-    pools = []
-    for i, g in enumerate(graphs):
-        per_k = {}
-        for k in (3, 4, 5):
-            ks = np.full(2048, k, np.uint8)
-            per_k[k] = W.kopt_tasks(seed + 17 * i + k, g, ks)
-        pools.append(per_k)
+    pools = W.config5_row_pools(graphs, seed)
     out = {"graphs": [g.dimension for g in graphs], "points": []}
     for qps in qps_list:
         rng = np.random.default_rng(seed + qps)
@@ -272,25 +266,7 @@ def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005, jv_share:
             # at least 10,000 requests, so that p99 rests on 100 of them
             count = max(int(qps * seconds), 10000)
             arrivals = np.cumsum(rng.exponential(1.0 / qps, count))
-        cells = np.clip(np.exp(rng.normal(np.log(2896.0), 1.04, count)), 128, 65536)
-        kinds = rng.choice([3, 4, 5], size=count, p=[0.5, 0.3, 0.2])
-        which = rng.integers(0, len(graphs), count)
-        is_jv = rng.random(count) < jv_share
-        jv_sizes = np.clip(np.exp(rng.normal(np.log(256.0), 1.0, count)), 16, 4096).astype(int)
-        requests = []
-        for i in range(count):
-            g = int(which[i])
-            if is_jv[i]:
-                m = graphs[g].edge_count
-                requests.append((g, 0, rng.choice(m, size=min(int(jv_sizes[i]), m),
-                                                  replace=False).astype(np.int32)))
-                continue
-            k = int(kinds[i])
-            rows = max(1, int(np.ceil(cells[i] / TEMPLATES[k])))
-            pool = pools[g][k]
-            rows = min(rows, len(pool))
-            start = int(rng.integers(0, len(pool) - rows + 1))
-            requests.append((g, k, pool[start:start + rows]))
+        requests, is_jv = W.config5_requests(graphs, pools, count, rng, jv_share)
         kopt_cells = [len(t) * TEMPLATES[k] for _, k, t in requests if k]
         point = {"qps": qps if qps else "burst", "requests": count, "jv_rechecks": int(is_jv.sum()),
                  "mean_cells": float(np.mean(kopt_cells)),
@@ -615,6 +591,9 @@ def main() -> int:
     ms_per_step = wall_pipelined * 1e3 / args.steps
     value = cells / (wall_pipelined / args.steps)
     gpu_out = {k: batch.segment_costs(i) for i, k in enumerate((3, 4, 5))}
+    # every pool the pipelined launches wrote, compared with the CPU below
+    pool_sha = [{k: sha(batch.segment_costs(i, pool)) for i, k in enumerate((3, 4, 5))}
+                for pool in range(min(PIPELINE_DEPTH, args.steps))]
     algo_bytes = kopt_algorithmic_bytes(wl)
     achieved = algo_bytes / (device_ms * 1e-3) / 1e9
     pipelined_achieved = algo_bytes / (pipelined_ms * 1e-3) / 1e9
@@ -703,8 +682,13 @@ def main() -> int:
         want_masks = np.concatenate([pack_improving(cpu_out[k], wl.bucket(k), TEMPLATES[k], masks["words_per_row"])
                                      for k in (3, 4, 5)])
         masks["equal_cpu"] = bool(np.array_equal(gpu_masks, want_masks))
-        result["parity"] = {"kopt_cells_equal_cpu": all(
-            np.array_equal(gpu_out[k], cpu_out[k]) for k in (3, 4, 5))}
+        cpu_sha = {k: sha(cpu_out[k]) for k in (3, 4, 5)}
+        result["parity"] = {
+            "kopt_cells_equal_cpu": all(np.array_equal(gpu_out[k], cpu_out[k]) for k in (3, 4, 5)),
+            # each pool of the pipelined (headline) launches, by SHA-256
+            "kopt_pools_checked": len(pool_sha),
+            "kopt_pools_equal_cpu": all(p[k] == cpu_sha[k] for p in pool_sha for k in (3, 4, 5)),
+        }
     else:
         result["cpu_baseline"] = None
 

@@ -144,7 +144,8 @@ class JvRunStats(C.Structure):
         ("proposed", C.c_uint64), ("committed", C.c_uint64), ("initial_hash", C.c_uint64),
         ("final_hash", C.c_uint64), ("device_ms", C.c_double), ("h2d_ms", C.c_double),
         ("d2h_ms", C.c_double), ("propose_ms", C.c_double), ("verify_ms", C.c_double),
-        ("commit_ms", C.c_double), ("total_ms", C.c_double),
+        ("commit_ms", C.c_double), ("total_ms", C.c_double), ("compact_ms", C.c_double),
+        ("hash_wait_ms", C.c_double),
     ]
 
 
@@ -209,6 +210,7 @@ PROTOTYPES = {
     "ee_kopt_batch_run": (C.c_int, [P, U32, C.c_int, C.POINTER(DBL)]),
     "ee_kopt_batch_set_pipeline_depth": (C.c_int, [P, U32]),
     "ee_kopt_batch_download": (C.c_int, [P, U64, U64, P]),
+    "ee_kopt_batch_download_pool": (C.c_int, [P, U32, U64, U64, P]),
     "ee_kopt_batch_set_output": (C.c_int, [P, U32]),
     "ee_kopt_batch_mask_words": (C.c_int, [P, C.POINTER(U32)]),
     "ee_kopt_batch_download_masks": (C.c_int, [P, U64, U64, P]),
@@ -217,6 +219,7 @@ PROTOTYPES = {
     "ee_jv_session_destroy": (None, [P]),
     "ee_jv_session_run": (C.c_int, [P, U32, C.POINTER(JvRunStats)]),
     "ee_jv_session_proof": (C.c_int, [P, C.c_char_p, SZ, C.POINTER(SZ)]),
+    "ee_jv_session_device_snapshot": (C.c_int, [P, P, P, P, P, SZ]),
     "ee_jv_session_active": (C.c_int, [P, P]),
     "ee_jv_fleet_create": (C.c_int, [P, C.POINTER(GraphView), U32, C.POINTER(P)]),
     "ee_jv_fleet_destroy": (None, [P]),
@@ -240,6 +243,7 @@ PROTOTYPES = {
     "ee_ht_hamilton_replies": (C.c_int, [P, P, I32, I32, P, SZ, P, P, U64, C.POINTER(U64),
                                          C.POINTER(HtUsage)]),
     "ee_ht_end_replies": (C.c_int, [P, P, P, SZ, P, P, U64, C.POINTER(U64), C.POINTER(HtUsage)]),
+    "ee_ht_copy_replies": (C.c_int, [P, U64, U64, P]),
     "ee_ht_clear_cache": (C.c_int, [P]),
     "ee_kopt_cursor_count": (C.c_int, [C.POINTER(TourView), U32, C.POINTER(U64)]),
     "ee_kopt_cursor_eval": (C.c_int, [P, P, P, I32, U8, C.POINTER(TourView), U32, P, U32, U64, U64,

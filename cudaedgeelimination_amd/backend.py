@@ -212,17 +212,19 @@ def evaluate_ht_hamilton_replies_cuda(graph: GraphSnapshot, target, centers,
     (center, first, second), usage)."""
     ctx = _ctx(ctx)
     centers = np.ascontiguousarray(centers, dtype=np.int32)
-    deg = np.diff(graph.row_offsets).astype(np.int64)[centers] if len(centers) else np.zeros(0)
-    bound = int((deg * (deg - 1) // 2).sum()) if len(centers) else 0
     offsets = np.zeros(len(centers) + 1, dtype=np.uint64)
-    replies = np.zeros((max(bound, 1), 3), dtype=np.int32)
     total, usage = C.c_uint64(), _lib.HtUsage()
     view = graph.view()
+    # the reply count is known after the device's count pass: size the
+    # buffer from it and copy the replies left on the device
     _lib.check(_lib.lib().ee_ht_hamilton_replies(
         ctx.handle, C.byref(view), int(target[0]), int(target[1]),
         _lib.ptr(centers) if len(centers) else None, len(centers), _lib.ptr(offsets),
-        _lib.ptr(replies), bound, C.byref(total), C.byref(usage)))
-    return offsets, replies[:total.value].copy(), _ht_usage(usage)
+        None, 0, C.byref(total), C.byref(usage)))
+    replies = np.empty((total.value, 3), dtype=np.int32)
+    if total.value:
+        _lib.check(_lib.lib().ee_ht_copy_replies(ctx.handle, 0, total.value, _lib.ptr(replies)))
+    return offsets, replies, _ht_usage(usage)
 
 
 def evaluate_ht_end_replies_cuda(graph: GraphSnapshot, tasks, ctx: Context | None = None):
@@ -231,16 +233,16 @@ def evaluate_ht_end_replies_cuda(graph: GraphSnapshot, tasks, ctx: Context | Non
     int32[total, 2] canonical edges, usage)."""
     ctx = _ctx(ctx)
     tasks = np.ascontiguousarray(np.asarray(tasks, dtype=np.int32).reshape(-1, 2))
-    deg = np.diff(graph.row_offsets).astype(np.int64)
-    bound = int((deg[tasks[:, 0]] - 1).clip(min=0).sum()) if len(tasks) else 0
     offsets = np.zeros(len(tasks) + 1, dtype=np.uint64)
-    replies = np.zeros((max(bound, 1), 2), dtype=np.int32)
     total, usage = C.c_uint64(), _lib.HtUsage()
     view = graph.view()
     _lib.check(_lib.lib().ee_ht_end_replies(
         ctx.handle, C.byref(view), _lib.ptr(tasks) if len(tasks) else None, len(tasks),
-        _lib.ptr(offsets), _lib.ptr(replies), bound, C.byref(total), C.byref(usage)))
-    return offsets, replies[:total.value].copy(), _ht_usage(usage)
+        _lib.ptr(offsets), None, 0, C.byref(total), C.byref(usage)))
+    replies = np.empty((total.value, 2), dtype=np.int32)
+    if total.value:
+        _lib.check(_lib.lib().ee_ht_copy_replies(ctx.handle, 0, total.value, _lib.ptr(replies)))
+    return offsets, replies, _ht_usage(usage)
 
 
 def clear_ht_reply_cuda_cache(ctx: Context | None = None) -> None:

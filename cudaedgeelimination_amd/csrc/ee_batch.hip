@@ -70,6 +70,7 @@ struct Batch {
     hipStream_t stream = nullptr;
     hipEvent_t ev_join = nullptr;
     DeviceArray<int64_t> costs;
+    bool written = false;  // a launch of the current work wrote this pool
   };
   uint32_t pipeline_depth = 2;
   std::vector<PipeLane> lanes;  // lanes 1 .. depth - 1
@@ -216,6 +217,7 @@ struct Batch {
       const uint32_t j = i % pipeline_depth;
       LaunchKoptResident(lane_args[j], d_args.get(), grid, tuning, shared,
                          j == 0 ? ctx->stream : lanes[j - 1].stream);
+      if (j > 0) lanes[j - 1].written = true;
     }
     for (uint32_t j = 1; j < pipeline_depth; ++j) {
       CheckHip(hipEventRecord(lanes[j - 1].ev_join, lanes[j - 1].stream), "event record");
@@ -226,6 +228,11 @@ struct Batch {
     float ms = 0.0f;
     CheckHip(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end), "event elapsed");
     return static_cast<double>(ms) / launches;
+  }
+
+  // Forgets which extra pools hold results (new work or output mode).
+  void ClearWritten() {
+    for (PipeLane& l : lanes) l.written = false;
   }
 
   uint64_t ResidentBytes() const {
@@ -550,6 +557,7 @@ ee_status ee_kopt_batch_set_work(ee_kopt_batch* batch, const ee_kopt_segment_des
     if (cell_offsets) cell_offsets[segment_count] = cells;
     b.ctx->Activate();
     b.DropGraph();
+    b.ClearWritten();
     b.d_tasks.Allocate(std::max<uint64_t>(rows, 1), b.ctx->device);
     b.d_tasks.Upload(tasks, rows, b.ctx->stream);
     b.d_costs.Allocate(std::max<uint64_t>(cells, 1), b.ctx->device);
@@ -632,12 +640,37 @@ ee_status ee_kopt_batch_download(ee_kopt_batch* batch, uint64_t first_cell, uint
   });
 }
 
+ee_status ee_kopt_batch_download_pool(ee_kopt_batch* batch, uint32_t pool, uint64_t first_cell,
+                                      uint64_t cell_count, int64_t* costs) {
+  return Guarded([&] {
+    ee::Batch& b = ee::AsBatch(batch);
+    if (pool > 0 && (pool > b.lanes.size() || !b.lanes[pool - 1].written)) {
+      throw std::logic_error("no launch of the current work wrote this cost pool");
+    }
+    if (first_cell > b.cell_count || cell_count > b.cell_count - first_cell) {
+      throw std::invalid_argument("cell range outside the batch output");
+    }
+    if (cell_count > 0 && costs == nullptr) throw std::invalid_argument("costs is NULL");
+    b.ctx->Activate();
+    if (pool == 0) {
+      b.d_costs.DownloadAt(first_cell, costs, cell_count, b.ctx->stream);
+    } else {
+      ee::Batch::PipeLane& l = b.lanes[pool - 1];
+      // the lane's launches ran on its own stream; wait for them first
+      ee::CheckHip(hipStreamSynchronize(l.stream), "hipStreamSynchronize");
+      l.costs.DownloadAt(first_cell, costs, cell_count, b.ctx->stream);
+    }
+    b.ctx->Sync();
+  });
+}
+
 ee_status ee_kopt_batch_set_output(ee_kopt_batch* batch, uint32_t mode) {
   return Guarded([&] {
     ee::Batch& b = ee::AsBatch(batch);
     if (mode > 1) throw std::invalid_argument("output mode must be 0 (costs) or 1 (improving masks)");
     b.ctx->Activate();
     b.mask_output = mode == 1;
+    b.ClearWritten();
     if (b.mask_output) {
       b.d_masks.Allocate(std::max<uint64_t>(b.task_count * b.MaskWords(), 1), b.ctx->device);
     }

@@ -4,12 +4,14 @@
 // proof) exactly as a standalone session does, so its proof equals the
 // standalone one (src/cpu/jv_elimination.cpp:250-368 per graph).  The device
 // side is shared: the graphs' snapshots sit back to back in one union (node
-// ids, edge slots and adjacency offset per graph), and every epoch is one
-// upload of the union's active flags and CSR, one JV launch over all edge
-// slots and one download of the witnesses -- replayable as a captured HIP
-// graph whose copies read fixed-capacity pinned staging.  A graph that has
-// reached its fixed point has its slots marked inactive, so the kernel skips
-// them, and its run is not advanced again.
+// ids, edge slots and adjacency offset per graph).  The first epoch uploads
+// the union's active flags and CSR; every later epoch uploads only the
+// union's commit list (the slots every graph committed, plus every live slot
+// of a graph that reached its fixed point, so the kernel skips it), and the
+// device clears those flags, compacts the union CSR (ee_csr.h), runs one JV
+// launch over all edge slots and downloads the witnesses -- a sequence
+// replayable as a captured HIP graph (one per CSR buffer).  Meanwhile the
+// host applies each graph's commit to its own snapshot.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -17,6 +19,7 @@
 #include <string>
 #include <vector>
 
+#include "ee_csr.h"
 #include "ee_jvhost.h"
 #include "ee_runtime.h"
 
@@ -28,7 +31,7 @@ namespace {
 
 struct Fleet {
   explicit Fleet(Context* c) : ctx(c) {}
-  ~Fleet() { DropGraph(); }
+  ~Fleet() { DropGraphs(); }
 
   Context* ctx;
   std::vector<JvHost> hosts;
@@ -39,17 +42,22 @@ struct Fleet {
   DeviceArray<int64_t> d_weight;
   DeviceArray<longlong2> d_xy;
   DeviceArray<uint8_t> d_mode;
-  // device: per epoch
-  DeviceArray<int32_t> d_active, d_row_offsets, d_neighbors, d_edge_ids, d_witnesses;
+  DeviceArray<int32_t> d_witnesses;
+  // device: active flags and CSR, updated by commits
+  DeviceCsr csr;
   Pinned<int32_t> h_active, h_row_offsets, h_neighbors, h_edge_ids, h_witnesses;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // one captured epoch (commit, launch, download) per CSR buffer
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  std::vector<uint8_t> retired;  // graph done and its slots deactivated on the device
 
-  void DropGraph() {
-    if (exec != nullptr) static_cast<void>(hipGraphExecDestroy(exec));
-    if (graph != nullptr) static_cast<void>(hipGraphDestroy(graph));
-    exec = nullptr;
-    graph = nullptr;
+  void DropGraphs() {
+    for (int p = 0; p < 2; ++p) {
+      if (exec[p] != nullptr) static_cast<void>(hipGraphExecDestroy(exec[p]));
+      if (graph[p] != nullptr) static_cast<void>(hipGraphDestroy(graph[p]));
+      exec[p] = nullptr;
+      graph[p] = nullptr;
+    }
   }
 
   void Load(const ee_graph_view* graphs, uint32_t count) {
@@ -67,7 +75,7 @@ struct Fleet {
     nodes = node_base[count];
     slots = edge_base[count];
     adjacency = std::max<size_t>(adj, 1);
-    if (nodes >= (size_t{1} << 31) || slots >= (size_t{1} << 31) || adjacency >= (size_t{1} << 31)) {
+    if (nodes >= (size_t{1} << 31) || slots >= (size_t{1} << 31) - 1 || adjacency >= (size_t{1} << 31)) {
       throw std::invalid_argument("JV fleet union exceeds int32 ids");
     }
     std::vector<int2> uv(std::max<size_t>(slots, 1));
@@ -91,18 +99,14 @@ struct Fleet {
     d_weight.Allocate(weight.size(), dev);
     d_mode.Allocate(mode.size(), dev);
     d_xy.Allocate(xy.size(), dev);
-    d_active.Allocate(uv.size(), dev);
     d_witnesses.Allocate(uv.size(), dev);
-    d_row_offsets.Allocate(nodes + 1, dev);
-    d_neighbors.Allocate(adjacency, dev);
-    d_edge_ids.Allocate(adjacency, dev);
+    // a commit deactivates each slot at most once
+    csr.Allocate(nodes, slots, adjacency, slots, dev);
     h_active.Reserve(uv.size());
     h_witnesses.Reserve(uv.size());
     h_row_offsets.Reserve(nodes + 1);
     h_neighbors.Reserve(adjacency);
     h_edge_ids.Reserve(adjacency);
-    std::fill(h_neighbors.get(), h_neighbors.get() + adjacency, 0);
-    std::fill(h_edge_ids.get(), h_edge_ids.get() + adjacency, 0);
     d_uv.Upload(uv.data(), uv.size(), ctx->stream);
     d_weight.Upload(weight.data(), weight.size(), ctx->stream);
     d_mode.Upload(mode.data(), mode.size(), ctx->stream);
@@ -110,9 +114,8 @@ struct Fleet {
     ctx->Sync();
   }
 
-  // Writes the union's active flags and CSR into the pinned staging; graphs
-  // whose run is done contribute inactive slots.
-  void Assemble() {
+  // The union's active flags and CSR from the host snapshots (first epoch).
+  size_t AssembleFull() {
     std::vector<size_t> adj_base(hosts.size() + 1, 0);
     for (size_t i = 0; i < hosts.size(); ++i) adj_base[i + 1] = adj_base[i] + hosts[i].neighbors.size();
     const size_t adj = adj_base[hosts.size()];
@@ -120,7 +123,7 @@ struct Fleet {
     ParallelFor(hosts.size(), [&](size_t i) {
       const JvHost& h = hosts[i];
       const size_t eb = edge_base[i], nb = node_base[i], ab = adj_base[i];
-      for (int32_t e = 0; e < h.m; ++e) h_active.get()[eb + e] = (!h.done && h.active[e]) ? 1 : 0;
+      for (int32_t e = 0; e < h.m; ++e) h_active.get()[eb + e] = h.active[e] ? 1 : 0;
       for (int32_t vtx = 0; vtx < h.n; ++vtx) {
         h_row_offsets.get()[nb + vtx] = static_cast<int32_t>(ab) + h.row_offsets[vtx];
       }
@@ -130,6 +133,41 @@ struct Fleet {
       }
     });
     h_row_offsets.get()[nodes] = static_cast<int32_t>(adj);
+    return adj;
+  }
+
+  // The union commit list of this epoch into the CSR's pinned staging: each
+  // graph's committed slots, and every live slot of a graph that stopped.
+  void StageCommits(const std::vector<const std::vector<int32_t>*>& commits) {
+    std::vector<size_t> at(hosts.size() + 1, 0);
+    std::vector<uint8_t> retire(hosts.size(), 0);
+    for (size_t i = 0; i < hosts.size(); ++i) {
+      size_t c = 0;
+      if (!retired[i]) {
+        if (commits[i] != nullptr && !commits[i]->empty()) {
+          c = commits[i]->size();
+        } else {
+          retire[i] = 1;
+          for (int32_t e = 0; e < hosts[i].m; ++e) c += hosts[i].active[e] != 0;
+        }
+      }
+      at[i + 1] = at[i] + c;
+    }
+    int32_t* staged = csr.StageCommits();
+    staged[0] = static_cast<int32_t>(at[hosts.size()]);
+    ParallelFor(hosts.size(), [&](size_t i) {
+      int32_t* out = staged + 1 + at[i];
+      const int32_t eb = static_cast<int32_t>(edge_base[i]);
+      if (retire[i]) {
+        const JvHost& h = hosts[i];
+        for (int32_t e = 0; e < h.m; ++e) {
+          if (h.active[e]) *out++ = eb + e;
+        }
+      } else if (!retired[i] && commits[i] != nullptr) {
+        for (const int32_t e : *commits[i]) *out++ = eb + e;
+      }
+    });
+    for (size_t i = 0; i < hosts.size(); ++i) retired[i] = retired[i] || retire[i];
   }
 
   JvLaunch Args() {
@@ -138,34 +176,55 @@ struct Fleet {
     a.distance_type = 0;
     a.edge_uv = d_uv.get();
     a.edge_weight = d_weight.get();
-    a.edge_active = d_active.get();
-    a.row_offsets = d_row_offsets.get();
-    a.neighbors = d_neighbors.get();
-    a.csr_edge_ids = d_edge_ids.get();
+    a.edge_active = csr.active();
+    a.row_offsets = csr.row_offsets();
+    a.neighbors = csr.neighbors();
+    a.csr_edge_ids = csr.edge_ids();
     a.xy = d_xy.get();
     a.witnesses = d_witnesses.get();
     a.edge_mode = d_mode.get();
     return a;
   }
 
-  // Upload, launch, download: the whole device side of one epoch.
-  void Enqueue() {
-    const size_t m = std::max<size_t>(slots, 1);
-    CheckHip(hipMemcpyAsync(d_active.get(), h_active.get(), m * 4, hipMemcpyHostToDevice, ctx->stream),
-             "fleet upload");
-    CheckHip(hipMemcpyAsync(d_row_offsets.get(), h_row_offsets.get(), (nodes + 1) * 4,
-                            hipMemcpyHostToDevice, ctx->stream),
-             "fleet upload");
-    CheckHip(hipMemcpyAsync(d_neighbors.get(), h_neighbors.get(), adjacency * 4, hipMemcpyHostToDevice,
-                            ctx->stream),
-             "fleet upload");
-    CheckHip(hipMemcpyAsync(d_edge_ids.get(), h_edge_ids.get(), adjacency * 4, hipMemcpyHostToDevice,
-                            ctx->stream),
-             "fleet upload");
+  // Launch and download (after a commit, which selects the CSR buffer).
+  void EnqueueLaunch() {
     LaunchJv(Args(), ctx->jv_grid, ctx->stream);
-    CheckHip(hipMemcpyAsync(h_witnesses.get(), d_witnesses.get(), m * 4, hipMemcpyDeviceToHost,
-                            ctx->stream),
+    CheckHip(hipMemcpyAsync(h_witnesses.get(), d_witnesses.get(), std::max<size_t>(slots, 1) * 4,
+                            hipMemcpyDeviceToHost, ctx->stream),
              "fleet download");
+  }
+
+  // The device side of one epoch; the first uploads the union, later ones
+  // commit the staged list on the device.
+  void EnqueueEpoch(bool first, bool use_graph, ee_jv_fleet_stats* st) {
+    if (first) {
+      const size_t adj = AssembleFull();
+      csr.Upload(h_active.get(), h_row_offsets.get(), h_neighbors.get(), h_edge_ids.get(), adj, ctx->stream);
+      CheckHip(hipEventRecord(ctx->ev_begin, ctx->stream), "event record");
+      EnqueueLaunch();
+      return;
+    }
+    csr.UploadCommits(ctx->stream);
+    CheckHip(hipEventRecord(ctx->ev_begin, ctx->stream), "event record");
+    if (!use_graph) {
+      csr.EnqueueCompact(ctx->stream);
+      EnqueueLaunch();
+      return;
+    }
+    const int p = csr.current();
+    if (exec[p] == nullptr) {
+      // the captured compaction reads the commit count from device memory,
+      // so the graph serves every later commit of this buffer parity
+      CheckHip(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+      csr.EnqueueCompact(ctx->stream);
+      EnqueueLaunch();
+      CheckHip(hipStreamEndCapture(ctx->stream, &graph[p]), "end capture");
+      CheckHip(hipGraphInstantiate(&exec[p], graph[p], nullptr, nullptr, 0), "graph instantiate");
+    } else {
+      csr.SetCurrent(p ^ 1);  // what the replayed compaction does
+    }
+    CheckHip(hipGraphLaunch(exec[p], ctx->stream), "graph launch");
+    ++st->graph_launches;
   }
 
   void Run(uint32_t max_rounds, bool use_graph, ee_jv_fleet_stats* out) {
@@ -175,43 +234,29 @@ struct Fleet {
     ee_jv_fleet_stats st{};
     st.instances = static_cast<uint32_t>(hosts.size());
     st.edge_slots = slots;
+    retired.assign(hosts.size(), 0);
     // the graphs' host runs are independent: each phase runs over the host cores
-    ParallelFor(hosts.size(), [&](size_t i) { hosts[i].StartRun(); });
-    for (uint32_t epoch = 0; epoch < max_rounds; ++epoch) {
-      bool live = false;
-      for (const JvHost& h : hosts) live = live || !h.done;
-      if (!live) break;
-      ParallelFor(hosts.size(), [&](size_t i) {
-        if (!hosts[i].done) hosts[i].BeginEpoch(epoch);
-      });
-      const auto t0 = std::chrono::steady_clock::now();
-      Assemble();
-      st.assemble_ms += MsSince(t0);
-      if (slots > 0) {
-        if (use_graph && exec == nullptr) {
-          CheckHip(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
-          Enqueue();
-          CheckHip(hipStreamEndCapture(ctx->stream, &graph), "end capture");
-          CheckHip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
-        }
-        CheckHip(hipEventRecord(ctx->ev_begin, ctx->stream), "event record");
-        if (use_graph) {
-          CheckHip(hipGraphLaunch(exec, ctx->stream), "graph launch");
-          ++st.graph_launches;
-        } else {
-          Enqueue();
-        }
-        CheckHip(hipEventRecord(ctx->ev_end, ctx->stream), "event record");
-        CheckHip(hipEventSynchronize(ctx->ev_end), "event synchronize");
-        float ms = 0.0f;
-        CheckHip(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end), "event elapsed");
-        st.device_ms += ms;
-      }
+    ParallelFor(hosts.size(), [&](size_t i) {
+      hosts[i].StartRun();
+      hosts[i].BeginEpoch(0);
+    });
+    std::vector<const std::vector<int32_t>*> commits(hosts.size(), nullptr);
+    auto t0 = std::chrono::steady_clock::now();
+    EnqueueEpoch(true, use_graph, &st);
+    st.assemble_ms += MsSince(t0);
+    for (uint32_t epoch = 0;; ++epoch) {
+      CheckHip(hipEventRecord(ctx->ev_end, ctx->stream), "event record");
+      CheckHip(hipEventSynchronize(ctx->ev_end), "event synchronize");
+      float ms = 0.0f;
+      CheckHip(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end), "event elapsed");
+      st.device_ms += ms;
       const double propose_ms = MsSince(t0);
       st.propose_ms += propose_ms;
-      const auto t1 = std::chrono::steady_clock::now();
+      ++st.epochs;
+      auto t1 = std::chrono::steady_clock::now();
       ParallelFor(hosts.size(), [&](size_t i) {
         JvHost& h = hosts[i];
+        commits[i] = nullptr;
         if (h.done) return;
         // witnesses are union node ids: back to the graph's own ids
         int32_t* w = h_witnesses.get() + edge_base[i];
@@ -219,10 +264,31 @@ struct Fleet {
         for (int32_t e = 0; e < h.m; ++e) {
           if (w[e] >= 0) w[e] -= nb;
         }
-        h.FinishEpoch(w, propose_ms);
+        commits[i] = &h.VerifyAndGate(w, propose_ms);
+      });
+      bool live = false;
+      for (size_t i = 0; i < hosts.size(); ++i) live = live || (commits[i] != nullptr && !commits[i]->empty());
+      const bool next = live && epoch + 1 < max_rounds;
+      st.host_ms += MsSince(t1);
+      if (next) {
+        // the device commits and proposes while the hosts apply the commits
+        t0 = std::chrono::steady_clock::now();
+        StageCommits(commits);
+        EnqueueEpoch(false, use_graph, &st);
+        st.assemble_ms += MsSince(t0);
+      }
+      t1 = std::chrono::steady_clock::now();
+      ParallelFor(hosts.size(), [&](size_t i) {
+        if (commits[i] != nullptr) hosts[i].ApplyCommit();
+      });
+      if (!next) {
+        st.host_ms += MsSince(t1);
+        break;
+      }
+      ParallelFor(hosts.size(), [&](size_t i) {
+        if (!hosts[i].done) hosts[i].BeginEpoch(epoch + 1);
       });
       st.host_ms += MsSince(t1);
-      ++st.epochs;
     }
     const double total = MsSince(t_total);
     ParallelFor(hosts.size(), [&](size_t i) { hosts[i].EndRun(total); });

@@ -66,12 +66,19 @@ __device__ inline bool Survives(const longlong2* xy, int32_t tu, int32_t tv, int
 // First index of row i of the pairs (i, j > i) of d items, first-major.
 __device__ inline uint64_t PairRowStart(uint64_t i, uint64_t d) { return i * (2 * d - i - 1) / 2; }
 
+// Centers per scan chunk: the count pass adds every center's reply count to
+// its chunk's total, and the scan pass gives each chunk to one workgroup.
+constexpr uint32_t kScanThreads = 256;
+constexpr uint32_t kScanPerThread = 4;
+constexpr uint32_t kScanChunk = kScanThreads * kScanPerThread;
+
 template <bool kWrite, bool kNear>
 __global__ __launch_bounds__(kHtBlock) void HtHamiltonKernel(
     const longlong2* __restrict__ xy, const int32_t* __restrict__ row_offsets,
     const int32_t* __restrict__ neighbors, int type, int32_t tu, int32_t tv,
     const int32_t* __restrict__ centers, uint64_t center_count, uint64_t* __restrict__ counts,
-    const uint64_t* __restrict__ offsets, int32_t* __restrict__ replies) {
+    unsigned long long* __restrict__ chunk_sums, const uint64_t* __restrict__ offsets,
+    int32_t* __restrict__ replies) {
   __shared__ HtNeighbor cache[kHtWaves][kHtLdsNeighbors];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const uint64_t item = static_cast<uint64_t>(blockIdx.x) * kHtWaves + wave;
@@ -93,6 +100,7 @@ __global__ __launch_bounds__(kHtBlock) void HtHamiltonKernel(
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const uint64_t pairs = d < 2 ? 0 : d * (d - 1) / 2;
   uint64_t out = kWrite ? offsets[item] : 0;
+  const uint64_t out_end = kWrite ? offsets[item + 1] : 0;
   uint64_t found = 0;
   for (uint64_t base = 0; base < pairs; base += 64) {
     const uint64_t p = base + lane;
@@ -125,71 +133,68 @@ __global__ __launch_bounds__(kHtBlock) void HtHamiltonKernel(
       }
       if (kWrite && keep) {
         const uint64_t ballot_below = __ballot(keep) & ((uint64_t{1} << lane) - 1);
-        int32_t* r = replies + 3 * (out + __popcll(ballot_below));
-        r[0] = c;
-        r[1] = f.node;
-        r[2] = s.node;
+        const uint64_t at = out + __popcll(ballot_below);
+        if (at < out_end) {  // the count pass found exactly out_end - offsets[item]
+          int32_t* r = replies + 3 * at;
+          r[0] = c;
+          r[1] = f.node;
+          r[2] = s.node;
+        }
       }
     }
     const uint64_t ballot = __ballot(keep);
     found += __popcll(ballot);
     if (kWrite) out += __popcll(ballot);
   }
-  if (!kWrite && lane == 0) counts[item] = found;
+  if (!kWrite && lane == 0) {
+    counts[item] = found;
+    if (found != 0) atomicAdd(&chunk_sums[item / kScanChunk], static_cast<unsigned long long>(found));
+  }
 }
 
-// One workgroup: offsets[0] = 0, offsets[i + 1] = counts[0] + ... + counts[i].
-// Chunks of 8,192 counts: each thread reads 8 consecutive counts (a
-// wavefront reads 4 KB in one sweep), the threads' sums are scanned across
-// the wavefront with shuffles and across the 16 wavefronts in LDS, and a
-// running carry joins the chunks.
-constexpr uint32_t kScanThreads = 1024;
-constexpr uint32_t kScanPerThread = 8;
-
+// offsets[0] = 0, offsets[i + 1] = counts[0] + ... + counts[i], one
+// workgroup per chunk of kScanChunk counts: the workgroup first sums the
+// totals of the chunks before its own (chunk_sums, filled by the count
+// pass), then scans its chunk, each thread holding 4 consecutive counts
+// (wavefront shuffles, then the 4 wavefront totals in LDS).
 __global__ __launch_bounds__(kScanThreads) void HtScanKernel(const uint64_t* __restrict__ counts,
                                                               uint64_t n,
+                                                              const unsigned long long* __restrict__ chunk_sums,
                                                               uint64_t* __restrict__ offsets) {
+  __shared__ unsigned long long partial[kScanThreads / 64];
   __shared__ unsigned long long wave_sums[kScanThreads / 64];
-  __shared__ unsigned long long carry_s;
   const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
-  if (t == 0) {
-    offsets[0] = 0;
-    carry_s = 0;
+  const uint64_t chunk = blockIdx.x;
+  // prefix of the chunks before this one
+  unsigned long long before = 0;
+  for (uint64_t c = t; c < chunk; c += kScanThreads) before += chunk_sums[c];
+  for (uint32_t d = 32; d >= 1; d >>= 1) before += __shfl_xor(before, d, 64);
+  if (lane == 0) partial[wave] = before;
+  const uint64_t first = chunk * kScanChunk + static_cast<uint64_t>(t) * kScanPerThread;
+  unsigned long long v[kScanPerThread];
+  unsigned long long own = 0;
+#pragma unroll
+  for (uint32_t e = 0; e < kScanPerThread; ++e) {
+    v[e] = first + e < n ? counts[first + e] : 0;
+    own += v[e];
   }
+  unsigned long long inc = own;  // inclusive scan of the threads' sums in the wavefront
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const unsigned long long o = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += o;
+  }
+  if (lane == 63) wave_sums[wave] = inc;
   __syncthreads();
-  for (uint64_t base = 0; base < n; base += kScanThreads * kScanPerThread) {
-    const uint64_t first = base + static_cast<uint64_t>(t) * kScanPerThread;
-    unsigned long long v[kScanPerThread];
-    unsigned long long own = 0;
-    for (uint32_t e = 0; e < kScanPerThread; ++e) {
-      v[e] = first + e < n ? counts[first + e] : 0;
-      own += v[e];
-    }
-    unsigned long long inc = own;  // inclusive scan of the threads' sums in the wavefront
-    for (uint32_t d = 1; d < 64; d <<= 1) {
-      const unsigned long long o = __shfl_up(inc, d, 64);
-      if (lane >= d) inc += o;
-    }
-    if (lane == 63) wave_sums[wave] = inc;
-    __syncthreads();
-    if (wave == 0) {
-      unsigned long long w = lane < kScanThreads / 64 ? wave_sums[lane] : 0;
-      for (uint32_t d = 1; d < kScanThreads / 64; d <<= 1) {
-        const unsigned long long o = __shfl_up(w, d, 64);
-        if (lane >= d) w += o;
-      }
-      if (lane < kScanThreads / 64) wave_sums[lane] = w;
-    }
-    __syncthreads();
-    const unsigned long long carry = carry_s;
-    unsigned long long run = carry + (wave > 0 ? wave_sums[wave - 1] : 0) + inc - own;
-    for (uint32_t e = 0; e < kScanPerThread; ++e) {
-      run += v[e];
-      if (first + e < n) offsets[first + e + 1] = run;
-    }
-    __syncthreads();
-    if (t == 0) carry_s = carry + wave_sums[kScanThreads / 64 - 1];
-    __syncthreads();
+  unsigned long long run = inc - own;
+  for (uint32_t w = 0; w < kScanThreads / 64; ++w) {
+    run += partial[w];
+    if (w < wave) run += wave_sums[w];
+  }
+  if (chunk == 0 && t == 0) offsets[0] = 0;
+#pragma unroll
+  for (uint32_t e = 0; e < kScanPerThread; ++e) {
+    run += v[e];
+    if (first + e < n) offsets[first + e + 1] = run;
   }
 }
 
@@ -308,7 +313,7 @@ template <typename T> bool Reserve(DeviceArray<T>& a, size_t count, int device) 
 
 uint64_t HtResidentBytes(const HtCache& c) {
   return c.d_xy.bytes() + c.d_row_offsets.bytes() + c.d_neighbors.bytes() + c.d_items.bytes() +
-         c.d_counts.bytes() + c.d_offsets.bytes() + c.d_replies.bytes();
+         c.d_counts.bytes() + c.d_offsets.bytes() + c.d_chunk_sums.bytes() + c.d_replies.bytes();
 }
 
 void HamiltonReplies(Context& ctx, const ee_graph_view* graph, int32_t target_u, int32_t target_v,
@@ -329,31 +334,34 @@ void HamiltonReplies(Context& ctx, const ee_graph_view* graph, int32_t target_u,
   if (tu < 0 || tv >= g.dimension || tu == tv || !RowHas(g, tu, tv)) {
     throw std::invalid_argument("HT Hamilton replies: the target is not an active edge");
   }
-  uint64_t bound = 0;  // every neighbour pair of every center
   for (size_t i = 0; i < count; ++i) {
     const int32_t c = centers[i];
     if (c < 0 || c >= g.dimension || c == tu || c == tv) {
       throw std::invalid_argument("HT Hamilton replies: invalid center");
     }
-    const uint64_t d = static_cast<uint64_t>(g.row_offsets[c + 1] - g.row_offsets[c]);
-    bound += d < 2 ? 0 : d * (d - 1) / 2;
   }
   offsets[0] = 0;
   *total = 0;
   HtCache& c = ctx.ht;
+  c.last_kind = 0;
+  c.last_total = 0;
   if (count == 0) {
     if (usage) *usage = u;
     return;
   }
+  const uint64_t chunks = (count + kScanChunk - 1) / kScanChunk;
   bool ws = Reserve(c.d_items, count, ctx.device);
   ws = Reserve(c.d_counts, count, ctx.device) && ws;
   ws = Reserve(c.d_offsets, count + 1, ctx.device) && ws;
-  ws = Reserve(c.d_replies, 3 * bound, ctx.device) && ws;
-  u.workspace_hit = ws ? 1 : 0;
+  ws = Reserve(c.d_chunk_sums, chunks, ctx.device) && ws;
   c.d_items.Upload(centers, count, ctx.stream);
+  CheckHip(hipMemsetAsync(c.d_chunk_sums.get(), 0, chunks * sizeof(unsigned long long), ctx.stream),
+           "hipMemsetAsync chunk sums");
   ctx.Sync();
   u.h2d_ms = MsSince(t0);
 
+  // count pass and offset scan, then the exact reply count (the reference
+  // sizes its reply buffer the same way, ht_replies.cu:626-637)
   t0 = std::chrono::steady_clock::now();
   const uint64_t blocks = (count + kHtWaves - 1) / kHtWaves;
   const auto count_kernel = c.near ? HtHamiltonKernel<false, true> : HtHamiltonKernel<false, false>;
@@ -361,24 +369,37 @@ void HamiltonReplies(Context& ctx, const ee_graph_view* graph, int32_t target_u,
   hipLaunchKernelGGL(count_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(kHtBlock), 0,
                      ctx.stream, c.d_xy.get(), c.d_row_offsets.get(), c.d_neighbors.get(),
                      static_cast<int>(g.distance_type), tu, tv, c.d_items.get(),
-                     static_cast<uint64_t>(count), c.d_counts.get(), nullptr, nullptr);
+                     static_cast<uint64_t>(count), c.d_counts.get(), c.d_chunk_sums.get(), nullptr,
+                     nullptr);
   CheckHip(hipGetLastError(), "HtHamiltonKernel count launch");
-  hipLaunchKernelGGL(HtScanKernel, dim3(1), dim3(kScanThreads), 0, ctx.stream, c.d_counts.get(),
-                     static_cast<uint64_t>(count), c.d_offsets.get());
+  hipLaunchKernelGGL(HtScanKernel, dim3(static_cast<uint32_t>(chunks)), dim3(kScanThreads), 0,
+                     ctx.stream, c.d_counts.get(), static_cast<uint64_t>(count),
+                     c.d_chunk_sums.get(), c.d_offsets.get());
   CheckHip(hipGetLastError(), "HtScanKernel launch");
-  hipLaunchKernelGGL(write_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(kHtBlock), 0,
-                     ctx.stream, c.d_xy.get(), c.d_row_offsets.get(), c.d_neighbors.get(),
-                     static_cast<int>(g.distance_type), tu, tv, c.d_items.get(),
-                     static_cast<uint64_t>(count), nullptr, c.d_offsets.get(), c.d_replies.get());
-  CheckHip(hipGetLastError(), "HtHamiltonKernel write launch");
-  ctx.Sync();
-  u.kernel_ms = MsSince(t0);
-
-  t0 = std::chrono::steady_clock::now();
   c.d_offsets.Download(offsets, count + 1, ctx.stream);
   ctx.Sync();
   *total = offsets[count];
+  uint64_t bound = 0;  // every neighbour pair of every center
+  for (size_t i = 0; i < count; ++i) {
+    const uint64_t d = static_cast<uint64_t>(g.row_offsets[centers[i] + 1] - g.row_offsets[centers[i]]);
+    bound += d < 2 ? 0 : d * (d - 1) / 2;
+  }
   if (*total > bound) throw std::logic_error("HT Hamilton replies exceed the pair bound");
+  if (*total > UINT64_MAX / 3) throw std::overflow_error("HT Hamilton reply count overflows");
+  ws = Reserve(c.d_replies, 3 * *total, ctx.device) && ws;
+  u.workspace_hit = ws ? 1 : 0;
+  hipLaunchKernelGGL(write_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(kHtBlock), 0,
+                     ctx.stream, c.d_xy.get(), c.d_row_offsets.get(), c.d_neighbors.get(),
+                     static_cast<int>(g.distance_type), tu, tv, c.d_items.get(),
+                     static_cast<uint64_t>(count), nullptr, nullptr, c.d_offsets.get(),
+                     c.d_replies.get());
+  CheckHip(hipGetLastError(), "HtHamiltonKernel write launch");
+  ctx.Sync();
+  u.kernel_ms = MsSince(t0);
+  c.last_kind = 1;
+  c.last_total = *total;
+
+  t0 = std::chrono::steady_clock::now();
   const uint64_t copy = std::min(*total, capacity);
   if (copy > 0) {
     CheckHip(hipMemcpyAsync(replies, c.d_replies.get(), copy * sizeof(ee_ht_neighbor_pair),
@@ -389,6 +410,23 @@ void HamiltonReplies(Context& ctx, const ee_graph_view* graph, int32_t target_u,
   u.d2h_ms = MsSince(t0);
   u.resident_bytes = HtResidentBytes(c);
   if (usage) *usage = u;
+}
+
+// Replies [first, first + count) of the context's last reply call.
+void CopyReplies(Context& ctx, uint64_t first, uint64_t count, void* out) {
+  HtCache& c = ctx.ht;
+  if (c.last_kind == 0) throw std::logic_error("HT copy replies: no reply call has completed");
+  if (first > c.last_total || count > c.last_total - first) {
+    throw std::invalid_argument("HT copy replies: range beyond the last call's replies");
+  }
+  if (count == 0) return;
+  if (out == nullptr) throw std::invalid_argument("HT copy replies: NULL buffer");
+  ctx.Activate();
+  const size_t record = c.last_kind == 1 ? sizeof(ee_ht_neighbor_pair) : sizeof(ee_node_edge);
+  CheckHip(hipMemcpyAsync(out, reinterpret_cast<const char*>(c.d_replies.get()) + first * record,
+                          count * record, hipMemcpyDeviceToHost, ctx.stream),
+           "hipMemcpyAsync HT replies");
+  ctx.Sync();
 }
 
 void EndReplies(Context& ctx, const ee_graph_view* graph, const ee_ht_end_task* tasks, size_t count,
@@ -415,6 +453,8 @@ void EndReplies(Context& ctx, const ee_graph_view* graph, const ee_ht_end_task* 
   }
   *total = offsets[count];
   HtCache& c = ctx.ht;
+  c.last_kind = 0;
+  c.last_total = 0;
   if (count == 0) {
     if (usage) *usage = u;
     return;
@@ -436,6 +476,8 @@ void EndReplies(Context& ctx, const ee_graph_view* graph, const ee_ht_end_task* 
   CheckHip(hipGetLastError(), "HtEndKernel launch");
   ctx.Sync();
   u.kernel_ms = MsSince(t0);
+  c.last_kind = 2;
+  c.last_total = *total;
   t0 = std::chrono::steady_clock::now();
   const uint64_t copy = std::min(*total, capacity);
   if (copy > 0) {
@@ -481,6 +523,13 @@ ee_status ee_ht_end_replies(ee_ctx* ctx, const ee_graph_view* graph, const ee_ht
     if (ctx == nullptr) throw std::invalid_argument("NULL context");
     ee::EndReplies(*reinterpret_cast<ee::Context*>(ctx), graph, tasks, task_count, offsets, replies,
                    capacity, total, usage);
+  });
+}
+
+ee_status ee_ht_copy_replies(ee_ctx* ctx, uint64_t first, uint64_t count, void* replies) {
+  return ee::Guarded([&] {
+    if (ctx == nullptr) throw std::invalid_argument("NULL context");
+    ee::CopyReplies(*reinterpret_cast<ee::Context*>(ctx), first, count, replies);
   });
 }
 

@@ -69,8 +69,8 @@ void ParallelFor(size_t n, const std::function<void(size_t)>& body, unsigned thr
 }
 
 void JvHost::Load(const ee_graph_view* g) {
-  if (pending_hash_.valid()) pending_hash_.wait();  // it reads the arrays replaced here
-  pending_hash_ = std::future<uint64_t>();
+  for (SnapHash& h : snaps_) h.Get();  // pending hash jobs read the arrays replaced here
+  snaps_.clear();
   n = g->dimension;
   m = g->edge_count;
   type = g->distance_type;
@@ -117,33 +117,31 @@ void JvHost::RebuildCsr() {
                                  neighbors.data(), edge_ids.data()));
 }
 
-uint64_t JvHost::ComputeHash() const {
-  return ContentHashEdges(prefix_hash_, m, u.data(), v.data(), weight.data(), active.data());
+uint64_t JvHost::ComputeHash(const uint8_t* active_flags) const {
+  return ContentHashEdges(prefix_hash_, m, u.data(), v.data(), weight.data(), active_flags);
 }
 
-uint64_t JvHost::Hash() const {
-  if (pending_hash_.valid()) {
-    const uint64_t h = pending_hash_.get();
-    if (pending_generation_ == generation_) {
-      hash_ = h;
-      hash_generation_ = generation_;
-    }
+uint64_t JvHost::SnapHash::Get() {
+  if (!ready) {
+    value = job.get();
+    ready = true;
   }
-  if (hash_generation_ != generation_) {
-    hash_ = ComputeHash();
-    hash_generation_ = generation_;
-  }
-  return hash_;
+  return value;
 }
 
-void JvHost::HashAsync() {
-  if (!async_hash || hash_generation_ == generation_) return;
-  if (pending_hash_.valid()) {
-    if (pending_generation_ == generation_) return;
-    pending_hash_.wait();
+JvHost::SnapHash JvHost::CurrentSnap() {
+  if (!snaps_.empty() && snaps_.back().generation == generation_) return snaps_.back();
+  SnapHash h;
+  h.generation = generation_;
+  if (async_hash) {
+    // the job hashes a copy of this generation's flags, so commits may
+    // change the snapshot while it runs
+    std::vector<uint8_t> flags(active);
+    h.job = std::async(std::launch::async,
+                       [this, flags = std::move(flags)] { return ComputeHash(flags.data()); })
+                .share();
   }
-  pending_generation_ = generation_;
-  pending_hash_ = std::async(std::launch::async, [this] { return ComputeHash(); });
+  return h;
 }
 
 bool JvHost::Verify(int32_t e, int32_t c) const {
@@ -162,11 +160,13 @@ bool JvHost::Verify(int32_t e, int32_t c) const {
 }
 
 void JvHost::StartRun() {
+  for (SnapHash& h : snaps_) {
+    if (h.job.valid()) h.Get();
+  }
   records_.clear();
   metrics_.clear();
+  snaps_.clear();
   stats = ee_jv_run_stats{};
-  initial_pending_ = true;  // the snapshot hash of the first commit or the end
-  HashAsync();
   done = false;
   proof.clear();
 }
@@ -176,12 +176,16 @@ void JvHost::BeginEpoch(uint32_t epoch) {
   cur_.epoch = epoch;
   for (int32_t e = 0; e < m; ++e) cur_.before += active[e] != 0;
   snap_generation_ = generation_;
-  HashAsync();
+  snaps_.push_back(CurrentSnap());
+  gated_.clear();
+  gated_edges_.clear();
 }
 
-size_t JvHost::FinishEpoch(const int32_t* witnesses, double propose_ms) {
+const std::vector<int32_t>& JvHost::VerifyAndGate(const int32_t* witnesses, double propose_ms) {
   Metrics& mt = cur_;
-  std::vector<std::pair<int32_t, int32_t>> proposed;
+  std::vector<std::pair<int32_t, int32_t>>& proposed = gated_;
+  proposed.clear();
+  gated_edges_.clear();
   for (int32_t e = 0; e < m; ++e) {
     if (witnesses[e] >= 0) proposed.emplace_back(e, witnesses[e]);
   }
@@ -190,19 +194,15 @@ size_t JvHost::FinishEpoch(const int32_t* witnesses, double propose_ms) {
 
   // host re-verification on the same snapshot
   auto t0 = std::chrono::steady_clock::now();
-  std::vector<std::pair<int32_t, int32_t>> verified;
   for (const auto& c : proposed) {
-    if (Verify(c.first, c.second)) {
-      verified.push_back(c);
-    } else {
-      ++mt.rejected;
-    }
+    if (!Verify(c.first, c.second)) ++mt.rejected;
   }
   mt.verify_ms = MsSince(t0);
-  mt.verified = verified.size();
+  mt.verified = proposed.size() - mt.rejected;
   stats.propose_ms += mt.propose_ms;
   stats.verify_ms += mt.verify_ms;
   if (mt.rejected != 0) {
+    proposed.clear();
     throw std::runtime_error("HIP JV candidate failed CPU re-verification; epoch stopped, nothing committed");
   }
 
@@ -212,37 +212,47 @@ size_t JvHost::FinishEpoch(const int32_t* witnesses, double propose_ms) {
   // the snapshot must be the one proposed on (elimination_commit.cpp:16-18);
   // it changes only through commits, which bump the generation
   if (generation_ != snap_generation_) throw std::runtime_error("snapshot changed before commit");
-  const uint64_t snap = Hash();
-  if (initial_pending_) {
-    stats.initial_hash = snap;
-    initial_pending_ = false;
-  }
-  std::sort(verified.begin(), verified.end(), [&](const auto& l, const auto& r) {
+  std::sort(proposed.begin(), proposed.end(), [&](const auto& l, const auto& r) {
     return std::make_tuple(u[l.first], v[l.first], l.second, l.first) <
            std::make_tuple(u[r.first], v[r.first], r.second, r.first);
   });
-  verified.erase(std::unique(verified.begin(), verified.end(),
+  proposed.erase(std::unique(proposed.begin(), proposed.end(),
                              [](const auto& l, const auto& r) { return l.first == r.first; }),
-                 verified.end());
+                 proposed.end());
   std::vector<int32_t> degree(static_cast<size_t>(n));
   for (int32_t vtx = 0; vtx < n; ++vtx) degree[vtx] = Degree(vtx);
-  std::vector<std::pair<int32_t, int32_t>> committed;
-  for (const auto& c : verified) {
+  size_t kept = 0;
+  for (const auto& c : proposed) {
     int32_t& du = degree[u[c.first]];
     int32_t& dv = degree[v[c.first]];
     if (du <= 2 || dv <= 2) continue;
     --du;
     --dv;
-    committed.push_back(c);
+    proposed[kept++] = c;
+    gated_edges_.push_back(c.first);
   }
-  if (!committed.empty()) {
-    for (const auto& c : committed) active[c.first] = 0;
+  proposed.resize(kept);
+  stats.commit_ms += MsSince(t0);
+  return gated_edges_;
+}
+
+size_t JvHost::ApplyCommit() {
+  Metrics& mt = cur_;
+  const auto t0 = std::chrono::steady_clock::now();
+  // records carry the hash of the snapshot proposed on
+  const size_t snap = snaps_.size() - 1;
+  if (!async_hash && !gated_.empty() && !snaps_[snap].ready) {
+    snaps_[snap].value = ComputeHash(active.data());  // before the snapshot changes
+    snaps_[snap].ready = true;
+  }
+  if (!gated_.empty()) {
+    for (const auto& c : gated_) active[c.first] = 0;
     ++generation_;
     RebuildCsr();
   }
   stats.commit_ms += MsSince(t0);
-  mt.committed = committed.size();
-  for (const auto& c : committed) {
+  mt.committed = gated_.size();
+  for (const auto& c : gated_) {
     records_.push_back({mt.epoch, snap, c.first, u[c.first], v[c.first], c.second});
   }
   metrics_.push_back(mt);
@@ -250,24 +260,40 @@ size_t JvHost::FinishEpoch(const int32_t* witnesses, double propose_ms) {
   stats.proposed += mt.proposed;
   stats.committed += mt.committed;
   ++stats.epochs;
-  if (committed.empty()) done = true;
-  return committed.size();
+  const size_t committed = gated_.size();
+  if (committed == 0) done = true;
+  gated_.clear();
+  return committed;
 }
 
 void JvHost::EndRun(double total_ms) {
   done = true;
-  if (initial_pending_) {
-    stats.initial_hash = Hash();
-    initial_pending_ = false;
+  const auto t0 = std::chrono::steady_clock::now();
+  // the snapshot of the first epoch is the run's initial snapshot; the
+  // final one is the current snapshot
+  SnapHash final_snap = CurrentSnap();
+  if (!async_hash && !final_snap.ready) {
+    final_snap.value = ComputeHash(active.data());
+    final_snap.ready = true;
   }
-  stats.final_hash = Hash();
+  if (snaps_.empty()) snaps_.push_back(final_snap);
+  for (SnapHash& h : snaps_) {
+    if (!h.ready && !h.job.valid()) {  // an epoch that committed nothing (synchronous mode)
+      if (h.generation != generation_) throw std::logic_error("JV snapshot hash of a past generation is missing");
+      h.value = final_snap.Get();
+      h.ready = true;
+    }
+  }
+  stats.initial_hash = snaps_.front().Get();
+  stats.final_hash = final_snap.Get();
+  stats.hash_wait_ms = MsSince(t0);
   stats.total_ms = total_ms;
   // V1 proof text (src/cpu/jv_elimination.cpp:324-368)
   std::ostringstream out;
   out << "CUDAEE_PROOF_V1\nbackend hip\ninitial_hash " << Hex(stats.initial_hash) << '\n';
   for (const Record& r : records_) {
-    out << "record " << r.epoch << ' ' << Hex(r.hash) << ' ' << r.edge << ' ' << r.a << ' ' << r.b
-        << " JV " << r.witness << '\n';
+    out << "record " << r.epoch << ' ' << Hex(snaps_[r.snap].Get()) << ' ' << r.edge << ' ' << r.a << ' '
+        << r.b << " JV " << r.witness << '\n';
   }
   for (const Metrics& mt : metrics_) {
     out << "metrics " << mt.epoch << ' ' << mt.before << ' ' << mt.proposed << ' ' << mt.verified

@@ -36,25 +36,33 @@ struct JvHost {
   int32_t Degree(int32_t vtx) const { return row_offsets[vtx + 1] - row_offsets[vtx]; }
   int64_t Dist(int32_t a, int32_t b) const;
   void RebuildCsr();
-  // ContentHash of the current snapshot; recomputed only after a commit
-  // changed it (the point prefix is hashed once, at Load).
-  uint64_t Hash() const;
-  // With async_hash set, a stale snapshot hash is computed on a helper thread
-  // from the start of a run or epoch until the commit needs it, overlapping
-  // the device propose step (FNV-1a is byte-serial).
+  // With async_hash set, the content hash of every epoch's snapshot is
+  // computed on a helper thread from a copy of the active flags, so it never
+  // waits on the critical path; the proof records get their hashes when the
+  // run ends.  Without it, the hash is computed when the epoch commits
+  // (before the snapshot changes), on the calling thread.
   bool async_hash = false;
   // IsJvWitness (jv_elimination.cpp:86-117) with 128-bit sums.
   bool Verify(int32_t e, int32_t c) const;
 
   // ---- one run
   void StartRun();
-  // Active edge count at the start of `epoch`; the snapshot hash is taken
-  // when the epoch commits (the snapshot does not change in between).
+  // Active edge count at the start of `epoch`; takes the snapshot hash job.
   void BeginEpoch(uint32_t epoch);
   // Re-verifies every proposal of witnesses[0..m) (witness or -1), throws if
-  // any is rejected, commits and records; returns the committed count.  The
-  // run is done when nothing was committed.
-  size_t FinishEpoch(const int32_t* witnesses, double propose_ms);
+  // any is rejected (nothing is committed), and runs the deterministic
+  // commit rule: sort by (u, v, method, witness, edge), one per edge, running
+  // degree gate > 2 (src/cpu/elimination_commit.cpp:13-75).  Returns the
+  // edge slots it commits, in commit order; the snapshot is unchanged.
+  const std::vector<int32_t>& VerifyAndGate(const int32_t* witnesses, double propose_ms);
+  // Applies the commit of the last VerifyAndGate to the snapshot (active
+  // flags, CSR), records it for the proof and ends the epoch; returns the
+  // committed count.  The run is done when nothing was committed.
+  size_t ApplyCommit();
+  size_t FinishEpoch(const int32_t* witnesses, double propose_ms) {
+    VerifyAndGate(witnesses, propose_ms);
+    return ApplyCommit();
+  }
   // Final hash and the V1 proof text with backend name "hip".
   void EndRun(double total_ms);
 
@@ -65,7 +73,7 @@ struct JvHost {
  private:
   struct Record {
     uint32_t epoch;
-    uint64_t hash;
+    size_t snap;  // index into snaps_
     int32_t edge, a, b, witness;
   };
   struct Metrics {
@@ -73,21 +81,30 @@ struct JvHost {
     size_t before, proposed, verified, rejected, committed;
     double propose_ms, verify_ms;
   };
-  void HashAsync();
-  uint64_t ComputeHash() const;
+  // The content hash of one snapshot generation: a value, or a job computing
+  // it from a copy of that generation's active flags.
+  struct SnapHash {
+    uint64_t generation = 0;
+    bool ready = false;
+    uint64_t value = 0;
+    std::shared_future<uint64_t> job;
+    uint64_t Get();
+  };
+  uint64_t ComputeHash(const uint8_t* active_flags) const;
+  // The hash holder of the current generation (reuses the last one when the
+  // snapshot has not changed since).
+  SnapHash CurrentSnap();
 
   std::vector<Record> records_;
   std::vector<Metrics> metrics_;
+  std::vector<SnapHash> snaps_;  // one per epoch of the run
   Metrics cur_{};
+  std::vector<std::pair<int32_t, int32_t>> gated_;  // (edge, witness) of the pending commit
+  std::vector<int32_t> gated_edges_;
   uint64_t snap_generation_ = 0;
-  bool initial_pending_ = false;
   // every change of the snapshot bumps the generation
   uint64_t generation_ = 1;
   uint64_t prefix_hash_ = 0;
-  mutable uint64_t hash_ = 0, hash_generation_ = 0;
-  // the hash being computed for snapshot generation pending_generation_
-  mutable std::future<uint64_t> pending_hash_;
-  mutable uint64_t pending_generation_ = 0;
 };
 
 }  // namespace ee

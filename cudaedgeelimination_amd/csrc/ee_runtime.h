@@ -226,7 +226,10 @@ struct HtCache {
   DeviceArray<int32_t> d_items;      // centers, or (endpoint, internal) task pairs
   DeviceArray<uint64_t> d_counts;    // per item
   DeviceArray<uint64_t> d_offsets;   // per item + 1
-  DeviceArray<int32_t> d_replies;    // triples or pairs
+  DeviceArray<unsigned long long> d_chunk_sums;  // reply totals per scan chunk of centers
+  DeviceArray<int32_t> d_replies;    // triples or pairs, exactly the last call's replies
+  int last_kind = 0;                 // last completed call: 1 Hamilton, 2 end replies
+  uint64_t last_total = 0;           // its reply count
 };
 
 struct Context {

@@ -458,7 +458,9 @@ KoptPlan BuildPlan(uint32_t k, const ee_endpoint_matching* templates, uint32_t c
   const uint32_t ends = 2 * k;
   int pair_index[kMaxPorts][kMaxPorts];
   for (auto& row : pair_index) std::fill(row, row + kMaxPorts, -1);
-  std::vector<std::pair<uint8_t, uint8_t>> pairs;
+  // at most C(10, 2) = 45 distinct pairs; fixed storage, no heap
+  std::pair<uint8_t, uint8_t> pairs[kMaxPairs];
+  uint32_t pair_count = 0;
   KoptPlan plan;
   plan.k = k;
   plan.templates = count;
@@ -473,14 +475,14 @@ KoptPlan BuildPlan(uint32_t k, const ee_endpoint_matching* templates, uint32_t c
       }
       if (p > q) continue;
       if (pair_index[p][q] < 0) {
-        pair_index[p][q] = static_cast<int>(pairs.size());
-        pairs.emplace_back(static_cast<uint8_t>(p), static_cast<uint8_t>(q));
+        pair_index[p][q] = static_cast<int>(pair_count);
+        pairs[pair_count++] = {static_cast<uint8_t>(p), static_cast<uint8_t>(q)};
       }
       plan.tpl[t] |= static_cast<uint64_t>(pair_index[p][q]) << (8 * used++);
     }
   }
-  plan.pairs = static_cast<uint32_t>(pairs.size());
-  for (size_t i = 0; i < pairs.size(); ++i) {
+  plan.pairs = pair_count;
+  for (uint32_t i = 0; i < pair_count; ++i) {
     plan.pair_pq[i] = static_cast<uint16_t>(pairs[i].first | (pairs[i].second << 8));
   }
   return plan;

@@ -134,9 +134,17 @@ class KoptBatch:
                                                            _lib.ptr(out) if count else None))
         return out
 
-    def segment_costs(self, index: int) -> np.ndarray:
+    def segment_costs(self, index: int, pool: int = 0) -> np.ndarray:
+        """Cells of segment `index`, from cost pool `pool` of pipelined runs
+        (0: the pool ``download`` reads)."""
         _, k, count = self.segments[index]
-        return self.download(int(self.cell_offsets[index]), count * TEMPLATES_PER_K[k])
+        first, cells = int(self.cell_offsets[index]), count * TEMPLATES_PER_K[k]
+        if pool == 0:
+            return self.download(first, cells)
+        out = np.empty(cells, np.int64)
+        _lib.check(_lib.lib().ee_kopt_batch_download_pool(self._h, pool, first, cells,
+                                                          _lib.ptr(out) if cells else None))
+        return out
 
     def resident_bytes(self) -> int:
         return int(_lib.lib().ee_kopt_batch_resident_bytes(self._h))
@@ -158,6 +166,8 @@ class JvRun:
     commit_ms: float
     total_ms: float
     proof: str
+    compact_ms: float = 0.0    # device commit (deactivate + CSR compaction)
+    hash_wait_ms: float = 0.0  # end-of-run wait for the snapshot hashes
 
 
 class JvSession:
@@ -192,12 +202,26 @@ class JvSession:
         _lib.check(_lib.lib().ee_jv_session_proof(self._h, buf, len(buf), C.byref(length)))
         return JvRun(st.epochs, st.edges_scanned, st.proposed, st.committed, st.initial_hash,
                      st.final_hash, st.device_ms, st.h2d_ms, st.d2h_ms, st.propose_ms,
-                     st.verify_ms, st.commit_ms, st.total_ms, buf.value.decode())
+                     st.verify_ms, st.commit_ms, st.total_ms, buf.value.decode(), st.compact_ms,
+                     st.hash_wait_ms)
 
     def active(self) -> np.ndarray:
         out = np.empty(self.graph.edge_count, np.uint8)
         _lib.check(_lib.lib().ee_jv_session_active(self._h, _lib.ptr(out)))
         return out
+
+    def device_snapshot(self) -> dict:
+        """The device's active flags and CSR after the last run."""
+        m, n = self.graph.edge_count, self.graph.dimension
+        active = np.empty(max(m, 1), np.uint8)
+        ro = np.empty(n + 1, np.int32)
+        nb = np.empty(max(2 * m, 1), np.int32)
+        eid = np.empty(max(2 * m, 1), np.int32)
+        _lib.check(_lib.lib().ee_jv_session_device_snapshot(
+            self._h, _lib.ptr(active), _lib.ptr(ro), _lib.ptr(nb), _lib.ptr(eid), 2 * m))
+        a = int(ro[n])
+        return {"edge_active": active[:m], "row_offsets": ro, "neighbors": nb[:a],
+                "csr_edge_ids": eid[:a]}
 
 
 class JvFleet:

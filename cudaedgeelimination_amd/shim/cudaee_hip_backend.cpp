@@ -27,6 +27,7 @@
 #include "cuda_edge_elimination/hamilton_tutte.hpp"
 #include "cuda_edge_elimination/local_search.hpp"
 
+#include <algorithm>
 #include <cstddef>
 #include <cstdint>
 #include <limits>
@@ -212,7 +213,7 @@ thread_local ThreadContext t_ht;
 
 // The snapshot's edge arrays and a view over them and its CSR.
 struct HtGraphView {
-  explicit HtGraphView(const GraphSnapshot& graph) : xy(graph) {
+  explicit HtGraphView(const GraphSnapshot& graph) : xy(CheckedShape(graph)) {
     const std::size_t m = graph.edges.size();
     if (m > static_cast<std::size_t>(std::numeric_limits<std::int32_t>::max())) {
       throw std::overflow_error("HIP HT reply edge count exceeds int32 indexing");
@@ -246,6 +247,23 @@ struct HtGraphView {
   std::vector<std::int64_t> w;
   std::vector<std::uint8_t> active;
   ee_graph_view view{};
+
+ private:
+  // The container-size and coordinate parts of the reference's
+  // ValidateGraphShape (src/cuda/ht_replies.cu:521-531), checked before any
+  // row is read; the C ABI checks the CSR contents.  The reference throws
+  // invalid_argument for every one of them.
+  static const GraphSnapshot& CheckedShape(const GraphSnapshot& graph) {
+    const auto n = static_cast<std::size_t>(std::max(graph.dimension, 0));
+    if (!graph.integer_coordinates || !graph.integer_distance_safe || graph.dimension < 4 ||
+        graph.points.size() != n || graph.row_offsets.size() != n + 1 ||
+        graph.row_offsets.front() != 0 || graph.row_offsets.back() < 0 ||
+        static_cast<std::size_t>(graph.row_offsets.back()) != graph.neighbors.size() ||
+        graph.csr_edge_ids.size() != graph.neighbors.size()) {
+      throw std::invalid_argument("HIP HT reply snapshot shape is invalid");
+    }
+    return graph;
+  }
 };
 
 void FillUsage(const ee_ht_usage& usage, HtReplyCudaCacheUsage* const cache_usage) {
@@ -266,7 +284,14 @@ ee_ctx* HtContext(int* const selected_device) {
 }  // namespace
 
 static_assert(sizeof(HtNeighborPair) == sizeof(ee_ht_neighbor_pair), "reply layout");
+static_assert(offsetof(HtNeighborPair, center) == offsetof(ee_ht_neighbor_pair, center) &&
+                  offsetof(HtNeighborPair, first) == offsetof(ee_ht_neighbor_pair, first) &&
+                  offsetof(HtNeighborPair, second) == offsetof(ee_ht_neighbor_pair, second),
+              "reply layout");
 static_assert(sizeof(NodeEdge) == sizeof(ee_node_edge), "edge layout");
+static_assert(offsetof(NodeEdge, u) == offsetof(ee_node_edge, u) &&
+                  offsetof(NodeEdge, v) == offsetof(ee_node_edge, v),
+              "edge layout");
 static_assert(sizeof(HtEndReplyTask) == sizeof(ee_ht_end_task), "task layout");
 
 bool HtHamiltonReplyCudaAvailable(std::string* const reason) {
@@ -279,28 +304,19 @@ HtHamiltonReplyDeviceBatch EvaluateHtHamiltonRepliesCuda(const GraphSnapshot& gr
                                                          int* const selected_device,
                                                          HtReplyCudaCacheUsage* const cache_usage) {
   if (cache_usage != nullptr) *cache_usage = {};
+  const HtGraphView g(graph);  // shape checks first, as the reference validates before SelectDevice
   ee_ctx* ctx = HtContext(selected_device);
-  const HtGraphView g(graph);
-  std::uint64_t bound = 0;  // every neighbour pair of every center
-  for (const std::int32_t c : centers) {
-    if (c >= 0 && c < graph.dimension) {
-      const auto d = static_cast<std::uint64_t>(graph.Degree(c));
-      bound += d < 2 ? 0 : d * (d - 1) / 2;
-    }
-  }
   HtHamiltonReplyDeviceBatch out;
   out.offsets.assign(centers.size() + 1, 0);
-  std::vector<ee_ht_neighbor_pair> replies(bound);
   std::uint64_t total = 0;
   ee_ht_usage usage{};
+  // the reply count comes from the device's count pass; the vector is then
+  // sized to it and filled in place (same record layout)
   Throw(ee_ht_hamilton_replies(ctx, &g.view, target_edge.u, target_edge.v, centers.data(),
-                               centers.size(), out.offsets.data(), replies.data(), bound, &total,
-                               &usage),
+                               centers.size(), out.offsets.data(), nullptr, 0, &total, &usage),
         "HIP HT Hamilton replies");
   out.replies.resize(total);
-  for (std::uint64_t i = 0; i < total; ++i) {
-    out.replies[i] = {replies[i].center, replies[i].first, replies[i].second};
-  }
+  Throw(ee_ht_copy_replies(ctx, 0, total, out.replies.data()), "HIP HT Hamilton replies");
   FillUsage(usage, cache_usage);
   return out;
 }
@@ -312,24 +328,17 @@ HtEndReplyDeviceBatch EvaluateHtEndRepliesCuda(const GraphSnapshot& graph,
                                                int* const selected_device,
                                                HtReplyCudaCacheUsage* const cache_usage) {
   if (cache_usage != nullptr) *cache_usage = {};
+  const HtGraphView g(graph);  // shape checks first, as the reference validates before SelectDevice
   ee_ctx* ctx = HtContext(selected_device);
-  const HtGraphView g(graph);
-  std::uint64_t bound = 0;
-  for (const HtEndReplyTask& t : tasks) {
-    if (t.endpoint >= 0 && t.endpoint < graph.dimension) {
-      bound += static_cast<std::uint64_t>(std::max(graph.Degree(t.endpoint) - 1, 0));
-    }
-  }
   HtEndReplyDeviceBatch out;
   out.offsets.assign(tasks.size() + 1, 0);
-  std::vector<ee_node_edge> replies(bound);
   std::uint64_t total = 0;
   ee_ht_usage usage{};
   Throw(ee_ht_end_replies(ctx, &g.view, reinterpret_cast<const ee_ht_end_task*>(tasks.data()),
-                          tasks.size(), out.offsets.data(), replies.data(), bound, &total, &usage),
+                          tasks.size(), out.offsets.data(), nullptr, 0, &total, &usage),
         "HIP HT end replies");
   out.replies.resize(total);
-  for (std::uint64_t i = 0; i < total; ++i) out.replies[i] = {replies[i].u, replies[i].v};
+  Throw(ee_ht_copy_replies(ctx, 0, total, out.replies.data()), "HIP HT end replies");
   FillUsage(usage, cache_usage);
   return out;
 }

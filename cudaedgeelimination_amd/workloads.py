@@ -175,3 +175,46 @@ def config5_graphs(seed: int = SEEDS[5], sizes=(1000, 5000, 15000, 30000)):
         x, y = _points_uniform(seed + i, n, 0, 1_000_000)
         graphs.append(knn_graph(x, y, EUC_2D, 10))
     return graphs
+
+
+SERVED_TEMPLATES = {3: 4, 4: 25, 5: 208}
+
+
+def config5_row_pools(graphs, seed: int = SEEDS[5], rows: int = 2048):
+    """Per served graph and k, `rows` task rows on random walks of the graph
+    (the rows that leaf-cost queries take contiguous slices of)."""
+    pools = []
+    for i, g in enumerate(graphs):
+        pools.append({k: kopt_tasks(seed + 17 * i + k, g, np.full(rows, k, np.uint8))
+                      for k in (3, 4, 5)})
+    return pools
+
+
+def config5_requests(graphs, pools, count: int, rng, jv_share: float = 0.2):
+    """A stream of served requests: (graph index, k, rows) leaf-cost queries
+    whose size is lognormal in cells (median 2,896, sigma 1.04) clipped to
+    [128, 65,536], k = 3 / 4 / 5 at 50/30/20%, rows a contiguous slice of the
+    graph's pool; and, with probability `jv_share`, (graph index, 0, edge
+    slots) JV rechecks of lognormally many (median 256, sigma 1.0, clipped to
+    [16, 4,096]) distinct random edge slots."""
+    cells = np.clip(np.exp(rng.normal(np.log(2896.0), 1.04, count)), 128, 65536)
+    kinds = rng.choice([3, 4, 5], size=count, p=[0.5, 0.3, 0.2])
+    which = rng.integers(0, len(graphs), count)
+    is_jv = rng.random(count) < jv_share
+    jv_sizes = np.clip(np.exp(rng.normal(np.log(256.0), 1.0, count)), 16, 4096).astype(int)
+    requests = []
+    for i in range(count):
+        g = int(which[i])
+        if is_jv[i]:
+            m = graphs[g].edge_count
+            requests.append((g, 0, rng.choice(m, size=min(int(jv_sizes[i]), m),
+                                              replace=False).astype(np.int32)))
+            continue
+        k = int(kinds[i])
+        rows = max(1, int(np.ceil(cells[i] / SERVED_TEMPLATES[k])))
+        pool = pools[g][k]
+        rows = min(rows, len(pool))
+        start = int(rng.integers(0, len(pool) - rows + 1))
+        requests.append((g, k, pool[start:start + rows]))
+    return requests, is_jv
+

@@ -207,8 +207,9 @@ typedef struct {
  * (src/cpu/hamilton_tutte.cpp:230-262).  The target may be given in either
  * order.  offsets receives center_count + 1 entries (center i's replies are
  * [offsets[i], offsets[i+1])), replies the first min(*total, capacity)
- * replies, *total their count (at most the sum over centers of
- * C(degree, 2)).  The snapshot must pass the reference's HT graph checks
+ * replies (ee_ht_copy_replies copies the rest), *total their count.  A count
+ * pass runs first, so the device holds exactly *total replies, never the
+ * sum over centers of C(degree, 2).  The snapshot must pass the reference's HT graph checks
  * (ht_replies.cu:521-553): a symmetric CSR of exactly the active edges,
  * rows strictly ascending. */
 ee_status ee_ht_hamilton_replies(ee_ctx* ctx, const ee_graph_view* graph, int32_t target_u,
@@ -222,6 +223,15 @@ ee_status ee_ht_hamilton_replies(ee_ctx* ctx, const ee_graph_view* graph, int32_
 ee_status ee_ht_end_replies(ee_ctx* ctx, const ee_graph_view* graph, const ee_ht_end_task* tasks,
                             size_t task_count, uint64_t* offsets, ee_node_edge* replies,
                             uint64_t capacity, uint64_t* total, ee_ht_usage* usage);
+/* Copies replies [first, first + count) of the context's last completed
+ * ee_ht_hamilton_replies (12-byte ee_ht_neighbor_pair records) or
+ * ee_ht_end_replies (8-byte ee_node_edge records) call into `replies`; they
+ * stay on the device until the next reply call.  A caller that does not know
+ * the reply count beforehand passes capacity 0 (or a guess) to the reply
+ * call, sizes its buffer from *total and copies the rest here, as the
+ * reference sizes its vector from the scanned offsets
+ * (src/cuda/ht_replies.cu:626-647). */
+ee_status ee_ht_copy_replies(ee_ctx* ctx, uint64_t first, uint64_t count, void* replies);
 /* Replaces cudaee::ClearHtReplyCudaCache (src/cuda/ht_replies.cu:753). */
 ee_status ee_ht_clear_cache(ee_ctx* ctx);
 
@@ -342,6 +352,13 @@ ee_status ee_kopt_batch_run(ee_kopt_batch* batch, uint32_t launches, int flags, 
 ee_status ee_kopt_batch_set_pipeline_depth(ee_kopt_batch* batch, uint32_t depth);
 ee_status ee_kopt_batch_download(ee_kopt_batch* batch, uint64_t first_cell, uint64_t cell_count,
                                  int64_t* costs);
+/* Cells of cost pool `pool` of pipelined runs (0 is the pool that
+ * ee_kopt_batch_download reads; 1 .. depth - 1 the further streams' pools).
+ * Fails with EE_ERR_LOGIC for a pool that no launch of the current work has
+ * written.  No reference counterpart: it lets a caller check every pool a
+ * pipelined run filled. */
+ee_status ee_kopt_batch_download_pool(ee_kopt_batch* batch, uint32_t pool, uint64_t first_cell,
+                                      uint64_t cell_count, int64_t* costs);
 /* Output mode of later runs: 0 = the int64 cost cells (default); 1 = per
  * task row (rows numbered in segment order) the bit mask of its improving
  * templates, bit j of word j / 32 set iff cell (row, j) < the row's
@@ -359,11 +376,15 @@ uint64_t ee_kopt_batch_resident_bytes(const ee_kopt_batch* batch);
 
 /* ------------------------------------------------------- resident JV epochs */
 /* Replaces the propose step of cudaee::RunJvElimination with backend cuda
- * (src/cpu/jv_elimination.cpp:250-322): the snapshot stays resident on the
+ * (src/cpu/jv_elimination.cpp:250-322) and the CSR refresh of its commit
+ * (src/cpu/elimination_commit.cpp:64-72): the snapshot stays resident on the
  * device across epochs; each epoch proposes on the device, re-verifies every
  * candidate on the host (an epoch with a rejected candidate fails closed,
- * nothing committed), commits through the deterministic degree gate, rebuilds
- * the CSR and records the proof.  The proof is the reference's V1 text
+ * nothing committed) and runs the deterministic commit rule there; the
+ * committed edge slots go to the device, which clears their active flags and
+ * compacts its CSR while the host applies the same commit to its own copy
+ * (kept for the re-verification) and hashes snapshots on a helper thread;
+ * the proof is recorded.  The proof is the reference's V1 text
  * (src/cpu/jv_elimination.cpp:324-368) with backend name "hip". */
 typedef struct ee_jv_session ee_jv_session;
 
@@ -376,12 +397,14 @@ typedef struct {
   uint64_t initial_hash;
   uint64_t final_hash;
   double device_ms;         /* JV kernel time, HIP events */
-  double h2d_ms;
+  double h2d_ms;            /* the first epoch's upload of active flags and CSR */
   double d2h_ms;
-  double propose_ms;        /* host wall time of the propose step */
-  double verify_ms;
-  double commit_ms;
+  double propose_ms;        /* host wall time from enqueueing an epoch's device work to its witnesses */
+  double verify_ms;         /* host re-verification */
+  double commit_ms;         /* host commit rule plus applying it to the host snapshot */
   double total_ms;
+  double compact_ms;        /* device commit: deactivate and CSR compaction, HIP events */
+  double hash_wait_ms;      /* time the end of the run waited for snapshot hashes */
 } ee_jv_run_stats;
 
 ee_status ee_jv_session_create(ee_ctx* ctx, const ee_graph_view* graph, ee_jv_session** out);
@@ -393,6 +416,17 @@ ee_status ee_jv_session_proof(const ee_jv_session* session, char* buffer, size_t
                               size_t* length);
 /* Active flags after the last run (edge_count entries). */
 ee_status ee_jv_session_active(const ee_jv_session* session, uint8_t* active);
+/* No reference counterpart: the session's device-resident snapshot, the one
+ * the last epoch of the last run proposed on (the final snapshot when the
+ * run reached its fixed point; one commit behind the host's when max_rounds
+ * cut the run after a committing epoch, which the next run's upload
+ * resolves), for checks against the host snapshot.  active: edge_count flags; row_offsets:
+ * dimension + 1 entries; neighbors and edge_ids: row_offsets[dimension]
+ * entries each, which must not exceed `capacity`.  Before any run the CSR
+ * buffers hold no snapshot. */
+ee_status ee_jv_session_device_snapshot(const ee_jv_session* session, uint8_t* active,
+                                        int32_t* row_offsets, int32_t* neighbors, int32_t* edge_ids,
+                                        size_t capacity);
 
 /* ------------------------------------------------------- many JV graphs */
 /* No single reference counterpart: runs RunJvElimination
@@ -412,10 +446,10 @@ typedef struct {
   uint64_t edges_scanned;   /* summed over graphs and epochs */
   uint64_t proposed;
   uint64_t committed;
-  double device_ms;         /* per epoch: the launch, or the whole replayed graph */
-  double assemble_ms;       /* host packing of the union's per-epoch arrays */
+  double device_ms;         /* per epoch: device commit + JV launch + download, or the replayed graph */
+  double assemble_ms;       /* host packing of the union's arrays (first epoch) and commit lists */
   double propose_ms;        /* assemble + device round trip */
-  double host_ms;           /* re-verification, commit and CSR rebuild of all graphs */
+  double host_ms;           /* re-verification, commit rule and host CSR rebuild of all graphs */
   double total_ms;
   uint32_t graph_launches;
   uint32_t reserved;

@@ -197,14 +197,17 @@ class OGraph:
 
 def jv_run(graph: OGraph, max_rounds=100, backend="cpu") -> str:
     """RunJvElimination(kCpu) + WriteProof V1 on a clone; returns the text."""
-    g = graph.clone()
     lib = oracle()
-    buf = C.create_string_buffer(1 << 26)
-    size = lib.orc_jv_run(g.h, max_rounds, backend.encode(), buf, len(buf))
-    if size < 0:
-        _raise(lib.orc_last_error)
-    assert size < len(buf)
-    return buf.value.decode()
+    capacity = 1 << 20
+    while True:
+        g = graph.clone()
+        buf = C.create_string_buffer(capacity)
+        size = lib.orc_jv_run(g.h, max_rounds, backend.encode(), buf, len(buf))
+        if size < 0:
+            _raise(lib.orc_last_error)
+        if size < len(buf):
+            return buf.value.decode()
+        capacity = size + 1
 
 
 def jv_replay(graph: OGraph, proof: str) -> None:

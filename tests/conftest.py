@@ -26,7 +26,7 @@ def ctx():
     """One HIP context for the GPU tests of this process."""
     from cudaedgeelimination_amd import backend
 
-    context = backend.Context(-1)
+    context = backend.Context(0)
     yield context
     context.close()
 

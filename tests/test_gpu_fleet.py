@@ -87,3 +87,56 @@ def test_many_instance_kopt_batch_equals_oracle(ctx, instances):
     for i, want in enumerate(expect):
         assert np.array_equal(b.segment_costs(i), want), i
     b.close()
+
+
+@pytest.fixture(scope="module")
+def full_instances():
+    """The many-instance workload at its full size: 256 instances, n
+    log-uniform in [200, 4000], 4,096 rows each (2,048 / 1,229 / 819)."""
+    return [W.config4_instance(i) for i in range(256)]
+
+
+def test_full_scale_fleet_and_batch_equal_oracle(ctx, full_instances):
+    """Every one of the 256 proofs and all 768 cost segments of the
+    many-instance workload equal the CPU oracle (JV epochs replayed as HIP
+    graphs, as the bench runs them)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    graphs = [g for g, _ in full_instances]
+    assert len(graphs) == 256
+    assert min(g.dimension for g in graphs) >= 200 and max(g.dimension for g in graphs) <= 4000
+    say("full-scale fleet: 256 graphs")
+    fleet = JvFleet(ctx, graphs)
+    st = fleet.run(100, use_graph=True)
+    assert st["instances"] == 256 and st["committed"] > 0
+    got = [fleet.proof(i) for i in range(256)]
+    with ThreadPoolExecutor(8) as pool:
+        want = list(pool.map(lambda g: O.jv_run(O.OGraph.from_snapshot(g)), graphs))
+    bad = [i for i in range(256) if O.canonical_proof(got[i]) != O.canonical_proof(want[i])]
+    assert not bad, f"proofs differ for instances {bad[:10]}"
+    for i in (0, 1, 2, 3, 127, 255):  # final snapshots, by replaying the device proof
+        final = O.OGraph.from_snapshot(graphs[i])
+        O.oracle().orc_jv_replay(final.h, got[i].encode())
+        assert np.array_equal(fleet.active(i), final.arrays()["edge_active"]), i
+    fleet.close()
+
+    say("full-scale k-opt batch: 768 segments, 1,048,576 rows")
+    b = KoptBatch(ctx)
+    segs, tasks, expect = [], [], []
+    for g, wl in full_instances:
+        inst = b.add_instance(g)
+        for k in (3, 4, 5):
+            rows = wl.bucket(k)
+            segs.append((inst, k, len(rows)))
+            tasks.append(rows)
+            expect.append((g, k, rows))
+    b.set_work(segs, np.concatenate(tasks))
+    assert sum(c for _, _, c in segs) == 256 * 4096
+    assert b.cells == 53_572_864
+    b.run(1, use_graph=True)
+    bad = []
+    for i, (g, k, rows) in enumerate(expect):
+        if not np.array_equal(b.segment_costs(i), O.kopt_costs(g.x, g.y, g.distance_type, k, rows)[0]):
+            bad.append(i)
+    assert not bad, f"segments differ: {bad[:10]}"
+    b.close()

@@ -31,13 +31,7 @@ def say(msg):
 
 @pytest.fixture(scope="module")
 def snapshots():
-    gs = graphs()
-    x, y = W._points_uniform(0xEE0000D2, 300, 0, 10**5)
-    gs["dense90"] = W.knn_graph(x, y, EUC_2D, 90)  # rows of 90+ neighbours
-    # coordinate extents beyond 2^25: the 64-bit integer distance path
-    x, y = W._points_uniform(0xEE0000D3, 400, 0, 1 << 30)
-    gs["wide"] = W.knn_graph(x, y, EUC_2D, 10)
-    return gs
+    return graphs()  # includes dense90 (rows beyond the LDS cache) and wide (64-bit distances)
 
 
 @pytest.mark.parametrize("name", ["uniform300", "lattice", "ceil400", "dense90", "wide"])

@@ -85,20 +85,29 @@ def test_batch_full_config3_graph_replay(ctx):
 
 
 def test_batch_pipelined_runs_equal_reference(ctx):
-    """Launches dealt over 2 to 4 streams and cost pools: the pool that
-    downloads read holds the whole result after 1, 2 and 5 launches, also
-    between ordinary launches; mask output and graph replay refuse it."""
+    """Launches dealt over 2 to 4 streams and cost pools: every pool a
+    pipelined run wrote holds the whole result after 1, 2, depth and 5
+    launches, also between ordinary launches; a pool no launch of the current
+    work wrote is refused; mask output and graph replay refuse pipelining."""
     say("batch config3 pipelined runs")
     wl = W.config3()
     b = single_instance_batch(ctx, wl)
     for depth in (2, 3, 4):
         b.set_pipeline_depth(depth)
-        for launches in (1, 2, 5):
+        for launches in sorted({1, 2, depth, 5}):
             assert b.run(launches, pipelined=True) > 0
-            for i, k in enumerate((3, 4, 5)):
-                assert sha(b.segment_costs(i)) == G["config3"][f"k{k}"]["sha256"], \
-                    f"depth {depth}, {launches} launches, k={k}"
+            for pool in range(min(launches, depth)):
+                for i, k in enumerate((3, 4, 5)):
+                    assert sha(b.segment_costs(i, pool)) == G["config3"][f"k{k}"]["sha256"], \
+                        f"depth {depth}, {launches} launches, pool {pool}, k={k}"
             b.run(1)
+    # new work forgets which pools hold results
+    b.set_work(b.segments, np.concatenate([wl.bucket(k) for k in (3, 4, 5)]))
+    with pytest.raises(_lib.EeError):
+        b.segment_costs(0, 1)
+    b.run(4, pipelined=True)
+    for pool in range(4):
+        assert sha(b.segment_costs(2, pool)) == G["config3"]["k5"]["sha256"], pool
     for bad in (1, 5):
         with pytest.raises(_lib.InvalidArgument):
             b.set_pipeline_depth(bad)
@@ -193,6 +202,15 @@ def test_batch_rejects_bad_work(ctx):
 
 
 # ---------------------------------------------------------------- sessions
+def assert_device_snapshot(session, og):
+    """The session's device-resident active flags and CSR (compacted on the
+    device after every commit) equal the oracle snapshot's RebuildCsr."""
+    dev = session.device_snapshot()
+    want = og.arrays()
+    for key in ("edge_active", "row_offsets", "neighbors", "csr_edge_ids"):
+        assert np.array_equal(dev[key], want[key]), key
+
+
 @pytest.mark.parametrize("name", ["config1", "config2", "config1_ceil"])
 def test_session_proof_equals_reference(ctx, name):
     say(f"session {name}")
@@ -211,6 +229,35 @@ def test_session_proof_equals_reference(ctx, name):
     final = og.clone()
     O.oracle().orc_jv_replay(final.h, run.proof.encode())
     assert np.array_equal(s.active(), final.arrays()["edge_active"])
+    # the last epoch committed nothing, so the device holds the final snapshot
+    assert_device_snapshot(s, final)
+    assert run.epochs >= 2 and run.compact_ms > 0
+    # a second run starts from the final snapshot: one epoch, nothing to commit
+    again = s.run(100)
+    assert again.epochs == 1 and again.committed == 0
+    assert_device_snapshot(s, final)
+    s.close()
+
+
+def test_session_rounds_limit_leaves_device_on_the_proposed_snapshot(ctx):
+    """max_rounds cuts a run after an epoch that committed: the host applied
+    that commit, the device still holds the snapshot the epoch proposed on
+    (its next run uploads the host's); the next run continues from there and
+    together the two runs commit what one uncut run commits."""
+    g = W.config2()
+    og = O.OGraph.from_snapshot(g)
+    s = JvSession(ctx, g)
+    first = s.run(1)
+    assert first.epochs == 1 and first.committed > 0
+    assert_device_snapshot(s, og)  # not compacted: the cut run proposes no more
+    rest = s.run(100)
+    full = O.jv_run(og)
+    final = og.clone()
+    O.oracle().orc_jv_replay(final.h, full.encode())
+    assert np.array_equal(s.active(), final.arrays()["edge_active"])
+    assert_device_snapshot(s, final)
+    assert first.committed + rest.committed == int(
+        sum(1 for line in full.splitlines() if line.startswith("record ")))
     s.close()
 
 
@@ -239,7 +286,12 @@ def test_session_random_graphs(ctx, seed):
     g = W.knn_graph(x, y, seed % 2, 8 + 4 * seed)
     s = JvSession(ctx, g)
     run = s.run(100)
-    assert O.canonical_proof(run.proof) == O.canonical_proof(O.jv_run(O.OGraph.from_snapshot(g)))
+    og = O.OGraph.from_snapshot(g)
+    want = O.jv_run(og)
+    assert O.canonical_proof(run.proof) == O.canonical_proof(want)
+    final = og.clone()
+    O.oracle().orc_jv_replay(final.h, want.encode())
+    assert_device_snapshot(s, final)
     s.close()
 
 

@@ -214,3 +214,45 @@ def test_submit_many_equals_oracle(ctx, graphs_and_rows, batching):
         server.submit_many(bad)
     assert server.stats()["requests"] == before
     server.close()
+
+
+def test_served_graphs_at_full_size(ctx):
+    """The served workload at its full size: four resident graphs (n = 1k,
+    5k, 15k, 30k, uniform, 10-NN), lognormal leaf-cost queries of 128 to
+    65,536 cells with JV rechecks mixed in, one of them a 65,536-cell query
+    above the batch cell limit (it runs in a launch of its own).  Every
+    result equals the CPU oracle and the plain per-query server's."""
+    say("serving at full size: graphs 1k/5k/15k/30k")
+    graphs = W.config5_graphs()
+    assert [g.dimension for g in graphs] == [1000, 5000, 15000, 30000]
+    pools = W.config5_row_pools(graphs)
+    rng = np.random.default_rng(0xEE0005F5)
+    requests, _ = W.config5_requests(graphs, pools, 600, rng)
+    big = pools[3][5][:316]  # 316 x 208 = 65,728 cells: the largest query size
+    requests.append((3, 5, big))
+    requests.append((2, 4, pools[2][4][:2621]))  # 65,525 cells of k = 4
+    sizes = [len(t) * TEMPLATES_PER_K[k] for _, k, t in requests if k]
+    assert max(sizes) > 32768 and min(sizes) >= 128
+    oracles = [O.OGraph.from_snapshot(g) for g in graphs]
+    want_w = [o.witnesses() for o in oracles]
+    want = []
+    for g, k, t in requests:
+        want.append(want_w[g][t] if k == 0 else expected(graphs, g, k, t))
+    results = {}
+    for mode in ("batched", "plain"):
+        server = KoptServer(ctx, batching=(mode == "batched"), max_batch_cells=32768,
+                            hbm_budget_bytes=1 << 30)
+        kids = [server.add_graph(g) for g in graphs]
+        jids = [server.add_jv_graph(g) for g in graphs]
+        ids = server.submit_many([(kids[g], k, t) if k else (jids[g], 0, t) for g, k, t in requests])
+        outs = [out for out, _ in server.wait_many(ids, 120000)]
+        results[mode] = outs
+        stats = server.stats()
+        assert stats["requests"] == len(requests)
+        if mode == "batched":
+            assert stats["batches"] < len(requests)  # requests shared launches
+        server.close()
+        bad = [i for i, (o, w) in enumerate(zip(outs, want)) if not np.array_equal(o, w)]
+        assert not bad, f"{mode}: requests differ from the oracle: {bad[:10]}"
+    for a, b in zip(results["batched"], results["plain"]):
+        assert np.array_equal(a, b)

@@ -157,3 +157,21 @@ def test_missing_library_raises(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(_lib.BackendMissing):
         _lib.lib()
+
+
+def test_shim_rejects_malformed_ht_snapshots_before_any_device_call():
+    """The drop-in shim's HT reply symbols throw std::invalid_argument for a
+    snapshot whose containers disagree with its dimension or CSR (the
+    reference's ValidateGraphShape, src/cuda/ht_replies.cu:521-531), ahead of
+    device selection, so no GPU is needed (oracle/shim_shape_check.cpp)."""
+    import subprocess
+
+    exe = ROOT / "oracle" / "_ref" / "shim_shape_check"
+    if not exe.exists():
+        pytest.skip("the drop-in build (oracle/_ref) is not present")
+    fixture = ROOT / "tests" / "golden" / "recursive-point"
+    run = subprocess.run([str(exe), str(fixture / "recursive-point.tsp"),
+                          str(fixture / "recursive-point.edg")], capture_output=True, text=True,
+                         timeout=60)
+    assert run.returncode == 0, run.stdout + run.stderr
+    assert run.stdout.count("ok ") == 12

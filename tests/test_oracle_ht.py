@@ -6,7 +6,8 @@ AppendHtHamiltonRepliesUnchecked / AppendHtEndRepliesUnchecked, :230-277),
 built from the reference sources into oracle/_ref and called through
 oracle/ref_harness.cpp.  Graphs: the 300-node uniform instance, the
 drilling-grid lattice (many equal distances, so the >= and <= ties of the
-2-opt and cost tests matter), and CEIL_2D.  Closed forms beside it: end
+2-opt and cost tests matter), CEIL_2D, a dense graph with rows of 90+
+neighbours and one with coordinate extents beyond 2^25.  Closed forms beside it: end
 replies number sum(degree - 1) and never hold the internal edge; every
 Hamilton reply is a pair of the center's neighbours in CSR order.
 """
@@ -26,7 +27,16 @@ def graphs():
     g2 = W.config2()
     x, y = W._points_uniform(0xEE0000D1, 400, 0, 3000)
     g3 = W.knn_graph(x, y, CEIL_2D, 9)
-    return {"uniform300": g1, "lattice": g2, "ceil400": g3}
+    out = {"uniform300": g1, "lattice": g2, "ceil400": g3}
+    x, y = W._points_uniform(0xEE0000D2, 300, 0, 10**5)
+    out["dense90"] = W.knn_graph(x, y, EUC_2D, 90)  # rows of 90+ neighbours
+    # coordinate extents beyond 2^25: the device's 64-bit distance path
+    x, y = W._points_uniform(0xEE0000D3, 400, 0, 1 << 30)
+    out["wide"] = W.knn_graph(x, y, EUC_2D, 10)
+    return out
+
+
+NAMES = ["uniform300", "lattice", "ceil400", "dense90", "wide"]
 
 
 @pytest.fixture(scope="module")
@@ -76,12 +86,12 @@ def ref_args(g):
     return (g.x, g.y, g.distance_type, g.edge_u, g.edge_v, g.edge_weight, g.edge_active)
 
 
-@pytest.mark.parametrize("name", ["uniform300", "lattice", "ceil400"])
+@pytest.mark.parametrize("name", NAMES)
 def test_hamilton_replies_equal_reference(snapshots, name):
     require_ref()
     g = snapshots[name]
     og = O.OGraph.from_snapshot(g)
-    for target, centers in ht_cases(g, 0xA1):
+    for target, centers in ht_cases(g, 0xA1, centers=64 if name == "dense90" else 300):
         bound = pair_bound(g, centers)
         off, rep = O.ht_hamilton_replies(og, target, centers, bound)
         roff, rrep = O.ref_ht_hamilton_replies(*ref_args(g), target, centers, bound)
@@ -95,7 +105,7 @@ def test_hamilton_replies_equal_reference(snapshots, name):
                 assert r[0] == c and row.index(r[1]) < row.index(r[2])
 
 
-@pytest.mark.parametrize("name", ["uniform300", "lattice", "ceil400"])
+@pytest.mark.parametrize("name", NAMES)
 def test_end_replies_equal_reference(snapshots, name):
     require_ref()
     g = snapshots[name]
