@@ -451,7 +451,7 @@ typedef struct {
   double propose_ms;        /* assemble + device round trip */
   double host_ms;           /* re-verification, commit rule and host CSR rebuild of all graphs */
   double total_ms;
-  uint32_t graph_launches;
+  uint32_t graph_launches;  /* epochs replayed as a graph: every epoch after the first */
   uint32_t reserved;
 } ee_jv_fleet_stats;
 

@@ -39,7 +39,9 @@ def test_fleet_proofs_equal_standalone_and_oracle(ctx, instances, use_graph):
     assert st["instances"] == len(graphs)
     assert st["committed"] > 0
     if use_graph:
-        assert st["graph_launches"] == st["epochs"] and st["epochs"] >= 2
+        # the first epoch uploads the union eagerly; every later one (device
+        # commit, launch, download) is a replayed graph
+        assert st["graph_launches"] == st["epochs"] - 1 and st["epochs"] >= 2
     for i, g in enumerate(graphs):
         og = O.OGraph.from_snapshot(g)
         want = O.jv_run(og)
