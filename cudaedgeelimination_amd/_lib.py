@@ -125,6 +125,23 @@ class TourView(C.Structure):
     ]
 
 
+class LeafBlock(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("first", C.c_uint64), ("k", C.c_uint32), ("rows", C.c_uint32),
+                ("template_count", C.c_uint32), ("mask_words", C.c_uint32), ("tasks", C.c_void_p),
+                ("positions", C.c_void_p), ("costs", C.c_void_p), ("improving", C.c_void_p),
+                ("improving_cells", C.c_uint64)]
+
+
+class LeafStats(C.Structure):
+    _fields_ = [("blocks", C.c_uint64), ("rows", C.c_uint64), ("cells", C.c_uint64),
+                ("improving_cells", C.c_uint64), ("kernel_ms", C.c_double), ("copy_ms", C.c_double),
+                ("wait_ms", C.c_double), ("certify_ms", C.c_double), ("certify_threads", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
 class SegmentDesc(C.Structure):
     _fields_ = [("instance", C.c_uint32), ("k", C.c_uint32), ("task_count", C.c_uint64)]
 
@@ -251,6 +268,15 @@ PROTOTYPES = {
     "ee_kopt_eval_costs": (C.c_int, [P, P, P, I32, U8, U32, P, U32, U64, P, SZ, P,
                                      C.POINTER(KoptUsage)]),
     "ee_kopt_clear_cache": (C.c_int, [P]),
+    "ee_kopt_certify": (C.c_int, [P, P, I32, U8, U32, P, U32, P, SZ, P, P, C.POINTER(U64),
+                                  C.POINTER(U64)]),
+    "ee_leaf_create": (C.c_int, [P, P, P, I32, U8, U32, U32, C.POINTER(P)]),
+    "ee_leaf_destroy": (None, [P]),
+    "ee_leaf_set_tour": (C.c_int, [P, C.POINTER(TourView)]),
+    "ee_leaf_submit_tasks": (C.c_int, [P, U32, P, U32, U64]),
+    "ee_leaf_submit_cursor": (C.c_int, [P, U32, U64, U32, U64, C.POINTER(U32)]),
+    "ee_leaf_next": (C.c_int, [P, C.POINTER(LeafBlock)]),
+    "ee_leaf_stats_get": (C.c_int, [P, C.POINTER(LeafStats)]),
     "ee_gen_points_uniform": (C.c_int, [U64, I32, I64, I64, P, P]),
     "ee_gen_points_lattice": (C.c_int, [U64, I32, I32, I64, DBL, I64, C.POINTER(I32), P, P]),
     "ee_gen_points_clustered": (C.c_int, [U64, I32, I32, DBL, I64, I64, P, P]),

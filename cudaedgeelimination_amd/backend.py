@@ -190,6 +190,35 @@ def evaluate_kopt_template_costs_cuda(graph: GraphSnapshot, table: KOptReconnect
         int(usage.resident_bytes), usage.h2d_ms, usage.kernel_ms, usage.d2h_ms)
 
 
+def certify_kopt_costs(graph: GraphSnapshot, table: KOptReconnectTable, tasks: np.ndarray,
+                       cells: np.ndarray):
+    """ee_kopt_certify: the host's exact CPU matrix of `tasks` compared cell
+    by cell with `cells` (the certification of EvaluateKOptTemplateCosts,
+    src/cpu/kopt_search.cpp:1455-1475).  Returns (improving masks uint32
+    [rows, ceil(T / 32)], improving cell count); raises EeError
+    (EE_ERR_RUNTIME) with .mismatch = the first differing cell otherwise."""
+    tasks = np.ascontiguousarray(tasks, dtype=_lib.KOPT_TASK_DTYPE)
+    templates = np.ascontiguousarray(table.templates, dtype=_lib.MATCHING_DTYPE)
+    T = len(templates)
+    cells = np.ascontiguousarray(cells, dtype=np.int64).reshape(-1)
+    if cells.size != len(tasks) * T:
+        raise ValueError("cells must hold rows x templates values")
+    words = (T + 31) // 32
+    improving = np.zeros((len(tasks), words), np.uint32)
+    mismatch, count = C.c_uint64(), C.c_uint64()
+    status = _lib.lib().ee_kopt_certify(
+        _lib.ptr(graph.x), _lib.ptr(graph.y), graph.dimension, graph.distance_type, table.k,
+        _lib.ptr(templates), T, _lib.ptr(tasks) if len(tasks) else None, len(tasks),
+        _lib.ptr(cells) if cells.size else None, _lib.ptr(improving) if improving.size else None,
+        C.byref(mismatch), C.byref(count))
+    try:
+        _lib.check(status)
+    except _lib.EeError as err:
+        err.mismatch = int(mismatch.value)
+        raise
+    return improving, int(count.value)
+
+
 @dataclasses.dataclass
 class HtReplyCudaCacheUsage:
     graph_hit: bool = False

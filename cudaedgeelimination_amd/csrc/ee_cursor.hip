@@ -252,6 +252,23 @@ template <typename F> ee_status Guarded(F&& body) {
 }
 
 }  // namespace
+
+uint64_t CursorSetCount(uint32_t selectable_count, uint32_t k) { return SetCount(selectable_count, k); }
+
+void CheckCursorTour(const ee_tour_view* tour, int32_t n) { CheckTourView(tour, n); }
+
+void LaunchCursorRows(const int32_t* d_tour, const int64_t* d_edge_costs, const uint64_t* d_selectable,
+                      uint32_t tour_length, uint32_t selectable_count, uint32_t k, uint64_t first,
+                      uint32_t count, ee_kopt_task* d_tasks, uint32_t* d_positions, uint32_t* d_status,
+                      hipStream_t stream) {
+  if (count == 0) return;
+  const uint32_t blocks = (count + kCursorBlock - 1) / kCursorBlock;
+  hipLaunchKernelGGL(KoptCursorKernel, dim3(blocks), dim3(kCursorBlock), 0, stream, d_tour, d_edge_costs,
+                     d_selectable, tour_length, selectable_count - 1, k, first, count, d_tasks, d_positions,
+                     d_status);
+  CheckHip(hipGetLastError(), "KoptCursorKernel launch");
+}
+
 }  // namespace ee
 
 extern "C" {

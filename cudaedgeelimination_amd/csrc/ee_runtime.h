@@ -64,6 +64,7 @@ template <typename T> class Pinned {
     count_ = count;
   }
   T* get() { return ptr_; }
+  const T* get() const { return ptr_; }
   size_t count() const { return count_; }
 
  private:
@@ -252,6 +253,18 @@ struct Context {
   KoptCursorCache cursor;
   HtCache ht;
 };
+
+// The deletion-set producer of ee_cursor.hip: the number of sets of size k
+// the reference cursor enumerates over `selectable_count` positions, the
+// checks of a tour view, and the row-building launch (sets [first, first +
+// count) into d_tasks and, when not NULL, their sorted positions into
+// d_positions; bit 0 of *d_status is set where a deleted cost overflows).
+uint64_t CursorSetCount(uint32_t selectable_count, uint32_t k);
+void CheckCursorTour(const ee_tour_view* tour, int32_t n);
+void LaunchCursorRows(const int32_t* d_tour, const int64_t* d_edge_costs, const uint64_t* d_selectable,
+                      uint32_t tour_length, uint32_t selectable_count, uint32_t k, uint64_t first,
+                      uint32_t count, ee_kopt_task* d_tasks, uint32_t* d_positions, uint32_t* d_status,
+                      hipStream_t stream);
 
 std::vector<ee_endpoint_matching> CanonicalTemplates(uint32_t k, uint64_t* hash);
 KoptPlan BuildPlan(uint32_t k, const ee_endpoint_matching* templates, uint32_t count);

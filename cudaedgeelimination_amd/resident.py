@@ -1,6 +1,8 @@
 """Device-resident forms of the two hot paths: many k-opt segments per launch
-(``KoptBatch``) and multi-epoch JV with the snapshot kept on the device
-(``JvSession``).  Thin owners over the C ABI in include/ee_hip.h."""
+(``KoptBatch``), multi-epoch JV with the snapshot kept on the device
+(``JvSession``, ``JvFleet``), served queries (``KoptServer``) and the
+certified leaf pipeline (``LeafPipeline``).  Thin owners over the C ABI in
+include/ee_hip.h."""
 from __future__ import annotations
 
 import ctypes as C
@@ -9,7 +11,7 @@ import dataclasses
 import numpy as np
 
 from . import _lib
-from .backend import Context
+from .backend import Context, TourContext
 from .graph import GraphSnapshot
 
 TEMPLATES_PER_K = {3: 4, 4: 25, 5: 208}
@@ -394,3 +396,83 @@ class KoptServer:
         return {"requests": st.requests, "batches": st.batches, "cells": st.cells,
                 "jv_edges": st.jv_edges, "peak_inflight_bytes": st.peak_inflight_bytes,
                 "device_ms": st.device_ms}
+
+
+@dataclasses.dataclass
+class LeafBlock:
+    """One certified block of ee_leaf_next (copies of the pipeline's views)."""
+    tag: int
+    first: int
+    k: int
+    tasks: np.ndarray             # KOPT_TASK_DTYPE [rows]
+    positions: np.ndarray | None  # uint32 [rows, k] (cursor blocks)
+    costs: np.ndarray             # int64 [rows, T], certified
+    improving: np.ndarray         # uint32 [rows, ceil(T / 32)]
+    improving_cells: int
+
+
+class LeafPipeline:
+    """The certified leaf loop with device work ahead of host certification:
+    submit up to `depth` blocks, then take them back in order with next()."""
+
+    def __init__(self, ctx: Context, graph: GraphSnapshot, depth: int = 2, max_rows: int = 4096):
+        self.ctx = ctx
+        self.graph = graph
+        handle = C.c_void_p()
+        _lib.check(_lib.lib().ee_leaf_create(ctx.handle, _lib.ptr(graph.x), _lib.ptr(graph.y),
+                                             graph.dimension, graph.distance_type, depth, max_rows,
+                                             C.byref(handle)))
+        self._h = handle
+        ctx.adopt(self)
+
+    def close(self):
+        if self._h is not None:
+            _lib.lib().ee_leaf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tour(self, tour: TourContext) -> None:
+        self._tour = tour
+        view = tour.view()
+        _lib.check(_lib.lib().ee_leaf_set_tour(self._h, C.byref(view)))
+
+    def submit_tasks(self, k: int, tasks: np.ndarray, tag: int = 0) -> None:
+        tasks = np.ascontiguousarray(tasks, dtype=_lib.KOPT_TASK_DTYPE)
+        _lib.check(_lib.lib().ee_leaf_submit_tasks(self._h, k, _lib.ptr(tasks) if len(tasks) else None,
+                                                   len(tasks), tag))
+
+    def submit_cursor(self, k: int, first: int, count: int, tag: int = 0) -> int:
+        rows = C.c_uint32()
+        _lib.check(_lib.lib().ee_leaf_submit_cursor(self._h, k, first, count, tag, C.byref(rows)))
+        return int(rows.value)
+
+    def next(self, copy: bool = True) -> LeafBlock:
+        b = _lib.LeafBlock()
+        _lib.check(_lib.lib().ee_leaf_next(self._h, C.byref(b)))
+        rows, T, W = b.rows, b.template_count, b.mask_words
+
+        def view(ptr, dtype, shape):
+            count = int(np.prod(shape))
+            if count == 0:
+                return np.zeros(shape, dtype)
+            buf = (C.c_char * (count * np.dtype(dtype).itemsize)).from_address(ptr)
+            a = np.frombuffer(buf, dtype=dtype, count=count).reshape(shape)
+            return a.copy() if copy else a
+
+        return LeafBlock(
+            int(b.tag), int(b.first), int(b.k),
+            view(b.tasks, _lib.KOPT_TASK_DTYPE, (rows,)),
+            view(b.positions, np.uint32, (rows, b.k)) if b.positions else None,
+            view(b.costs, np.int64, (rows, T)),
+            view(b.improving, np.uint32, (rows, W)),
+            int(b.improving_cells))
+
+    def stats(self) -> dict:
+        st = _lib.LeafStats()
+        _lib.check(_lib.lib().ee_leaf_stats_get(self._h, C.byref(st)))
+        return st.as_dict()

@@ -173,6 +173,7 @@ ee_status ee_kopt_eval_costs(ee_ctx* ctx, const int64_t* x, const int64_t* y, in
 /* Replaces cudaee::detail::ClearKOptCostCudaCache (src/cuda/kopt_cost.cu:472). */
 ee_status ee_kopt_clear_cache(ee_ctx* ctx);
 
+
 /* -------------------------------------------------------- HT replies */
 /* Reference HtNeighborPair, NodeEdge, HtEndReplyTask
  * (include/cuda_edge_elimination/hamilton_tutte.hpp:26-32, 76-81) and
@@ -279,6 +280,94 @@ ee_status ee_kopt_cursor_eval(ee_ctx* ctx, const int64_t* x, const int64_t* y, i
                               uint64_t generator_hash, uint64_t first, uint32_t count,
                               ee_kopt_task* tasks, uint32_t* positions, int64_t* costs,
                               uint32_t* masks, uint32_t* produced, ee_kopt_usage* usage);
+
+/* ------------------------------------------------ certified leaf pipeline */
+/* Replaces the certification step of cudaee::EvaluateKOptTemplateCosts
+ * (src/cpu/kopt_search.cpp:1455-1475) and the row scan of
+ * TryReconnectFromCostRow (kopt_search.cpp:677-734): recomputes every cell of
+ * `cells` (task_count x template_count, row-major, e.g. a device matrix) on
+ * the host by the reference CPU scorer's rules (KOptCostTaskCpuScorer,
+ * kopt_search.cpp:962-1042) and compares them.  When all are equal it returns
+ * EE_OK and, when `improving` is not NULL, writes per row
+ * ceil(template_count / 32) words whose bit j is set iff cell j is below the
+ * row's deleted_cost (the templates the consumer rechecks, in template
+ * order).  Otherwise it fails with EE_ERR_RUNTIME and *mismatch (may be NULL)
+ * receives the first differing cell, as the reference throws with the cell
+ * index.  Runs on every core the process may use; pure host code. */
+ee_status ee_kopt_certify(const int64_t* x, const int64_t* y, int32_t dimension,
+                          uint8_t distance_type, uint32_t k, const ee_endpoint_matching* templates,
+                          uint32_t template_count, const ee_kopt_task* tasks, size_t task_count,
+                          const int64_t* cells, uint32_t* improving, uint64_t* mismatch,
+                          uint64_t* improving_count);
+
+/* No reference counterpart: the certified leaf loop of
+ * ProvePathSystemsByKOptImpl (src/cpu/kopt_search.cpp:2651-2764) as a
+ * pipeline.  The reference evaluates one block of task rows per k on the
+ * device, waits for it, downloads every cell, recomputes the whole matrix on
+ * the host to certify it and only then builds the next block, so device and
+ * host time add up.  Here the caller submits up to `depth` blocks ahead
+ * (rows uploaded from the host, or built on the device from the cost
+ * cursor's deletion-set ranks); each runs its cost launch and downloads its
+ * rows and cells into pinned host memory on its own, while ee_leaf_next
+ * certifies the oldest block on the host threads (ee_kopt_certify's rule,
+ * plus, for device-built rows, a host rebuild of every row from the cursor
+ * order) and hands the consumer its certified cells and improving masks.
+ * Blocks come back in submission order.  A block that fails certification,
+ * or whose rows overflow their deleted cost, fails ee_leaf_next with
+ * EE_ERR_RUNTIME and its cells are never returned.  Templates are the
+ * canonical ones (ee_kopt_build_templates). */
+typedef struct ee_leaf_pipeline ee_leaf_pipeline;
+
+typedef struct {
+  uint64_t tag;                 /* the submitter's tag */
+  uint64_t first;               /* cursor blocks: rank of the first deletion set; else 0 */
+  uint32_t k;
+  uint32_t rows;
+  uint32_t template_count;
+  uint32_t mask_words;          /* ceil(template_count / 32) */
+  const ee_kopt_task* tasks;    /* [rows] */
+  const uint32_t* positions;    /* cursor blocks: [rows x k] sorted tour positions; else NULL */
+  const int64_t* costs;         /* [rows x template_count], certified */
+  const uint32_t* improving;    /* [rows x mask_words] */
+  uint64_t improving_cells;     /* set bits of `improving` */
+} ee_leaf_block;
+
+typedef struct {
+  uint64_t blocks;
+  uint64_t rows;
+  uint64_t cells;               /* certified */
+  uint64_t improving_cells;
+  double kernel_ms;             /* device: row build + cost launch, summed over blocks */
+  double copy_ms;               /* device: download of rows and cells, summed */
+  double wait_ms;               /* host time ee_leaf_next spent waiting for the device */
+  double certify_ms;            /* host time certifying (rows and cells) */
+  uint32_t certify_threads;
+  uint32_t reserved;
+} ee_leaf_stats;
+
+/* depth in [1, 4]: blocks outstanding (submitted, not yet returned by
+ * ee_leaf_next); max_rows in [1, 65536]: rows per block.  The coordinates are
+ * copied (device and host) and must be distance safe. */
+ee_status ee_leaf_create(ee_ctx* ctx, const int64_t* x, const int64_t* y, int32_t dimension,
+                         uint8_t distance_type, uint32_t depth, uint32_t max_rows,
+                         ee_leaf_pipeline** out);
+void ee_leaf_destroy(ee_leaf_pipeline* pipeline);
+/* The tour context of later cursor blocks (as ee_kopt_cursor_eval); waits
+ * for the blocks in flight first. */
+ee_status ee_leaf_set_tour(ee_leaf_pipeline* pipeline, const ee_tour_view* tour);
+/* Enqueues one block of host task rows (validated like ee_kopt_eval_costs). */
+ee_status ee_leaf_submit_tasks(ee_leaf_pipeline* pipeline, uint32_t k, const ee_kopt_task* tasks,
+                               uint32_t rows, uint64_t tag);
+/* Enqueues the deletion sets [first, first + count) of size k of the tour
+ * context, clamped to the sets left; *rows receives the rows (0: nothing was
+ * enqueued). */
+ee_status ee_leaf_submit_cursor(ee_leaf_pipeline* pipeline, uint32_t k, uint64_t first,
+                                uint32_t count, uint64_t tag, uint32_t* rows);
+/* Waits for the oldest outstanding block and certifies it; *block views stay
+ * valid until the next ee_leaf_next or ee_leaf_destroy.  EE_ERR_LOGIC when
+ * nothing is outstanding. */
+ee_status ee_leaf_next(ee_leaf_pipeline* pipeline, ee_leaf_block* block);
+ee_status ee_leaf_stats_get(const ee_leaf_pipeline* pipeline, ee_leaf_stats* stats);
 
 /* --------------------------------------------------- resident k-opt batch */
 /* No reference counterpart.  The reference evaluates one (graph, k) matrix
