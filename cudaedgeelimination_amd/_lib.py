@@ -267,6 +267,8 @@ PROTOTYPES = {
                                       U32, P, P, P, P, C.POINTER(U32), C.POINTER(KoptUsage)]),
     "ee_kopt_eval_costs": (C.c_int, [P, P, P, I32, U8, U32, P, U32, U64, P, SZ, P,
                                      C.POINTER(KoptUsage)]),
+    "ee_kopt_eval_costs_noalloc": (C.c_int, [P, P, P, I32, U8, U32, P, U32, U64, P, SZ, P,
+                                             C.POINTER(KoptUsage)]),
     "ee_kopt_clear_cache": (C.c_int, [P]),
     "ee_kopt_certify": (C.c_int, [P, P, I32, U8, U32, P, U32, P, SZ, P, P, C.POINTER(U64),
                                   C.POINTER(U64)]),

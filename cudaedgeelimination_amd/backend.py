@@ -172,15 +172,19 @@ def make_tasks(port_nodes, deleted_cost=None) -> np.ndarray:
 
 
 def evaluate_kopt_template_costs_cuda(graph: GraphSnapshot, table: KOptReconnectTable,
-                                      tasks: np.ndarray, ctx: Context | None = None):
+                                      tasks: np.ndarray, ctx: Context | None = None,
+                                      no_alloc: bool = False):
     """detail::EvaluateKOptTemplateCostsCuda: (costs[R*T] int64 row-major,
-    selected_device, usage)."""
+    selected_device, usage).  no_alloc: the same evaluation through
+    ee_kopt_eval_costs_noalloc, staged in the device's fixed scratch array
+    (no allocation, no cache, at most EE_NOALLOC_SCRATCH_BYTES of staging)."""
     ctx = _ctx(ctx)
     tasks = np.ascontiguousarray(tasks, dtype=_lib.KOPT_TASK_DTYPE)
     templates = np.ascontiguousarray(table.templates, dtype=_lib.MATCHING_DTYPE)
     costs = np.empty(len(tasks) * len(templates), dtype=np.int64)
     usage = _lib.KoptUsage()
-    _lib.check(_lib.lib().ee_kopt_eval_costs(
+    entry = _lib.lib().ee_kopt_eval_costs_noalloc if no_alloc else _lib.lib().ee_kopt_eval_costs
+    _lib.check(entry(
         ctx.handle, _lib.ptr(graph.x), _lib.ptr(graph.y), graph.dimension, graph.distance_type,
         table.k, _lib.ptr(templates), len(templates), table.generator_hash,
         _lib.ptr(tasks) if len(tasks) else None, len(tasks),

@@ -266,6 +266,10 @@ void LaunchCursorRows(const int32_t* d_tour, const int64_t* d_edge_costs, const 
                       uint32_t count, ee_kopt_task* d_tasks, uint32_t* d_positions, uint32_t* d_status,
                       hipStream_t stream);
 
+// Looks up (once per device, with that device current) the device address of
+// the no-allocation scratch array of ee_kopt_eval_costs_noalloc.
+void InitNoAllocScratch(int device);
+
 std::vector<ee_endpoint_matching> CanonicalTemplates(uint32_t k, uint64_t* hash);
 KoptPlan BuildPlan(uint32_t k, const ee_endpoint_matching* templates, uint32_t count);
 void ValidateTasks(int32_t n, uint32_t k, const ee_kopt_task* tasks, size_t count);

@@ -158,6 +158,7 @@ int SelectDevice(int requested) {
 
 Context::Context(int requested) {
   device = SelectDevice(requested);
+  InitNoAllocScratch(device);
   CheckHip(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
   CheckHip(hipEventCreate(&ev_begin), "hipEventCreate");
   CheckHip(hipEventCreate(&ev_end), "hipEventCreate");

@@ -173,6 +173,27 @@ ee_status ee_kopt_eval_costs(ee_ctx* ctx, const int64_t* x, const int64_t* y, in
 /* Replaces cudaee::detail::ClearKOptCostCudaCache (src/cuda/kopt_cost.cu:472). */
 ee_status ee_kopt_clear_cache(ee_ctx* ctx);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): ee_kopt_eval_costs with the same arguments and the same
+ * cells, bit for bit, but no allocation and no cache.  Its device staging
+ * (the reconnect plan, x and y, the interleaved coordinates, the task rows
+ * and the cost cells; about 23 KB + 32 n + 48 task_count + 8 task_count x
+ * template_count bytes, each part rounded up to 256 bytes) lives in one
+ * static array of EE_NOALLOC_SCRATCH_BYTES per GPU, which each GPU the
+ * library runs on holds until the process ends; a call that needs more fails
+ * with EE_ERR_INVALID_ARGUMENT before touching the device.  Every call on a
+ * GPU uses that GPU's one array: calls that overlap on one GPU (from several
+ * threads or contexts) race and must be serialised by the caller.  usage
+ * reports resident_bytes = the scratch bytes the call used and no cache
+ * hits. */
+#define EE_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+ee_status ee_kopt_eval_costs_noalloc(ee_ctx* ctx, const int64_t* x, const int64_t* y,
+                                     int32_t dimension, uint8_t distance_type, uint32_t k,
+                                     const ee_endpoint_matching* templates,
+                                     uint32_t template_count, uint64_t generator_hash,
+                                     const ee_kopt_task* tasks, size_t task_count,
+                                     int64_t* costs, ee_kopt_usage* usage);
 
 /* -------------------------------------------------------- HT replies */
 /* Reference HtNeighborPair, NodeEdge, HtEndReplyTask
