@@ -29,7 +29,7 @@ sys.path.insert(0, str(ROOT / "tests"))
 import numpy as np  # noqa: E402
 
 from cudaedgeelimination_amd import backend, workloads as W  # noqa: E402
-from cudaedgeelimination_amd.resident import JvSession, KoptBatch, KoptServer  # noqa: E402
+from cudaedgeelimination_amd.resident import JvSession, KoptBatch, KoptServer, LeafPipeline  # noqa: E402
 
 METRIC = "exact k-opt cost cells/s + JV edges/s; p50/p99 query latency vs plain GPU path"
 # streams (and cost pools) the timed k-opt launches are dealt over
@@ -461,6 +461,82 @@ def cursor_leg(ctx, graph, blocks: int = 40) -> dict:
             "masks_speedup_over_plain": plain["ms_per_block"] / dev_masks["ms_per_block"]}
 
 
+def certified_leaf_leg(ctx, wl, block_rows: int = 4096, depth: int = 3, reps: int = 5) -> dict:
+    """The leaf search's certified cost loop (src/cpu/kopt_search.cpp:2651-2764)
+    over the 65,536-row workload: blocks of `block_rows` rows per k (the
+    reference's cost_batch_size), each evaluated on the device and certified
+    on the host against the exact CPU matrix (every cell recomputed and
+    compared) before its improving masks reach the consumer, with `depth`
+    blocks in flight on the device.  Also the cursor form: the deletion sets of
+    a 400-node tour built on the device."""
+    blocks = [(k, wl.bucket(k)[s:s + block_rows]) for k in (3, 4, 5)
+              for s in range(0, len(wl.bucket(k)), block_rows)]
+    cells = sum(len(rows) * TEMPLATES[k] for k, rows in blocks)
+    p = LeafPipeline(ctx, wl.graph, depth=depth, max_rows=block_rows)
+
+    def run_host():
+        submitted = improving = 0
+        for i in range(len(blocks)):
+            while submitted < len(blocks) and submitted < i + depth:
+                k, rows = blocks[submitted]
+                p.submit_tasks(k, rows, tag=submitted)
+                submitted += 1
+            improving += p.next(copy=False).improving_cells
+        return improving
+
+    def stats_delta(fn):
+        before = p.stats()
+        t0 = time.perf_counter()
+        value = fn()
+        wall = time.perf_counter() - t0
+        after = p.stats()
+        return wall, value, {key: after[key] - before[key] for key in
+                             ("blocks", "cells", "kernel_ms", "copy_ms", "wait_ms", "certify_ms")}
+
+    run_host()  # warm-up: host pool threads, pattern caches, pinned pages
+    runs = sorted((stats_delta(run_host) for _ in range(reps)), key=lambda r: r[0])
+    wall, improving, st = runs[len(runs) // 2]
+    out = {
+        "workload": f"{len(blocks)} blocks of <= {block_rows} rows per k over the 65536-row workload",
+        "cells": cells, "blocks": len(blocks), "depth": depth,
+        "certified_cells_per_s": cells / wall, "ms": wall * 1e3,
+        "improving_cells": improving,
+        # per run: device time per block summed (rows + launch; downloads),
+        # host time waiting on the device, host time certifying
+        "device_kernel_ms": st["kernel_ms"], "device_copy_ms": st["copy_ms"],
+        "host_wait_ms": st["wait_ms"], "certify_ms": st["certify_ms"],
+        "certify_threads": p.stats()["certify_threads"],
+    }
+
+    # cursor blocks: the first 40 blocks of k = 5 sets of a 400-node tour
+    t = W.tour_context(5, wl.graph, 400, 120)
+    p.set_tour(t)
+    n_cursor = 40
+
+    def run_cursor():
+        submitted = improving = 0
+        for i in range(n_cursor):
+            while submitted < n_cursor and submitted < i + depth:
+                p.submit_cursor(5, submitted * block_rows, block_rows, tag=submitted)
+                submitted += 1
+            improving += p.next(copy=False).improving_cells
+        return improving
+
+    run_cursor()
+    runs = sorted((stats_delta(run_cursor) for _ in range(reps)), key=lambda r: r[0])
+    wall, improving, st = runs[len(runs) // 2]
+    c_cells = n_cursor * block_rows * TEMPLATES[5]
+    out["cursor_rows"] = {
+        "workload": f"{n_cursor} blocks of {block_rows} k=5 deletion sets of a 400-node tour, "
+                    "rows built on the device and rebuilt on the host",
+        "cells": c_cells, "certified_cells_per_s": c_cells / wall, "ms": wall * 1e3,
+        "improving_cells": improving, "device_kernel_ms": st["kernel_ms"],
+        "device_copy_ms": st["copy_ms"], "host_wait_ms": st["wait_ms"], "certify_ms": st["certify_ms"],
+    }
+    p.close()
+    return out
+
+
 def ht_leg(ctx, cpu: bool, reps: int = 20) -> dict:
     """HT Hamilton replies of one target edge against every other node of the
     30,000-node served graph (10-NN), and end replies of 30,000 tasks: device
@@ -535,6 +611,7 @@ def main() -> int:
     ap.add_argument("--no-many", action="store_true", help="skip the many-instance workload")
     ap.add_argument("--no-cursor", action="store_true", help="skip the deletion-set cursor leg")
     ap.add_argument("--no-ht", action="store_true", help="skip the HT reply leg")
+    ap.add_argument("--no-leaf", action="store_true", help="skip the certified leaf leg")
     ap.add_argument("--serve-qps", default="2000,8000,20000,0",
                     help="Poisson rates of the serving leg; 0 is a burst of --serve-burst requests")
     ap.add_argument("--serve-burst", type=int, default=20000)
@@ -721,6 +798,20 @@ def main() -> int:
             jv["cpu_baseline"] = cpu_jv
             jv["proof_equals_cpu"] = O.canonical_proof(text) == O.canonical_proof(best.proof)
         result["jv"] = jv
+
+    if not args.no_leaf:
+        log("certified leaf pipeline (device blocks ahead of host certification)")
+        leaf = certified_leaf_leg(ctx, wl)
+        # the reference's synchronous shape of the same certified loop: the
+        # plain device call, then its CPU matrix of the same rows, per block
+        # (plain_path.ms + the CPU baseline's matrix time, summed over k)
+        if result.get("cpu_baseline"):
+            cpu_s = wl.cells() / result["cpu_baseline"]["value"]
+            leaf["reference_synchronous_cells_per_s"] = wl.cells() / (plain_s + cpu_s)
+            leaf["reference_cpu_matrix_cells_per_s"] = result["cpu_baseline"]["value"]
+            leaf["speedup_over_reference_cpu_matrix"] = (leaf["certified_cells_per_s"]
+                                                         / result["cpu_baseline"]["value"])
+        result["certified_leaf"] = leaf
 
     if not args.no_ht:
         log("HT replies (30,000 centers of one target)")
