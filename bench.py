@@ -639,16 +639,16 @@ def main() -> int:
     if args.tuning:
         v = [int(t) for t in args.tuning.split(",")]
         batch.set_tuning(v[:3], v[3], v[4], v[5])
-        tuning = batch.tuning()
+        serial_tuning = batch.tuning()
     else:
-        log("autotuning (pipelined runs)")
-        batch.set_pipeline_depth(PIPELINE_DEPTH)
-        tuning = batch.autotune(40, pipelined=True)  # enough launches to fill the pipeline
+        log("autotuning (single-stream launches)")
+        serial_tuning = batch.autotune(10)
     batch.run(max(args.warmup, 1))
     log(f"timing {args.steps} steps")
     ctx_sync = backend._lib.lib().ee_ctx_synchronize
     ctx_sync(ctx.handle)
-    # serial launches on one stream: the kernel's own duration (roofline)
+    # serial launches on one stream at their own best shape: the kernel's own
+    # duration (roofline)
     t0 = time.perf_counter()
     device_ms = batch.run(args.steps)
     ctx_sync(ctx.handle)
@@ -658,12 +658,34 @@ def main() -> int:
     # each still a full pass over the batch, so a launch's start (dependent
     # gathers) overlaps the previous ones' drain
     batch.set_pipeline_depth(PIPELINE_DEPTH)
+    if args.tuning:
+        tuning = serial_tuning
+    else:
+        log("autotuning (pipelined runs)")
+        tuning = batch.autotune(40, pipelined=True)  # enough launches to fill the pipeline
     batch.run(max(args.warmup, 1), pipelined=True)
     ctx_sync(ctx.handle)
     t0 = time.perf_counter()
     pipelined_ms = batch.run(args.steps, pipelined=True)
     ctx_sync(ctx.handle)
     wall_pipelined = time.perf_counter() - t0
+    # evidence of the overlap, from the device clock: every wavefront stamps
+    # its start and end, so each launch spans its first start to its last
+    # end; the same shape alone on one stream beside it
+    n_stamped = min(max(args.steps, 64), 512)
+    s0, e0 = batch.run_stamped(n_stamped, pipelined=False)
+    s1, e1 = batch.run_stamped(n_stamped, pipelined=True)
+    overlap = {
+        "launches": n_stamped,
+        "alone_median_launch_us": float(np.median(e0 - s0)),
+        "alone_wall_us_per_launch": float((e0.max() - s0.min()) / n_stamped),
+        "pipelined_median_launch_us": float(np.median(e1 - s1)),
+        "pipelined_wall_us_per_launch": float((e1.max() - s1.min()) / n_stamped),
+        # summed launch spans over the run's span: launches running at once
+        "launches_in_flight": float((e1 - s1).sum() / (e1.max() - s1.min())),
+        "fraction_starting_before_previous_ends": float(np.mean(s1[1:] < e1[:-1])),
+        "clock": "s_memrealtime (constant rate, hipDeviceAttributeWallClockRate)",
+    }
     cells = batch.cells
     ms_per_step = wall_pipelined * 1e3 / args.steps
     value = cells / (wall_pipelined / args.steps)
@@ -721,9 +743,11 @@ def main() -> int:
             "tuning": tuning, "launches_per_step": 1,
             "streams": PIPELINE_DEPTH, "output_pools": PIPELINE_DEPTH,
         },
-        # the same steps launched back to back on one stream
+        # the same steps launched back to back on one stream, at the best
+        # single-stream shape
         "serial": {"value": cells / (wall / args.steps), "ms_per_step": wall * 1e3 / args.steps,
-                   "kernel_ms": device_ms},
+                   "kernel_ms": device_ms, "tuning": serial_tuning},
+        "pipelined_overlap": overlap,
         # the kernel's own duration comes from the serial launches: rocprof's
         # per-kernel durations, which the committed summary must match, are
         # defined only for launches that do not overlap.  The pipelined

@@ -226,6 +226,7 @@ PROTOTYPES = {
     "ee_kopt_batch_autotune_runs": (C.c_int, [P, U32, C.c_int, C.c_char_p, C.POINTER(KoptTuning)]),
     "ee_kopt_batch_run": (C.c_int, [P, U32, C.c_int, C.POINTER(DBL)]),
     "ee_kopt_batch_set_pipeline_depth": (C.c_int, [P, U32]),
+    "ee_kopt_batch_run_stamped": (C.c_int, [P, U32, C.c_int, P, P]),
     "ee_kopt_batch_download": (C.c_int, [P, U64, U64, P]),
     "ee_kopt_batch_download_pool": (C.c_int, [P, U32, U64, U64, P]),
     "ee_kopt_batch_set_output": (C.c_int, [P, U32]),

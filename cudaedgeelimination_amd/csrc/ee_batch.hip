@@ -167,12 +167,18 @@ struct Batch {
 
   // Mean device time of one launch over `launches` back-to-back launches,
   // pipelined over two streams when asked.
-  double Time(uint32_t launches, bool use_graph, bool pipelined = false) {
+  // Clock stamps per launch: two per wavefront of the grid.
+  size_t StampsPerLaunch() const { return 2 * static_cast<size_t>(grid) * (tuning.block / 64); }
+
+  // stamps (measurement only, may be null): StampsPerLaunch() entries per
+  // launch, launch i at stamps + i x StampsPerLaunch().
+  double Time(uint32_t launches, bool use_graph, bool pipelined = false,
+              unsigned long long* stamps = nullptr) {
     if (use_graph && pipelined) throw std::invalid_argument("graph replay and pipelined runs do not combine");
     Prepare();
     if (tile_count == 0 || launches == 0) return 0.0;
     SyncArgs();  // any upload happens here, never inside the capture
-    if (pipelined) return TimePipelined(launches);
+    if (pipelined) return TimePipelined(launches, stamps);
     if (use_graph && exec == nullptr) {
       CheckHip(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
       LaunchOnce();
@@ -183,6 +189,10 @@ struct Batch {
     for (uint32_t i = 0; i < launches; ++i) {
       if (use_graph) {
         CheckHip(hipGraphLaunch(exec, ctx->stream), "graph launch");
+      } else if (stamps != nullptr) {
+        SyncArgs();
+        LaunchKoptResident(host_args, d_args.get(), grid, tuning, shared, ctx->stream,
+                           stamps + static_cast<size_t>(i) * StampsPerLaunch());
       } else {
         LaunchOnce();
       }
@@ -194,7 +204,7 @@ struct Batch {
     return static_cast<double>(ms) / launches;
   }
 
-  double TimePipelined(uint32_t launches) {
+  double TimePipelined(uint32_t launches, unsigned long long* stamps = nullptr) {
     if (mask_output) throw std::invalid_argument("pipelined runs write cost cells only");
     if (lanes.size() + 1 < pipeline_depth) lanes.resize(pipeline_depth - 1);
     std::vector<KoptLaunch> lane_args(pipeline_depth, host_args);
@@ -215,8 +225,9 @@ struct Batch {
     }
     for (uint32_t i = 0; i < launches; ++i) {
       const uint32_t j = i % pipeline_depth;
-      LaunchKoptResident(lane_args[j], d_args.get(), grid, tuning, shared,
-                         j == 0 ? ctx->stream : lanes[j - 1].stream);
+      const hipStream_t stream = j == 0 ? ctx->stream : lanes[j - 1].stream;
+      LaunchKoptResident(lane_args[j], d_args.get(), grid, tuning, shared, stream,
+                         stamps != nullptr ? stamps + static_cast<size_t>(i) * StampsPerLaunch() : nullptr);
       if (j > 0) lanes[j - 1].written = true;
     }
     for (uint32_t j = 1; j < pipeline_depth; ++j) {
@@ -363,6 +374,18 @@ ee_kopt_tuning Autotune(Batch& b, uint32_t launches, bool pipelined) {
     return ms;
   };
   double best_ms = measure(best);
+  // start points: one tile per wavefront with small k = 5 tiles, and its
+  // neighbours, the shapes that won on the 65,536-row leaf workload and that
+  // coordinate descent from the default rarely reaches
+  const ee_kopt_tuning seeds[] = {{{40, 32, 5}, 0, 256, 1}, {{36, 20, 12}, 0, 256, 1}, {{32, 24, 6}, 0, 128, 1},
+                                  {{40, 26, 6}, 0, 128, 1}, {{32, 16, 5}, 0, 256, 1}, {{40, 32, 16}, 0, 256, 1}};
+  for (const ee_kopt_tuning& seed : seeds) {
+    const double ms = measure(seed);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = seed;
+    }
+  }
   const uint32_t cu = static_cast<uint32_t>(std::max(b.ctx->compute_units, 1));
   // residency cliffs (every block reserves the LDS of the launch's largest
   // k) make the time a bumpy function of the rows, hence the fine options
@@ -616,6 +639,41 @@ ee_status ee_kopt_batch_run(ee_kopt_batch* batch, uint32_t launches, int flags, 
     const double ms = ee::AsBatch(batch).Time(launches, (flags & EE_KOPT_RUN_GRAPH) != 0,
                                                (flags & EE_KOPT_RUN_PIPELINED) != 0);
     if (device_ms) *device_ms = ms;
+  });
+}
+
+ee_status ee_kopt_batch_run_stamped(ee_kopt_batch* batch, uint32_t launches, int flags, double* start_us,
+                                    double* end_us) {
+  return ee::Guarded([&] {
+    if (start_us == nullptr || end_us == nullptr) throw std::invalid_argument("NULL argument");
+    if (launches == 0 || launches > 1024) throw std::invalid_argument("launches must lie in [1, 1024]");
+    if (flags != 0 && flags != EE_KOPT_RUN_PIPELINED) throw std::invalid_argument("plain or pipelined runs only");
+    ee::Batch& b = ee::AsBatch(batch);
+    b.ctx->Activate();
+    b.Prepare();
+    b.SyncArgs();  // sets the grid the stamps are sized for
+    const size_t per = b.StampsPerLaunch();
+    if (per == 0) throw std::logic_error("the batch has no work");
+    ee::DeviceArray<unsigned long long> d_stamps(per * launches, b.ctx->device);
+    b.Time(launches, false, flags == EE_KOPT_RUN_PIPELINED, d_stamps.get());
+    std::vector<unsigned long long> h(per * launches);
+    d_stamps.Download(h.data(), h.size(), b.ctx->stream);
+    b.ctx->Sync();
+    int khz = 0;
+    ee::CheckHip(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b.ctx->device),
+                 "hipDeviceGetAttribute(wall clock rate)");
+    if (khz <= 0) throw ee::Unavailable("device reports no wall clock rate");
+    unsigned long long origin = ~0ull;
+    for (size_t w = 0; w < h.size(); w += 2) origin = std::min(origin, h[w]);
+    for (uint32_t i = 0; i < launches; ++i) {
+      unsigned long long first = ~0ull, last = 0;
+      for (size_t w = 0; w < per; w += 2) {
+        first = std::min(first, h[i * per + w]);
+        last = std::max(last, h[i * per + w + 1]);
+      }
+      start_us[i] = static_cast<double>(first - origin) * 1e3 / khz;
+      end_us[i] = static_cast<double>(last - origin) * 1e3 / khz;
+    }
   });
 }
 

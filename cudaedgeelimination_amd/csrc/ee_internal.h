@@ -174,9 +174,12 @@ void LaunchKopt(const KoptLaunch& args, const KoptTuning& tuning, size_t shared_
 uint32_t KoptFinalize(KoptLaunch* args, const KoptTuning& tuning, size_t shared_bytes);
 // Launches with the arguments read from `device_args`, a device copy of
 // `finalized` (a KoptFinalize result) that stays unchanged until the launch
-// has run.
+// has run.  stamps (measurement only, may be null): 2 x grid x waves per
+// block entries, wavefront w writing the constant-rate device clock
+// (s_memrealtime) at its start to stamps[2w] and at its end to stamps[2w+1].
 void LaunchKoptResident(const KoptLaunch& finalized, const KoptLaunch* device_args, uint32_t grid,
-                        const KoptTuning& tuning, size_t shared_bytes, hipStream_t stream);
+                        const KoptTuning& tuning, size_t shared_bytes, hipStream_t stream,
+                        unsigned long long* stamps = nullptr);
 
 // ------------------------------------------------------------------- JV
 // Edge slots edge_list[0..edge_count) (or 0..edge_count when edge_list is

@@ -58,6 +58,10 @@ struct KoptPointers {
   const KoptPlan* plans;
   int64_t* costs;
   uint32_t* masks;
+  // measurement only (null otherwise): wavefront w of the launch writes the
+  // constant-rate device clock at its start to stamps[2w] and at its end to
+  // stamps[2w + 1]
+  unsigned long long* stamps;
 };
 
 __device__ __forceinline__ void WaveSync() {
@@ -497,6 +501,8 @@ __device__ void TileLoop(const KoptLaunch& a, const KoptPointers& bufs, char* sm
   const uint32_t wib = tid / kWave;
   const uint32_t waves_per_block = bd / kWave;
   char* wave = smem + a.wave_base + wib * a.wave_bytes;
+  const uint32_t wave_id = __builtin_amdgcn_readfirstlane(blockIdx.x * waves_per_block + wib);
+  if (bufs.stamps != nullptr && lane == 0) bufs.stamps[2 * static_cast<uint64_t>(wave_id)] = __builtin_amdgcn_s_memrealtime();
   const uint32_t block = blockIdx.x - a.k_block_first[ki];
   const uint32_t blocks = a.k_block_count[ki];
   const uint64_t stride = static_cast<uint64_t>(blocks) * waves_per_block;
@@ -533,6 +539,7 @@ __device__ void TileLoop(const KoptLaunch& a, const KoptPointers& bufs, char* sm
       TileCells<K, false, kNonTemporal, kMask>(a, bufs, L, d, smem, wave, lane);
     }
   }
+  if (bufs.stamps != nullptr && lane == 0) bufs.stamps[2 * static_cast<uint64_t>(wave_id) + 1] = __builtin_amdgcn_s_memrealtime();
 }
 
 template <bool kNonTemporal, bool kMask>
@@ -550,7 +557,7 @@ __device__ __forceinline__ void KoptBody(const KoptLaunch& a, const KoptPointers
 template <bool kNonTemporal, bool kMask>
 __global__ void __launch_bounds__(256) KoptKernel(KoptLaunch a) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
-  const KoptPointers bufs{a.xy, a.tiles, a.tasks, a.plans, a.costs, a.masks};
+  const KoptPointers bufs{a.xy, a.tiles, a.tasks, a.plans, a.costs, a.masks, nullptr};
   KoptBody<kNonTemporal, kMask>(a, bufs, smem);
 }
 
@@ -744,12 +751,13 @@ void LaunchKopt(const KoptLaunch& args, const KoptTuning& tuning, size_t shared_
 }
 
 void LaunchKoptResident(const KoptLaunch& finalized, const KoptLaunch* device_args, uint32_t grid,
-                        const KoptTuning& tuning, size_t shared_bytes, hipStream_t stream) {
+                        const KoptTuning& tuning, size_t shared_bytes, hipStream_t stream,
+                        unsigned long long* stamps) {
   if (grid == 0) return;
   if (device_args == nullptr) throw std::invalid_argument("resident k-opt arguments are NULL");
   const dim3 gd(grid), b(tuning.block);
   const KoptPointers ptrs{finalized.xy, finalized.tiles, finalized.tasks, finalized.plans, finalized.costs,
-                          finalized.masks};
+                          finalized.masks, stamps};
   if (finalized.masks != nullptr) {
     hipLaunchKernelGGL((KoptKernelResident<false, true>), gd, b, shared_bytes, stream, device_args, ptrs);
   } else if (tuning.non_temporal) {

@@ -105,6 +105,15 @@ class KoptBatch:
         _lib.check(_lib.lib().ee_kopt_batch_run(self._h, launches, flags, C.byref(ms)))
         return ms.value
 
+    def run_stamped(self, launches: int, pipelined: bool = True):
+        """(start_us, end_us) of each launch: its first wavefront's start and
+        last wavefront's end on the device clock, from the run's first start."""
+        start = np.zeros(launches, np.float64)
+        end = np.zeros(launches, np.float64)
+        _lib.check(_lib.lib().ee_kopt_batch_run_stamped(
+            self._h, launches, _lib.KOPT_RUN_PIPELINED if pipelined else 0, _lib.ptr(start), _lib.ptr(end)))
+        return start, end
+
     def set_pipeline_depth(self, depth: int) -> None:
         """Streams and cost pools of pipelined runs (2 to 4)."""
         _lib.check(_lib.lib().ee_kopt_batch_set_pipeline_depth(self._h, depth))

@@ -456,6 +456,15 @@ ee_status ee_kopt_batch_autotune_runs(ee_kopt_batch* batch, uint32_t launches_pe
 #define EE_KOPT_RUN_GRAPH 1
 #define EE_KOPT_RUN_PIPELINED 2
 ee_status ee_kopt_batch_run(ee_kopt_batch* batch, uint32_t launches, int flags, double* device_ms);
+/* ee_kopt_batch_run of `launches` (at most 1024) plain (flags 0) or
+ * pipelined (EE_KOPT_RUN_PIPELINED) launches in which every wavefront
+ * records the device's constant-rate clock (s_memrealtime) when it starts
+ * and ends: start_us[i] and end_us[i] receive launch i's first wavefront
+ * start and last wavefront end, in microseconds from the run's first start.
+ * Launches whose intervals intersect ran at the same time.  No reference
+ * counterpart (a measurement aid). */
+ee_status ee_kopt_batch_run_stamped(ee_kopt_batch* batch, uint32_t launches, int flags, double* start_us,
+                                    double* end_us);
 /* Streams (and cost pools) of pipelined runs, 2 to 4 (default 2); each pool
  * beyond the first is allocated on the first pipelined run that needs it.
  * No reference counterpart. */

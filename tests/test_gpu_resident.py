@@ -119,6 +119,28 @@ def test_batch_pipelined_runs_equal_reference(ctx):
     b.close()
 
 
+def test_stamped_runs_time_every_launch(ctx):
+    """Device-clock stamps of every wavefront: each launch gets a positive
+    span, launches on one stream follow each other, and the stamped runs
+    leave every pool's cells unchanged."""
+    say("batch stamped runs")
+    wl = W.config3()
+    b = single_instance_batch(ctx, wl)
+    b.set_tuning([32, 16, 8], 0, 256, 1)
+    b.set_pipeline_depth(4)
+    for pipelined in (False, True):
+        start, end = b.run_stamped(24, pipelined=pipelined)
+        assert (end > start).all() and start.min() == 0.0
+        assert (end - start).max() < 10_000.0  # a launch of this batch takes microseconds
+        if not pipelined:
+            assert (start[1:] >= end[:-1] - 0.05).all()  # one clock tick of slack
+        for pool in range(4 if pipelined else 1):
+            assert sha(b.segment_costs(2, pool)) == G["config3"]["k5"]["sha256"], (pipelined, pool)
+    with pytest.raises(_lib.InvalidArgument):
+        b.run_stamped(0)
+    b.close()
+
+
 def test_batch_autotune_and_cache(ctx, config3_small, tmp_path):
     say("batch autotune")
     b = single_instance_batch(ctx, config3_small)
