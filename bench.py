@@ -815,6 +815,20 @@ def main() -> int:
             "verify_ms": best.verify_ms, "commit_ms": best.commit_ms,
             "final_hash": f"{best.final_hash:016x}",
         }
+        # every resident byte touched once per epoch: 24 B per edge slot (u,
+        # v, weight, active, witness), 20 B per node + 4 (row offsets,
+        # coordinates), 8 B per adjacency entry (2 per active edge)
+        jv_bytes = best.epochs * (24 * graph.edge_count + 20 * graph.dimension + 4) + 16 * best.edges_scanned
+        jv_achieved = jv_bytes / (best.device_ms * 1e-3) / 1e9
+        jv["roofline"] = {
+            "bound": "hbm", "achieved": jv_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": jv_achieved / HBM_PEAK_GBS, "kernel": "JvKernel",
+            "algorithmic_bytes": jv_bytes, "kernel_us_per_epoch": best.device_ms * 1e3 / best.epochs,
+            # from the committed rocprof summary of the same workload
+            "valu_utilisation": profile_figure("jv_kernel_lattice.md", "valu_utilisation"),
+            "salu_per_edge": profile_figure("jv_kernel_lattice.md", "salu_per_edge"),
+            "valu_per_edge": profile_figure("jv_kernel_lattice.md", "valu_per_edge"),
+        }
         if not args.no_cpu:
             import _oracle as O
 

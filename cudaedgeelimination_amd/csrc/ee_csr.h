@@ -54,8 +54,8 @@ class DeviceCsr {
   int cur_ = 0;
   DeviceArray<int32_t> active_;
   DeviceArray<int32_t> ro_[2], nb_[2], eid_[2];
-  DeviceArray<int32_t> counts_;
-  DeviceArray<unsigned long long> chunk_sums_;
+  DeviceArray<unsigned long long> masks_;  // live mask per 64 adjacency entries
+  DeviceArray<uint32_t> block_counts_, block_offsets_;
   DeviceArray<int32_t> commits_;
   Pinned<int32_t> h_commits_;
 };

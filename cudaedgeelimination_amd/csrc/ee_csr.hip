@@ -1,15 +1,17 @@
 // Commit on the device: clear the active flags of committed edge slots and
-// compact the CSR (see ee_csr.h).  Four launches, each a pass over its
-// arrays: deactivate, per-node live degree with per-chunk totals, offset
-// scan, scatter of the live entries.
+// compact the CSR (see ee_csr.h).  Four launches, every one spread over the
+// adjacency entries (or nodes) rather than over rows, so even a small graph
+// fills many workgroups: deactivate; the live mask of every 64 entries (one
+// ballot) with per-block counts; a scan of the block counts; the scatter of
+// the live entries and of the new row offsets.
 #include "ee_csr.h"
 
 namespace ee {
 namespace {
 
-constexpr uint32_t kThreads = 256;
-constexpr uint32_t kPerThread = 4;
-constexpr uint32_t kChunk = kThreads * kPerThread;  // nodes per workgroup of the count and scan passes
+constexpr uint32_t kThreads = 256;  // = entries per block of the count and scatter passes
+constexpr uint32_t kWavesPerBlock = kThreads / 64;
+constexpr uint32_t kScanThreads = 1024;
 
 __global__ __launch_bounds__(kThreads) void DeactivateKernel(const int32_t* __restrict__ commits,
                                                              int32_t capacity, int32_t* __restrict__ active,
@@ -22,99 +24,107 @@ __global__ __launch_bounds__(kThreads) void DeactivateKernel(const int32_t* __re
   }
 }
 
-// counts[v] = live entries of row v; chunk_sums[c] = their sum over chunk c.
-__global__ __launch_bounds__(kThreads) void LiveDegreeKernel(const int32_t* __restrict__ row_offsets,
-                                                             const int32_t* __restrict__ edge_ids,
-                                                             const int32_t* __restrict__ active, int32_t n,
-                                                             int32_t* __restrict__ counts,
-                                                             unsigned long long* __restrict__ chunk_sums) {
-  __shared__ unsigned long long wave_sums[kThreads / 64];
+// masks[w] = the live entries among entries [64 w, 64 w + 64) (bit j: entry
+// 64 w + j's edge is active); block_counts[b] = live entries of block b.  The
+// grid covers the buffer's capacity; the CSR holds row_offsets[n] entries,
+// and every entry past them (stale ids) counts as not live and is not read.
+__global__ __launch_bounds__(kThreads) void LiveMaskKernel(const int32_t* __restrict__ row_offsets, int32_t n,
+                                                           const int32_t* __restrict__ edge_ids,
+                                                           const int32_t* __restrict__ active,
+                                                           unsigned long long* __restrict__ masks,
+                                                           uint32_t* __restrict__ block_counts) {
+  __shared__ uint32_t wave_counts[kWavesPerBlock];
+  const int32_t entries = row_offsets[n];
   const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
-  const int64_t first = static_cast<int64_t>(blockIdx.x) * kChunk + static_cast<int64_t>(t) * kPerThread;
-  unsigned long long own = 0;
-#pragma unroll
-  for (uint32_t j = 0; j < kPerThread; ++j) {
-    const int64_t v = first + j;
-    if (v < n) {
-      int32_t live = 0;
-      for (int32_t a = row_offsets[v]; a < row_offsets[v + 1]; ++a) live += active[edge_ids[a]] != 0;
-      counts[v] = live;
-      own += static_cast<unsigned long long>(live);
-    }
+  const int64_t a = static_cast<int64_t>(blockIdx.x) * kThreads + t;
+  const bool live = a < entries && active[edge_ids[a]] != 0;
+  const unsigned long long m = __ballot(live);
+  if (lane == 0) {
+    masks[static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wave] = m;
+    wave_counts[wave] = static_cast<uint32_t>(__popcll(m));
   }
-  for (uint32_t d = 32; d >= 1; d >>= 1) own += __shfl_xor(own, d, 64);
-  if (lane == 0) wave_sums[wave] = own;
   __syncthreads();
   if (t == 0) {
-    unsigned long long s = 0;
-    for (uint32_t w = 0; w < kThreads / 64; ++w) s += wave_sums[w];
-    chunk_sums[blockIdx.x] = s;
+    uint32_t total = 0;
+    for (uint32_t w = 0; w < kWavesPerBlock; ++w) total += wave_counts[w];
+    block_counts[blockIdx.x] = total;
   }
 }
 
-// new_ro[0] = 0, new_ro[v + 1] = counts[0] + ... + counts[v]: each
-// workgroup sums the totals of the chunks before its own, then scans its
-// chunk (4 consecutive counts per thread, wavefront shuffles, wavefront
-// totals in LDS).
-__global__ __launch_bounds__(kThreads) void OffsetScanKernel(const int32_t* __restrict__ counts, int32_t n,
-                                                             const unsigned long long* __restrict__ chunk_sums,
-                                                             int32_t* __restrict__ new_ro) {
-  __shared__ unsigned long long partial[kThreads / 64];
-  __shared__ unsigned long long wave_sums[kThreads / 64];
+// offsets[b] = block_counts[0] + ... + block_counts[b - 1] for b <= blocks
+// (offsets[blocks] = all live entries): one workgroup, 4 counts per thread
+// per round, wavefront shuffles and wavefront totals in LDS.
+__global__ __launch_bounds__(kScanThreads) void BlockScanKernel(const uint32_t* __restrict__ counts, int32_t blocks,
+                                                                uint32_t* __restrict__ offsets) {
+  __shared__ uint32_t wave_sums[kScanThreads / 64];
+  __shared__ uint32_t carry;
   const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
-  const uint64_t chunk = blockIdx.x;
-  unsigned long long before = 0;
-  for (uint64_t c = t; c < chunk; c += kThreads) before += chunk_sums[c];
-  for (uint32_t d = 32; d >= 1; d >>= 1) before += __shfl_xor(before, d, 64);
-  if (lane == 0) partial[wave] = before;
-  const int64_t first = static_cast<int64_t>(chunk) * kChunk + static_cast<int64_t>(t) * kPerThread;
-  unsigned long long v[kPerThread];
-  unsigned long long own = 0;
-#pragma unroll
-  for (uint32_t j = 0; j < kPerThread; ++j) {
-    v[j] = first + j < n ? static_cast<unsigned long long>(counts[first + j]) : 0ull;
-    own += v[j];
-  }
-  unsigned long long inc = own;
-  for (uint32_t d = 1; d < 64; d <<= 1) {
-    const unsigned long long o = __shfl_up(inc, d, 64);
-    if (lane >= d) inc += o;
-  }
-  if (lane == 63) wave_sums[wave] = inc;
+  if (t == 0) carry = 0;
   __syncthreads();
-  unsigned long long run = inc - own;
-  for (uint32_t w = 0; w < kThreads / 64; ++w) {
-    run += partial[w];
-    if (w < wave) run += wave_sums[w];
-  }
-  if (chunk == 0 && t == 0) new_ro[0] = 0;
+  for (int64_t base = 0; base < blocks; base += 4 * kScanThreads) {
+    uint32_t v[4], own = 0;
+    const int64_t first = base + 4 * static_cast<int64_t>(t);
 #pragma unroll
-  for (uint32_t j = 0; j < kPerThread; ++j) {
-    run += v[j];
-    if (first + j < n) new_ro[first + j + 1] = static_cast<int32_t>(run);
+    for (uint32_t j = 0; j < 4; ++j) {
+      v[j] = first + j < blocks ? counts[first + j] : 0u;
+      own += v[j];
+    }
+    uint32_t inc = own;
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+      const uint32_t o = __shfl_up(inc, d, 64);
+      if (lane >= d) inc += o;
+    }
+    if (lane == 63) wave_sums[wave] = inc;
+    __syncthreads();
+    uint32_t run = carry + inc - own;
+    for (uint32_t w = 0; w < wave; ++w) run += wave_sums[w];
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) {
+      if (first + j < blocks) offsets[first + j] = run;
+      run += v[j];
+    }
+    __syncthreads();  // every thread has read carry and wave_sums
+    if (t == kScanThreads - 1) carry = run;
+    __syncthreads();
   }
+  if (t == 0) offsets[blocks] = carry;
 }
 
-// Row v's live entries, in row order, to [new_ro[v], new_ro[v + 1]).
-__global__ __launch_bounds__(kThreads) void ScatterLiveKernel(const int32_t* __restrict__ row_offsets,
-                                                              const int32_t* __restrict__ neighbors,
-                                                              const int32_t* __restrict__ edge_ids,
-                                                              const int32_t* __restrict__ active, int32_t n,
-                                                              const int32_t* __restrict__ new_ro,
-                                                              int32_t* __restrict__ new_nb,
-                                                              int32_t* __restrict__ new_eid) {
-  const int64_t v = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-  if (v >= n) return;
-  int32_t out = new_ro[v];
-  const int32_t end = new_ro[v + 1];
-  for (int32_t a = row_offsets[v]; a < row_offsets[v + 1] && out < end; ++a) {
-    const int32_t e = edge_ids[a];
-    if (active[e] != 0) {
-      new_nb[out] = neighbors[a];
-      new_eid[out] = e;
-      ++out;
-    }
+// The live entries before entry a (a <= entries).
+__device__ __forceinline__ uint32_t LiveBefore(int64_t a, int32_t entries,
+                                               const unsigned long long* __restrict__ masks,
+                                               const uint32_t* __restrict__ offsets) {
+  const int64_t block = a / kThreads;
+  if (a >= entries) return offsets[(entries + kThreads - 1) / kThreads];
+  uint32_t before = offsets[block];
+  const int64_t w0 = block * kWavesPerBlock, w = a / 64;
+  for (int64_t q = w0; q < w; ++q) before += static_cast<uint32_t>(__popcll(masks[q]));
+  const uint32_t bit = static_cast<uint32_t>(a & 63);
+  return before + static_cast<uint32_t>(__popcll(masks[w] & ((1ull << bit) - 1ull)));
+}
+
+// Blocks [0, entry_blocks): live entry a to position LiveBefore(a); the
+// rest: new_ro[v] = LiveBefore(row_offsets[v]) for v <= n.  Removing
+// entries keeps each row's order.
+__global__ __launch_bounds__(kThreads) void CompactKernel(
+    const int32_t* __restrict__ row_offsets, const int32_t* __restrict__ neighbors,
+    const int32_t* __restrict__ edge_ids, int32_t n, uint32_t entry_blocks,
+    const unsigned long long* __restrict__ masks, const uint32_t* __restrict__ offsets,
+    int32_t* __restrict__ new_ro, int32_t* __restrict__ new_nb, int32_t* __restrict__ new_eid) {
+  const int32_t entries = row_offsets[n];
+  const uint32_t t = threadIdx.x;
+  if (blockIdx.x < entry_blocks) {
+    const int64_t a = static_cast<int64_t>(blockIdx.x) * kThreads + t;
+    if (a >= entries) return;
+    if (((masks[a / 64] >> (a & 63)) & 1ull) == 0) return;
+    const uint32_t at = LiveBefore(a, entries, masks, offsets);
+    new_nb[at] = neighbors[a];
+    new_eid[at] = edge_ids[a];
+    return;
   }
+  const int64_t v = static_cast<int64_t>(blockIdx.x - entry_blocks) * kThreads + t;
+  if (v > n) return;
+  new_ro[v] = static_cast<int32_t>(LiveBefore(row_offsets[v], entries, masks, offsets));
 }
 
 }  // namespace
@@ -135,8 +145,10 @@ void DeviceCsr::Allocate(size_t n, size_t m, size_t adjacency, size_t max_commit
     nb_[b].Allocate(adjacency_, device);
     eid_[b].Allocate(adjacency_, device);
   }
-  counts_.Allocate(std::max<size_t>(n, 1), device);
-  chunk_sums_.Allocate(std::max<size_t>((n + kChunk - 1) / kChunk, 1), device);
+  const size_t blocks = (adjacency_ + kThreads - 1) / kThreads;
+  masks_.Allocate(blocks * kWavesPerBlock, device);
+  block_counts_.Allocate(blocks, device);
+  block_offsets_.Allocate(blocks + 1, device);
   commits_.Allocate(max_commits + 1, device);
   h_commits_.Reserve(max_commits + 1);
   h_commits_.get()[0] = 0;
@@ -171,18 +183,21 @@ void DeviceCsr::EnqueueCompact(hipStream_t stream) {
     cur_ = nxt;
     return;
   }
-  const uint32_t chunks = static_cast<uint32_t>((n_ + kChunk - 1) / kChunk);
-  hipLaunchKernelGGL(LiveDegreeKernel, dim3(chunks), dim3(kThreads), 0, stream, ro_[cur_].get(),
-                     eid_[cur_].get(), active_.get(), n, counts_.get(), chunk_sums_.get());
-  CheckHip(hipGetLastError(), "live degree launch");
-  hipLaunchKernelGGL(OffsetScanKernel, dim3(chunks), dim3(kThreads), 0, stream, counts_.get(), n,
-                     chunk_sums_.get(), ro_[nxt].get());
-  CheckHip(hipGetLastError(), "offset scan launch");
-  const uint32_t blocks = static_cast<uint32_t>((n_ + kThreads - 1) / kThreads);
-  hipLaunchKernelGGL(ScatterLiveKernel, dim3(blocks), dim3(kThreads), 0, stream, ro_[cur_].get(),
-                     nb_[cur_].get(), eid_[cur_].get(), active_.get(), n, ro_[nxt].get(), nb_[nxt].get(),
-                     eid_[nxt].get());
-  CheckHip(hipGetLastError(), "scatter launch");
+  // the passes are sized for the capacity; the kernels read the current
+  // CSR's entry count (its last row end) on the device, so one captured
+  // sequence serves every commit
+  const uint32_t entry_blocks = static_cast<uint32_t>((adjacency_ + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(LiveMaskKernel, dim3(entry_blocks), dim3(kThreads), 0, stream, ro_[cur_].get(), n,
+                     eid_[cur_].get(), active_.get(), masks_.get(), block_counts_.get());
+  CheckHip(hipGetLastError(), "live mask launch");
+  hipLaunchKernelGGL(BlockScanKernel, dim3(1), dim3(kScanThreads), 0, stream, block_counts_.get(),
+                     static_cast<int32_t>(entry_blocks), block_offsets_.get());
+  CheckHip(hipGetLastError(), "block scan launch");
+  const uint32_t node_blocks = static_cast<uint32_t>((n_ + 1 + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(CompactKernel, dim3(entry_blocks + node_blocks), dim3(kThreads), 0, stream,
+                     ro_[cur_].get(), nb_[cur_].get(), eid_[cur_].get(), n, entry_blocks,
+                     masks_.get(), block_offsets_.get(), ro_[nxt].get(), nb_[nxt].get(), eid_[nxt].get());
+  CheckHip(hipGetLastError(), "compact launch");
   cur_ = nxt;
 }
 

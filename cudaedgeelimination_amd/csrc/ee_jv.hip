@@ -7,14 +7,20 @@
 //   1. candidate scoring (jv_elimination.cpp:45-84): lane s scores the s-th
 //      entry of N(a) followed by N(b); x in {a, b} is skipped, and on the b
 //      side x is skipped when (a, x) is an edge (the reference's HasActiveEdge
-//      binary search; here a readlane scan over the a-side lanes when both
-//      rows fit one pass).  score = w(side, x) + D(x, other).
-//   2. top ten by (score, node): every valid lane computes its rank by
-//      counting smaller keys with readlane; ranks < 10 land in LDS in
-//      ascending order.  Rows longer than one pass merge into the running list.
+//      binary search; here a broadcast scan of N(a) staged in LDS when both
+//      rows fit one pass).  score = w(side, x) + D(x, other).  Each lane also
+//      fetches x's row bounds and D(a, x), D(b, x), which the witness test
+//      needs if x becomes a candidate.
+//   2. top ten by (score, node): when both rows fit one pass and every key
+//      packs into 64 bits, each valid lane counts the smaller keys with
+//      broadcast LDS reads; otherwise by readlane.  Ranks < 10 land in LDS in
+//      ascending order with their row bounds and distances.  Rows longer than
+//      one pass merge into the running list.
 //   3. witness test (jv_elimination.cpp:86-117): candidates in rank order,
-//      as many consecutive candidates per pass as their rows fit 64 lanes;
-//      lane j tests one neighbour d of its candidate c and a ballot tells,
+//      four per pass in 16-lane slots when every candidate row has at most 16
+//      entries, else as many consecutive candidates per pass as their rows
+//      fit 64 lanes; lane j tests one neighbour d of its candidate c and a
+//      ballot tells,
 //      per candidate, whether any d is compatible (w_ab + w_cd <= D(a,c) +
 //      D(d,b) or w_ab + w_cd <= D(a,d) + D(b,c)).  The first c with no
 //      compatible d is the witness.  Testing every d at once gives the same
@@ -78,9 +84,23 @@ __device__ __forceinline__ int64_t Dist(const longlong2 p, const longlong2 q, in
   return ExactDistance(p.x, p.y, q.x, q.y, type);
 }
 
+// Packed candidate keys: (score << 24) | node, ordered as (score, node) when
+// the score is below 2^40 and the node below 2^24.
+constexpr int kNodeBits = 24;
+constexpr int kScoreBits = 40;
+
 __global__ void __launch_bounds__(kBlock) JvKernel(JvLaunch g) {
   __shared__ uint64_t s_score[kWavesPerBlock][2][kCandidates];
   __shared__ int32_t s_node[kWavesPerBlock][2][kCandidates];
+  // one-pass rows: N(a) (padded with -1) and the packed keys of every lane
+  __shared__ __attribute__((aligned(16))) int32_t s_nbr[kWavesPerBlock][kWave];
+  __shared__ __attribute__((aligned(16))) uint64_t s_key[kWavesPerBlock][kWave];
+  // one-pass rows: the candidates' row bounds and D(a, c), D(b, c), stored
+  // by the lanes that scored them
+  __shared__ int32_t s_cr0[kWavesPerBlock][kCandidates];
+  __shared__ int32_t s_cdeg[kWavesPerBlock][kCandidates];
+  __shared__ uint64_t s_cac[kWavesPerBlock][kCandidates];
+  __shared__ uint64_t s_cbc[kWavesPerBlock][kCandidates];
 
   const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
   const int wib = static_cast<int>(threadIdx.x) / kWave;
@@ -110,7 +130,75 @@ __global__ void __launch_bounds__(kBlock) JvKernel(JvLaunch g) {
       const int32_t total = deg_a + deg_b;
       const bool one_pass = total <= kWave;
       int cnt = 0, cur = 0;
-      for (int32_t base = 0; base < total; base += kWave) {
+      bool ranked = false;
+      if (one_pass) {
+        // both rows in one pass: lane s takes entry s of N(a) then N(b).  The
+        // b side skips x in N(a) by a broadcast scan of N(a) in LDS; the keys
+        // are ranked by counting smaller keys, two per LDS read, all lanes
+        // reading the same address (a broadcast).
+        const int32_t s = lane;
+        const bool side_b = s >= deg_a;
+        int32_t raw = -1, eid = 0;
+        if (s < total) {
+          const int32_t slot = side_b ? rb0 + (s - deg_a) : ra0 + s;
+          raw = g.neighbors[slot];
+          eid = g.csr_edge_ids[slot];
+        }
+        s_nbr[wib][lane] = s < deg_a ? raw : -1;
+        WaveSync();
+        bool skip = s >= total || raw == a || raw == b;
+        if (side_b && !skip) {
+          const int4* row = reinterpret_cast<const int4*>(s_nbr[wib]);
+          bool in_a = false;
+          for (int j = 0; j < (deg_a + 7) / 8; ++j) {  // uniform; entries past deg_a hold -1
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+              const int4 q = row[2 * j + u];
+              in_a |= q.x == raw || q.y == raw || q.z == raw || q.w == raw;
+            }
+          }
+          skip = in_a;
+        }
+        uint64_t sc = ~0ull, d_ax = 0, d_bx = 0;
+        int32_t x_r0 = 0, x_r1 = 0;
+        if (!skip) {
+          // what the witness test needs of x if it becomes a candidate,
+          // fetched alongside the score's operands
+          x_r0 = g.row_offsets[raw];
+          x_r1 = g.row_offsets[raw + 1];
+          const longlong2 px = g.xy[raw];
+          d_ax = static_cast<uint64_t>(Dist(pa, px, type, near));
+          d_bx = static_cast<uint64_t>(Dist(pb, px, type, near));
+          sc = static_cast<uint64_t>(g.edge_weight[eid]) + (side_b ? d_ax : d_bx);
+        }
+        const bool valid = !skip;
+        const uint64_t wide = __ballot(valid && ((sc >> kScoreBits) != 0 || (raw >> kNodeBits) != 0));
+        if (wide == 0) {
+          const uint64_t key = valid ? (sc << kNodeBits) | static_cast<uint64_t>(raw) : ~0ull;
+          s_key[wib][lane] = key;
+          WaveSync();
+          const ulonglong2* keys = reinterpret_cast<const ulonglong2*>(s_key[wib]);
+          int rank = 0;
+          for (int j = 0; j < (total + 1) / 2; ++j) {  // uniform; lanes past total hold ~0
+            const ulonglong2 q = keys[j];
+            rank += (q.x < key ? 1 : 0) + (q.y < key ? 1 : 0);
+          }
+          if (valid && rank < kCandidates) {
+            s_score[wib][1][rank] = sc;
+            s_node[wib][1][rank] = raw;
+            s_cr0[wib][rank] = x_r0;
+            s_cdeg[wib][rank] = x_r1 - x_r0;
+            s_cac[wib][rank] = d_ax;
+            s_cbc[wib][rank] = d_bx;
+          }
+          WaveSync();
+          const int npass = __popcll(__ballot(valid));
+          cnt = npass < kCandidates ? npass : kCandidates;
+          cur = 1;
+          ranked = true;
+        }
+      }
+      for (int32_t base = 0; !ranked && base < total; base += kWave) {
         const int32_t s = base + lane;
         bool valid = false;
         uint64_t sc = ~0ull;
@@ -192,21 +280,55 @@ __global__ void __launch_bounds__(kBlock) JvKernel(JvLaunch g) {
       uint64_t c_ac = 0, c_bc = 0;
       if (lane < cnt) {
         c_node = s_node[wib][cur][lane];
-        const longlong2 pc = g.xy[c_node];
-        c_ac = static_cast<uint64_t>(Dist(pa, pc, type, near));
-        c_bc = static_cast<uint64_t>(Dist(pb, pc, type, near));
-        c_r0 = g.row_offsets[c_node];
-        c_deg = g.row_offsets[c_node + 1] - c_r0;
+        if (ranked) {
+          c_ac = s_cac[wib][lane];
+          c_bc = s_cbc[wib][lane];
+          c_r0 = s_cr0[wib][lane];
+          c_deg = s_cdeg[wib][lane];
+        } else {
+          const longlong2 pc = g.xy[c_node];
+          c_ac = static_cast<uint64_t>(Dist(pa, pc, type, near));
+          c_bc = static_cast<uint64_t>(Dist(pb, pc, type, near));
+          c_r0 = g.row_offsets[c_node];
+          c_deg = g.row_offsets[c_node + 1] - c_r0;
+        }
       }
       auto compatible_at = [&](int32_t slot, uint64_t cac, uint64_t cbc) -> bool {
         const int32_t d = g.neighbors[slot];
+        const int32_t eid = g.csr_edge_ids[slot];
         if (d == a || d == b) return false;
         const longlong2 pd = g.xy[d];
-        const uint64_t lhs = wab + static_cast<uint64_t>(g.edge_weight[g.csr_edge_ids[slot]]);
+        const uint64_t lhs = wab + static_cast<uint64_t>(g.edge_weight[eid]);
         return lhs <= cac + static_cast<uint64_t>(Dist(pd, pb, type, near)) ||
                lhs <= static_cast<uint64_t>(Dist(pa, pd, type, near)) + cbc;
       };
-      for (int i = 0; i < cnt;) {  // uniform
+      // rows of at most 16 entries (the usual case): candidate q takes lanes
+      // 16 (q % 4) .. 16 (q % 4) + 15 of pass q / 4, so one ballot gives the
+      // verdicts of four candidates as 16-bit chunks, with no per-candidate
+      // bookkeeping; longer rows take the packing below
+      constexpr int kSlot = 16;
+      const bool slots = __ballot(lane < cnt && c_deg > kSlot) == 0;
+      for (int base_q = 0; slots && base_q < cnt; base_q += kWave / kSlot) {  // uniform
+        const int q = base_q + lane / kSlot, k = lane % kSlot;
+        const int src = q < cnt ? q : 0;
+        const int32_t r0 = __shfl(c_r0, src);
+        const int32_t dq = __shfl(c_deg, src);
+        const uint64_t cac = (static_cast<uint64_t>(static_cast<uint32_t>(__shfl(static_cast<int>(c_ac >> 32), src))) << 32) |
+                             static_cast<uint32_t>(__shfl(static_cast<int>(c_ac), src));
+        const uint64_t cbc = (static_cast<uint64_t>(static_cast<uint32_t>(__shfl(static_cast<int>(c_bc >> 32), src))) << 32) |
+                             static_cast<uint32_t>(__shfl(static_cast<int>(c_bc), src));
+        const uint64_t hits = __ballot(q < cnt && k < dq && compatible_at(r0 + k, cac, cbc));
+        bool found = false;
+#pragma unroll 1
+        for (int t = 0; t < kWave / kSlot; ++t) {  // uniform, in candidate order
+          if (!found && base_q + t < cnt && ((hits >> (kSlot * t)) & 0xffffull) == 0) {
+            result = ReadLane(c_node, base_q + t);
+            found = true;
+          }
+        }
+        if (found) break;
+      }
+      for (int i = 0; !slots && i < cnt;) {  // uniform
         const int deg_i = ReadLane(c_deg, i);
         if (deg_i > kWave) {
           // a row longer than a wavefront: this candidate alone, pass by pass
