@@ -41,8 +41,10 @@ ee_status ee_gen_points_clustered(uint64_t seed, int32_t n, int32_t clusters, do
 
 /* Symmetric union of every node's k nearest neighbours (ties broken by
  * squared distance, then node id), canonical (u < v), sorted by (u, v), with
- * exact distance weights.  Returns EE_ERR_INVALID_ARGUMENT when `capacity`
- * is too small; *edge_count receives the number of edges either way. */
+ * exact distance weights.  With edge_u, edge_v and edge_weight all NULL the
+ * call only counts the edges (EE_OK); otherwise it returns
+ * EE_ERR_INVALID_ARGUMENT when `capacity` is too small.  *edge_count
+ * receives the number of edges either way. */
 ee_status ee_gen_knn_graph(const int64_t* x, const int64_t* y, int32_t n, uint8_t distance_type,
                            int32_t k, int32_t* edge_u, int32_t* edge_v, int64_t* edge_weight,
                            int64_t capacity, int64_t* edge_count);

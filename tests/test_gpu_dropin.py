@@ -6,6 +6,11 @@ translation units (oracle/Makefile, target dropin), run on the device.
   CPU path, cache-hit pattern, resident bytes 24m + 20n + 4 + 8A);
 * `gpu-eliminate --backend cuda` proofs equal `--backend cpu` proofs and
   replay with `verify` (the reference's CLI path, src/cli/main.cpp:249-275);
+* the reference's leaf-search scenarios (tests/unit/test_kopt.cpp:575-707,
+  842-897), restated in oracle/leaf_scenarios.cpp over the reference API:
+  the collinear 7-node exhaustion (25 sets, 1,330 templates), device-backed
+  proof bytes equal to CPU, the fused cursor batch counters and cache-hit
+  pattern, the budget boundary, and 12 seeded differential trials;
 * the reference's hybrid leaf scenario `ht-prove --leaf-backend cuda`
   (tests/CMakeLists.txt:146-184): every device k-opt matrix is certified cell
   by cell against the reference CPU matrix inside the reference code
@@ -27,6 +32,7 @@ ROOT = pathlib.Path(__file__).resolve().parent.parent
 CLI = ROOT / "oracle" / "_ref" / "cudaee_hip"
 TEST_CORE = ROOT / "oracle" / "_ref" / "test_core_hip"
 TEST_HT = ROOT / "oracle" / "_ref" / "test_ht_hip"
+LEAF = ROOT / "oracle" / "_ref" / "leaf_scenarios_hip"
 RP = ROOT / "tests" / "golden" / "recursive-point"
 
 
@@ -47,6 +53,13 @@ def test_reference_unit_test_core_passes_on_device(tmp_path):
     need(TEST_CORE)
     done = run([TEST_CORE], tmp_path)
     assert done.returncode == 0 and "unit tests passed" in done.stdout
+
+
+def test_reference_leaf_search_scenarios_on_device(tmp_path):
+    need(LEAF)
+    done = run([LEAF], tmp_path)
+    assert done.returncode == 0 and "all leaf scenarios passed" in done.stdout
+    assert "FAIL" not in done.stdout
 
 
 def test_reference_unit_test_hamilton_tutte_passes_on_device(tmp_path):

@@ -175,3 +175,21 @@ def test_shim_rejects_malformed_ht_snapshots_before_any_device_call():
                          timeout=60)
     assert run.returncode == 0, run.stdout + run.stderr
     assert run.stdout.count("ok ") == 12
+
+
+def test_host_units_under_address_and_undefined_behaviour_sanitizers():
+    """The host-only units (generator, CSR and hash, template generator, the
+    JV host run, the certifier) with ASan and UBSan, checked against the
+    oracle (oracle/sanitize_harness.cpp; built by `make -C oracle sanitize`)."""
+    import os
+    import subprocess
+
+    harness = ROOT / "oracle" / "_san" / "sanitize_harness"
+    if not harness.exists():
+        pytest.skip("sanitizer build not present (make -C oracle sanitize)")
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0",
+               UBSAN_OPTIONS="print_stacktrace=1")
+    done = subprocess.run([str(harness)], capture_output=True, text=True, timeout=300, env=env)
+    assert done.returncode == 0, done.stdout[-3000:] + done.stderr[-3000:]
+    assert "sanitized host checks passed" in done.stdout
+    assert "runtime error" not in done.stderr and "AddressSanitizer" not in done.stderr
