@@ -66,8 +66,8 @@ __device__ inline bool Survives(const longlong2* xy, int32_t tu, int32_t tv, int
 // First index of row i of the pairs (i, j > i) of d items, first-major.
 __device__ inline uint64_t PairRowStart(uint64_t i, uint64_t d) { return i * (2 * d - i - 1) / 2; }
 
-// Centers per scan chunk: the count pass adds every center's reply count to
-// its chunk's total, and the scan pass gives each chunk to one workgroup.
+// Centers per scan chunk: one workgroup sums each chunk's reply counts, and
+// the scan pass gives each chunk to one workgroup.
 constexpr uint32_t kScanThreads = 256;
 constexpr uint32_t kScanPerThread = 4;
 constexpr uint32_t kScanChunk = kScanThreads * kScanPerThread;
@@ -77,8 +77,7 @@ __global__ __launch_bounds__(kHtBlock) void HtHamiltonKernel(
     const longlong2* __restrict__ xy, const int32_t* __restrict__ row_offsets,
     const int32_t* __restrict__ neighbors, int type, int32_t tu, int32_t tv,
     const int32_t* __restrict__ centers, uint64_t center_count, uint64_t* __restrict__ counts,
-    unsigned long long* __restrict__ chunk_sums, const uint64_t* __restrict__ offsets,
-    int32_t* __restrict__ replies) {
+    const uint64_t* __restrict__ offsets, int32_t* __restrict__ replies) {
   __shared__ HtNeighbor cache[kHtWaves][kHtLdsNeighbors];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const uint64_t item = static_cast<uint64_t>(blockIdx.x) * kHtWaves + wave;
@@ -146,16 +145,33 @@ __global__ __launch_bounds__(kHtBlock) void HtHamiltonKernel(
     found += __popcll(ballot);
     if (kWrite) out += __popcll(ballot);
   }
-  if (!kWrite && lane == 0) {
-    counts[item] = found;
-    if (found != 0) atomicAdd(&chunk_sums[item / kScanChunk], static_cast<unsigned long long>(found));
+  if (!kWrite && lane == 0) counts[item] = found;
+}
+
+// chunk_sums[c] = counts of chunk c (kScanChunk centers), one workgroup per
+// chunk: plain stores, no atomics shared by the chunk's centers.
+__global__ __launch_bounds__(kScanThreads) void HtChunkSumKernel(const uint64_t* __restrict__ counts, uint64_t n,
+                                                                  unsigned long long* __restrict__ chunk_sums) {
+  __shared__ unsigned long long wave_sums[kScanThreads / 64];
+  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+  const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kScanChunk + static_cast<uint64_t>(t) * kScanPerThread;
+  unsigned long long own = 0;
+#pragma unroll
+  for (uint32_t e = 0; e < kScanPerThread; ++e) own += first + e < n ? counts[first + e] : 0;
+  for (uint32_t d = 32; d >= 1; d >>= 1) own += __shfl_xor(own, d, 64);
+  if (lane == 0) wave_sums[wave] = own;
+  __syncthreads();
+  if (t == 0) {
+    unsigned long long total = 0;
+    for (uint32_t w = 0; w < kScanThreads / 64; ++w) total += wave_sums[w];
+    chunk_sums[blockIdx.x] = total;
   }
 }
 
 // offsets[0] = 0, offsets[i + 1] = counts[0] + ... + counts[i], one
 // workgroup per chunk of kScanChunk counts: the workgroup first sums the
-// totals of the chunks before its own (chunk_sums, filled by the count
-// pass), then scans its chunk, each thread holding 4 consecutive counts
+// totals of the chunks before its own (chunk_sums, from HtChunkSumKernel),
+// then scans its chunk, each thread holding 4 consecutive counts
 // (wavefront shuffles, then the 4 wavefront totals in LDS).
 __global__ __launch_bounds__(kScanThreads) void HtScanKernel(const uint64_t* __restrict__ counts,
                                                               uint64_t n,
@@ -355,8 +371,6 @@ void HamiltonReplies(Context& ctx, const ee_graph_view* graph, int32_t target_u,
   ws = Reserve(c.d_offsets, count + 1, ctx.device) && ws;
   ws = Reserve(c.d_chunk_sums, chunks, ctx.device) && ws;
   c.d_items.Upload(centers, count, ctx.stream);
-  CheckHip(hipMemsetAsync(c.d_chunk_sums.get(), 0, chunks * sizeof(unsigned long long), ctx.stream),
-           "hipMemsetAsync chunk sums");
   ctx.Sync();
   u.h2d_ms = MsSince(t0);
 
@@ -369,9 +383,11 @@ void HamiltonReplies(Context& ctx, const ee_graph_view* graph, int32_t target_u,
   hipLaunchKernelGGL(count_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(kHtBlock), 0,
                      ctx.stream, c.d_xy.get(), c.d_row_offsets.get(), c.d_neighbors.get(),
                      static_cast<int>(g.distance_type), tu, tv, c.d_items.get(),
-                     static_cast<uint64_t>(count), c.d_counts.get(), c.d_chunk_sums.get(), nullptr,
-                     nullptr);
+                     static_cast<uint64_t>(count), c.d_counts.get(), nullptr, nullptr);
   CheckHip(hipGetLastError(), "HtHamiltonKernel count launch");
+  hipLaunchKernelGGL(HtChunkSumKernel, dim3(static_cast<uint32_t>(chunks)), dim3(kScanThreads), 0, ctx.stream,
+                     c.d_counts.get(), static_cast<uint64_t>(count), c.d_chunk_sums.get());
+  CheckHip(hipGetLastError(), "HtChunkSumKernel launch");
   hipLaunchKernelGGL(HtScanKernel, dim3(static_cast<uint32_t>(chunks)), dim3(kScanThreads), 0,
                      ctx.stream, c.d_counts.get(), static_cast<uint64_t>(count),
                      c.d_chunk_sums.get(), c.d_offsets.get());
@@ -391,8 +407,7 @@ void HamiltonReplies(Context& ctx, const ee_graph_view* graph, int32_t target_u,
   hipLaunchKernelGGL(write_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(kHtBlock), 0,
                      ctx.stream, c.d_xy.get(), c.d_row_offsets.get(), c.d_neighbors.get(),
                      static_cast<int>(g.distance_type), tu, tv, c.d_items.get(),
-                     static_cast<uint64_t>(count), nullptr, nullptr, c.d_offsets.get(),
-                     c.d_replies.get());
+                     static_cast<uint64_t>(count), nullptr, c.d_offsets.get(), c.d_replies.get());
   CheckHip(hipGetLastError(), "HtHamiltonKernel write launch");
   ctx.Sync();
   u.kernel_ms = MsSince(t0);

@@ -146,7 +146,13 @@ void DifferentialScenario(uint64_t min_device_cells) {
   }
   Expect(mismatches == 0, "12 seeded trials x 3 path systems, device threshold " + std::to_string(min_device_cells) +
                               " cells: auto, CPU and scalar proofs byte-identical");
-  Expect(device_batches > 0, "the auto backend ran " + std::to_string(device_batches) + " batches on the device");
+  // with the default threshold these 7-node leaves may stay on the CPU; the
+  // threshold-0 run puts every batch on the device
+  if (min_device_cells == 0) {
+    Expect(device_batches > 0, "the auto backend ran " + std::to_string(device_batches) + " batches on the device");
+  } else {
+    std::cout << "info the auto backend ran " << device_batches << " batches on the device\n";
+  }
 }
 
 }  // namespace
