@@ -813,6 +813,9 @@ def main() -> int:
             # where the rest of the run goes (host side of each epoch)
             "h2d_ms": best.h2d_ms, "d2h_ms": best.d2h_ms, "propose_ms": best.propose_ms,
             "verify_ms": best.verify_ms, "commit_ms": best.commit_ms,
+            # device commit (deactivate + CSR compaction) and the end of the
+            # run's wait for the helper thread's snapshot hashes
+            "device_commit_ms": best.compact_ms, "hash_wait_ms": best.hash_wait_ms,
             "final_hash": f"{best.final_hash:016x}",
         }
         # every resident byte touched once per epoch: 24 B per edge slot (u,
@@ -829,12 +832,34 @@ def main() -> int:
             "salu_per_edge": profile_figure("jv_kernel_lattice.md", "salu_per_edge"),
             "valu_per_edge": profile_figure("jv_kernel_lattice.md", "valu_per_edge"),
         }
+        # where a run's time goes at the scale of a 15,000-node instance
+        # (uniform points, 10-NN): device proposal and commit against the
+        # host's re-verification, commit rule and snapshot hashes
+        big = W.config5_graphs(sizes=(15000,))[0]
+        split_runs = []
+        for _ in range(3):
+            session = JvSession(ctx, big)
+            split_runs.append(session.run(100))
+            session.close()
+        sb = min(split_runs, key=lambda r: r.total_ms)
+        jv["split_uniform_15000"] = {
+            "workload": f"uniform n={big.dimension} m={big.edge_count} 10-NN",
+            "epochs": sb.epochs, "edges_scanned": sb.edges_scanned, "committed": sb.committed,
+            "total_ms": sb.total_ms, "algorithm_edges_per_s": sb.edges_scanned / (sb.total_ms * 1e-3),
+            "first_upload_ms": sb.h2d_ms, "kernel_ms": sb.device_ms,
+            "device_commit_ms": sb.compact_ms, "propose_ms": sb.propose_ms,
+            "verify_ms": sb.verify_ms, "commit_ms": sb.commit_ms, "hash_wait_ms": sb.hash_wait_ms,
+        }
         if not args.no_cpu:
             import _oracle as O
 
             cpu_jv, text = cpu_baseline_jv(graph)
             jv["cpu_baseline"] = cpu_jv
             jv["proof_equals_cpu"] = O.canonical_proof(text) == O.canonical_proof(best.proof)
+            cpu_big, big_text = cpu_baseline_jv(big)
+            jv["split_uniform_15000"]["cpu_baseline"] = cpu_big
+            jv["split_uniform_15000"]["proof_equals_cpu"] = (O.canonical_proof(big_text)
+                                                             == O.canonical_proof(sb.proof))
         result["jv"] = jv
 
     if not args.no_leaf:

@@ -58,6 +58,11 @@ class Context:
     def device(self) -> int:
         return int(_lib.lib().ee_ctx_device(self.handle))
 
+    def set_jv_edges_per_wave(self, edges: int) -> None:
+        """JV kernel shape for later launches: 2 (default) pairs edge slots
+        on 32-lane halves where both rows fit, 1 gives each a wavefront."""
+        _lib.check(_lib.lib().ee_ctx_set_jv_edges_per_wave(self.handle, edges))
+
     def close(self) -> None:
         if self._handle is not None:
             for child in list(self._children):

@@ -183,6 +183,7 @@ struct Fleet {
     a.xy = d_xy.get();
     a.witnesses = d_witnesses.get();
     a.edge_mode = d_mode.get();
+    a.one_per_wave = ctx->jv_edges_per_wave == 1 ? 1 : 0;
     return a;
   }
 

@@ -188,7 +188,7 @@ struct JvLaunch {
   int32_t edge_count;
   int32_t distance_type;
   int32_t near;                 // NearExtent of the coordinates: FP64 distances
-  int32_t reserved;
+  int32_t one_per_wave;         // 1: one edge slot per wavefront; 0: two when they fit
   const int2* edge_uv;          // [m]
   const int64_t* edge_weight;   // [m]
   const int32_t* edge_active;   // [m]

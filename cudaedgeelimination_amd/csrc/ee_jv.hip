@@ -3,7 +3,14 @@
 // Reference kernel: JvCandidatesKernel (src/cuda/jv_cuda.cu:221-288), one
 // thread per edge slot walking N(a), N(b) and every candidate's row serially.
 //
-// Here one 64-lane wavefront owns one edge slot (a, b):
+// Here a 64-lane wavefront owns two edge slots, one per 32-lane half, when
+// both rows of each fit its half (deg_a + deg_b <= 32, the usual case for
+// k-NN graphs), and one edge slot otherwise.  A half whose edge turns out
+// not to fit (a longer row, a key that does not pack into 64 bits, a
+// candidate row longer than 32) leaves that edge to a full wavefront after
+// the pair, which repeats the edge from the start on the one-edge path.
+//
+// One edge slot (a, b) on a full wavefront:
 //   1. candidate scoring (jv_elimination.cpp:45-84): lane s scores the s-th
 //      entry of N(a) followed by N(b); x in {a, b} is skipped, and on the b
 //      side x is skipped when (a, x) is an edge (the reference's HasActiveEdge
@@ -26,6 +33,10 @@
 //      compatible d is the witness.  Testing every d at once gives the same
 //      verdict as the reference's early exit at the first compatible d, and
 //      testing later candidates of a pass does not change which is first.
+// The pair path runs the same three steps in each half: scoring by lane
+// (N(a) staged in the half's LDS row), ranking by broadcast LDS reads of the
+// half's keys, and the witness test in 16-lane slots (two candidates per
+// half per pass) or 32-lane slots when a candidate row has 17-32 entries.
 // Sums stay exact in uint64: the host guarantees weights <= 2^62 and
 // distances <= 2^32, so no sum wraps and unsigned comparison equals the
 // reference CPU's 128-bit comparison.
@@ -89,298 +100,510 @@ __device__ __forceinline__ int64_t Dist(const longlong2 p, const longlong2 q, in
 constexpr int kNodeBits = 24;
 constexpr int kScoreBits = 40;
 
-__global__ void __launch_bounds__(kBlock) JvKernel(JvLaunch g) {
-  __shared__ uint64_t s_score[kWavesPerBlock][2][kCandidates];
-  __shared__ int32_t s_node[kWavesPerBlock][2][kCandidates];
-  // one-pass rows: N(a) (padded with -1) and the packed keys of every lane
-  __shared__ __attribute__((aligned(16))) int32_t s_nbr[kWavesPerBlock][kWave];
-  __shared__ __attribute__((aligned(16))) uint64_t s_key[kWavesPerBlock][kWave];
-  // one-pass rows: the candidates' row bounds and D(a, c), D(b, c), stored
-  // by the lanes that scored them
-  __shared__ int32_t s_cr0[kWavesPerBlock][kCandidates];
-  __shared__ int32_t s_cdeg[kWavesPerBlock][kCandidates];
-  __shared__ uint64_t s_cac[kWavesPerBlock][kCandidates];
-  __shared__ uint64_t s_cbc[kWavesPerBlock][kCandidates];
+constexpr int kHalf = kWave / 2;
+// pair path LDS rows, padded so that the two halves' broadcast reads fall in
+// different banks (16-byte multiples for the vector reads)
+constexpr int kHalfNbrStride = 36;  // int32 entries
+constexpr int kHalfKeyStride = 34;  // uint64 entries
 
+struct JvShared {
+  // one-edge path: the running top ten (double-buffered for the merge)
+  uint64_t score[kWavesPerBlock][2][kCandidates];
+  int32_t node[kWavesPerBlock][2][kCandidates];
+  // one-edge path, one-pass rows: N(a) (padded with -1) and every lane's key
+  alignas(16) int32_t nbr[kWavesPerBlock][kWave];
+  alignas(16) uint64_t key[kWavesPerBlock][kWave];
+  // the candidates' row bounds and D(a, c), D(b, c), stored by the lanes that
+  // scored them: index 0 on the one-edge path, the half on the pair path
+  int32_t cr0[kWavesPerBlock][2][kCandidates];
+  int32_t cdeg[kWavesPerBlock][2][kCandidates];
+  uint64_t cac[kWavesPerBlock][2][kCandidates];
+  uint64_t cbc[kWavesPerBlock][2][kCandidates];
+  // pair path: each half's N(a) and keys
+  alignas(16) int32_t pnbr[kWavesPerBlock][2][kHalfNbrStride];
+  alignas(16) uint64_t pkey[kWavesPerBlock][2][kHalfKeyStride];
+};
+
+// What the witness test needs of the edge (a, b) under test.
+struct EdgeUnderTest {
+  int32_t a = 0, b = 0;
+  longlong2 pa{}, pb{};
+  uint64_t wab = 0;
+  int type = 0;
+  bool near = false;
+};
+
+// Is neighbour d (adjacency entry `slot` of candidate c's row) compatible:
+// w_ab + w_cd <= D(a,c) + D(d,b) or w_ab + w_cd <= D(a,d) + D(b,c), d not in
+// {a, b} (jv_elimination.cpp:86-117).
+__device__ __forceinline__ bool Compatible(const JvLaunch& g, const EdgeUnderTest& t, int32_t slot,
+                                           uint64_t cac, uint64_t cbc) {
+  const int32_t d = g.neighbors[slot];
+  const int32_t eid = g.csr_edge_ids[slot];
+  if (d == t.a || d == t.b) return false;
+  const longlong2 pd = g.xy[d];
+  const uint64_t lhs = t.wab + static_cast<uint64_t>(g.edge_weight[eid]);
+  return lhs <= cac + static_cast<uint64_t>(Dist(pd, t.pb, t.type, t.near)) ||
+         lhs <= static_cast<uint64_t>(Dist(t.pa, pd, t.type, t.near)) + cbc;
+}
+
+__device__ __forceinline__ uint32_t HalfBits(uint64_t ballot, int h) {
+  return static_cast<uint32_t>(ballot >> (kHalf * h));
+}
+
+// One edge slot (item, wave-uniform) on the whole wavefront; returns the
+// witness or -1.
+__device__ __forceinline__ int32_t OneEdge(const JvLaunch& g, JvShared& sh, int wib, int lane,
+                                           int32_t item) {
+  const int32_t e = g.edge_list != nullptr ? g.edge_list[item] : item;
+  int32_t result = -1;
+  EdgeUnderTest t;
+  t.type = g.distance_type;
+  t.near = g.near != 0;
+  if (g.edge_mode != nullptr) {
+    const uint32_t mode = g.edge_mode[e];
+    t.type = static_cast<int>(mode & 1u);
+    t.near = (mode & 2u) != 0;
+  }
+  const int2 uv = g.edge_uv[e];
+  const int32_t a = uv.x, b = uv.y;
+  t.a = a;
+  t.b = b;
+  const int32_t ra0 = g.row_offsets[a], ra1 = g.row_offsets[a + 1];
+  const int32_t rb0 = g.row_offsets[b], rb1 = g.row_offsets[b + 1];
+  const int32_t deg_a = ra1 - ra0, deg_b = rb1 - rb0;
+  if (g.edge_active[e] != 0 && deg_a > 2 && deg_b > 2) {
+    const int type = t.type;
+    const bool near = t.near;
+    t.pa = g.xy[a];
+    t.pb = g.xy[b];
+    t.wab = static_cast<uint64_t>(g.edge_weight[e]);
+    const longlong2 pa = t.pa, pb = t.pb;
+
+    // ---- 1+2: score N(a) then N(b), keep the ten smallest (score, node)
+    const int32_t total = deg_a + deg_b;
+    const bool one_pass = total <= kWave;
+    int cnt = 0, cur = 0;
+    bool ranked = false;
+    if (one_pass) {
+      // both rows in one pass: lane s takes entry s of N(a) then N(b).  The
+      // b side skips x in N(a) by a broadcast scan of N(a) in LDS; the keys
+      // are ranked by counting smaller keys, two per LDS read, all lanes
+      // reading the same address (a broadcast).
+      const int32_t s = lane;
+      const bool side_b = s >= deg_a;
+      int32_t raw = -1, eid = 0;
+      if (s < total) {
+        const int32_t slot = side_b ? rb0 + (s - deg_a) : ra0 + s;
+        raw = g.neighbors[slot];
+        eid = g.csr_edge_ids[slot];
+      }
+      sh.nbr[wib][lane] = s < deg_a ? raw : -1;
+      WaveSync();
+      bool skip = s >= total || raw == a || raw == b;
+      if (side_b && !skip) {
+        const int4* row = reinterpret_cast<const int4*>(sh.nbr[wib]);
+        bool in_a = false;
+        for (int j = 0; j < (deg_a + 7) / 8; ++j) {  // uniform; entries past deg_a hold -1
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const int4 q = row[2 * j + u];
+            in_a |= q.x == raw || q.y == raw || q.z == raw || q.w == raw;
+          }
+        }
+        skip = in_a;
+      }
+      uint64_t sc = ~0ull, d_ax = 0, d_bx = 0;
+      int32_t x_r0 = 0, x_r1 = 0;
+      if (!skip) {
+        // what the witness test needs of x if it becomes a candidate,
+        // fetched alongside the score's operands
+        x_r0 = g.row_offsets[raw];
+        x_r1 = g.row_offsets[raw + 1];
+        const longlong2 px = g.xy[raw];
+        d_ax = static_cast<uint64_t>(Dist(pa, px, type, near));
+        d_bx = static_cast<uint64_t>(Dist(pb, px, type, near));
+        sc = static_cast<uint64_t>(g.edge_weight[eid]) + (side_b ? d_ax : d_bx);
+      }
+      const bool valid = !skip;
+      const uint64_t wide = __ballot(valid && ((sc >> kScoreBits) != 0 || (raw >> kNodeBits) != 0));
+      if (wide == 0) {
+        const uint64_t key = valid ? (sc << kNodeBits) | static_cast<uint64_t>(raw) : ~0ull;
+        sh.key[wib][lane] = key;
+        WaveSync();
+        const ulonglong2* keys = reinterpret_cast<const ulonglong2*>(sh.key[wib]);
+        int rank = 0;
+        for (int j = 0; j < (total + 1) / 2; ++j) {  // uniform; lanes past total hold ~0
+          const ulonglong2 q = keys[j];
+          rank += (q.x < key ? 1 : 0) + (q.y < key ? 1 : 0);
+        }
+        if (valid && rank < kCandidates) {
+          sh.score[wib][1][rank] = sc;
+          sh.node[wib][1][rank] = raw;
+          sh.cr0[wib][0][rank] = x_r0;
+          sh.cdeg[wib][0][rank] = x_r1 - x_r0;
+          sh.cac[wib][0][rank] = d_ax;
+          sh.cbc[wib][0][rank] = d_bx;
+        }
+        WaveSync();
+        const int npass = __popcll(__ballot(valid));
+        cnt = npass < kCandidates ? npass : kCandidates;
+        cur = 1;
+        ranked = true;
+      }
+    }
+    for (int32_t base = 0; !ranked && base < total; base += kWave) {
+      const int32_t s = base + lane;
+      bool valid = false;
+      uint64_t sc = ~0ull;
+      int32_t x = INT32_MAX;
+      int32_t raw = -1;
+      const bool side_b = s >= deg_a;
+      int32_t slot = 0;
+      if (s < total) {
+        slot = side_b ? rb0 + (s - deg_a) : ra0 + s;
+        raw = g.neighbors[slot];
+      }
+      bool skip = s >= total || raw == a || raw == b;
+      if (one_pass) {
+        // every lane joins the (uniform) scan; lanes 0..deg_a-1 hold N(a)
+        bool in_a = false;
+        for (int j = 0; j < deg_a; ++j) in_a |= ReadLane(raw, j) == raw;
+        skip |= side_b && in_a;
+      } else if (side_b && !skip) {
+        int32_t lo = ra0, hi = ra1;
+        while (lo < hi) {
+          const int32_t mid = lo + (hi - lo) / 2;
+          if (g.neighbors[mid] < raw) {
+            lo = mid + 1;
+          } else {
+            hi = mid;
+          }
+        }
+        skip = lo < ra1 && g.neighbors[lo] == raw;
+      }
+      if (!skip) {
+        const longlong2 px = g.xy[raw];
+        sc = static_cast<uint64_t>(g.edge_weight[g.csr_edge_ids[slot]]) +
+             static_cast<uint64_t>(Dist(px, side_b ? pa : pb, type, near));
+        x = raw;
+        valid = true;
+      }
+      const uint64_t vmask = __ballot(valid);
+      const int lanes = total - base < kWave ? total - base : kWave;
+      const int npass = __popcll(vmask);
+      // rank among this pass plus the running list
+      int rank = CountLessInWave(vmask, lanes, sc, x, sc, x);
+      for (int j = 0; j < cnt; ++j) {
+        rank += KeyLess(sh.score[wib][cur][j], sh.node[wib][cur][j], sc, x) ? 1 : 0;
+      }
+      // new positions of the running list entries
+      uint64_t ls = ~0ull;
+      int32_t ln = INT32_MAX;
+      int lrank = kCandidates;
+      if (lane < cnt) {
+        ls = sh.score[wib][cur][lane];
+        ln = sh.node[wib][cur][lane];
+      }
+      if (cnt > 0) {  // uniform: the first pass has no running list
+        const int c = CountLessInWave(vmask, lanes, sc, x, ls, ln);
+        if (lane < cnt) lrank = lane + c;
+      }
+      const int nxt = cur ^ 1;
+      if (valid && rank < kCandidates) {
+        sh.score[wib][nxt][rank] = sc;
+        sh.node[wib][nxt][rank] = x;
+      }
+      if (lane < cnt && lrank < kCandidates) {
+        sh.score[wib][nxt][lrank] = ls;
+        sh.node[wib][nxt][lrank] = ln;
+      }
+      WaveSync();
+      cnt = cnt + npass < kCandidates ? cnt + npass : kCandidates;
+      cur = nxt;
+    }
+
+    // ---- 3: witness test in candidate order.  Lane q < cnt holds
+    // candidate q (node, row, D(a,c), D(b,c)); consecutive candidates whose
+    // rows fit 64 lanes together are tested in one pass, lane j testing the
+    // j-th neighbour of its candidate, and the first candidate whose lanes
+    // have no compatible neighbour is the witness.
+    int32_t c_node = -1, c_r0 = 0, c_deg = 0;
+    uint64_t c_ac = 0, c_bc = 0;
+    if (lane < cnt) {
+      c_node = sh.node[wib][cur][lane];
+      if (ranked) {
+        c_ac = sh.cac[wib][0][lane];
+        c_bc = sh.cbc[wib][0][lane];
+        c_r0 = sh.cr0[wib][0][lane];
+        c_deg = sh.cdeg[wib][0][lane];
+      } else {
+        const longlong2 pc = g.xy[c_node];
+        c_ac = static_cast<uint64_t>(Dist(pa, pc, type, near));
+        c_bc = static_cast<uint64_t>(Dist(pb, pc, type, near));
+        c_r0 = g.row_offsets[c_node];
+        c_deg = g.row_offsets[c_node + 1] - c_r0;
+      }
+    }
+    // rows of at most 16 entries (the usual case): candidate q takes lanes
+    // 16 (q % 4) .. 16 (q % 4) + 15 of pass q / 4, so one ballot gives the
+    // verdicts of four candidates as 16-bit chunks, with no per-candidate
+    // bookkeeping; longer rows take the packing below
+    constexpr int kSlot = 16;
+    const bool slots = __ballot(lane < cnt && c_deg > kSlot) == 0;
+    for (int base_q = 0; slots && base_q < cnt; base_q += kWave / kSlot) {  // uniform
+      const int q = base_q + lane / kSlot, k = lane % kSlot;
+      const int src = q < cnt ? q : 0;
+      const int32_t r0 = __shfl(c_r0, src);
+      const int32_t dq = __shfl(c_deg, src);
+      const uint64_t cac = (static_cast<uint64_t>(static_cast<uint32_t>(__shfl(static_cast<int>(c_ac >> 32), src))) << 32) |
+                           static_cast<uint32_t>(__shfl(static_cast<int>(c_ac), src));
+      const uint64_t cbc = (static_cast<uint64_t>(static_cast<uint32_t>(__shfl(static_cast<int>(c_bc >> 32), src))) << 32) |
+                           static_cast<uint32_t>(__shfl(static_cast<int>(c_bc), src));
+      const uint64_t hits = __ballot(q < cnt && k < dq && Compatible(g, t, r0 + k, cac, cbc));
+      bool found = false;
+#pragma unroll 1
+      for (int u = 0; u < kWave / kSlot; ++u) {  // uniform, in candidate order
+        if (!found && base_q + u < cnt && ((hits >> (kSlot * u)) & 0xffffull) == 0) {
+          result = ReadLane(c_node, base_q + u);
+          found = true;
+        }
+      }
+      if (found) break;
+    }
+    for (int i = 0; !slots && i < cnt;) {  // uniform
+      const int deg_i = ReadLane(c_deg, i);
+      if (deg_i > kWave) {
+        // a row longer than a wavefront: this candidate alone, pass by pass
+        const int32_t r0 = ReadLane(c_r0, i), r1 = r0 + deg_i;
+        const uint64_t cac = ReadLane64(c_ac, i), cbc = ReadLane64(c_bc, i);
+        bool compatible = false;
+        for (int32_t base = r0; base < r1 && !compatible; base += kWave) {
+          const int32_t slot = base + lane;
+          compatible = __ballot(slot < r1 && Compatible(g, t, slot, cac, cbc)) != 0;
+        }
+        if (!compatible) {
+          result = ReadLane(c_node, i);
+          break;
+        }
+        ++i;
+        continue;
+      }
+      // candidates i .. j-1: lanes [start_q, start_q + deg_q) test candidate q
+      int j = i, used = 0, my_q = -1, my_k = 0;
+      while (j < cnt) {
+        const int dj = ReadLane(c_deg, j);
+        if (used + dj > kWave) break;
+        if (lane >= used && lane < used + dj) {
+          my_q = j;
+          my_k = lane - used;
+        }
+        used += dj;
+        ++j;
+      }
+      const int src = my_q < 0 ? 0 : my_q;
+      const int32_t r0 = __shfl(c_r0, src);
+      const uint64_t cac = (static_cast<uint64_t>(static_cast<uint32_t>(__shfl(static_cast<int>(c_ac >> 32), src))) << 32) |
+                           static_cast<uint32_t>(__shfl(static_cast<int>(c_ac), src));
+      const uint64_t cbc = (static_cast<uint64_t>(static_cast<uint32_t>(__shfl(static_cast<int>(c_bc >> 32), src))) << 32) |
+                           static_cast<uint32_t>(__shfl(static_cast<int>(c_bc), src));
+      const uint64_t hits = __ballot(my_q >= 0 && Compatible(g, t, r0 + my_k, cac, cbc));
+      bool found = false;
+      for (int q = i, start = 0; q < j; ++q) {  // uniform
+        const int dq = ReadLane(c_deg, q);
+        const uint64_t span = dq >= kWave ? ~0ull : ((1ull << dq) - 1ull);
+        if ((hits & (span << start)) == 0) {
+          result = ReadLane(c_node, q);
+          found = true;
+          break;
+        }
+        start += dq;
+      }
+      if (found) break;
+      i = j;
+    }
+  }
+  WaveSync();
+  return result;
+}
+
+// Edge slots 2 pair and 2 pair + 1 on the two 32-lane halves.  Lane 0 of a
+// half writes its edge's witness (or -1), unless the edge does not fit the
+// half; then the function returns true on that half's lanes and writes
+// nothing for it.
+__device__ __forceinline__ bool PairOfEdges(const JvLaunch& g, JvShared& sh, int wib, int lane,
+                                            int32_t pair) {
+  const int h = lane / kHalf, hl = lane % kHalf;
+  const int32_t item = 2 * pair + h;
+  const bool present = item < g.edge_count;
+  EdgeUnderTest t;
+  t.type = g.distance_type;
+  t.near = g.near != 0;
+  int32_t e = 0, ra0 = 0, rb0 = 0, deg_a = 0, deg_b = 0;
+  bool eligible = false;
+  if (present) {
+    e = g.edge_list != nullptr ? g.edge_list[item] : item;
+    if (g.edge_mode != nullptr) {
+      const uint32_t mode = g.edge_mode[e];
+      t.type = static_cast<int>(mode & 1u);
+      t.near = (mode & 2u) != 0;
+    }
+    const int2 uv = g.edge_uv[e];
+    t.a = uv.x;
+    t.b = uv.y;
+    ra0 = g.row_offsets[t.a];
+    rb0 = g.row_offsets[t.b];
+    deg_a = g.row_offsets[t.a + 1] - ra0;
+    deg_b = g.row_offsets[t.b + 1] - rb0;
+    eligible = g.edge_active[e] != 0 && deg_a > 2 && deg_b > 2;
+  }
+  const int32_t total = deg_a + deg_b;
+  bool fallback = eligible && total > kHalf;
+  bool work = eligible && !fallback;  // uniform within the half from here on
+  if (work) {
+    t.pa = g.xy[t.a];
+    t.pb = g.xy[t.b];
+    t.wab = static_cast<uint64_t>(g.edge_weight[e]);
+  }
+
+  // ---- 1: lane hl scores entry hl of N(a) then N(b)
+  const bool side_b = hl >= deg_a;
+  int32_t raw = -1, eid = 0;
+  if (work && hl < total) {
+    const int32_t slot = side_b ? rb0 + (hl - deg_a) : ra0 + hl;
+    raw = g.neighbors[slot];
+    eid = g.csr_edge_ids[slot];
+  }
+  sh.pnbr[wib][h][hl] = hl < deg_a ? raw : -1;
+  // both halves scan their N(a) for the same number of steps (deg_a <= 29
+  // when the edge fits, so 4 steps of 8 entries stay in the half's row)
+  const int32_t scan_a = work ? deg_a : 0;
+  const int32_t scan_max = max(ReadLane(scan_a, 0), ReadLane(scan_a, kHalf));
+  WaveSync();
+  bool skip = !work || hl >= total || raw == t.a || raw == t.b;
+  if (side_b && !skip) {
+    const int4* row = reinterpret_cast<const int4*>(sh.pnbr[wib][h]);
+    bool in_a = false;
+    for (int j = 0; j < (scan_max + 7) / 8; ++j) {  // uniform; entries past deg_a hold -1
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int4 q = row[2 * j + u];
+        in_a |= q.x == raw || q.y == raw || q.z == raw || q.w == raw;
+      }
+    }
+    skip = in_a;
+  }
+  uint64_t sc = ~0ull, d_ax = 0, d_bx = 0;
+  int32_t x_r0 = 0, x_deg = 0;
+  if (!skip) {
+    x_r0 = g.row_offsets[raw];
+    x_deg = g.row_offsets[raw + 1] - x_r0;
+    const longlong2 px = g.xy[raw];
+    const uint64_t wx = static_cast<uint64_t>(g.edge_weight[eid]);
+    d_ax = static_cast<uint64_t>(Dist(t.pa, px, t.type, t.near));
+    d_bx = static_cast<uint64_t>(Dist(t.pb, px, t.type, t.near));
+    sc = wx + (side_b ? d_ax : d_bx);
+  }
+  const bool scored = !skip;
+  if (HalfBits(__ballot(scored && ((sc >> kScoreBits) != 0 || (raw >> kNodeBits) != 0)), h) != 0) {
+    fallback = true;  // a key that does not pack: the one-edge path ranks it
+    work = false;
+  }
+
+  // ---- 2: rank the half's packed keys by broadcast reads
+  const bool valid = scored && work;
+  const uint64_t key = valid ? (sc << kNodeBits) | static_cast<uint64_t>(raw) : ~0ull;
+  sh.pkey[wib][h][hl] = key;
+  const int32_t rank_n = work ? total : 0;
+  const int32_t rank_max = max(ReadLane(rank_n, 0), ReadLane(rank_n, kHalf));
+  WaveSync();
+  const ulonglong2* keys = reinterpret_cast<const ulonglong2*>(sh.pkey[wib][h]);
+  int rank = 0;
+  for (int j = 0; j < (rank_max + 1) / 2; ++j) {  // uniform; lanes past total hold ~0
+    const ulonglong2 q = keys[j];
+    rank += (q.x < key ? 1 : 0) + (q.y < key ? 1 : 0);
+  }
+  if (valid && rank < kCandidates) {
+    sh.node[wib][h][rank] = raw;
+    sh.cr0[wib][h][rank] = x_r0;
+    sh.cdeg[wib][h][rank] = x_deg;
+    sh.cac[wib][h][rank] = d_ax;
+    sh.cbc[wib][h][rank] = d_bx;
+  }
+  const int in_half = __popc(HalfBits(__ballot(valid), h));
+  const int cnt = in_half < kCandidates ? in_half : kCandidates;
+  WaveSync();
+
+  // ---- 3: witness test, candidates in rank order: two per pass in 16-lane
+  // slots, or one per pass in the whole half when a candidate row has 17-32
+  // entries; a longer candidate row sends the edge to the one-edge path
+  const int32_t my_deg = hl < cnt ? sh.cdeg[wib][h][hl] : 0;
+  const uint32_t over16 = HalfBits(__ballot(my_deg > 16), h);
+  if (HalfBits(__ballot(my_deg > kHalf), h) != 0) {
+    fallback = true;
+    work = false;
+  }
+  const int width = over16 != 0 ? kHalf : 16;
+  int32_t result = -1;
+  int qbase = 0;
+  bool done = !work || cnt == 0;
+  while (__ballot(!done) != 0) {  // uniform
+    const int q = qbase + hl / width, k = hl % width;
+    const bool test = !done && q < cnt;
+    int32_t r0 = 0, dq = 0;
+    uint64_t cac = 0, cbc = 0;
+    if (test) {
+      r0 = sh.cr0[wib][h][q];
+      dq = sh.cdeg[wib][h][q];
+      cac = sh.cac[wib][h][q];
+      cbc = sh.cbc[wib][h][q];
+    }
+    const uint32_t hits = HalfBits(__ballot(test && k < dq && Compatible(g, t, r0 + k, cac, cbc)), h);
+    if (!done) {  // qbase < cnt here
+      if ((width == kHalf ? hits : (hits & 0xffffu)) == 0) {
+        result = sh.node[wib][h][qbase];
+        done = true;
+      } else if (width == 16 && qbase + 1 < cnt && (hits >> 16) == 0) {
+        result = sh.node[wib][h][qbase + 1];
+        done = true;
+      } else {
+        qbase += kHalf / width;
+        done = qbase >= cnt;
+      }
+    }
+  }
+  if (hl == 0 && present && !fallback) g.witnesses[item] = result;
+  WaveSync();
+  return fallback;
+}
+
+// Two edge slots per wavefront (edges that do not fit a half run afterwards
+// on the whole wavefront), or one per wavefront when g.one_per_wave is set.
+// Six wavefronts per SIMD (at most 80 VGPRs): the kernel is bound by its
+// chains of dependent gathers, and more wavefronts in flight hide more of
+// them (lattice epoch 24.3 -> 22.6 us against the compiler's own choice).
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) JvKernel(JvLaunch g) {
+  __shared__ JvShared sh;
   const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
   const int wib = static_cast<int>(threadIdx.x) / kWave;
   const int32_t waves = static_cast<int32_t>(gridDim.x) * kWavesPerBlock;
-
-  for (int32_t item = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock + wib);
-       item < g.edge_count; item += waves) {
-    const int32_t e = g.edge_list != nullptr ? g.edge_list[item] : item;
-    int32_t result = -1;
-    int type = g.distance_type;
-    bool near = g.near != 0;
-    if (g.edge_mode != nullptr) {
-      const uint32_t mode = g.edge_mode[e];
-      type = static_cast<int>(mode & 1u);
-      near = (mode & 2u) != 0;
+  const int32_t first = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock + wib);
+  if (g.one_per_wave == 0) {
+    const int32_t pairs = (g.edge_count + 1) / 2;
+    for (int32_t pair = first; pair < pairs; pair += waves) {
+      const uint64_t rest = __ballot(PairOfEdges(g, sh, wib, lane, pair));
+      for (int h = 0; h < 2; ++h) {  // uniform
+        if (((rest >> (kHalf * h)) & 1ull) != 0) {
+          const int32_t item = 2 * pair + h;
+          const int32_t result = OneEdge(g, sh, wib, lane, item);
+          if (lane == 0) g.witnesses[item] = result;
+        }
+      }
     }
-    const int2 uv = g.edge_uv[e];
-    const int32_t a = uv.x, b = uv.y;
-    const int32_t ra0 = g.row_offsets[a], ra1 = g.row_offsets[a + 1];
-    const int32_t rb0 = g.row_offsets[b], rb1 = g.row_offsets[b + 1];
-    const int32_t deg_a = ra1 - ra0, deg_b = rb1 - rb0;
-    if (g.edge_active[e] != 0 && deg_a > 2 && deg_b > 2) {
-      const longlong2 pa = g.xy[a], pb = g.xy[b];
-      const uint64_t wab = static_cast<uint64_t>(g.edge_weight[e]);
-
-      // ---- 1+2: score N(a) then N(b), keep the ten smallest (score, node)
-      const int32_t total = deg_a + deg_b;
-      const bool one_pass = total <= kWave;
-      int cnt = 0, cur = 0;
-      bool ranked = false;
-      if (one_pass) {
-        // both rows in one pass: lane s takes entry s of N(a) then N(b).  The
-        // b side skips x in N(a) by a broadcast scan of N(a) in LDS; the keys
-        // are ranked by counting smaller keys, two per LDS read, all lanes
-        // reading the same address (a broadcast).
-        const int32_t s = lane;
-        const bool side_b = s >= deg_a;
-        int32_t raw = -1, eid = 0;
-        if (s < total) {
-          const int32_t slot = side_b ? rb0 + (s - deg_a) : ra0 + s;
-          raw = g.neighbors[slot];
-          eid = g.csr_edge_ids[slot];
-        }
-        s_nbr[wib][lane] = s < deg_a ? raw : -1;
-        WaveSync();
-        bool skip = s >= total || raw == a || raw == b;
-        if (side_b && !skip) {
-          const int4* row = reinterpret_cast<const int4*>(s_nbr[wib]);
-          bool in_a = false;
-          for (int j = 0; j < (deg_a + 7) / 8; ++j) {  // uniform; entries past deg_a hold -1
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-              const int4 q = row[2 * j + u];
-              in_a |= q.x == raw || q.y == raw || q.z == raw || q.w == raw;
-            }
-          }
-          skip = in_a;
-        }
-        uint64_t sc = ~0ull, d_ax = 0, d_bx = 0;
-        int32_t x_r0 = 0, x_r1 = 0;
-        if (!skip) {
-          // what the witness test needs of x if it becomes a candidate,
-          // fetched alongside the score's operands
-          x_r0 = g.row_offsets[raw];
-          x_r1 = g.row_offsets[raw + 1];
-          const longlong2 px = g.xy[raw];
-          d_ax = static_cast<uint64_t>(Dist(pa, px, type, near));
-          d_bx = static_cast<uint64_t>(Dist(pb, px, type, near));
-          sc = static_cast<uint64_t>(g.edge_weight[eid]) + (side_b ? d_ax : d_bx);
-        }
-        const bool valid = !skip;
-        const uint64_t wide = __ballot(valid && ((sc >> kScoreBits) != 0 || (raw >> kNodeBits) != 0));
-        if (wide == 0) {
-          const uint64_t key = valid ? (sc << kNodeBits) | static_cast<uint64_t>(raw) : ~0ull;
-          s_key[wib][lane] = key;
-          WaveSync();
-          const ulonglong2* keys = reinterpret_cast<const ulonglong2*>(s_key[wib]);
-          int rank = 0;
-          for (int j = 0; j < (total + 1) / 2; ++j) {  // uniform; lanes past total hold ~0
-            const ulonglong2 q = keys[j];
-            rank += (q.x < key ? 1 : 0) + (q.y < key ? 1 : 0);
-          }
-          if (valid && rank < kCandidates) {
-            s_score[wib][1][rank] = sc;
-            s_node[wib][1][rank] = raw;
-            s_cr0[wib][rank] = x_r0;
-            s_cdeg[wib][rank] = x_r1 - x_r0;
-            s_cac[wib][rank] = d_ax;
-            s_cbc[wib][rank] = d_bx;
-          }
-          WaveSync();
-          const int npass = __popcll(__ballot(valid));
-          cnt = npass < kCandidates ? npass : kCandidates;
-          cur = 1;
-          ranked = true;
-        }
-      }
-      for (int32_t base = 0; !ranked && base < total; base += kWave) {
-        const int32_t s = base + lane;
-        bool valid = false;
-        uint64_t sc = ~0ull;
-        int32_t x = INT32_MAX;
-        int32_t raw = -1;
-        const bool side_b = s >= deg_a;
-        int32_t slot = 0;
-        if (s < total) {
-          slot = side_b ? rb0 + (s - deg_a) : ra0 + s;
-          raw = g.neighbors[slot];
-        }
-        bool skip = s >= total || raw == a || raw == b;
-        if (one_pass) {
-          // every lane joins the (uniform) scan; lanes 0..deg_a-1 hold N(a)
-          bool in_a = false;
-          for (int j = 0; j < deg_a; ++j) in_a |= ReadLane(raw, j) == raw;
-          skip |= side_b && in_a;
-        } else if (side_b && !skip) {
-          {
-            int32_t lo = ra0, hi = ra1;
-            while (lo < hi) {
-              const int32_t mid = lo + (hi - lo) / 2;
-              if (g.neighbors[mid] < raw) {
-                lo = mid + 1;
-              } else {
-                hi = mid;
-              }
-            }
-            skip = lo < ra1 && g.neighbors[lo] == raw;
-          }
-        }
-        if (!skip) {
-          const longlong2 px = g.xy[raw];
-          sc = static_cast<uint64_t>(g.edge_weight[g.csr_edge_ids[slot]]) +
-               static_cast<uint64_t>(Dist(px, side_b ? pa : pb, type, near));
-          x = raw;
-          valid = true;
-        }
-        const uint64_t vmask = __ballot(valid);
-        const int lanes = total - base < kWave ? total - base : kWave;
-        const int npass = __popcll(vmask);
-        // rank among this pass plus the running list
-        int rank = CountLessInWave(vmask, lanes, sc, x, sc, x);
-        for (int j = 0; j < cnt; ++j) {
-          rank += KeyLess(s_score[wib][cur][j], s_node[wib][cur][j], sc, x) ? 1 : 0;
-        }
-        // new positions of the running list entries
-        uint64_t ls = ~0ull;
-        int32_t ln = INT32_MAX;
-        int lrank = kCandidates;
-        if (lane < cnt) {
-          ls = s_score[wib][cur][lane];
-          ln = s_node[wib][cur][lane];
-        }
-        if (cnt > 0) {  // uniform: the first pass has no running list
-          const int c = CountLessInWave(vmask, lanes, sc, x, ls, ln);
-          if (lane < cnt) lrank = lane + c;
-        }
-        const int nxt = cur ^ 1;
-        if (valid && rank < kCandidates) {
-          s_score[wib][nxt][rank] = sc;
-          s_node[wib][nxt][rank] = x;
-        }
-        if (lane < cnt && lrank < kCandidates) {
-          s_score[wib][nxt][lrank] = ls;
-          s_node[wib][nxt][lrank] = ln;
-        }
-        WaveSync();
-        cnt = cnt + npass < kCandidates ? cnt + npass : kCandidates;
-        cur = nxt;
-      }
-
-      // ---- 3: witness test in candidate order.  Lane q < cnt holds
-      // candidate q (node, row, D(a,c), D(b,c)); consecutive candidates whose
-      // rows fit 64 lanes together are tested in one pass, lane j testing the
-      // j-th neighbour of its candidate, and the first candidate whose lanes
-      // have no compatible neighbour is the witness.
-      int32_t c_node = -1, c_r0 = 0, c_deg = 0;
-      uint64_t c_ac = 0, c_bc = 0;
-      if (lane < cnt) {
-        c_node = s_node[wib][cur][lane];
-        if (ranked) {
-          c_ac = s_cac[wib][lane];
-          c_bc = s_cbc[wib][lane];
-          c_r0 = s_cr0[wib][lane];
-          c_deg = s_cdeg[wib][lane];
-        } else {
-          const longlong2 pc = g.xy[c_node];
-          c_ac = static_cast<uint64_t>(Dist(pa, pc, type, near));
-          c_bc = static_cast<uint64_t>(Dist(pb, pc, type, near));
-          c_r0 = g.row_offsets[c_node];
-          c_deg = g.row_offsets[c_node + 1] - c_r0;
-        }
-      }
-      auto compatible_at = [&](int32_t slot, uint64_t cac, uint64_t cbc) -> bool {
-        const int32_t d = g.neighbors[slot];
-        const int32_t eid = g.csr_edge_ids[slot];
-        if (d == a || d == b) return false;
-        const longlong2 pd = g.xy[d];
-        const uint64_t lhs = wab + static_cast<uint64_t>(g.edge_weight[eid]);
-        return lhs <= cac + static_cast<uint64_t>(Dist(pd, pb, type, near)) ||
-               lhs <= static_cast<uint64_t>(Dist(pa, pd, type, near)) + cbc;
-      };
-      // rows of at most 16 entries (the usual case): candidate q takes lanes
-      // 16 (q % 4) .. 16 (q % 4) + 15 of pass q / 4, so one ballot gives the
-      // verdicts of four candidates as 16-bit chunks, with no per-candidate
-      // bookkeeping; longer rows take the packing below
-      constexpr int kSlot = 16;
-      const bool slots = __ballot(lane < cnt && c_deg > kSlot) == 0;
-      for (int base_q = 0; slots && base_q < cnt; base_q += kWave / kSlot) {  // uniform
-        const int q = base_q + lane / kSlot, k = lane % kSlot;
-        const int src = q < cnt ? q : 0;
-        const int32_t r0 = __shfl(c_r0, src);
-        const int32_t dq = __shfl(c_deg, src);
-        const uint64_t cac = (static_cast<uint64_t>(static_cast<uint32_t>(__shfl(static_cast<int>(c_ac >> 32), src))) << 32) |
-                             static_cast<uint32_t>(__shfl(static_cast<int>(c_ac), src));
-        const uint64_t cbc = (static_cast<uint64_t>(static_cast<uint32_t>(__shfl(static_cast<int>(c_bc >> 32), src))) << 32) |
-                             static_cast<uint32_t>(__shfl(static_cast<int>(c_bc), src));
-        const uint64_t hits = __ballot(q < cnt && k < dq && compatible_at(r0 + k, cac, cbc));
-        bool found = false;
-#pragma unroll 1
-        for (int t = 0; t < kWave / kSlot; ++t) {  // uniform, in candidate order
-          if (!found && base_q + t < cnt && ((hits >> (kSlot * t)) & 0xffffull) == 0) {
-            result = ReadLane(c_node, base_q + t);
-            found = true;
-          }
-        }
-        if (found) break;
-      }
-      for (int i = 0; !slots && i < cnt;) {  // uniform
-        const int deg_i = ReadLane(c_deg, i);
-        if (deg_i > kWave) {
-          // a row longer than a wavefront: this candidate alone, pass by pass
-          const int32_t r0 = ReadLane(c_r0, i), r1 = r0 + deg_i;
-          const uint64_t cac = ReadLane64(c_ac, i), cbc = ReadLane64(c_bc, i);
-          bool compatible = false;
-          for (int32_t base = r0; base < r1 && !compatible; base += kWave) {
-            const int32_t slot = base + lane;
-            compatible = __ballot(slot < r1 && compatible_at(slot, cac, cbc)) != 0;
-          }
-          if (!compatible) {
-            result = ReadLane(c_node, i);
-            break;
-          }
-          ++i;
-          continue;
-        }
-        // candidates i .. j-1: lanes [start_q, start_q + deg_q) test candidate q
-        int j = i, used = 0, my_q = -1, my_k = 0;
-        while (j < cnt) {
-          const int dj = ReadLane(c_deg, j);
-          if (used + dj > kWave) break;
-          if (lane >= used && lane < used + dj) {
-            my_q = j;
-            my_k = lane - used;
-          }
-          used += dj;
-          ++j;
-        }
-        const int src = my_q < 0 ? 0 : my_q;
-        const int32_t r0 = __shfl(c_r0, src);
-        const uint64_t cac = (static_cast<uint64_t>(static_cast<uint32_t>(__shfl(static_cast<int>(c_ac >> 32), src))) << 32) |
-                             static_cast<uint32_t>(__shfl(static_cast<int>(c_ac), src));
-        const uint64_t cbc = (static_cast<uint64_t>(static_cast<uint32_t>(__shfl(static_cast<int>(c_bc >> 32), src))) << 32) |
-                             static_cast<uint32_t>(__shfl(static_cast<int>(c_bc), src));
-        const uint64_t hits = __ballot(my_q >= 0 && compatible_at(r0 + my_k, cac, cbc));
-        bool found = false;
-        for (int q = i, start = 0; q < j; ++q) {  // uniform
-          const int dq = ReadLane(c_deg, q);
-          const uint64_t span = dq >= kWave ? ~0ull : ((1ull << dq) - 1ull);
-          if ((hits & (span << start)) == 0) {
-            result = ReadLane(c_node, q);
-            found = true;
-            break;
-          }
-          start += dq;
-        }
-        if (found) break;
-        i = j;
-      }
-      WaveSync();
-    }
+    return;
+  }
+  for (int32_t item = first; item < g.edge_count; item += waves) {
+    const int32_t result = OneEdge(g, sh, wib, lane, item);
     if (lane == 0) g.witnesses[item] = result;
   }
 }
@@ -389,7 +612,10 @@ __global__ void __launch_bounds__(kBlock) JvKernel(JvLaunch g) {
 
 void LaunchJv(const JvLaunch& args, uint32_t grid, hipStream_t stream) {
   if (args.edge_count <= 0) return;
-  const uint64_t needed = (static_cast<uint64_t>(args.edge_count) + kWavesPerBlock - 1) / kWavesPerBlock;
+  // one wavefront per edge slot, or per pair of edge slots
+  const uint64_t items = args.one_per_wave != 0 ? static_cast<uint64_t>(args.edge_count)
+                                                : (static_cast<uint64_t>(args.edge_count) + 1) / 2;
+  const uint64_t needed = (items + kWavesPerBlock - 1) / kWavesPerBlock;
   uint64_t blocks = grid == 0 ? needed : grid;
   if (blocks > needed) blocks = needed;
   hipLaunchKernelGGL(JvKernel, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, stream, args);

@@ -246,7 +246,8 @@ struct Context {
   int compute_units = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
-  uint32_t jv_grid = 0;  // 0: one wavefront per edge slot
+  uint32_t jv_grid = 0;  // 0: one wavefront per edge slot or pair of slots
+  uint32_t jv_edges_per_wave = 2;  // 2: pairs of edge slots on 32-lane halves
   KoptTuning kopt_tuning;
   JvCache jv;
   KoptCache kopt;

@@ -248,6 +248,7 @@ JvLaunch JvArgs(Context& ctx, int32_t m, int type) {
   a.csr_edge_ids = c.d_edge_ids.get();
   a.xy = c.d_xy.get();
   a.witnesses = c.d_witnesses.get();
+  a.one_per_wave = ctx.jv_edges_per_wave == 1 ? 1 : 0;
   return a;
 }
 
@@ -486,6 +487,17 @@ void ee_ctx_destroy(ee_ctx* ctx) { delete reinterpret_cast<ee::Context*>(ctx); }
 
 int ee_ctx_device(const ee_ctx* ctx) {
   return ctx ? reinterpret_cast<const ee::Context*>(ctx)->device : -1;
+}
+
+ee_status ee_ctx_set_jv_edges_per_wave(ee_ctx* ctx, uint32_t edges) {
+  try {
+    if (ctx == nullptr) throw std::invalid_argument("context is NULL");
+    if (edges != 1 && edges != 2) throw std::invalid_argument("JV edges per wavefront must be 1 or 2");
+    reinterpret_cast<ee::Context*>(ctx)->jv_edges_per_wave = edges;
+    return EE_OK;
+  } catch (...) {
+    return StatusFromException();
+  }
 }
 
 ee_status ee_jv_find_candidates(ee_ctx* ctx, const ee_graph_view* graph, int32_t* witnesses,

@@ -81,6 +81,7 @@ struct Session {
     a.csr_edge_ids = csr.edge_ids();
     a.xy = d_xy.get();
     a.witnesses = d_witnesses.get();
+    a.one_per_wave = ctx->jv_edges_per_wave == 1 ? 1 : 0;
     CheckHip(hipEventRecord(ctx->ev_begin, ctx->stream), "event record");
     LaunchJv(a, ctx->jv_grid, ctx->stream);
     CheckHip(hipEventRecord(ctx->ev_end, ctx->stream), "event record");

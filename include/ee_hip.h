@@ -77,6 +77,13 @@ void ee_ctx_destroy(ee_ctx* ctx);
 int ee_ctx_device(const ee_ctx* ctx);
 /* Waits for all work the context enqueued. */
 ee_status ee_ctx_synchronize(ee_ctx* ctx);
+/* JV kernel shape for the context's later JV launches (direct calls,
+ * sessions, fleets): 2 (the default) puts two edge slots on one wavefront,
+ * one per 32-lane half, when both rows of an edge fit the half, and runs the
+ * others on a whole wavefront; 1 gives every edge slot a whole wavefront.
+ * Both give the same witnesses.  No counterpart in the reference, whose
+ * kernel (src/cuda/jv_cuda.cu:221-288) has one thread per edge slot. */
+ee_status ee_ctx_set_jv_edges_per_wave(ee_ctx* ctx, uint32_t edges);
 
 /* ---------------------------------------------------------------------- JV */
 
