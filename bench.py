@@ -700,6 +700,13 @@ def main() -> int:
     # consumer-side output: per row the mask of improving templates
     # (cell < deleted_cost) instead of the cells, evaluated and downloaded
     batch.set_output("masks")
+    if args.tuning:
+        batch.set_tuning(serial_tuning["rows_per_tile"], serial_tuning["grid"], serial_tuning["block"],
+                         serial_tuning["non_temporal"])
+        mask_tuning = batch.tuning()
+    else:
+        log("autotuning (improving-template masks)")
+        mask_tuning = batch.autotune(10)  # its own store pattern, tuned on its own
     batch.run(args.warmup)
     mask_ms = batch.run(args.steps)
     t_dl = time.perf_counter()
@@ -707,7 +714,9 @@ def main() -> int:
     mask_dl_s = time.perf_counter() - t_dl
     batch.set_output("costs")
     masks = {
-        "words_per_row": int(gpu_masks.shape[1]), "kernel_ms": mask_ms,
+        "words_per_row": int(gpu_masks.shape[1]), "kernel_ms": mask_ms, "tuning": mask_tuning,
+        # against the cells' own single-stream launch
+        "kernel_vs_cells": mask_ms / device_ms,
         "download_ms": mask_dl_s * 1e3, "bytes": int(gpu_masks.nbytes),
         "improving_cells": int(np.unpackbits(gpu_masks.view(np.uint8)).sum()),
         "cells_download_ms": None,

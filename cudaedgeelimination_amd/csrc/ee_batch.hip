@@ -18,7 +18,7 @@ namespace {
 
 constexpr uint32_t kTemplateCount[3] = {4, 25, 208};
 // Bumped whenever the k-opt kernel changes, so cached tunings are retuned.
-constexpr int kKernelRevision = 6;
+constexpr int kKernelRevision = 7;
 
 struct Batch {
   explicit Batch(Context* c) : ctx(c) {}
@@ -92,6 +92,7 @@ struct Batch {
   size_t SharedBytes() const {
     KoptLaunch probe{};
     probe.masks = mask_output ? const_cast<uint32_t*>(d_masks.get()) : nullptr;  // mask mode stages deleted costs
+    probe.mask_words = mask_output ? MaskWords() : 0u;                              // and the tile's mask words
     return KoptPrepareLayout(&probe, plans.data(), k_present, tuning, NearOnly());
   }
 
@@ -313,6 +314,7 @@ std::string TuningKey(const Batch& b, bool pipelined) {
     key << '-' << (r == 0 ? -1 : bucket);
   }
   if (pipelined) key << "/pipelined" << b.pipeline_depth;
+  if (b.mask_output) key << "/masks";  // a different store pattern, tuned on its own
   std::string s = key.str();
   std::replace(s.begin(), s.end(), ' ', '_');
   return s;

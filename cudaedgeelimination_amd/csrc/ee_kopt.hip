@@ -248,36 +248,20 @@ __device__ __forceinline__ void PairRecords(const KoptKLayout& L, uint32_t rows,
 // and the high halves to the counters and sentinels.  kG is G when it is a
 // compile-time constant, else 0.
 //
-// Mask mode (m.words > 0): no cells are stored; a ballot per row gathers the
-// lanes whose cell is below the row's deleted cost, and the first lane of each
-// lane group writes that row's bits of its templates (m.tbase .. m.tbase +
-// templates of the pass) into the row's mask words.
+// Mask mode (m.words > 0): no cells are stored; a lane whose cell is below
+// the row's deleted cost sets its template's bit in the row's mask words,
+// staged in LDS (zeroed first) and copied to the output once per tile with
+// full-width stores.  Improving cells are rare, so most steps write nothing.
 struct MaskOut {
-  uint32_t* masks;       // mask of tile row 0 (the tile's first task row)
+  uint32_t* stage;       // LDS: mask of tile row 0
   uint32_t words;        // words per row; 0 = cost output
   const int64_t* dcost;  // LDS: deleted cost of each tile row
-  uint32_t tbase;        // first template of the pass
-  uint32_t tcount;       // templates per lane group in this pass (<= 64)
+  uint32_t t;            // the lane's template
   uint32_t g;            // lane group (its rows are g, g + G, ...)
-  bool writer;           // first lane of its group
-  bool single_pass;      // T <= 64: one pass writes every word of a row
 };
 
 __device__ __forceinline__ void WriteMaskRow(const MaskOut& m, uint32_t row, bool improving) {
-  const uint64_t ballot = __ballot(improving);
-  if (!m.writer) return;
-  const uint64_t bits = (ballot >> (m.g * m.tcount)) & (m.tcount >= 64 ? ~0ull : ((1ull << m.tcount) - 1ull));
-  uint32_t* dst = m.masks + static_cast<uint64_t>(row) * m.words + m.tbase / 32;
-  if (m.single_pass) {
-    // all templates in one pass: this write covers every word of the row
-    for (uint32_t w = 0; w < m.words; ++w) {
-      dst[w] = w == 0 ? static_cast<uint32_t>(bits) : (w == 1 ? static_cast<uint32_t>(bits >> 32) : 0u);
-    }
-    return;
-  }
-  // passes of 64 templates: words 2p and 2p + 1 of the row
-  dst[0] = static_cast<uint32_t>(bits);
-  if (m.tbase / 32 + 1 < m.words) dst[1] = static_cast<uint32_t>(bits >> 32);
+  if (improving) atomicOr(m.stage + row * m.words + m.t / 32, 1u << (m.t % 32));
 }
 
 template <int K, bool kNear, bool kNonTemporal, uint32_t kG, bool kMask>
@@ -287,7 +271,28 @@ __device__ __forceinline__ void CellSteps(const char* pairs, const uint32_t (&of
   const uint32_t G = kG != 0 ? kG : g_rows;
   auto steps = [&](auto unroll, uint32_t i) {
     constexpr uint32_t U = decltype(unroll)::value;
-    if (kMask) {
+    if (kMask && kNear) {
+      // near coordinates: as for the cells below, the cost is the low word
+      uint32_t lo[U], hi[U];
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) lo[u] = hi[u] = 0;
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        const uint2* rec = reinterpret_cast<const uint2*>(pairs + off[j] + i * G * 8);
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+          const uint2 v = rec[u * G];
+          lo[u] += v.x;
+          hi[u] += v.y;
+        }
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        const uint32_t row = m.g + (i + u) * G;
+        const bool bad = (hi[u] & static_cast<uint32_t>(kBadMask >> 32)) != 0;
+        WriteMaskRow(m, row, !bad && static_cast<int64_t>(lo[u]) < m.dcost[row]);
+      }
+    } else if (kMask) {
       uint64_t sum[U];
 #pragma unroll
       for (uint32_t u = 0; u < U; ++u) sum[u] = 0;
@@ -420,6 +425,14 @@ __device__ void TileCells(const KoptLaunch& a, const KoptPointers& bufs, const K
   const uint32_t n = __builtin_amdgcn_readfirstlane(rows / G);  // rows every group has
   const uint32_t extra = rows - n * G;                          // groups g < extra have one more
   const uint32_t step_bytes = G * T * 8;
+  // mask mode: the tile's mask words are staged in the port records, which
+  // phase C no longer reads (the layout sizes them for rows x mask words)
+  uint32_t* const w_mask = reinterpret_cast<uint32_t*>(wave + L.xy_off);
+  const uint32_t mask_count = kMask ? rows * a.mask_words : 0u;
+  if (kMask) {
+    for (uint32_t q = lane; q < mask_count; q += kWave) w_mask[q] = 0u;
+    WaveSync();
+  }
   for (uint32_t pass = 0; pass < L.passes; ++pass) {
     uint32_t g = 0, t = pass * kWave + static_cast<uint32_t>(lane);
     if (T <= static_cast<uint32_t>(kWave)) {
@@ -439,15 +452,16 @@ __device__ void TileCells(const KoptLaunch& a, const KoptPointers& bufs, const K
     constexpr uint32_t kStdG = K == 3 ? 16u : (K == 4 ? 2u : 1u);
     if (kMask) {
       MaskOut m;
-      m.masks = bufs.masks + (d.task_index * a.mask_words);
+      m.stage = w_mask;
       m.words = a.mask_words;
       m.dcost = reinterpret_cast<const int64_t*>(wave + L.dcost_off);
-      m.tbase = pass * kWave;
-      m.tcount = T <= static_cast<uint32_t>(kWave) ? T : (T - m.tbase < kWave ? T - m.tbase : kWave);
+      m.t = t;
       m.g = g;
-      m.writer = (T <= static_cast<uint32_t>(kWave) ? t : t - m.tbase) == 0;
-      m.single_pass = T <= static_cast<uint32_t>(kWave);
-      CellSteps<K, kNear, kNonTemporal, 0, true>(pairs, off, out, at, step_bytes, G, n, g < extra, m);
+      if (G == kStdG) {
+        CellSteps<K, kNear, kNonTemporal, kStdG, true>(pairs, off, out, at, step_bytes, G, n, g < extra, m);
+      } else {
+        CellSteps<K, kNear, kNonTemporal, 0, true>(pairs, off, out, at, step_bytes, G, n, g < extra, m);
+      }
     } else if (G == kStdG) {
       CellSteps<K, kNear, kNonTemporal, kStdG, false>(pairs, off, out, at, step_bytes, G, n, g < extra, MaskOut{});
     } else {
@@ -455,6 +469,12 @@ __device__ void TileCells(const KoptLaunch& a, const KoptPointers& bufs, const K
     }
   }
   WaveSync();
+  if (kMask) {
+    // the tile's rows are consecutive task rows: one contiguous block
+    uint32_t* const dst = bufs.masks + Uniform64(d.task_index) * a.mask_words;
+    for (uint32_t q = lane; q < mask_count; q += kWave) dst[q] = w_mask[q];
+    WaveSync();
+  }
 }
 
 // Descriptor of `tile` (a tile of this k): from the inline segment table
@@ -619,7 +639,10 @@ size_t KoptPrepareLayout(KoptLaunch* a, const KoptPlan plans[3], const bool pres
     // per-wave arrays are kept to what each phase needs: the same-node masks
     // of A2 live in the port records, the deleted costs only in mask mode
     L.xy_off = 0;
-    L.row_off = Align16(rows * ports * (near_only ? kNearPortBytes : kFarPortBytes));
+    // in mask mode the port records also hold the tile's mask words in phase C
+    const uint32_t port_bytes = rows * ports * (near_only ? kNearPortBytes : kFarPortBytes);
+    const uint32_t stage_bytes = a->masks != nullptr ? rows * a->mask_words * 4 : 0u;
+    L.row_off = Align16(std::max(port_bytes, stage_bytes));
     L.dcost_off = L.row_off + Align16(rows * 4);
     L.pair_off = L.dcost_off + (a->masks != nullptr ? Align16(rows * 8) : 0u);
     // the port node ids of phase A share the pair record area
