@@ -238,8 +238,74 @@ def many_instance_leg(ctx, cpu: bool, sample: int = 16):
     return out
 
 
+def serve_stream(ctx, graphs, pools, requests, is_jv, arrivals, mode: str, burst: bool):
+    """One request stream against a fresh server of `mode` ("batched" or
+    "plain"): latency from each request's scheduled arrival to completion."""
+    import threading
+
+    count = len(requests)
+    kopt_cells = sum(len(t) * TEMPLATES[k] for _, k, t in requests if k)
+    server = KoptServer(ctx, batching=(mode == "batched"), hbm_budget_bytes=2 << 30,
+                        max_batch_cells=1 << 21)
+    gids = [server.add_graph(g) for g in graphs]
+    jids = [server.add_jv_graph(g) for g in graphs]
+    warm = server.submit(gids[0], 3, pools[0][3][:8])
+    server.wait(warm)
+    latencies = np.zeros(count)
+    late = np.zeros(count)
+    if burst:
+        # the burst arrives as one message: queued in one call
+        t0 = time.perf_counter()
+        ids = server.submit_many([(gids[g], k, tasks) if k else (jids[g], 0, tasks)
+                                  for g, k, tasks in requests])
+        done = server.wait_many(ids)
+        wall = time.perf_counter() - t0
+        latencies[:] = [lat for _, lat in done]
+    else:
+        ids = [0] * count
+        submitted = threading.Semaphore(0)
+
+        def collect():
+            for i in range(count):
+                submitted.acquire()
+                _, info = server.wait(ids[i])
+                latencies[i] = info["latency_ms"] + late[i]
+
+        waiter = threading.Thread(target=collect)
+        waiter.start()
+        t0 = time.perf_counter()
+        for i, (g, k, tasks) in enumerate(requests):
+            due = t0 + arrivals[i]
+            while True:
+                now = time.perf_counter()
+                if now >= due:
+                    break
+                if due - now > 2e-4:
+                    time.sleep(1e-4)
+            ids[i] = server.submit(gids[g], k, tasks) if k else server.submit_jv(jids[g], tasks)
+            late[i] = (time.perf_counter() - due) * 1e3  # late submit counts
+            submitted.release()
+        waiter.join()
+        wall = time.perf_counter() - t0
+    stats = server.stats()
+    server.close()
+    return {
+        "p50_ms": float(np.percentile(latencies, 50)),
+        "p99_ms": float(np.percentile(latencies, 99)),
+        "mean_ms": float(latencies.mean()),
+        "jv_p99_ms": float(np.percentile(latencies[is_jv], 99)) if is_jv.any() else None,
+        "cells_per_s": kopt_cells / wall,
+        "jv_edges_per_s": stats["jv_edges"] / wall,
+        "launches": stats["batches"],
+        "device_cells_per_s": stats["cells"] / (stats["device_ms"] * 1e-3) if stats["device_ms"] > 0 else None,
+        "wall_s": wall,
+        # how far behind schedule the client submitted (its own limit)
+        "client_late_p50_ms": float(np.percentile(late, 50)) if not burst else None,
+    }
+
+
 def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005, jv_share: float = 0.2,
-                burst: int = 20000):
+                burst: int = 20000, saturation_p99_ms: float = 0.5, saturation_rates=()):
     """Open-loop Poisson queries against four resident graphs (n = 1k, 5k,
     15k, 30k, uniform, 10-NN).  Leaf-cost queries: sizes lognormal (median
     2,896 cells, sigma 1.04) clipped to [128, 65,536] cells, k = 3 / 4 / 5 at
@@ -249,13 +315,14 @@ def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005, jv_share:
     so queueing counts (no coordinated omission).  Same request stream for the
     batched server and the plain per-query path.  A rate of 0 is a burst of
     `burst` requests all due at once, queued with one submit_many call: its
-    cells/s is each path's capacity."""
-    import threading
-
+    cells/s is each path's capacity.  Then, per path, the saturation rate:
+    the highest of `saturation_rates` (ascending) whose p99 stays within
+    `saturation_p99_ms`, each rate run while the previous one held."""
     graphs = W.config5_graphs()
     pools = W.config5_row_pools(graphs, seed)
     out = {"graphs": [g.dimension for g in graphs], "points": []}
-    for qps in qps_list:
+
+    def stream(qps, count_seconds):
         rng = np.random.default_rng(seed + qps)
         if qps == 0:
             # burst: every request is due at once, so the wall time measures
@@ -264,88 +331,41 @@ def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005, jv_share:
             arrivals = np.zeros(count)
         else:
             # at least 10,000 requests, so that p99 rests on 100 of them
-            count = max(int(qps * seconds), 10000)
+            count = max(int(qps * count_seconds), 10000)
             arrivals = np.cumsum(rng.exponential(1.0 / qps, count))
         requests, is_jv = W.config5_requests(graphs, pools, count, rng, jv_share)
+        return requests, is_jv, arrivals
+
+    for qps in qps_list:
+        requests, is_jv, arrivals = stream(qps, seconds)
         kopt_cells = [len(t) * TEMPLATES[k] for _, k, t in requests if k]
-        point = {"qps": qps if qps else "burst", "requests": count, "jv_rechecks": int(is_jv.sum()),
+        point = {"qps": qps if qps else "burst", "requests": len(requests), "jv_rechecks": int(is_jv.sum()),
                  "mean_cells": float(np.mean(kopt_cells)),
                  "mean_jv_edges": float(np.mean([len(t) for _, k, t in requests if not k] or [0]))}
         for mode in ("batched", "plain"):
-            server = KoptServer(ctx, batching=(mode == "batched"), hbm_budget_bytes=2 << 30,
-                                max_batch_cells=1 << 21)
-            gids = [server.add_graph(g) for g in graphs]
-            jids = [server.add_jv_graph(g) for g in graphs]
-            warm = server.submit(gids[0], 3, pools[0][3][:8])
-            server.wait(warm)
-            latencies = np.zeros(count)
-            if qps == 0:
-                # the burst arrives as one message: queued in one call
-                t0 = time.perf_counter()
-                ids = server.submit_many([(gids[g], k, tasks) if k else (jids[g], 0, tasks)
-                                          for g, k, tasks in requests])
-                done = server.wait_many(ids)
-                wall = time.perf_counter() - t0
-                latencies[:] = [lat for _, lat in done]
-                stats = server.stats()
-                server.close()
-                total_cells = sum(kopt_cells)
-                point[mode] = {
-                    "p50_ms": float(np.percentile(latencies, 50)),
-                    "p99_ms": float(np.percentile(latencies, 99)),
-                    "mean_ms": float(latencies.mean()),
-                    "jv_p99_ms": float(np.percentile(latencies[is_jv], 99)) if is_jv.any() else None,
-                    "cells_per_s": total_cells / wall,
-                    "jv_edges_per_s": stats["jv_edges"] / wall,
-                    "launches": stats["batches"],
-                    "device_cells_per_s": stats["cells"] / (stats["device_ms"] * 1e-3)
-                    if stats["device_ms"] > 0 else None,
-                    "wall_s": wall,
-                }
-                continue
-            ids = [0] * count
-            submitted = threading.Semaphore(0)
-
-            def collect():
-                for i in range(count):
-                    submitted.acquire()
-                    _, info = server.wait(ids[i])
-                    latencies[i] = info["latency_ms"] + info_offsets[i]
-
-            info_offsets = np.zeros(count)
-            waiter = threading.Thread(target=collect)
-            waiter.start()
-            t0 = time.perf_counter()
-            for i, (g, k, tasks) in enumerate(requests):
-                due = t0 + arrivals[i]
-                while True:
-                    now = time.perf_counter()
-                    if now >= due:
-                        break
-                    if due - now > 2e-4:
-                        time.sleep(1e-4)
-                ids[i] = server.submit(gids[g], k, tasks) if k else server.submit_jv(jids[g], tasks)
-                info_offsets[i] = (time.perf_counter() - due) * 1e3  # late submit counts
-                submitted.release()
-            waiter.join()
-            wall = time.perf_counter() - t0
-            stats = server.stats()
-            server.close()
-            total_cells = sum(kopt_cells)
-            point[mode] = {
-                "p50_ms": float(np.percentile(latencies, 50)),
-                "p99_ms": float(np.percentile(latencies, 99)),
-                "mean_ms": float(latencies.mean()),
-                "jv_p99_ms": float(np.percentile(latencies[is_jv], 99)) if is_jv.any() else None,
-                "cells_per_s": total_cells / wall,
-                "jv_edges_per_s": stats["jv_edges"] / wall,
-                "launches": stats["batches"],
-                "device_cells_per_s": stats["cells"] / (stats["device_ms"] * 1e-3)
-                if stats["device_ms"] > 0 else None,
-                "wall_s": wall,
-            }
+            point[mode] = serve_stream(ctx, graphs, pools, requests, is_jv, arrivals, mode, qps == 0)
         point["batched_over_plain_cells_per_s"] = point["batched"]["cells_per_s"] / point["plain"]["cells_per_s"]
         out["points"].append(point)
+
+    if saturation_rates:
+        sat = {"p99_bound_ms": saturation_p99_ms, "rates": list(saturation_rates)}
+        for mode in ("batched", "plain"):
+            held, tried = None, []
+            for qps in saturation_rates:
+                requests, is_jv, arrivals = stream(qps, seconds)
+                r = serve_stream(ctx, graphs, pools, requests, is_jv, arrivals, mode, False)
+                tried.append({"qps": qps, "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
+                              "client_late_p50_ms": r["client_late_p50_ms"], "launches": r["launches"]})
+                log(f"saturation {mode} {qps} QPS: p99 {r['p99_ms']:.3f} ms")
+                if r["p99_ms"] > saturation_p99_ms:
+                    break
+                held = qps
+            sat[mode] = {"saturation_qps": held, "points": tried,
+                         # the last rate held with nothing left to try: a lower bound
+                         "at_last_rate": held == saturation_rates[-1]}
+        if sat["batched"]["saturation_qps"] and sat["plain"]["saturation_qps"]:
+            sat["batched_over_plain_qps"] = sat["batched"]["saturation_qps"] / sat["plain"]["saturation_qps"]
+        out["saturation"] = sat
     return out
 
 
@@ -615,6 +635,9 @@ def main() -> int:
     ap.add_argument("--serve-qps", default="2000,8000,20000,0",
                     help="Poisson rates of the serving leg; 0 is a burst of --serve-burst requests")
     ap.add_argument("--serve-burst", type=int, default=20000)
+    ap.add_argument("--saturation-qps", default="10000,14000,20000,28000,40000,56000,80000",
+                    help="ascending Poisson rates of the saturation search (empty: none)")
+    ap.add_argument("--saturation-p99-ms", type=float, default=0.5)
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--tuning", default=None,
                     help="r3,r4,r5,grid,block,nt: skip the autotuner and use this tuning")
@@ -899,8 +922,10 @@ def main() -> int:
 
     if not args.no_serve:
         log("serving workload (open-loop Poisson queries)")
-        result["serving"] = serving_leg(ctx, [int(q) for q in args.serve_qps.split(",")],
-                                        args.serve_seconds, burst=args.serve_burst)
+        result["serving"] = serving_leg(
+            ctx, [int(q) for q in args.serve_qps.split(",")], args.serve_seconds, burst=args.serve_burst,
+            saturation_p99_ms=args.saturation_p99_ms,
+            saturation_rates=[int(q) for q in args.saturation_qps.split(",") if q])
 
     batch.close()
     ctx.close()
