@@ -217,6 +217,7 @@ PROTOTYPES = {
     "ee_ctx_device": (C.c_int, [P]),
     "ee_ctx_synchronize": (C.c_int, [P]),
     "ee_ctx_set_jv_edges_per_wave": (C.c_int, [P, U32]),
+    "ee_graph_content_hash_hip": (C.c_int, [P, I32, U8, P, P, I32, P, P, P, P, C.POINTER(C.c_uint64)]),
     "ee_kopt_batch_create": (C.c_int, [P, C.POINTER(P)]),
     "ee_kopt_batch_destroy": (None, [P]),
     "ee_kopt_batch_add_instance": (C.c_int, [P, P, P, I32, U8, C.POINTER(U32)]),

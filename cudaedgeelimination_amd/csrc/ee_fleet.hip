@@ -291,8 +291,9 @@ struct Fleet {
       });
       st.host_ms += MsSince(t1);
     }
-    const double total = MsSince(t_total);
-    ParallelFor(hosts.size(), [&](size_t i) { hosts[i].EndRun(total); });
+    const double loop_ms = MsSince(t_total);
+    ParallelFor(hosts.size(), [&](size_t i) { hosts[i].EndRun(loop_ms); });
+    const double total = MsSince(t_total);  // including the wait for hashes
     for (const JvHost& h : hosts) {
       st.edges_scanned += h.stats.edges_scanned;
       st.proposed += h.stats.proposed;

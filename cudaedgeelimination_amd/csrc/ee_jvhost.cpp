@@ -133,6 +133,7 @@ JvHost::SnapHash JvHost::CurrentSnap() {
   if (!snaps_.empty() && snaps_.back().generation == generation_) return snaps_.back();
   SnapHash h;
   h.generation = generation_;
+  if (hash_source && hash_source(&h.job)) return h;
   if (async_hash) {
     // the job hashes a copy of this generation's flags, so commits may
     // change the snapshot while it runs
@@ -287,7 +288,7 @@ void JvHost::EndRun(double total_ms) {
   stats.initial_hash = snaps_.front().Get();
   stats.final_hash = final_snap.Get();
   stats.hash_wait_ms = MsSince(t0);
-  stats.total_ms = total_ms;
+  stats.total_ms = total_ms + stats.hash_wait_ms;  // the run ends when its hashes are known
   // V1 proof text (src/cpu/jv_elimination.cpp:324-368)
   std::ostringstream out;
   out << "CUDAEE_PROOF_V1\nbackend hip\ninitial_hash " << Hex(stats.initial_hash) << '\n';

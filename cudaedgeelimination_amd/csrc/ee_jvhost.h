@@ -42,6 +42,12 @@ struct JvHost {
   // run ends.  Without it, the hash is computed when the epoch commits
   // (before the snapshot changes), on the calling thread.
   bool async_hash = false;
+  // When set, asked for the hash of each new snapshot generation first: it
+  // returns true with a future of that hash (computed elsewhere, e.g. on the
+  // device), or false to leave the generation to the host.
+  std::function<bool(std::shared_future<uint64_t>*)> hash_source;
+  // FNV-1a state after the header and coordinates (the edge part follows).
+  uint64_t prefix_hash() const { return prefix_hash_; }
   // IsJvWitness (jv_elimination.cpp:86-117) with 128-bit sums.
   bool Verify(int32_t e, int32_t c) const;
 
@@ -63,7 +69,8 @@ struct JvHost {
     VerifyAndGate(witnesses, propose_ms);
     return ApplyCommit();
   }
-  // Final hash and the V1 proof text with backend name "hip".
+  // Final hash and the V1 proof text with backend name "hip"; total_ms is
+  // the run's time up to this call, to which the wait for hashes is added.
   void EndRun(double total_ms);
 
   bool done = false;

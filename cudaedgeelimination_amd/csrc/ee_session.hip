@@ -9,9 +9,11 @@
 // 13-75).  The committed edge slots go to the device, which clears their
 // active flags and compacts its CSR (ee_csr.h) and proposes the next epoch,
 // while the host applies the same commit to its own snapshot (needed by the
-// next re-verification) and the snapshot hashes run on a helper thread.  The
-// static part of the snapshot, the active flags and the CSR stay on the
-// device; only the first epoch of a run uploads them.
+// next re-verification).  Each snapshot's content hash is computed on the
+// device (ee_hash.h) on a second stream, beside the proposal, from the
+// device's active flags; the next commit waits for it.  The static part of
+// the snapshot, the active flags and the CSR stay on the device; only the
+// first epoch of a run uploads them.
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -24,6 +26,7 @@
 
 #include "ee_csr.h"
 #include "ee_exact.h"
+#include "ee_hash.h"
 #include "ee_gen.h"
 #include "ee_jvhost.h"
 #include "ee_runtime.h"
@@ -34,7 +37,11 @@ namespace {
 struct Session {
   explicit Session(Context* c) : ctx(c) {}
   ~Session() {
+    if (hash_stream != nullptr) static_cast<void>(hipStreamSynchronize(hash_stream));
     if (ev_compact != nullptr) static_cast<void>(hipEventDestroy(ev_compact));
+    if (ev_flags != nullptr) static_cast<void>(hipEventDestroy(ev_flags));
+    if (ev_hashed != nullptr) static_cast<void>(hipEventDestroy(ev_hashed));
+    if (hash_stream != nullptr) static_cast<void>(hipStreamDestroy(hash_stream));
   }
 
   Context* ctx;
@@ -42,6 +49,15 @@ struct Session {
   std::vector<int32_t> active32;
   Pinned<int32_t> witnesses;
   hipEvent_t ev_compact = nullptr;
+  // snapshot hashes on the device: one slot per generation of a run
+  DeviceContentHash dhash;
+  DeviceArray<uint64_t> d_hashes;
+  std::vector<uint64_t> h_hashes;
+  hipStream_t hash_stream = nullptr;
+  hipEvent_t ev_flags = nullptr, ev_hashed = nullptr;
+  size_t hash_slots = 0;      // enqueued this run
+  bool hash_pending = false;  // the newest slot is not yet handed to the host
+  bool hashes_fetched = false;
   // device
   DeviceArray<int2> d_uv;
   DeviceArray<int64_t> d_weight;
@@ -60,14 +76,51 @@ struct Session {
     *h2d_ms += MsSince(t0);
   }
 
+  // The hash of the device's current snapshot into the run's next slot, on
+  // the hash stream once the context's stream has the flags in place.
+  void EnqueueHash() {
+    if (hash_slots >= d_hashes.count()) throw std::logic_error("JV session: hash slots exhausted");
+    CheckHip(hipEventRecord(ev_flags, ctx->stream), "event record");
+    CheckHip(hipStreamWaitEvent(hash_stream, ev_flags, 0), "stream wait");
+    dhash.Enqueue(host.prefix_hash(), d_uv.get(), d_weight.get(), csr.active(), d_hashes.get() + hash_slots,
+                  hash_stream);
+    CheckHip(hipEventRecord(ev_hashed, hash_stream), "event record");
+    ++hash_slots;
+    hash_pending = true;
+  }
+
+  // The host's request for the hash of a new generation: the slot enqueued
+  // for it, read when the proof is written.
+  bool HashOfNewGeneration(std::shared_future<uint64_t>* job) {
+    if (!hash_pending) return false;  // a generation the device has not hashed
+    hash_pending = false;
+    const size_t slot = hash_slots - 1;
+    *job = std::async(std::launch::deferred, [this, slot] { return FetchHash(slot); }).share();
+    return true;
+  }
+
+  uint64_t FetchHash(size_t slot) {
+    if (!hashes_fetched) {
+      CheckHip(hipStreamSynchronize(hash_stream), "hipStreamSynchronize");
+      h_hashes.resize(hash_slots);
+      d_hashes.Download(h_hashes.data(), hash_slots, hash_stream);
+      CheckHip(hipStreamSynchronize(hash_stream), "hipStreamSynchronize");
+      hashes_fetched = true;
+    }
+    return h_hashes.at(slot);
+  }
+
   // The device side of one epoch on the context's stream: [commit of the
-  // previous epoch,] JV kernel, witness download.  Events bracket the commit
-  // and the kernel.
+  // previous epoch and the hash of its result,] JV kernel, witness download.
+  // Events bracket the commit and the kernel.
   void Enqueue(bool compact) {
     if (compact) {
+      // the previous snapshot's hash reads the flags this commit changes
+      CheckHip(hipStreamWaitEvent(ctx->stream, ev_hashed, 0), "stream wait");
       csr.UploadCommits(ctx->stream);
       CheckHip(hipEventRecord(ev_compact, ctx->stream), "event record");
       csr.EnqueueCompact(ctx->stream);
+      EnqueueHash();
     }
     JvLaunch a{};
     a.edge_count = host.m;
@@ -104,14 +157,24 @@ struct Session {
     if (max_rounds == 0) throw std::invalid_argument("max_rounds must be > 0");
     ctx->Activate();
     const auto t_total = std::chrono::steady_clock::now();
-    host.StartRun();
-    host.BeginEpoch(0);
+    host.StartRun();  // settles the previous run's hashes first
+    // a run that stopped on an error may have left hashes in flight
+    if (hash_stream != nullptr) CheckHip(hipStreamSynchronize(hash_stream), "hipStreamSynchronize");
+    hash_slots = 0;
+    hash_pending = false;
+    hashes_fetched = false;
     if (host.m == 0) {  // nothing to propose: one empty epoch
+      host.BeginEpoch(0);
       std::vector<int32_t> none(1, -1);
       host.FinishEpoch(none.data(), 0.0);
     } else {
+      // one hash per generation: the first, and one after each commit
+      const size_t slots = static_cast<size_t>(std::min<uint64_t>(max_rounds, static_cast<uint64_t>(host.m))) + 1;
+      if (d_hashes.count() < slots) d_hashes.Allocate(slots, ctx->device);
       // the first epoch uploads the host's snapshot (a run may follow another)
       UploadFull(&host.stats.h2d_ms);
+      EnqueueHash();
+      host.BeginEpoch(0);
       auto t0 = std::chrono::steady_clock::now();
       Enqueue(false);
       bool compacted = false;
@@ -166,7 +229,7 @@ ee_status ee_jv_session_create(ee_ctx* handle, const ee_graph_view* g, ee_jv_ses
     auto* ctx = reinterpret_cast<ee::Context*>(handle);
     ctx->Activate();
     auto s = std::make_unique<ee::Session>(ctx);
-    s->host.async_hash = true;  // one graph: hash beside the device step
+    s->host.async_hash = true;  // a generation the device does not hash: on a helper thread
     s->host.Load(g);
     const ee::JvHost& h = s->host;
     const size_t n = static_cast<size_t>(h.n), m = static_cast<size_t>(h.m);
@@ -185,6 +248,14 @@ ee_status ee_jv_session_create(ee_ctx* handle, const ee_graph_view* g, ee_jv_ses
     s->d_uv.Upload(uv.data(), m, ctx->stream);
     s->d_weight.Upload(h.weight.data(), m, ctx->stream);
     s->d_xy.Upload(xy.data(), n, ctx->stream);
+    if (m > 0) {
+      s->dhash.Allocate(m, ctx->device);
+      ee::CheckHip(hipStreamCreateWithFlags(&s->hash_stream, hipStreamNonBlocking), "hipStreamCreate");
+      ee::CheckHip(hipEventCreateWithFlags(&s->ev_flags, hipEventDisableTiming), "hipEventCreate");
+      ee::CheckHip(hipEventCreateWithFlags(&s->ev_hashed, hipEventDisableTiming), "hipEventCreate");
+      ee::Session* raw = s.get();
+      s->host.hash_source = [raw](std::shared_future<uint64_t>* job) { return raw->HashOfNewGeneration(job); };
+    }
     ctx->Sync();
     *out = reinterpret_cast<ee_jv_session*>(s.release());
   });

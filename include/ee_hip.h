@@ -77,6 +77,16 @@ void ee_ctx_destroy(ee_ctx* ctx);
 int ee_ctx_device(const ee_ctx* ctx);
 /* Waits for all work the context enqueued. */
 ee_status ee_ctx_synchronize(ee_ctx* ctx);
+/* The snapshot content hash of ee_graph_content_hash (src/common/graph.cpp:
+ * 323-346: FNV-1a 64 over the header, coordinates and edge slots) with the
+ * edge part evaluated on the device: per chunk of slots the permutation of
+ * the hash's low byte, walks giving every chunk's starting low byte, then
+ * per chunk its weighted FNV sum.  Equal to the byte-serial hash; the JV
+ * session hashes its snapshots this way.  No counterpart in the reference,
+ * which hashes on the host. */
+ee_status ee_graph_content_hash_hip(ee_ctx* ctx, int32_t n, uint8_t distance_type, const int64_t* x,
+                                    const int64_t* y, int32_t m, const int32_t* edge_u, const int32_t* edge_v,
+                                    const int64_t* edge_weight, const uint8_t* edge_active, uint64_t* hash);
 /* JV kernel shape for the context's later JV launches (direct calls,
  * sessions, fleets): 2 (the default) puts two edge slots on one wavefront,
  * one per 32-lane half, when both rows of an edge fit the half, and runs the
