@@ -117,6 +117,28 @@ void JvHost::RebuildCsr() {
                                  neighbors.data(), edge_ids.data()));
 }
 
+// Removes the entries of edges that are no longer active, row by row.  Rows
+// keep their order, so the result equals RebuildCsr on the new flags (rows
+// are filled in edge-slot order there too).
+void JvHost::DropInactiveFromCsr() {
+  int32_t out = 0, start = row_offsets[0];
+  for (int32_t vtx = 0; vtx < n; ++vtx) {
+    const int32_t end = row_offsets[static_cast<size_t>(vtx) + 1];
+    row_offsets[static_cast<size_t>(vtx)] = out;
+    for (int32_t s = start; s < end; ++s) {
+      if (active[static_cast<size_t>(edge_ids[static_cast<size_t>(s)])] != 0) {
+        neighbors[static_cast<size_t>(out)] = neighbors[static_cast<size_t>(s)];
+        edge_ids[static_cast<size_t>(out)] = edge_ids[static_cast<size_t>(s)];
+        ++out;
+      }
+    }
+    start = end;
+  }
+  row_offsets[static_cast<size_t>(n)] = out;
+  neighbors.resize(static_cast<size_t>(out));
+  edge_ids.resize(static_cast<size_t>(out));
+}
+
 uint64_t JvHost::ComputeHash(const uint8_t* active_flags) const {
   return ContentHashEdges(prefix_hash_, m, u.data(), v.data(), weight.data(), active_flags);
 }
@@ -195,8 +217,21 @@ const std::vector<int32_t>& JvHost::VerifyAndGate(const int32_t* witnesses, doub
 
   // host re-verification on the same snapshot
   auto t0 = std::chrono::steady_clock::now();
-  for (const auto& c : proposed) {
-    if (!Verify(c.first, c.second)) ++mt.rejected;
+  constexpr size_t kVerifyBlock = 256, kParallelVerify = 2048;
+  if (verify_threads != 1 && proposed.size() >= kParallelVerify) {
+    std::atomic<size_t> rejected{0};
+    const size_t blocks = (proposed.size() + kVerifyBlock - 1) / kVerifyBlock;
+    ParallelFor(blocks, [&](size_t b) {
+      size_t r = 0;
+      const size_t end = std::min(proposed.size(), (b + 1) * kVerifyBlock);
+      for (size_t i = b * kVerifyBlock; i < end; ++i) r += Verify(proposed[i].first, proposed[i].second) ? 0 : 1;
+      rejected += r;
+    }, verify_threads);
+    mt.rejected = rejected.load();
+  } else {
+    for (const auto& c : proposed) {
+      if (!Verify(c.first, c.second)) ++mt.rejected;
+    }
   }
   mt.verify_ms = MsSince(t0);
   mt.verified = proposed.size() - mt.rejected;
@@ -213,10 +248,15 @@ const std::vector<int32_t>& JvHost::VerifyAndGate(const int32_t* witnesses, doub
   // the snapshot must be the one proposed on (elimination_commit.cpp:16-18);
   // it changes only through commits, which bump the generation
   if (generation_ != snap_generation_) throw std::runtime_error("snapshot changed before commit");
-  std::sort(proposed.begin(), proposed.end(), [&](const auto& l, const auto& r) {
+  const auto key_less = [&](const auto& l, const auto& r) {
     return std::make_tuple(u[l.first], v[l.first], l.second, l.first) <
            std::make_tuple(u[r.first], v[r.first], r.second, r.first);
-  });
+  };
+  // proposals come in edge-slot order, which for a snapshot sorted by (u, v)
+  // (as the reference loads them) is already the commit order
+  if (!std::is_sorted(proposed.begin(), proposed.end(), key_less)) {
+    std::sort(proposed.begin(), proposed.end(), key_less);
+  }
   proposed.erase(std::unique(proposed.begin(), proposed.end(),
                              [](const auto& l, const auto& r) { return l.first == r.first; }),
                  proposed.end());
@@ -249,7 +289,7 @@ size_t JvHost::ApplyCommit() {
   if (!gated_.empty()) {
     for (const auto& c : gated_) active[c.first] = 0;
     ++generation_;
-    RebuildCsr();
+    DropInactiveFromCsr();
   }
   stats.commit_ms += MsSince(t0);
   mt.committed = gated_.size();

@@ -36,12 +36,16 @@ struct JvHost {
   int32_t Degree(int32_t vtx) const { return row_offsets[vtx + 1] - row_offsets[vtx]; }
   int64_t Dist(int32_t a, int32_t b) const;
   void RebuildCsr();
+  void DropInactiveFromCsr();
   // With async_hash set, the content hash of every epoch's snapshot is
   // computed on a helper thread from a copy of the active flags, so it never
   // waits on the critical path; the proof records get their hashes when the
   // run ends.  Without it, the hash is computed when the epoch commits
   // (before the snapshot changes), on the calling thread.
   bool async_hash = false;
+  // Host threads of the re-verification of a large epoch (each candidate is
+  // checked on its own; 1 = the calling thread only, 0 = every core).
+  unsigned verify_threads = 1;
   // When set, asked for the hash of each new snapshot generation first: it
   // returns true with a future of that hash (computed elsewhere, e.g. on the
   // device), or false to leave the generation to the host.

@@ -230,6 +230,7 @@ ee_status ee_jv_session_create(ee_ctx* handle, const ee_graph_view* g, ee_jv_ses
     ctx->Activate();
     auto s = std::make_unique<ee::Session>(ctx);
     s->host.async_hash = true;  // a generation the device does not hash: on a helper thread
+    s->host.verify_threads = 0;  // one graph: a large epoch's re-verification over the host cores
     s->host.Load(g);
     const ee::JvHost& h = s->host;
     const size_t n = static_cast<size_t>(h.n), m = static_cast<size_t>(h.m);

@@ -149,3 +149,17 @@ def test_shape_setting_is_validated(ctx):
     for bad in (0, 3):
         with pytest.raises(_lib.InvalidArgument):
             ctx.set_jv_edges_per_wave(bad)
+
+
+def test_large_epoch_session_equals_oracle(ctx):
+    """A 15,000-node 10-NN graph: the first epoch commits about 7,600 edges,
+    so the host re-verifies on several threads; the device commits, compacts
+    and hashes every generation.  The proof equals the oracle's."""
+    g = W.config5_graphs(sizes=(15000,))[0]
+    expected = O.jv_run(O.OGraph.from_snapshot(g), max_rounds=100)
+    s = JvSession(ctx, g)
+    run = s.run(100)
+    s.close()
+    say(f"JV 15000: epochs={run.epochs} committed={run.committed} total_ms={run.total_ms:.3f}")
+    assert run.committed > 2048
+    assert O.canonical_proof(run.proof) == O.canonical_proof(expected)
