@@ -217,6 +217,8 @@ PROTOTYPES = {
     "ee_ctx_device": (C.c_int, [P]),
     "ee_ctx_synchronize": (C.c_int, [P]),
     "ee_ctx_set_jv_edges_per_wave": (C.c_int, [P, U32]),
+    "ee_leaf_inject_cell": (C.c_int, [P, U64, I64]),
+    "ee_jv_session_inject_witness": (C.c_int, [P, I32, I32]),
     "ee_graph_content_hash_hip": (C.c_int, [P, I32, U8, P, P, I32, P, P, P, P, C.POINTER(C.c_uint64)]),
     "ee_kopt_batch_create": (C.c_int, [P, C.POINTER(P)]),
     "ee_kopt_batch_destroy": (None, [P]),

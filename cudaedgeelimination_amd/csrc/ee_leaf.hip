@@ -153,6 +153,11 @@ struct Leaf {
   LeafTour tour;
   HostPool pool;
   ee_leaf_stats stats{};
+  // a fault for tests of the fail-closed path: one cell of the next block's
+  // host copy replaced before certification
+  bool inject = false;
+  uint64_t inject_cell = 0;
+  int64_t inject_value = 0;
 
   void Create(const int64_t* xs, const int64_t* ys, int32_t dim, uint8_t dist_type) {
     if (xs == nullptr || ys == nullptr || dim <= 0 || (dist_type != EE_EUC_2D && dist_type != EE_CEIL_2D)) {
@@ -382,6 +387,10 @@ struct Leaf {
     }
     t0 = std::chrono::steady_clock::now();
     const CertifyPlan& plan = cert[s.k - 3];
+    if (inject) {
+      inject = false;
+      if (inject_cell < static_cast<uint64_t>(s.rows) * plan.templates) s.h_costs.get()[inject_cell] = inject_value;
+    }
     const std::function<void()> rows_check = [&] { CheckCursorRows(s); };
     const CertifyResult r = CertifyCells(pool, plan, x.data(), y.data(), n, type, s.h_tasks.get(), s.rows,
                                          s.h_costs.get(), s.improving.data(), s.cursor ? &rows_check : nullptr);
@@ -467,6 +476,15 @@ ee_status ee_leaf_submit_cursor(ee_leaf_pipeline* pipeline, uint32_t k, uint64_t
 
 ee_status ee_leaf_next(ee_leaf_pipeline* pipeline, ee_leaf_block* block) {
   return ee::Guarded([&] { ee::AsLeaf(pipeline).Next(block); });
+}
+
+ee_status ee_leaf_inject_cell(ee_leaf_pipeline* pipeline, uint64_t cell, int64_t value) {
+  return ee::Guarded([&] {
+    ee::Leaf& l = ee::AsLeaf(pipeline);
+    l.inject = true;
+    l.inject_cell = cell;
+    l.inject_value = value;
+  });
 }
 
 ee_status ee_leaf_stats_get(const ee_leaf_pipeline* pipeline, ee_leaf_stats* stats) {

@@ -55,6 +55,10 @@ struct Session {
   std::vector<uint64_t> h_hashes;
   hipStream_t hash_stream = nullptr;
   hipEvent_t ev_flags = nullptr, ev_hashed = nullptr;
+  // a fault for tests of the fail-closed path: one witness of the next
+  // epoch's host copy replaced before re-verification
+  bool inject = false;
+  int32_t inject_edge = 0, inject_witness = -1;
   size_t hash_slots = 0;      // enqueued this run
   bool hash_pending = false;  // the newest slot is not yet handed to the host
   bool hashes_fetched = false;
@@ -180,6 +184,10 @@ struct Session {
       bool compacted = false;
       for (uint32_t epoch = 0;; ++epoch) {
         Wait(compacted);
+        if (inject) {
+          inject = false;
+          if (inject_edge >= 0 && inject_edge < host.m) witnesses.get()[inject_edge] = inject_witness;
+        }
         const std::vector<int32_t>& commit = host.VerifyAndGate(witnesses.get(), MsSince(t0));
         const bool next = !commit.empty() && epoch + 1 < max_rounds;
         if (next) {
@@ -274,6 +282,16 @@ ee_status ee_jv_session_run(ee_jv_session* session, uint32_t max_rounds, ee_jv_r
   return Guarded([&] {
     if (session == nullptr) throw std::invalid_argument("session is NULL");
     reinterpret_cast<ee::Session*>(session)->Run(max_rounds, stats);
+  });
+}
+
+ee_status ee_jv_session_inject_witness(ee_jv_session* session, int32_t edge, int32_t witness) {
+  return Guarded([&] {
+    if (session == nullptr) throw std::invalid_argument("session is NULL");
+    auto* s = reinterpret_cast<ee::Session*>(session);
+    s->inject = true;
+    s->inject_edge = edge;
+    s->inject_witness = witness;
   });
 }
 

@@ -216,6 +216,11 @@ class JvSession:
                      st.verify_ms, st.commit_ms, st.total_ms, buf.value.decode(), st.compact_ms,
                      st.hash_wait_ms)
 
+    def inject_witness(self, edge: int, witness: int) -> None:
+        """Test hook: the next run's first epoch sees `witness` for `edge` in
+        place of the device's, before host re-verification."""
+        _lib.check(_lib.lib().ee_jv_session_inject_witness(self._h, edge, witness))
+
     def active(self) -> np.ndarray:
         out = np.empty(self.graph.edge_count, np.uint8)
         _lib.check(_lib.lib().ee_jv_session_active(self._h, _lib.ptr(out)))
@@ -480,6 +485,11 @@ class LeafPipeline:
             view(b.costs, np.int64, (rows, T)),
             view(b.improving, np.uint32, (rows, W)),
             int(b.improving_cells))
+
+    def inject_cell(self, cell: int, value: int) -> None:
+        """Test hook: the next block handed out has `cell` of its host copy
+        set to `value` before certification."""
+        _lib.check(_lib.lib().ee_leaf_inject_cell(self._h, cell, value))
 
     def stats(self) -> dict:
         st = _lib.LeafStats()

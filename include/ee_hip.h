@@ -406,6 +406,12 @@ ee_status ee_leaf_submit_cursor(ee_leaf_pipeline* pipeline, uint32_t k, uint64_t
  * nothing is outstanding. */
 ee_status ee_leaf_next(ee_leaf_pipeline* pipeline, ee_leaf_block* block);
 ee_status ee_leaf_stats_get(const ee_leaf_pipeline* pipeline, ee_leaf_stats* stats);
+/* Fault injection for tests of the fail-closed path (no counterpart in the
+ * reference, whose certification is tested by construction): the next block
+ * handed out by ee_leaf_next has cell `cell` of its host copy set to `value`
+ * before certification, which must then refuse the block with
+ * EE_ERR_RUNTIME naming the cell (unless the value is the true one). */
+ee_status ee_leaf_inject_cell(ee_leaf_pipeline* pipeline, uint64_t cell, int64_t value);
 
 /* --------------------------------------------------- resident k-opt batch */
 /* No reference counterpart.  The reference evaluates one (graph, k) matrix
@@ -548,6 +554,13 @@ void ee_jv_session_destroy(ee_jv_session* session);
 ee_status ee_jv_session_run(ee_jv_session* session, uint32_t max_rounds, ee_jv_run_stats* stats);
 /* V1 proof text of the last run (NUL-terminated when capacity > 0);
  * *length receives the full length. */
+/* Fault injection for tests of the fail-closed path (no counterpart in the
+ * reference): in the next run's first epoch, the host copy of the device's
+ * witness for `edge` is replaced by `witness` before re-verification, which
+ * must stop the run with EE_ERR_RUNTIME and commit nothing, as the
+ * reference stops on a rejected candidate (src/cpu/jv_elimination.cpp:
+ * 301-303), unless the replacement is itself a valid witness. */
+ee_status ee_jv_session_inject_witness(ee_jv_session* session, int32_t edge, int32_t witness);
 ee_status ee_jv_session_proof(const ee_jv_session* session, char* buffer, size_t capacity,
                               size_t* length);
 /* Active flags after the last run (edge_count entries). */

@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 import _oracle as O
-from cudaedgeelimination_amd import backend, workloads as W
+from cudaedgeelimination_amd import _lib, backend, workloads as W
 from cudaedgeelimination_amd.graph import CEIL_2D, EUC_2D, GraphSnapshot, exact_distances
 from cudaedgeelimination_amd.resident import JvSession
 
@@ -162,4 +162,27 @@ def test_large_epoch_session_equals_oracle(ctx):
     s.close()
     say(f"JV 15000: epochs={run.epochs} committed={run.committed} total_ms={run.total_ms:.3f}")
     assert run.committed > 2048
+    assert O.canonical_proof(run.proof) == O.canonical_proof(expected)
+
+
+def test_corrupted_witness_stops_the_run_and_commits_nothing(ctx):
+    """A device witness corrupted on the host before re-verification (the
+    reference stops the run on a rejected candidate, jv_elimination.cpp:
+    301-303): the run fails, nothing is committed on the host or the device,
+    and the next run gives the oracle's proof."""
+    g = W.config2()
+    expected = O.jv_run(O.OGraph.from_snapshot(g), max_rounds=100)
+    s = JvSession(ctx, g)
+    before = s.active().copy()
+    e = int(np.flatnonzero(g.degrees()[g.edge_u] > 2)[0])
+    s.inject_witness(e, int(g.edge_u[e]))  # a candidate equal to an endpoint is never a witness
+    with pytest.raises(_lib.EeError) as err:
+        s.run(100)
+    assert err.value.status == 3 and "re-verification" in err.value.message
+    assert np.array_equal(s.active(), before)  # host snapshot
+    dev = s.device_snapshot()
+    assert np.array_equal(dev["edge_active"], before)  # the device committed nothing either
+    assert np.array_equal(dev["row_offsets"], g.row_offsets)
+    run = s.run(100)
+    s.close()
     assert O.canonical_proof(run.proof) == O.canonical_proof(expected)

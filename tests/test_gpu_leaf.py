@@ -202,3 +202,31 @@ def test_failures_leave_the_pipeline_usable(ctx, wl):
     st = p.stats()
     assert st["blocks"] == 4  # the failed block is not counted
     p.close()
+
+
+def test_corrupted_cell_fails_certification_and_only_its_block(ctx, wl):
+    """A cell of a device block corrupted on the host before certification
+    (the fail-closed path of the reference's certify-every-cell contract):
+    ee_leaf_next refuses that block naming the cell; the next block, and a
+    corrupted value equal to the true one, pass."""
+    say("leaf certification fault")
+    g = wl.graph
+    p = LeafPipeline(ctx, g, depth=2, max_rows=256)
+    rows = wl.bucket(5)[:100]
+    cells, _, _ = O.kopt_costs(g.x, g.y, g.distance_type, 5, rows)
+    cell = 37 * T_OF[5] + 101
+    p.submit_tasks(5, rows, tag=1)
+    p.submit_tasks(5, rows, tag=2)
+    p.inject_cell(cell, int(cells[cell]) + 1)
+    with pytest.raises(_lib.EeError) as err:
+        p.next()
+    assert err.value.status == 3 and "certification" in err.value.message
+    assert f"cell={cell}" in err.value.message
+    b = p.next()
+    assert b.tag == 2
+    check_block(g, b, rows)
+    # the true value is no fault
+    p.submit_tasks(5, rows, tag=3)
+    p.inject_cell(cell, int(cells[cell]))
+    check_block(g, p.next(), rows)
+    p.close()
