@@ -33,10 +33,14 @@
 //      compatible d is the witness.  Testing every d at once gives the same
 //      verdict as the reference's early exit at the first compatible d, and
 //      testing later candidates of a pass does not change which is first.
+//      (The pair path below first probes three neighbours of every candidate
+//      at once, then finishes the rows of the candidates still open in rank
+//      order.)
 // The pair path runs the same three steps in each half: scoring by lane
 // (N(a) staged in the half's LDS row), ranking by broadcast LDS reads of the
-// half's keys, and the witness test in 16-lane slots (two candidates per
-// half per pass) or 32-lane slots when a candidate row has 17-32 entries.
+// half's keys, and the witness test as a probe pass (three lanes per
+// candidate, the first three entries of its row) followed by one pass per
+// candidate still open, lowest rank first, over the rest of its row.
 // Sums stay exact in uint64: the host guarantees weights <= 2^62 and
 // distances <= 2^32, so no sum wraps and unsigned comparison equals the
 // reference CPU's 128-bit comparison.
@@ -534,41 +538,61 @@ __device__ __forceinline__ bool PairOfEdges(const JvLaunch& g, JvShared& sh, int
   const int cnt = in_half < kCandidates ? in_half : kCandidates;
   WaveSync();
 
-  // ---- 3: witness test, candidates in rank order: two per pass in 16-lane
-  // slots, or one per pass in the whole half when a candidate row has 17-32
-  // entries; a longer candidate row sends the edge to the one-edge path
+  // ---- 3: witness test.  Most candidates have a compatible neighbour among
+  // the first three of their row (94% on k-NN graphs), so one probe pass
+  // gives every candidate 3 lanes (candidate hl / 3, neighbours 0..2); then,
+  // in rank order, the lowest candidate without a compatible neighbour yet
+  // gets its whole remaining row (at most 29 entries, one per lane) per pass.
+  // The first candidate whose row has no compatible neighbour is the
+  // witness, as in the reference's scan in rank order with early exit.  A
+  // candidate row longer than 32 sends the edge to the one-edge path.
+  constexpr int kProbe = 3;
   const int32_t my_deg = hl < cnt ? sh.cdeg[wib][h][hl] : 0;
-  const uint32_t over16 = HalfBits(__ballot(my_deg > 16), h);
   if (HalfBits(__ballot(my_deg > kHalf), h) != 0) {
     fallback = true;
     work = false;
   }
-  const int width = over16 != 0 ? kHalf : 16;
+  uint32_t hitmask = 0;  // per half: candidates with a compatible neighbour
+  {
+    const int q = (hl * 11) >> 5;  // hl / 3 for hl < 32
+    const int k = hl - kProbe * q;
+    bool hit = false;
+    if (work && q < cnt) {
+      const int32_t dq = sh.cdeg[wib][h][q];
+      if (k < dq) hit = Compatible(g, t, sh.cr0[wib][h][q] + k, sh.cac[wib][h][q], sh.cbc[wib][h][q]);
+    }
+    const uint32_t lanes = HalfBits(__ballot(hit), h);
+    // lane q < cnt gathers candidate q's three probe lanes
+    hitmask = HalfBits(__ballot(hl < cnt && ((lanes >> (kProbe * hl)) & 7u) != 0), h);
+  }
   int32_t result = -1;
-  int qbase = 0;
   bool done = !work || cnt == 0;
   while (__ballot(!done) != 0) {  // uniform
-    const int q = qbase + hl / width, k = hl % width;
-    const bool test = !done && q < cnt;
-    int32_t r0 = 0, dq = 0;
-    uint64_t cac = 0, cbc = 0;
-    if (test) {
-      r0 = sh.cr0[wib][h][q];
-      dq = sh.cdeg[wib][h][q];
-      cac = sh.cac[wib][h][q];
-      cbc = sh.cbc[wib][h][q];
-    }
-    const uint32_t hits = HalfBits(__ballot(test && k < dq && Compatible(g, t, r0 + k, cac, cbc)), h);
-    if (!done) {  // qbase < cnt here
-      if ((width == kHalf ? hits : (hits & 0xffffu)) == 0) {
-        result = sh.node[wib][h][qbase];
-        done = true;
-      } else if (width == 16 && qbase + 1 < cnt && (hits >> 16) == 0) {
-        result = sh.node[wib][h][qbase + 1];
-        done = true;
+    const uint32_t open = ~hitmask & ((1u << cnt) - 1u);
+    int q = 0;
+    int32_t dq = 0;
+    if (!done) {
+      if (open == 0) {
+        done = true;  // every candidate has a compatible neighbour: no witness
       } else {
-        qbase += kHalf / width;
-        done = qbase >= cnt;
+        q = __builtin_ctz(open);
+        dq = sh.cdeg[wib][h][q];
+        if (dq <= kProbe) {  // the probe saw its whole row
+          result = sh.node[wib][h][q];
+          done = true;
+        }
+      }
+    }
+    const bool test = !done && kProbe + hl < dq;
+    bool hit = false;
+    if (test) hit = Compatible(g, t, sh.cr0[wib][h][q] + kProbe + hl, sh.cac[wib][h][q], sh.cbc[wib][h][q]);
+    const bool any = HalfBits(__ballot(hit), h) != 0;
+    if (!done) {
+      if (any) {
+        hitmask |= 1u << q;
+      } else {
+        result = sh.node[wib][h][q];
+        done = true;
       }
     }
   }
@@ -581,7 +605,9 @@ __device__ __forceinline__ bool PairOfEdges(const JvLaunch& g, JvShared& sh, int
 // on the whole wavefront), or one per wavefront when g.one_per_wave is set.
 // Six wavefronts per SIMD (at most 80 VGPRs): the kernel is bound by its
 // chains of dependent gathers, and more wavefronts in flight hide more of
-// them (lattice epoch 24.3 -> 22.6 us against the compiler's own choice).
+// them (lattice epoch 24.3 -> 22.6 us against the compiler's own choice; with
+// the probe witness pass, 55 us against 58 us at five per SIMD on a
+// 15,000-node graph, 19.5 against 18.9 us on the lattice).
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) JvKernel(JvLaunch g) {
   __shared__ JvShared sh;
   const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
