@@ -11,15 +11,21 @@
 // device clears those flags, compacts the union CSR (ee_csr.h), runs one JV
 // launch over all edge slots and downloads the witnesses -- a sequence
 // replayable as a captured HIP graph (one per CSR buffer).  Meanwhile the
-// host applies each graph's commit to its own snapshot.
+// host applies each graph's commit to its own snapshot.  After every device
+// commit, one device content hash of the union (ee_hash.h, one segment per
+// graph) on a second stream gives every graph's new snapshot hash; the next
+// commit waits for it.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <future>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "ee_csr.h"
+#include "ee_hash.h"
 #include "ee_jvhost.h"
 #include "ee_runtime.h"
 
@@ -31,7 +37,13 @@ namespace {
 
 struct Fleet {
   explicit Fleet(Context* c) : ctx(c) {}
-  ~Fleet() { DropGraphs(); }
+  ~Fleet() {
+    DropGraphs();
+    if (hash_stream != nullptr) static_cast<void>(hipStreamSynchronize(hash_stream));
+    if (ev_flags != nullptr) static_cast<void>(hipEventDestroy(ev_flags));
+    if (ev_hashed != nullptr) static_cast<void>(hipEventDestroy(ev_hashed));
+    if (hash_stream != nullptr) static_cast<void>(hipStreamDestroy(hash_stream));
+  }
 
   Context* ctx;
   std::vector<JvHost> hosts;
@@ -50,6 +62,51 @@ struct Fleet {
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   std::vector<uint8_t> retired;  // graph done and its slots deactivated on the device
+  // snapshot hashes on the device: slot e holds every graph's hash after the
+  // device's e-th commit of the run (slot 0: the uploaded snapshots)
+  DeviceContentHash dhash;
+  DeviceArray<uint64_t> d_hashes;
+  std::vector<uint64_t> h_hashes;
+  hipStream_t hash_stream = nullptr;
+  hipEvent_t ev_flags = nullptr, ev_hashed = nullptr;
+  size_t hash_slots = 0;      // enqueued this run
+  bool hash_pending = false;  // new generations of this step take the newest slot
+  bool hashes_fetched = false;
+  std::mutex hash_mu;
+
+  // The hash of the device's union snapshot into the run's next slot, on the
+  // hash stream once the context's stream has the flags in place.
+  void EnqueueHash() {
+    const size_t n = hosts.size();
+    if ((hash_slots + 1) * n > d_hashes.count()) throw std::logic_error("JV fleet: hash slots exhausted");
+    CheckHip(hipEventRecord(ev_flags, ctx->stream), "event record");
+    CheckHip(hipStreamWaitEvent(hash_stream, ev_flags, 0), "stream wait");
+    dhash.Enqueue(d_uv.get(), d_weight.get(), csr.active(), d_hashes.get() + hash_slots * n, hash_stream);
+    CheckHip(hipEventRecord(ev_hashed, hash_stream), "event record");
+    ++hash_slots;
+  }
+
+  // A graph's request for the hash of its new generation (from the host
+  // threads of a parallel step): the newest slot, if the device holds it.
+  bool HashOfNewGeneration(size_t graph, std::shared_future<uint64_t>* job) {
+    if (!hash_pending || hash_slots == 0) return false;
+    const size_t slot = hash_slots - 1;
+    *job = std::async(std::launch::deferred, [this, slot, graph] { return FetchHash(slot, graph); }).share();
+    return true;
+  }
+
+  uint64_t FetchHash(size_t slot, size_t graph) {
+    std::lock_guard<std::mutex> lk(hash_mu);
+    if (!hashes_fetched) {
+      ctx->Activate();
+      CheckHip(hipStreamSynchronize(hash_stream), "hipStreamSynchronize");
+      h_hashes.resize(hash_slots * hosts.size());
+      d_hashes.Download(h_hashes.data(), h_hashes.size(), hash_stream);
+      CheckHip(hipStreamSynchronize(hash_stream), "hipStreamSynchronize");
+      hashes_fetched = true;
+    }
+    return h_hashes.at(slot * hosts.size() + graph);
+  }
 
   void DropGraphs() {
     for (int p = 0; p < 2; ++p) {
@@ -112,6 +169,20 @@ struct Fleet {
     d_mode.Upload(mode.data(), mode.size(), ctx->stream);
     d_xy.Upload(xy.data(), xy.size(), ctx->stream);
     ctx->Sync();
+    // one hash segment per graph, each continuing from its own prefix
+    std::vector<int32_t> node_offset(count);
+    for (uint32_t i = 0; i < count; ++i) node_offset[i] = static_cast<int32_t>(node_base[i]);
+    dhash.Allocate(edge_base, dev, ctx->stream, node_offset);
+    std::vector<uint64_t> prefixes(count);
+    for (uint32_t i = 0; i < count; ++i) prefixes[i] = hosts[i].prefix_hash();
+    dhash.SetPrefixes(prefixes.data(), ctx->stream);
+    CheckHip(hipStreamCreateWithFlags(&hash_stream, hipStreamNonBlocking), "hipStreamCreate");
+    CheckHip(hipEventCreateWithFlags(&ev_flags, hipEventDisableTiming), "hipEventCreate");
+    CheckHip(hipEventCreateWithFlags(&ev_hashed, hipEventDisableTiming), "hipEventCreate");
+    for (uint32_t i = 0; i < count; ++i) {
+      hosts[i].async_hash = true;  // a generation the device does not hash: on a helper thread
+      hosts[i].hash_source = [this, i](std::shared_future<uint64_t>* job) { return HashOfNewGeneration(i, job); };
+    }
   }
 
   // The union's active flags and CSR from the host snapshots (first epoch).
@@ -205,6 +276,8 @@ struct Fleet {
       EnqueueLaunch();
       return;
     }
+    // the last snapshot's hash reads the flags this commit changes
+    CheckHip(hipStreamWaitEvent(ctx->stream, ev_hashed, 0), "stream wait");
     csr.UploadCommits(ctx->stream);
     CheckHip(hipEventRecord(ctx->ev_begin, ctx->stream), "event record");
     if (!use_graph) {
@@ -236,15 +309,24 @@ struct Fleet {
     st.instances = static_cast<uint32_t>(hosts.size());
     st.edge_slots = slots;
     retired.assign(hosts.size(), 0);
-    // the graphs' host runs are independent: each phase runs over the host cores
-    ParallelFor(hosts.size(), [&](size_t i) {
-      hosts[i].StartRun();
-      hosts[i].BeginEpoch(0);
-    });
+    // the graphs' host runs are independent: each phase runs over the host
+    // cores; the previous run's hashes are settled first
+    ParallelFor(hosts.size(), [&](size_t i) { hosts[i].StartRun(); });
+    CheckHip(hipStreamSynchronize(hash_stream), "hipStreamSynchronize");
+    hash_slots = 0;
+    hashes_fetched = false;
+    size_t max_m = 0;
+    for (const JvHost& h : hosts) max_m = std::max(max_m, static_cast<size_t>(h.m));
+    const size_t hash_capacity = (std::min<size_t>(max_rounds, max_m) + 1) * hosts.size();
+    if (d_hashes.count() < hash_capacity) d_hashes.Allocate(hash_capacity, ctx->device);
     std::vector<const std::vector<int32_t>*> commits(hosts.size(), nullptr);
     auto t0 = std::chrono::steady_clock::now();
     EnqueueEpoch(true, use_graph, &st);
+    EnqueueHash();
     st.assemble_ms += MsSince(t0);
+    hash_pending = true;  // the uploaded snapshots are generation 0 of every graph
+    ParallelFor(hosts.size(), [&](size_t i) { hosts[i].BeginEpoch(0); });
+    hash_pending = false;
     for (uint32_t epoch = 0;; ++epoch) {
       CheckHip(hipEventRecord(ctx->ev_end, ctx->stream), "event record");
       CheckHip(hipEventSynchronize(ctx->ev_end), "event synchronize");
@@ -276,6 +358,7 @@ struct Fleet {
         t0 = std::chrono::steady_clock::now();
         StageCommits(commits);
         EnqueueEpoch(false, use_graph, &st);
+        EnqueueHash();
         st.assemble_ms += MsSince(t0);
       }
       t1 = std::chrono::steady_clock::now();
@@ -286,9 +369,11 @@ struct Fleet {
         st.host_ms += MsSince(t1);
         break;
       }
+      hash_pending = true;  // a graph that committed takes the newest slot
       ParallelFor(hosts.size(), [&](size_t i) {
         if (!hosts[i].done) hosts[i].BeginEpoch(epoch + 1);
       });
+      hash_pending = false;
       st.host_ms += MsSince(t1);
     }
     const double loop_ms = MsSince(t_total);

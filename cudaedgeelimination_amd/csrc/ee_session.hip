@@ -86,8 +86,7 @@ struct Session {
     if (hash_slots >= d_hashes.count()) throw std::logic_error("JV session: hash slots exhausted");
     CheckHip(hipEventRecord(ev_flags, ctx->stream), "event record");
     CheckHip(hipStreamWaitEvent(hash_stream, ev_flags, 0), "stream wait");
-    dhash.Enqueue(host.prefix_hash(), d_uv.get(), d_weight.get(), csr.active(), d_hashes.get() + hash_slots,
-                  hash_stream);
+    dhash.Enqueue(d_uv.get(), d_weight.get(), csr.active(), d_hashes.get() + hash_slots, hash_stream);
     CheckHip(hipEventRecord(ev_hashed, hash_stream), "event record");
     ++hash_slots;
     hash_pending = true;
@@ -258,7 +257,9 @@ ee_status ee_jv_session_create(ee_ctx* handle, const ee_graph_view* g, ee_jv_ses
     s->d_weight.Upload(h.weight.data(), m, ctx->stream);
     s->d_xy.Upload(xy.data(), n, ctx->stream);
     if (m > 0) {
-      s->dhash.Allocate(m, ctx->device);
+      s->dhash.Allocate({0, m}, ctx->device, ctx->stream);
+      const uint64_t prefix = s->host.prefix_hash();
+      s->dhash.SetPrefixes(&prefix, ctx->stream);
       ee::CheckHip(hipStreamCreateWithFlags(&s->hash_stream, hipStreamNonBlocking), "hipStreamCreate");
       ee::CheckHip(hipEventCreateWithFlags(&s->ev_flags, hipEventDisableTiming), "hipEventCreate");
       ee::CheckHip(hipEventCreateWithFlags(&s->ev_hashed, hipEventDisableTiming), "hipEventCreate");
