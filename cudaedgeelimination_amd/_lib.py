@@ -202,6 +202,16 @@ class ServerStats(C.Structure):
                 ("device_ms", C.c_double)]
 
 
+class QueueConfig(C.Structure):
+    _fields_ = [("task_rows", C.c_uint64), ("cost_cells", C.c_uint64), ("max_nodes", C.c_uint64),
+                ("ring_entries", C.c_uint32), ("workers", C.c_uint32), ("idle_exit_ms", C.c_double)]
+
+
+class QueueStats(C.Structure):
+    _fields_ = [("requests", C.c_uint64), ("cells", C.c_uint64), ("launches", C.c_uint64),
+                ("peak_inflight_cells", C.c_uint64)]
+
+
 P = C.c_void_p
 I32, U32, I64, U64, U8, SZ, DBL = (C.c_int32, C.c_uint32, C.c_int64, C.c_uint64, C.c_uint8,
                                    C.c_size_t, C.c_double)
@@ -259,6 +269,14 @@ PROTOTYPES = {
     "ee_server_submit_many": (C.c_int, [P, U32, P, P]),
     "ee_server_wait_many": (C.c_int, [P, U32, P, DBL, P]),
     "ee_server_stats_get": (C.c_int, [P, C.POINTER(ServerStats)]),
+    "ee_queue_create": (C.c_int, [P, C.POINTER(QueueConfig), C.POINTER(P)]),
+    "ee_queue_destroy": (None, [P]),
+    "ee_queue_add_graph": (C.c_int, [P, P, P, I32, U8, C.POINTER(U32)]),
+    "ee_queue_submit": (C.c_int, [P, U32, U32, P, U64, P, C.POINTER(U64)]),
+    "ee_queue_submit_many": (C.c_int, [P, U32, P, P]),
+    "ee_queue_wait": (C.c_int, [P, U64, DBL, C.POINTER(RequestResult)]),
+    "ee_queue_wait_many": (C.c_int, [P, U32, P, DBL, P]),
+    "ee_queue_stats_get": (C.c_int, [P, C.POINTER(QueueStats)]),
     "ee_jv_find_candidates": (C.c_int, [P, C.POINTER(GraphView), P, C.POINTER(JvUsage)]),
     "ee_jv_clear_cache": (C.c_int, [P]),
     "ee_kopt_build_templates": (C.c_int, [U32, P, U32, C.POINTER(U32), C.POINTER(U64)]),

@@ -181,6 +181,84 @@ void LaunchKoptResident(const KoptLaunch& finalized, const KoptLaunch* device_ar
                         const KoptTuning& tuning, size_t shared_bytes, hipStream_t stream,
                         unsigned long long* stamps = nullptr);
 
+// ------------------------------------------------------- k-opt work queue
+// A persistent kernel serves leaf-cost requests from a ring of entries in
+// coherent host memory.  Workgroup 0 (one wavefront) polls the ring's tail,
+// copies new entries to a device mirror and publishes their count; every
+// other workgroup (one wavefront each) claims tiles of the published entries
+// and, after the last tile of an entry, stores entry + 1 into the entry's
+// completion flag in host memory.  Each tile wave also stores entry + 1
+// into the tile's mark in host memory after its own cells: a wave's stores
+// reach host memory in order, while the flag may arrive before another
+// wave's cells do, so the host takes an entry's cells once its flag and all
+// its tile marks are there.  Every wave ends on its own clock: the
+// poller after `idle_ticks` without new entries, after `life_ticks`, or when
+// the host sets `stop`; the tile waves once the poller has ended and every
+// published tile is claimed.  The host launches the kernel again whenever it
+// publishes into a queue whose kernel has ended.
+//
+// One entry (64 bytes): task rows [task_offset, task_offset + task_count) of
+// the task arena, cells from cost_offset of the cost arena.
+struct KoptQueueEntry {
+  uint64_t seq;          // entry index; ring slot = seq % ring
+  uint64_t task_offset;
+  uint64_t cost_offset;
+  uint64_t coord_base;
+  uint64_t done_target;  // the slot's done counter once this entry's last tile ended
+  uint32_t task_count;
+  uint32_t tiles;
+  uint32_t k;
+  uint32_t node_count;
+  uint16_t distance_type;
+  uint16_t near;
+  uint32_t reserved;
+};
+static_assert(sizeof(KoptQueueEntry) == 64, "a queue entry is 64 bytes");
+// Flag value of an entry the kernel refused (bounds checks): entry + 1 with
+// this bit; the queue is then out of service.
+constexpr uint64_t kQueueRefused = uint64_t{1} << 63;
+
+// Written by the host, read by the poller (coherent host memory).
+struct KoptQueueHost {
+  uint64_t tail;          // entries published
+  uint32_t stop;          // nonzero: the poller ends at its next look
+  uint32_t reserved;
+  uint64_t ended_epoch;   // written by the poller: the launch it ended
+};
+
+// Device memory, persisting across launches.
+struct KoptQueueDevice {
+  uint64_t claim;         // entry << 32 | next tile of that entry
+  uint64_t published;     // entries copied to the mirror
+  uint32_t exit_epoch;    // the launch whose poller has ended
+  uint32_t refused;       // nonzero once an entry failed the bounds checks
+};
+
+struct KoptQueueArgs {
+  KoptQueueHost* host;             // device address of the host control words
+  const KoptQueueEntry* ring;      // device address of the host ring
+  uint64_t* flags;                 // device address of the host completion flags
+  uint64_t* marks;                 // device address of the host tile marks [task_capacity]
+  KoptQueueDevice* dev;
+  KoptQueueEntry* mirror;          // [ring]
+  uint64_t* done;                  // [ring] tiles ended, per slot, cumulative
+  uint64_t task_capacity;          // rows of the task arena
+  uint64_t cost_capacity;          // cells of the cost arena
+  uint64_t coord_count;            // coordinates in the pool
+  uint64_t idle_ticks, life_ticks; // s_memrealtime ticks
+  uint32_t ring_size;
+  uint32_t epoch;                  // this launch, counted from 1
+};
+
+// LDS layout for the queue kernel: one wavefront per workgroup, the plans of
+// all three k side by side.  Returns the dynamic LDS bytes.
+size_t KoptQueueLayout(KoptLaunch* a, const KoptPlan plans[3], const KoptTuning& tuning);
+// Launches 1 + workers one-wavefront workgroups.  `a` carries the layout,
+// the coordinate pool, the plans, the task arena (tasks) and the cost arena
+// (costs).
+void LaunchKoptQueue(const KoptLaunch& a, const KoptQueueArgs& q, uint32_t workers, size_t shared_bytes,
+                     hipStream_t stream);
+
 // ------------------------------------------------------------------- JV
 // Edge slots edge_list[0..edge_count) (or 0..edge_count when edge_list is
 // NULL) are tested; item i's witness goes to witnesses[i].

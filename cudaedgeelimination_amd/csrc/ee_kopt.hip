@@ -101,9 +101,11 @@ __device__ __forceinline__ void StoreCell(int64_t* out, uint32_t at, uint64_t su
 
 // Phase A: the tile's port node ids (into the pair record area, which is
 // free until phase B) and their coordinates, two gathers in flight per lane.
-template <int K, bool kNear, bool kMask>
+// kBounded: node ids at or above node_count read node 0's coordinates (task
+// rows that another agent wrote, whose checks this kernel cannot see).
+template <int K, bool kNear, bool kMask, bool kBounded = false>
 __device__ __forceinline__ void StagePorts(const KoptLaunch& a, const KoptPointers& bufs, const KoptKLayout& L,
-                                           const KoptTile& d, char* wave, int lane) {
+                                           const KoptTile& d, char* wave, int lane, uint32_t node_count = 0) {
   constexpr uint32_t kPorts = 2 * K;
   constexpr uint32_t kBatch = 2;
   char* w_port = wave + L.xy_off;
@@ -120,6 +122,7 @@ __device__ __forceinline__ void StagePorts(const KoptLaunch& a, const KoptPointe
       const uint32_t s = base + i * kWave + static_cast<uint32_t>(lane);
       const uint32_t r = s / kPorts, p = s - r * kPorts;
       node[i] = ports[s < total ? r * 12 + p : 0u];  // a task row is 12 int32 words
+      if (kBounded && static_cast<uint32_t>(node[i]) >= node_count) node[i] = 0;
     }
     int64_t cx[kBatch], cy[kBatch];
 #pragma unroll
@@ -598,9 +601,262 @@ __global__ void __launch_bounds__(256) KoptKernelResident(const KoptLaunch* __re
   KoptBody<kNonTemporal, kMask>(*ap, bufs, smem);
 }
 
+// ------------------------------------------------------------ work queue
+// Control words are read with relaxed atomic loads: those are never served
+// from the scalar cache, and at agent or system scope they miss the
+// non-coherent caches, so a poll sees the other side's latest store.
+__device__ __forceinline__ uint64_t PollAgent(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t PollAgent(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t PollSystem(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ uint32_t PollSystem(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Copies entries [seen, tail) of the host ring to the mirror, then
+// publishes tail to the tile waves.
+__device__ __forceinline__ void PublishEntries(const KoptQueueArgs& q, int lane, uint64_t seen, uint64_t tail) {
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // the host's entries and task rows
+  const uint64_t words = (tail - seen) * 8;
+  for (uint64_t w = static_cast<uint64_t>(lane); w < words; w += kWave) {
+    const uint64_t slot = (seen + w / 8) % q.ring_size;
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(q.ring + slot) + w % 8;
+    uint64_t* dst = reinterpret_cast<uint64_t*>(q.mirror + slot) + w % 8;
+    __hip_atomic_store(dst, PollSystem(src), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  if (lane == 0) __hip_atomic_store(&q.dev->published, tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Workgroup 0: copies entries the host has published to the device mirror
+// and publishes their count, until the host sets stop, no entry arrives for
+// idle_ticks, or the launch has lived life_ticks.
+__device__ void QueuePoller(const KoptQueueArgs& q, int lane, uint64_t start) {
+  uint64_t seen = Uniform64(PollAgent(&q.dev->published));
+  uint64_t last = start;
+  for (;;) {
+    const uint64_t tail = Uniform64(PollSystem(&q.host->tail));
+    if (tail != seen) {
+      if (tail < seen || tail - seen > q.ring_size) {  // never published so by the host
+        if (lane == 0) __hip_atomic_store(&q.dev->refused, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      PublishEntries(q, lane, seen, tail);
+      seen = tail;
+      last = __builtin_amdgcn_s_memrealtime();
+    }
+    const uint64_t now = __builtin_amdgcn_s_memrealtime();
+    const bool expired = now - start > q.life_ticks;
+    const bool idle = now - last > q.idle_ticks;
+    const uint32_t stop = idle || expired ? 0u : __builtin_amdgcn_readfirstlane(PollSystem(&q.host->stop));
+    if (stop != 0 || expired || idle) {
+      // Announce the end, then look at the tail once more.  The host stores
+      // the tail before it reads the announcement, so an entry it published
+      // without seeing the announcement is seen here; one it published later
+      // goes to the launch it then starts.
+      if (lane == 0) {
+        __hip_atomic_store(&q.host->ended_epoch, static_cast<uint64_t>(q.epoch), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "");
+      const uint64_t again = Uniform64(PollSystem(&q.host->tail));
+      if (again != seen && again > seen && again - seen <= q.ring_size) {
+        if (idle && !expired && stop == 0) {
+          // new work ends the idle spell: withdraw the announcement
+          if (lane == 0) {
+            __hip_atomic_store(&q.host->ended_epoch, uint64_t{0}, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+          }
+          continue;
+        }
+        PublishEntries(q, lane, seen, again);  // the last entries of this launch
+      }
+      break;
+    }
+    if (tail == seen) __builtin_amdgcn_s_sleep(1);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  if (lane == 0) __hip_atomic_store(&q.dev->exit_epoch, q.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Entry `entry` from the device mirror, every field uniform.
+__device__ __forceinline__ KoptQueueEntry MirrorEntry(const KoptQueueArgs& q, uint64_t entry) {
+  const uint64_t* src = reinterpret_cast<const uint64_t*>(q.mirror + entry % q.ring_size);
+  uint64_t w[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) w[i] = Uniform64(PollAgent(src + i));
+  KoptQueueEntry e;
+  __builtin_memcpy(&e, w, sizeof(e));
+  return e;
+}
+
+// The entry's fields stay inside the arenas and the coordinate pool, so a
+// tile of it reads and writes in bounds whatever its task rows hold.
+__device__ __forceinline__ bool EntryInBounds(const KoptLaunch& a, const KoptQueueArgs& q, const KoptQueueEntry& e) {
+  if (e.k < 3 || e.k > 5 || e.task_count == 0 || e.node_count == 0) return false;
+  const KoptKLayout& L = a.layout[e.k - 3];
+  return e.task_offset <= q.task_capacity && e.task_count <= q.task_capacity - e.task_offset &&
+         e.cost_offset <= q.cost_capacity &&
+         static_cast<uint64_t>(e.task_count) * L.T <= q.cost_capacity - e.cost_offset &&
+         e.coord_base <= q.coord_count && e.node_count <= q.coord_count - e.coord_base &&
+         e.tiles == (e.task_count + L.rows - 1) / L.rows;
+}
+
+template <int K>
+__device__ __forceinline__ void QueueTile(const KoptLaunch& a, const KoptPointers& bufs, const KoptQueueEntry& e,
+                                          uint32_t tile, const char* smem, char* wave, int lane) {
+  const KoptKLayout& L = a.layout[K - 3];
+  const uint32_t row0 = tile * L.rows;
+  KoptTile d;
+  d.task_index = e.task_offset + row0;
+  d.cost_index = e.cost_offset + static_cast<uint64_t>(row0) * L.T;
+  d.coord_base = e.coord_base;
+  d.rows = e.task_count - row0 < L.rows ? e.task_count - row0 : L.rows;
+  d.distance_type = e.distance_type;
+  d.near = e.near;
+  if (d.near != 0) {
+    StagePorts<K, true, false, true>(a, bufs, L, d, wave, lane, e.node_count);
+    TileCells<K, true, false, false>(a, bufs, L, d, smem, wave, lane);
+  } else {
+    StagePorts<K, false, false, true>(a, bufs, L, d, wave, lane, e.node_count);
+    TileCells<K, false, false, false>(a, bufs, L, d, smem, wave, lane);
+  }
+}
+
+// One wavefront per workgroup.  Workgroup 0 polls the host ring; the others
+// claim tiles (one compare-and-swap on the claim word per tile, or per entry
+// to step past it) and run them with the batched kernel's tile code.
+__global__ void __launch_bounds__(kWave) KoptQueueKernel(KoptLaunch a, KoptQueueArgs q) {
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  const int lane = static_cast<int>(threadIdx.x);
+  const uint64_t start = __builtin_amdgcn_s_memrealtime();
+  if (blockIdx.x == 0) {
+    QueuePoller(q, lane, start);
+    return;
+  }
+  for (int i = 0; i < 3; ++i) {
+    const KoptKLayout& L = a.layout[i];
+    uint16_t* s_pq = reinterpret_cast<uint16_t*>(smem + L.pq_off);
+    uint64_t* s_tpl = reinterpret_cast<uint64_t*>(smem + L.tpl_off);
+    for (uint32_t t = lane; t < L.T; t += kWave) s_tpl[t] = a.plans[i].tpl[t];
+    for (uint32_t t = lane; t < L.P; t += kWave) s_pq[t] = a.plans[i].pair_pq[t];
+  }
+  __syncthreads();
+  const KoptPointers bufs{a.xy, nullptr, a.tasks, a.plans, a.costs, nullptr, nullptr};
+  char* const wave = smem + a.wave_base;
+  // the claim word's entry as last seen: while published stays at or below
+  // it there is nothing to claim, and only published is polled
+  uint64_t known = 0;
+  uint32_t idle = 0;
+  for (;;) {
+    const uint64_t published = Uniform64(PollAgent(&q.dev->published));
+    if (published > known) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the mirror's entries
+      const uint64_t claim = Uniform64(PollAgent(&q.dev->claim));
+      const uint64_t entry = claim >> 32;
+      const uint32_t tile = static_cast<uint32_t>(claim);
+      known = entry;
+      if (entry >= published) continue;
+      idle = 0;
+      const KoptQueueEntry e = MirrorEntry(q, entry);
+      const bool in_bounds = EntryInBounds(a, q, e);
+      const bool past = e.seq != entry || (in_bounds && tile >= e.tiles);
+      const uint64_t next = past || !in_bounds ? (entry + 1) << 32 : claim + 1;
+      uint32_t won = 0;
+      if (lane == 0) {
+        uint64_t expected = claim;
+        won = __hip_atomic_compare_exchange_strong(&q.dev->claim, &expected, next, __ATOMIC_RELAXED,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                  ? 1u
+                  : 0u;
+      }
+      won = __builtin_amdgcn_readfirstlane(won);
+      if (won == 0 || past) continue;
+      const uint64_t slot = entry % q.ring_size;
+      if (!in_bounds) {
+        if (lane == 0) {
+          __hip_atomic_store(&q.dev->refused, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(&q.flags[slot], (entry + 1) | kQueueRefused, __ATOMIC_RELEASE,
+                             __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        continue;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // the host's task rows
+      if (e.k == 3) {
+        QueueTile<3>(a, bufs, e, tile, smem, wave, lane);
+      } else if (e.k == 4) {
+        QueueTile<4>(a, bufs, e, tile, smem, wave, lane);
+      } else {
+        QueueTile<5>(a, bufs, e, tile, smem, wave, lane);
+      }
+      // the tile's cells, then its mark (the same wave's stores reach host
+      // memory in order), then its count
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+      if (lane == 0) {
+        __hip_atomic_store(&q.marks[e.task_offset + tile], entry + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+      uint64_t before = 0;
+      if (lane == 0) before = __hip_atomic_fetch_add(&q.done[slot], uint64_t{1}, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      before = Uniform64(before);
+      if (before + 1 == e.done_target) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+        if (lane == 0) __hip_atomic_store(&q.flags[slot], entry + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
+      continue;
+    }
+    if (__builtin_amdgcn_readfirstlane(PollAgent(&q.dev->exit_epoch)) == q.epoch) {
+      // the poller has ended: published no longer grows; leave once it is drained
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      if (Uniform64(PollAgent(&q.dev->published)) > known) continue;
+      break;
+    }
+    // a poller that never ran ends nothing: leave on the launch's own clock
+    if (__builtin_amdgcn_s_memrealtime() - start > 2 * q.life_ticks + q.idle_ticks) break;
+    // nap briefly while work came recently, longer once idle for a while
+    if (idle < 256) {
+      ++idle;
+      __builtin_amdgcn_s_sleep(2);
+    } else {
+      __builtin_amdgcn_s_sleep(16);
+    }
+  }
+}
+
 uint32_t Align16(uint32_t v) { return (v + 15u) & ~15u; }
 
 }  // namespace
+
+size_t KoptQueueLayout(KoptLaunch* a, const KoptPlan plans[3], const KoptTuning& tuning) {
+  KoptTuning t = tuning;
+  t.block = kWave;
+  const bool present[3] = {true, true, true};
+  a->masks = nullptr;
+  a->mask_words = 0;
+  KoptPrepareLayout(a, plans, present, t, false);
+  // one wavefront holds the plans of all three k
+  uint32_t off = 0;
+  for (int i = 0; i < 3; ++i) {
+    KoptKLayout& L = a->layout[i];
+    L.pq_off = off;
+    L.tpl_off = off + Align16(2 * kMaxPairs);
+    off = Align16(L.tpl_off + L.T * 8);
+  }
+  a->wave_base = off;
+  return static_cast<size_t>(off) + a->wave_bytes;
+}
+
+void LaunchKoptQueue(const KoptLaunch& a, const KoptQueueArgs& q, uint32_t workers, size_t shared_bytes,
+                     hipStream_t stream) {
+  if (workers == 0 || q.ring_size == 0 || q.epoch == 0) throw std::invalid_argument("bad k-opt queue launch");
+  if (shared_bytes > 65536) throw std::invalid_argument("k-opt queue launch needs more than 64 KiB of LDS");
+  hipLaunchKernelGGL(KoptQueueKernel, dim3(workers + 1), dim3(kWave), shared_bytes, stream, a, q);
+  CheckHip(hipGetLastError(), "k-opt queue kernel launch");
+}
 
 size_t KoptPrepareLayout(KoptLaunch* a, const KoptPlan plans[3], const bool present[3],
                          const KoptTuning& tuning, bool near_only) {

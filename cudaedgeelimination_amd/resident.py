@@ -412,6 +412,98 @@ class KoptServer:
                 "device_ms": st.device_ms}
 
 
+class DeviceQueue:
+    """Leaf-cost queries through the device work queue (ee_queue_* in
+    ee_hip.h): a persistent kernel polls a request ring in coherent host
+    memory and raises a completion flag per request; no launch or copy per
+    request.  Same submit / wait surface as KoptServer (leaf-cost queries
+    only)."""
+
+    def __init__(self, ctx: Context, task_rows: int = 1 << 18, cost_cells: int = 1 << 23,
+                 max_nodes: int = 1 << 20, ring_entries: int = 4096, workers: int = 256,
+                 idle_exit_ms: float = 20.0):
+        cfg = _lib.QueueConfig()
+        cfg.task_rows, cfg.cost_cells, cfg.max_nodes = task_rows, cost_cells, max_nodes
+        cfg.ring_entries, cfg.workers, cfg.idle_exit_ms = ring_entries, workers, idle_exit_ms
+        handle = C.c_void_p()
+        _lib.check(_lib.lib().ee_queue_create(ctx.handle, C.byref(cfg), C.byref(handle)))
+        self._h = handle
+        self.ctx = ctx
+        ctx.adopt(self)
+        self._buffers: dict[int, tuple[np.ndarray, np.ndarray]] = {}
+
+    def close(self):
+        if self._h is not None:
+            _lib.lib().ee_queue_destroy(self._h)
+            self._h = None
+            self._buffers.clear()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_graph(self, graph: GraphSnapshot) -> int:
+        gid = C.c_uint32()
+        _lib.check(_lib.lib().ee_queue_add_graph(self._h, _lib.ptr(graph.x), _lib.ptr(graph.y),
+                                                 graph.dimension, graph.distance_type, C.byref(gid)))
+        return gid.value
+
+    def submit(self, graph: int, k: int, tasks: np.ndarray, out: np.ndarray | None = None) -> int:
+        tasks = np.ascontiguousarray(tasks, dtype=_lib.KOPT_TASK_DTYPE)
+        if out is None:
+            out = np.empty(len(tasks) * TEMPLATES_PER_K[k], np.int64)
+        rid = C.c_uint64()
+        _lib.check(_lib.lib().ee_queue_submit(self._h, graph, k, _lib.ptr(tasks), len(tasks),
+                                              _lib.ptr(out), C.byref(rid)))
+        self._buffers[rid.value] = (tasks, out)
+        return rid.value
+
+    def wait(self, request_id: int, timeout_ms: float = -1.0):
+        """(costs, result dict) of a completed request."""
+        res = _lib.RequestResult()
+        status = _lib.lib().ee_queue_wait(self._h, request_id, timeout_ms, C.byref(res))
+        _, out = self._buffers.pop(request_id, (None, None))
+        _lib.check(status)
+        return out, {"batch_size": res.batch_size, "batch_id": res.batch_id,
+                     "queue_ms": res.queue_ms, "service_ms": res.service_ms,
+                     "latency_ms": res.latency_ms}
+
+    def submit_many(self, requests) -> list[int]:
+        """Queues (graph, k, task rows) requests in order; the inputs are
+        copied into the task arena, the outputs kept until wait_many()."""
+        descs = (_lib.ServerRequest * max(len(requests), 1))()
+        keep = []
+        for i, (graph, k, data) in enumerate(requests):
+            data = np.ascontiguousarray(data, dtype=_lib.KOPT_TASK_DTYPE)
+            out = np.empty(len(data) * TEMPLATES_PER_K[k], np.int64)
+            descs[i].kind, descs[i].graph, descs[i].k = 0, graph, k
+            descs[i].input, descs[i].count, descs[i].output = _lib.ptr(data), len(data), _lib.ptr(out)
+            keep.append((data, out))
+        ids = np.zeros(max(len(requests), 1), np.uint64)
+        _lib.check(_lib.lib().ee_queue_submit_many(self._h, len(requests), descs, _lib.ptr(ids)))
+        ids = [int(v) for v in ids[:len(requests)]]
+        for rid, bufs in zip(ids, keep):
+            self._buffers[rid] = bufs
+        return ids
+
+    def wait_many(self, request_ids, timeout_ms: float = -1.0):
+        """[(costs, latency_ms)] of the requests, in the given order."""
+        ids = np.ascontiguousarray(request_ids, dtype=np.uint64)
+        res = (_lib.RequestResult * max(len(ids), 1))()
+        status = _lib.lib().ee_queue_wait_many(self._h, len(ids), _lib.ptr(ids), timeout_ms, res)
+        outs = [self._buffers.pop(int(rid), (None, None))[1] for rid in ids]
+        _lib.check(status)
+        return [(outs[i], res[i].latency_ms) for i in range(len(ids))]
+
+    def stats(self) -> dict:
+        st = _lib.QueueStats()
+        _lib.check(_lib.lib().ee_queue_stats_get(self._h, C.byref(st)))
+        return {"requests": st.requests, "cells": st.cells, "launches": st.launches,
+                "peak_inflight_cells": st.peak_inflight_cells}
+
+
 @dataclasses.dataclass
 class LeafBlock:
     """One certified block of ee_leaf_next (copies of the pipeline's views)."""
