@@ -29,7 +29,7 @@ sys.path.insert(0, str(ROOT / "tests"))
 import numpy as np  # noqa: E402
 
 from cudaedgeelimination_amd import backend, workloads as W  # noqa: E402
-from cudaedgeelimination_amd.resident import JvSession, KoptBatch, KoptServer, LeafPipeline  # noqa: E402
+from cudaedgeelimination_amd.resident import DeviceQueue, JvSession, KoptBatch, KoptServer, LeafPipeline  # noqa: E402
 
 METRIC = "exact k-opt cost cells/s + JV edges/s; p50/p99 query latency vs plain GPU path"
 # streams (and cost pools) the timed k-opt launches are dealt over
@@ -238,21 +238,30 @@ def many_instance_leg(ctx, cpu: bool, sample: int = 16):
     return out
 
 
-def serve_stream(ctx, graphs, pools, requests, is_jv, arrivals, mode: str, burst: bool):
-    """One request stream against a fresh server of `mode` ("batched" or
-    "plain"): latency from each request's scheduled arrival to completion."""
+def serve_stream(ctx, graphs, pools, requests, is_jv, arrivals, mode: str, burst: bool, keep=False):
+    """One request stream against a fresh server of `mode` ("batched",
+    "plain", or "queue": the device work queue, leaf-cost queries only):
+    latency from each request's scheduled arrival to completion.  keep: also
+    return the burst's results (key "outputs")."""
     import threading
 
     count = len(requests)
     kopt_cells = sum(len(t) * TEMPLATES[k] for _, k, t in requests if k)
-    server = KoptServer(ctx, batching=(mode == "batched"), hbm_budget_bytes=2 << 30,
-                        max_batch_cells=1 << 21)
+    if mode == "queue":
+        server = DeviceQueue(ctx, task_rows=1 << 20, cost_cells=1 << 25, max_nodes=1 << 16,
+                             ring_entries=1 << 15)
+        jids = None
+    else:
+        server = KoptServer(ctx, batching=(mode == "batched"), hbm_budget_bytes=2 << 30,
+                            max_batch_cells=1 << 21)
     gids = [server.add_graph(g) for g in graphs]
-    jids = [server.add_jv_graph(g) for g in graphs]
+    if mode != "queue":
+        jids = [server.add_jv_graph(g) for g in graphs]
     warm = server.submit(gids[0], 3, pools[0][3][:8])
     server.wait(warm)
     latencies = np.zeros(count)
     late = np.zeros(count)
+    outputs = None
     if burst:
         # the burst arrives as one message: queued in one call
         t0 = time.perf_counter()
@@ -261,6 +270,8 @@ def serve_stream(ctx, graphs, pools, requests, is_jv, arrivals, mode: str, burst
         done = server.wait_many(ids)
         wall = time.perf_counter() - t0
         latencies[:] = [lat for _, lat in done]
+        if keep:
+            outputs = [out for out, _ in done]
     else:
         ids = [0] * count
         submitted = threading.Semaphore(0)
@@ -289,19 +300,24 @@ def serve_stream(ctx, graphs, pools, requests, is_jv, arrivals, mode: str, burst
         wall = time.perf_counter() - t0
     stats = server.stats()
     server.close()
-    return {
+    out = {
         "p50_ms": float(np.percentile(latencies, 50)),
         "p99_ms": float(np.percentile(latencies, 99)),
         "mean_ms": float(latencies.mean()),
         "jv_p99_ms": float(np.percentile(latencies[is_jv], 99)) if is_jv.any() else None,
         "cells_per_s": kopt_cells / wall,
-        "jv_edges_per_s": stats["jv_edges"] / wall,
-        "launches": stats["batches"],
-        "device_cells_per_s": stats["cells"] / (stats["device_ms"] * 1e-3) if stats["device_ms"] > 0 else None,
+        "jv_edges_per_s": stats.get("jv_edges", 0) / wall,
+        # the queue launches once per busy period, the servers once per batch
+        "launches": stats["launches"] if mode == "queue" else stats["batches"],
+        "device_cells_per_s": (stats["cells"] / (stats["device_ms"] * 1e-3)
+                               if stats.get("device_ms", 0) > 0 else None),
         "wall_s": wall,
         # how far behind schedule the client submitted (its own limit)
         "client_late_p50_ms": float(np.percentile(late, 50)) if not burst else None,
     }
+    if outputs is not None:
+        out["outputs"] = outputs
+    return out
 
 
 def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005, jv_share: float = 0.2,
@@ -322,7 +338,7 @@ def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005, jv_share:
     pools = W.config5_row_pools(graphs, seed)
     out = {"graphs": [g.dimension for g in graphs], "points": []}
 
-    def stream(qps, count_seconds):
+    def stream(qps, count_seconds, share):
         rng = np.random.default_rng(seed + qps)
         if qps == 0:
             # burst: every request is due at once, so the wall time measures
@@ -333,26 +349,39 @@ def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005, jv_share:
             # at least 10,000 requests, so that p99 rests on 100 of them
             count = max(int(qps * count_seconds), 10000)
             arrivals = np.cumsum(rng.exponential(1.0 / qps, count))
-        requests, is_jv = W.config5_requests(graphs, pools, count, rng, jv_share)
+        requests, is_jv = W.config5_requests(graphs, pools, count, rng, share)
         return requests, is_jv, arrivals
 
-    for qps in qps_list:
-        requests, is_jv, arrivals = stream(qps, seconds)
-        kopt_cells = [len(t) * TEMPLATES[k] for _, k, t in requests if k]
-        point = {"qps": qps if qps else "burst", "requests": len(requests), "jv_rechecks": int(is_jv.sum()),
-                 "mean_cells": float(np.mean(kopt_cells)),
-                 "mean_jv_edges": float(np.mean([len(t) for _, k, t in requests if not k] or [0]))}
-        for mode in ("batched", "plain"):
-            point[mode] = serve_stream(ctx, graphs, pools, requests, is_jv, arrivals, mode, qps == 0)
-        point["batched_over_plain_cells_per_s"] = point["batched"]["cells_per_s"] / point["plain"]["cells_per_s"]
-        out["points"].append(point)
+    def points(share, modes):
+        pts = []
+        for qps in qps_list:
+            requests, is_jv, arrivals = stream(qps, seconds, share)
+            kopt_cells = [len(t) * TEMPLATES[k] for _, k, t in requests if k]
+            point = {"qps": qps if qps else "burst", "requests": len(requests), "jv_rechecks": int(is_jv.sum()),
+                     "mean_cells": float(np.mean(kopt_cells)),
+                     "mean_jv_edges": float(np.mean([len(t) for _, k, t in requests if not k] or [0]))}
+            for mode in modes:
+                point[mode] = serve_stream(ctx, graphs, pools, requests, is_jv, arrivals, mode, qps == 0,
+                                           keep=(qps == 0))
+            if qps == 0:
+                # every path's burst results against the first path's
+                first = point[modes[0]].pop("outputs")
+                same = True
+                for mode in modes[1:]:
+                    same = same and all(np.array_equal(a, b) for a, b in zip(first, point[mode].pop("outputs")))
+                point["results_equal_across_paths"] = bool(same)
+            for mode in modes:
+                if mode != "plain":
+                    point[f"{mode}_over_plain_cells_per_s"] = point[mode]["cells_per_s"] / point["plain"]["cells_per_s"]
+            pts.append(point)
+        return pts
 
-    if saturation_rates:
+    def saturation(share, modes):
         sat = {"p99_bound_ms": saturation_p99_ms, "rates": list(saturation_rates)}
-        for mode in ("batched", "plain"):
+        for mode in modes:
             held, tried = None, []
             for qps in saturation_rates:
-                requests, is_jv, arrivals = stream(qps, seconds)
+                requests, is_jv, arrivals = stream(qps, seconds, share)
                 r = serve_stream(ctx, graphs, pools, requests, is_jv, arrivals, mode, False)
                 tried.append({"qps": qps, "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
                               "client_late_p50_ms": r["client_late_p50_ms"], "launches": r["launches"]})
@@ -363,9 +392,19 @@ def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005, jv_share:
             sat[mode] = {"saturation_qps": held, "points": tried,
                          # the last rate held with nothing left to try: a lower bound
                          "at_last_rate": held == saturation_rates[-1]}
-        if sat["batched"]["saturation_qps"] and sat["plain"]["saturation_qps"]:
-            sat["batched_over_plain_qps"] = sat["batched"]["saturation_qps"] / sat["plain"]["saturation_qps"]
-        out["saturation"] = sat
+        for mode in modes:
+            if mode != "plain" and sat[mode]["saturation_qps"] and sat["plain"]["saturation_qps"]:
+                sat[f"{mode}_over_plain_qps"] = sat[mode]["saturation_qps"] / sat["plain"]["saturation_qps"]
+        return sat
+
+    out["points"] = points(jv_share, ("batched", "plain"))
+    if saturation_rates:
+        out["saturation"] = saturation(jv_share, ("batched", "plain"))
+    # leaf-cost queries alone, which the device work queue also serves
+    lo = {"jv_share": 0.0, "points": points(0.0, ("queue", "batched", "plain"))}
+    if saturation_rates:
+        lo["saturation"] = saturation(0.0, ("queue", "batched", "plain"))
+    out["leaf_only"] = lo
     return out
 
 

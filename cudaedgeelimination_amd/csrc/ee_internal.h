@@ -184,18 +184,22 @@ void LaunchKoptResident(const KoptLaunch& finalized, const KoptLaunch* device_ar
 // ------------------------------------------------------- k-opt work queue
 // A persistent kernel serves leaf-cost requests from a ring of entries in
 // coherent host memory.  Workgroup 0 (one wavefront) polls the ring's tail,
-// copies new entries to a device mirror and publishes their count; every
-// other workgroup (one wavefront each) claims tiles of the published entries
-// and, after the last tile of an entry, stores entry + 1 into the entry's
-// completion flag in host memory.  Each tile wave also stores entry + 1
-// into the tile's mark in host memory after its own cells: a wave's stores
+// copies new entries to a device mirror and writes one ticket per tile,
+// (entry + 1) << 24 | tile, into a device ticket ring.  Every other
+// workgroup (one wavefront each) takes the next ticket index with one atomic
+// add, polls that ticket's own slot until the poller fills it, clears it and
+// runs the tile; so waves never retry a claim and idle waves poll distinct
+// addresses.  After its cells each tile wave stores entry + 1 into the
+// tile's mark in host memory, and the wave that ends an entry's last tile
+// stores entry + 1 into the entry's completion flag there.  A wave's stores
 // reach host memory in order, while the flag may arrive before another
 // wave's cells do, so the host takes an entry's cells once its flag and all
-// its tile marks are there.  Every wave ends on its own clock: the
-// poller after `idle_ticks` without new entries, after `life_ticks`, or when
-// the host sets `stop`; the tile waves once the poller has ended and every
-// published tile is claimed.  The host launches the kernel again whenever it
-// publishes into a queue whose kernel has ended.
+// its tile marks are there.  Every wave ends on its own clock: the poller
+// after `idle_ticks` without new entries, after `life_ticks`, or when the
+// host sets `stop`; a tile wave once the poller has ended with its ticket
+// unfilled.  The host launches the kernel again whenever it publishes into a
+// queue whose kernel has ended; the new poller first rewinds the ticket
+// index past the tickets the ended launch handed out but never filled.
 //
 // One entry (64 bytes): task rows [task_offset, task_offset + task_count) of
 // the task arena, cells from cost_offset of the cost arena.
@@ -228,11 +232,16 @@ struct KoptQueueHost {
 
 // Device memory, persisting across launches.
 struct KoptQueueDevice {
-  uint64_t claim;         // entry << 32 | next tile of that entry
+  uint64_t taken;         // ticket indices handed to tile waves
+  uint64_t written;       // tickets the poller has written
   uint64_t published;     // entries copied to the mirror
   uint32_t exit_epoch;    // the launch whose poller has ended
+  uint32_t open_epoch;    // the launch whose poller has rewound `taken`
   uint32_t refused;       // nonzero once an entry failed the bounds checks
+  uint32_t reserved;
 };
+// Tiles of one entry and entries are numbered in a ticket's 24 and 40 bits.
+constexpr uint64_t kQueueMaxTiles = (uint64_t{1} << 24) - 1;
 
 struct KoptQueueArgs {
   KoptQueueHost* host;             // device address of the host control words
@@ -242,9 +251,11 @@ struct KoptQueueArgs {
   KoptQueueDevice* dev;
   KoptQueueEntry* mirror;          // [ring]
   uint64_t* done;                  // [ring] tiles ended, per slot, cumulative
+  uint64_t* tickets;               // [ticket_capacity], 0 = empty
   uint64_t task_capacity;          // rows of the task arena
   uint64_t cost_capacity;          // cells of the cost arena
   uint64_t coord_count;            // coordinates in the pool
+  uint64_t ticket_capacity;        // at least the task arena's rows: every tile has a row
   uint64_t idle_ticks, life_ticks; // s_memrealtime ticks
   uint32_t ring_size;
   uint32_t epoch;                  // this launch, counted from 1

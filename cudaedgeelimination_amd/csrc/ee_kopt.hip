@@ -617,33 +617,104 @@ __device__ __forceinline__ uint64_t PollSystem(const uint64_t* p) {
 __device__ __forceinline__ uint32_t PollSystem(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
-
-// Copies entries [seen, tail) of the host ring to the mirror, then
-// publishes tail to the tile waves.
-__device__ __forceinline__ void PublishEntries(const KoptQueueArgs& q, int lane, uint64_t seen, uint64_t tail) {
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // the host's entries and task rows
-  const uint64_t words = (tail - seen) * 8;
-  for (uint64_t w = static_cast<uint64_t>(lane); w < words; w += kWave) {
-    const uint64_t slot = (seen + w / 8) % q.ring_size;
-    const uint64_t* src = reinterpret_cast<const uint64_t*>(q.ring + slot) + w % 8;
-    uint64_t* dst = reinterpret_cast<uint64_t*>(q.mirror + slot) + w % 8;
-    __hip_atomic_store(dst, PollSystem(src), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  if (lane == 0) __hip_atomic_store(&q.dev->published, tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void StoreAgent(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void StoreAgent(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Workgroup 0: copies entries the host has published to the device mirror
-// and publishes their count, until the host sets stop, no entry arrives for
-// idle_ticks, or the launch has lived life_ticks.
+// Entry `entry` from the device mirror, every field uniform.
+__device__ __forceinline__ KoptQueueEntry MirrorEntry(const KoptQueueArgs& q, uint64_t entry) {
+  const uint64_t* src = reinterpret_cast<const uint64_t*>(q.mirror + entry % q.ring_size);
+  uint64_t w[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) w[i] = Uniform64(PollAgent(src + i));
+  KoptQueueEntry e;
+  __builtin_memcpy(&e, w, sizeof(e));
+  return e;
+}
+
+// The entry's fields stay inside the arenas and the coordinate pool, so a
+// tile of it reads and writes in bounds whatever its task rows hold.
+__device__ __forceinline__ bool EntryInBounds(const KoptLaunch& a, const KoptQueueArgs& q, const KoptQueueEntry& e) {
+  if (e.k < 3 || e.k > 5 || e.task_count == 0 || e.node_count == 0) return false;
+  const KoptKLayout& L = a.layout[e.k - 3];
+  return e.task_offset <= q.task_capacity && e.task_count <= q.task_capacity - e.task_offset &&
+         e.cost_offset <= q.cost_capacity &&
+         static_cast<uint64_t>(e.task_count) * L.T <= q.cost_capacity - e.cost_offset &&
+         e.coord_base <= q.coord_count && e.node_count <= q.coord_count - e.coord_base &&
+         e.tiles == (e.task_count + L.rows - 1) / L.rows && e.tiles <= kQueueMaxTiles;
+}
+
+// Refuses an entry: its flag says so and the queue goes out of service.
+__device__ __forceinline__ void Refuse(const KoptQueueArgs& q, uint64_t entry, int lane) {
+  if (lane == 0) {
+    StoreAgent(&q.dev->refused, 1u);
+    __hip_atomic_store(&q.flags[entry % q.ring_size], (entry + 1) | kQueueRefused, __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+// Copies entries [seen, tail) of the host ring to the mirror, eight at a
+// time (lane = 8 x entry + word), writes their tiles' tickets and publishes
+// both.  An entry whose sequence number or tile count is off is refused
+// here; the tile waves check the rest of it.
+__device__ void PublishEntries(const KoptQueueArgs& q, int lane, uint64_t seen, uint64_t tail) {
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // the host's entries and task rows
+  uint64_t written = Uniform64(PollAgent(&q.dev->written));
+  const uint32_t word = static_cast<uint32_t>(lane) & 7u, which = static_cast<uint32_t>(lane) >> 3;
+  for (uint64_t base = seen; base < tail; base += 8) {
+    const uint32_t count = tail - base < 8 ? static_cast<uint32_t>(tail - base) : 8u;
+    uint64_t v = 0;
+    if (which < count) {
+      const uint64_t slot = (base + which) % q.ring_size;
+      v = PollSystem(reinterpret_cast<const uint64_t*>(q.ring + slot) + word);
+      StoreAgent(reinterpret_cast<uint64_t*>(q.mirror + slot) + word, v);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // entries before their tickets
+    for (uint32_t j = 0; j < count; ++j) {
+      const uint64_t entry = base + j;
+      const uint64_t seq = static_cast<uint64_t>(__shfl(static_cast<long long>(v), static_cast<int>(8 * j)));
+      // word 5: task_count in the low half, tiles in the high half
+      const uint64_t w5 = static_cast<uint64_t>(__shfl(static_cast<long long>(v), static_cast<int>(8 * j + 5)));
+      const uint32_t tiles = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(w5 >> 32));
+      if (Uniform64(seq) != entry || tiles == 0 || tiles > kQueueMaxTiles || tiles > q.ticket_capacity) {
+        Refuse(q, entry, lane);
+        continue;
+      }
+      const uint64_t head = (entry + 1) << 24;
+      for (uint32_t t = static_cast<uint32_t>(lane); t < tiles; t += kWave) {
+        StoreAgent(&q.tickets[(written + t) % q.ticket_capacity], head | t);
+      }
+      written += tiles;
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  if (lane == 0) {
+    StoreAgent(&q.dev->written, written);
+    StoreAgent(&q.dev->published, tail);
+  }
+}
+
+// Workgroup 0: rewinds the ticket index past the tickets an ended launch
+// handed out but never filled, opens the launch, then copies entries the
+// host publishes to the mirror and writes their tickets, until the host sets
+// stop, no entry arrives for idle_ticks, or the launch has lived life_ticks.
 __device__ void QueuePoller(const KoptQueueArgs& q, int lane, uint64_t start) {
+  // every wave of earlier launches has ended (one stream): no ticket of theirs is outstanding
+  if (lane == 0) {
+    StoreAgent(&q.dev->taken, PollAgent(&q.dev->written));
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    StoreAgent(&q.dev->open_epoch, q.epoch);
+  }
   uint64_t seen = Uniform64(PollAgent(&q.dev->published));
   uint64_t last = start;
   for (;;) {
     const uint64_t tail = Uniform64(PollSystem(&q.host->tail));
     if (tail != seen) {
       if (tail < seen || tail - seen > q.ring_size) {  // never published so by the host
-        if (lane == 0) __hip_atomic_store(&q.dev->refused, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (lane == 0) StoreAgent(&q.dev->refused, 1u);
         break;
       }
       PublishEntries(q, lane, seen, tail);
@@ -680,30 +751,7 @@ __device__ void QueuePoller(const KoptQueueArgs& q, int lane, uint64_t start) {
     if (tail == seen) __builtin_amdgcn_s_sleep(1);
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  if (lane == 0) __hip_atomic_store(&q.dev->exit_epoch, q.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Entry `entry` from the device mirror, every field uniform.
-__device__ __forceinline__ KoptQueueEntry MirrorEntry(const KoptQueueArgs& q, uint64_t entry) {
-  const uint64_t* src = reinterpret_cast<const uint64_t*>(q.mirror + entry % q.ring_size);
-  uint64_t w[8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) w[i] = Uniform64(PollAgent(src + i));
-  KoptQueueEntry e;
-  __builtin_memcpy(&e, w, sizeof(e));
-  return e;
-}
-
-// The entry's fields stay inside the arenas and the coordinate pool, so a
-// tile of it reads and writes in bounds whatever its task rows hold.
-__device__ __forceinline__ bool EntryInBounds(const KoptLaunch& a, const KoptQueueArgs& q, const KoptQueueEntry& e) {
-  if (e.k < 3 || e.k > 5 || e.task_count == 0 || e.node_count == 0) return false;
-  const KoptKLayout& L = a.layout[e.k - 3];
-  return e.task_offset <= q.task_capacity && e.task_count <= q.task_capacity - e.task_offset &&
-         e.cost_offset <= q.cost_capacity &&
-         static_cast<uint64_t>(e.task_count) * L.T <= q.cost_capacity - e.cost_offset &&
-         e.coord_base <= q.coord_count && e.node_count <= q.coord_count - e.coord_base &&
-         e.tiles == (e.task_count + L.rows - 1) / L.rows;
+  if (lane == 0) StoreAgent(&q.dev->exit_epoch, q.epoch);
 }
 
 template <int K>
@@ -728,8 +776,8 @@ __device__ __forceinline__ void QueueTile(const KoptLaunch& a, const KoptPointer
 }
 
 // One wavefront per workgroup.  Workgroup 0 polls the host ring; the others
-// claim tiles (one compare-and-swap on the claim word per tile, or per entry
-// to step past it) and run them with the batched kernel's tile code.
+// take a ticket index, wait for the ticket and run its tile with the batched
+// kernel's tile code.
 __global__ void __launch_bounds__(kWave) KoptQueueKernel(KoptLaunch a, KoptQueueArgs q) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const int lane = static_cast<int>(threadIdx.x);
@@ -748,81 +796,71 @@ __global__ void __launch_bounds__(kWave) KoptQueueKernel(KoptLaunch a, KoptQueue
   __syncthreads();
   const KoptPointers bufs{a.xy, nullptr, a.tasks, a.plans, a.costs, nullptr, nullptr};
   char* const wave = smem + a.wave_base;
-  // the claim word's entry as last seen: while published stays at or below
-  // it there is nothing to claim, and only published is polled
-  uint64_t known = 0;
-  uint32_t idle = 0;
+  // a poller that never ran ends nothing: every wait also ends on the launch's own clock
+  const uint64_t limit = 2 * q.life_ticks + q.idle_ticks;
+  // wait until this launch's poller has rewound the ticket index
+  while (__builtin_amdgcn_readfirstlane(PollAgent(&q.dev->open_epoch)) != q.epoch) {
+    if (__builtin_amdgcn_s_memrealtime() - start > limit) return;
+    __builtin_amdgcn_s_sleep(4);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   for (;;) {
-    const uint64_t published = Uniform64(PollAgent(&q.dev->published));
-    if (published > known) {
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the mirror's entries
-      const uint64_t claim = Uniform64(PollAgent(&q.dev->claim));
-      const uint64_t entry = claim >> 32;
-      const uint32_t tile = static_cast<uint32_t>(claim);
-      known = entry;
-      if (entry >= published) continue;
-      idle = 0;
-      const KoptQueueEntry e = MirrorEntry(q, entry);
-      const bool in_bounds = EntryInBounds(a, q, e);
-      const bool past = e.seq != entry || (in_bounds && tile >= e.tiles);
-      const uint64_t next = past || !in_bounds ? (entry + 1) << 32 : claim + 1;
-      uint32_t won = 0;
-      if (lane == 0) {
-        uint64_t expected = claim;
-        won = __hip_atomic_compare_exchange_strong(&q.dev->claim, &expected, next, __ATOMIC_RELAXED,
-                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                  ? 1u
-                  : 0u;
-      }
-      won = __builtin_amdgcn_readfirstlane(won);
-      if (won == 0 || past) continue;
-      const uint64_t slot = entry % q.ring_size;
-      if (!in_bounds) {
-        if (lane == 0) {
-          __hip_atomic_store(&q.dev->refused, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_store(&q.flags[slot], (entry + 1) | kQueueRefused, __ATOMIC_RELEASE,
-                             __HIP_MEMORY_SCOPE_SYSTEM);
+    uint64_t index = 0;
+    if (lane == 0) index = __hip_atomic_fetch_add(&q.dev->taken, uint64_t{1}, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    index = Uniform64(index);
+    uint64_t* const slot = q.tickets + index % q.ticket_capacity;
+    uint64_t ticket = 0;
+    for (uint32_t polls = 0;; ++polls) {
+      ticket = Uniform64(PollAgent(slot));
+      if (ticket != 0) break;
+      // every 16th look: has the poller ended (then no ticket comes), or the clock run out?
+      if ((polls & 15u) == 15u) {
+        const bool ended = __builtin_amdgcn_readfirstlane(PollAgent(&q.dev->exit_epoch)) == q.epoch;
+        if (ended || __builtin_amdgcn_s_memrealtime() - start > limit) {
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+          ticket = Uniform64(PollAgent(slot));
+          break;
         }
-        continue;
       }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // the host's task rows
-      if (e.k == 3) {
-        QueueTile<3>(a, bufs, e, tile, smem, wave, lane);
-      } else if (e.k == 4) {
-        QueueTile<4>(a, bufs, e, tile, smem, wave, lane);
+      // nap briefly at first, longer once the wait has lasted
+      if (polls < 512) {
+        __builtin_amdgcn_s_sleep(2);
       } else {
-        QueueTile<5>(a, bufs, e, tile, smem, wave, lane);
+        __builtin_amdgcn_s_sleep(16);
       }
-      // the tile's cells, then its mark (the same wave's stores reach host
-      // memory in order), then its count
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
-      if (lane == 0) {
-        __hip_atomic_store(&q.marks[e.task_offset + tile], entry + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
-      uint64_t before = 0;
-      if (lane == 0) before = __hip_atomic_fetch_add(&q.done[slot], uint64_t{1}, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      before = Uniform64(before);
-      if (before + 1 == e.done_target) {
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
-        if (lane == 0) __hip_atomic_store(&q.flags[slot], entry + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-      }
+    }
+    if (ticket == 0) return;
+    if (lane == 0) StoreAgent(slot, uint64_t{0});
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the mirror's entry
+    const uint64_t entry = (ticket >> 24) - 1;
+    const uint32_t tile = static_cast<uint32_t>(ticket & kQueueMaxTiles);
+    const KoptQueueEntry e = MirrorEntry(q, entry);
+    if (e.seq != entry || !EntryInBounds(a, q, e) || tile >= e.tiles) {  // never so from the poller
+      Refuse(q, entry, lane);
       continue;
     }
-    if (__builtin_amdgcn_readfirstlane(PollAgent(&q.dev->exit_epoch)) == q.epoch) {
-      // the poller has ended: published no longer grows; leave once it is drained
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      if (Uniform64(PollAgent(&q.dev->published)) > known) continue;
-      break;
-    }
-    // a poller that never ran ends nothing: leave on the launch's own clock
-    if (__builtin_amdgcn_s_memrealtime() - start > 2 * q.life_ticks + q.idle_ticks) break;
-    // nap briefly while work came recently, longer once idle for a while
-    if (idle < 256) {
-      ++idle;
-      __builtin_amdgcn_s_sleep(2);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // the host's task rows
+    if (e.k == 3) {
+      QueueTile<3>(a, bufs, e, tile, smem, wave, lane);
+    } else if (e.k == 4) {
+      QueueTile<4>(a, bufs, e, tile, smem, wave, lane);
     } else {
-      __builtin_amdgcn_s_sleep(16);
+      QueueTile<5>(a, bufs, e, tile, smem, wave, lane);
+    }
+    // the tile's cells, then its mark: the same wave's stores reach host
+    // memory in order
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+    const uint64_t slot_of_entry = entry % q.ring_size;
+    uint64_t before = 0;
+    if (lane == 0) {
+      __hip_atomic_store(&q.marks[e.task_offset + tile], entry + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      before = __hip_atomic_fetch_add(&q.done[slot_of_entry], uint64_t{1}, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+    }
+    before = Uniform64(before);
+    // the last tile's wave raises the flag; the host still waits for every mark
+    if (before + 1 == e.done_target && lane == 0) {
+      __hip_atomic_store(&q.flags[slot_of_entry], entry + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
   }
 }

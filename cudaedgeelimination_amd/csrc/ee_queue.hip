@@ -1,25 +1,26 @@
 // Leaf-cost queries served through a device work queue.
 //
 // A persistent kernel (KoptQueueKernel, ee_kopt.hip) drains a ring of
-// requests that lives in coherent host memory and raises one completion flag
-// per request there.  A submit copies the task rows into a task arena in the
-// same memory, writes the request's entry into the ring and stores the new
-// tail; the kernel's poller sees the tail within about a microsecond, its
-// tile wavefronts claim the entry's tiles and write the cells straight into
-// a cost arena in host memory, and the wavefront that ends the last tile
-// stores the flag.  A waiter polls the flag and copies the cells into the
-// caller's buffer.  Between a submit and its cells there is no launch, no
-// copy command and no hand-over between host threads.
+// requests that lives in coherent host memory.  A submit copies the task
+// rows into a task arena in the same memory, writes the request's entry into
+// the ring and stores the new tail; the kernel's poller sees the tail within
+// a microsecond or two and hands out one ticket per tile, its tile
+// wavefronts write the cells straight into a cost arena in host memory, mark
+// each tile and raise the entry's completion flag.  A completing thread
+// polls the flags and marks of the requests in flight, copies each served
+// request's cells to the caller's buffer and wakes its waiter.  Between a
+// submit and its cells there is no launch and no copy command.
 //
 // The kernel ends by itself after `idle_exit_ms` without a request (and
 // after a few seconds in any case); a submit that finds it ended launches it
-// again, and so does a waiter that finds the stream idle with its request
-// unserved.  Every graph's coordinates stay in one device pool sized at
-// creation, so adding a graph never moves memory the kernel reads.
+// again, and so does the completing thread when it finds the stream idle
+// with requests unserved.  Every graph's coordinates stay in one device pool
+// sized at creation, so adding a graph never moves memory the kernel reads.
 #include <algorithm>
 #include <array>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <memory>
@@ -39,8 +40,10 @@ constexpr uint32_t kTemplates[3] = {4, 25, 208};
 // the launch's hard limit on its own lifetime; a queue under steady load is
 // launched again this often
 constexpr double kLifeMs = 2000.0;
-// how often a waiter checks that the kernel still runs
+// how often the completing thread checks that the kernel still runs
 constexpr double kLivenessMs = 1.0;
+// in-flight requests the completing thread looks at per pass, oldest first
+constexpr size_t kCollectWindow = 512;
 
 double Ms(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); }
 
@@ -135,7 +138,7 @@ struct Prepared {
 struct QRequest {
   uint64_t id = 0, entry = 0, slot = 0;
   uint64_t task_at = 0, cost_at = 0, cells = 0, tiles = 0;
-  uint64_t marked = 0;  // tiles whose marks the host has seen
+  uint64_t marked = 0;  // tiles whose marks the completer has seen
   int64_t* out = nullptr;
   Clock::time_point submit, done;
   std::atomic<bool> complete{false};  // collected: cells in the caller's buffer
@@ -161,15 +164,24 @@ struct Queue {
   DeviceArray<KoptQueueDevice> dev;
   DeviceArray<KoptQueueEntry> mirror;
   DeviceArray<uint64_t> done;
-  std::vector<uint64_t> slot_tiles;              // tiles ever queued per ring slot
+  DeviceArray<uint64_t> tickets;
+  std::vector<uint64_t> slot_tiles;                // tiles ever queued per ring slot
   std::vector<std::shared_ptr<QRequest>> by_slot;  // the request holding each slot
-  std::deque<std::shared_ptr<QRequest>> inflight;  // published, not collected, oldest first
+  // published requests the completing thread has not yet taken over
+  std::vector<std::shared_ptr<QRequest>> incoming;
+  std::atomic<uint64_t> incoming_count{0};  // bumped with every push to `incoming`
+  uint64_t completed = 0;                   // requests collected
   KoptLaunch layout{};
   size_t lds = 0;
   uint64_t idle_ticks = 0, life_ticks = 0;
   hipStream_t stream = nullptr, copy_stream = nullptr;
   RingArena task_ring, cost_ring;
   std::mutex mu;
+  std::condition_variable cv_work;  // the completer: something was published, or stop
+  std::condition_variable cv_room;  // publishers: arena space or a ring slot came free
+  std::condition_variable cv_done;  // waiters: a request completed
+  std::thread completer;
+  bool stopping = false;
   uint32_t epoch = 0;  // the latest launch
   uint64_t tail = 0;   // entries published
   uint64_t next_id = 1;
@@ -179,27 +191,22 @@ struct Queue {
   std::string broken;  // non-empty: the kernel refused an entry; the queue is out of service
 
   uint64_t FlagOf(const QRequest& r) { return __atomic_load_n(&flags.host()[r.slot], __ATOMIC_ACQUIRE); }
-  // Served: the flag, then every tile's mark (each stored after that
-  // tile's cells by the wave that wrote them).  r.marked only grows, and
-  // only the thread holding mu or the request's one waiter moves it.
+  // Served: the flag, then every tile's mark (each stored after that tile's
+  // cells by the wave that wrote them).  Only the completer calls this.
   bool Ready(QRequest& r) {
     const uint64_t f = FlagOf(r);
     if (f == ((r.entry + 1) | kQueueRefused)) return true;
     if (f != r.entry + 1) return false;
     const uint64_t* m = marks.host() + r.task_at;
-    uint64_t t = __atomic_load_n(&r.marked, __ATOMIC_RELAXED);
-    while (t < r.tiles && __atomic_load_n(&m[t], __ATOMIC_ACQUIRE) == r.entry + 1) ++t;
-    __atomic_store_n(&r.marked, t, __ATOMIC_RELAXED);
-    return t == r.tiles;
+    while (r.marked < r.tiles && __atomic_load_n(&m[r.marked], __ATOMIC_ACQUIRE) == r.entry + 1) ++r.marked;
+    return r.marked == r.tiles;
   }
   // under mu
   void Launch();
   void EnsureRunning(bool stream_check);
-  void Collect(const std::shared_ptr<QRequest>& r);
-  bool Reap();
-  void AwaitOldest(std::unique_lock<std::mutex>& lk);
   Prepared Prepare(uint32_t graph, uint32_t k, const ee_kopt_task* tasks, uint64_t count, int64_t* costs);
   uint64_t Publish(const Prepared& p);
+  void CompleteLoop();
   void Wait(const std::shared_ptr<QRequest>& r, double timeout_ms);
   void Drain();
 };
@@ -213,6 +220,8 @@ void Queue::Launch() {
   q.dev = dev.get();
   q.mirror = mirror.get();
   q.done = done.get();
+  q.tickets = tickets.get();
+  q.ticket_capacity = tickets.count();
   q.task_capacity = task_arena.count();
   q.cost_capacity = cost_arena.count();
   q.coord_count = d_xy.count();
@@ -237,55 +246,6 @@ void Queue::EnsureRunning(bool stream_check) {
   if (ended == epoch || (stream_check && hipStreamQuery(stream) == hipSuccess)) Launch();
 }
 
-// Copies a served request's cells out and frees its arena blocks and slot.
-void Queue::Collect(const std::shared_ptr<QRequest>& rp) {
-  QRequest& r = *rp;
-  const uint64_t f = FlagOf(r);
-  if ((f & kQueueRefused) != 0) {
-    r.status = EE_ERR_RUNTIME;
-    r.error = "the device queue refused the request's entry";
-    broken = r.error;
-  } else {
-    std::memcpy(r.out, cost_arena.host() + r.cost_at, r.cells * sizeof(int64_t));
-  }
-  task_ring.Free(r.task_at);
-  cost_ring.Free(r.cost_at);
-  by_slot[r.slot].reset();
-  inflight.erase(std::find(inflight.begin(), inflight.end(), rp));
-  inflight_cells -= r.cells;
-  r.done = Clock::now();
-  ++stats.requests;
-  stats.cells += r.cells;
-  r.complete.store(true, std::memory_order_release);
-}
-
-// Collects every served request; true if there was one.
-bool Queue::Reap() {
-  std::vector<std::shared_ptr<QRequest>> ready;
-  for (auto& r : inflight) {
-    if (Ready(*r)) ready.push_back(r);
-  }
-  for (auto& r : ready) Collect(r);
-  return !ready.empty();
-}
-
-// Waits (the lock released) until the oldest request in flight is served.
-void Queue::AwaitOldest(std::unique_lock<std::mutex>& lk) {
-  if (inflight.empty()) throw std::logic_error("queue full with nothing in flight");
-  const std::shared_ptr<QRequest> oldest = inflight.front();
-  lk.unlock();
-  auto check = Clock::now();
-  while (!Ready(*oldest)) {
-    if (Ms(check, Clock::now()) > kLivenessMs) {
-      std::lock_guard<std::mutex> g(mu);
-      EnsureRunning(true);
-      check = Clock::now();
-    }
-    std::this_thread::yield();
-  }
-  lk.lock();
-}
-
 Prepared Queue::Prepare(uint32_t graph, uint32_t k, const ee_kopt_task* tasks, uint64_t count, int64_t* costs) {
   if (k < 3 || k > 5) throw std::invalid_argument("request k must be in [3,5]");
   if (count == 0 || tasks == nullptr || costs == nullptr) {
@@ -304,6 +264,7 @@ Prepared Queue::Prepare(uint32_t graph, uint32_t k, const ee_kopt_task* tasks, u
   }
   ValidateTasks(static_cast<int32_t>(inst.n), k, tasks, count);
   const uint32_t rows = tuning.rows_per_tile[k - 3];
+  if ((count + rows - 1) / rows > kQueueMaxTiles) throw std::invalid_argument("request has too many tiles");
   return Prepared{tasks, count, cells, cell_span, (count + rows - 1) / rows, costs, k, inst};
 }
 
@@ -314,6 +275,7 @@ uint64_t Queue::Publish(const Prepared& p) {
   auto r = std::make_shared<QRequest>();
   for (;;) {
     if (!broken.empty()) throw std::runtime_error(broken);
+    if (stopping) throw std::runtime_error("queue is stopping");
     const uint64_t slot = tail % cfg.ring_entries;
     if (!by_slot[slot]) {
       uint64_t task_at = 0, cost_at = 0;
@@ -327,8 +289,8 @@ uint64_t Queue::Publish(const Prepared& p) {
         task_ring.Free(task_at);
       }
     }
-    // no room: collect what has been served, else wait for the oldest
-    if (!Reap()) AwaitOldest(lk);
+    // no room: the completer frees space as requests complete
+    cv_room.wait_for(lk, std::chrono::milliseconds(1));
   }
   std::memcpy(task_arena.host() + r->task_at, p.tasks, count * sizeof(ee_kopt_task));
   r->entry = tail;
@@ -352,7 +314,8 @@ uint64_t Queue::Publish(const Prepared& p) {
   e.near = inst.near;
   ring.host()[r->slot] = e;
   by_slot[r->slot] = r;
-  inflight.push_back(r);
+  incoming.push_back(r);
+  incoming_count.fetch_add(1, std::memory_order_release);
   live[r->id] = r;
   inflight_cells += p.cells;
   stats.peak_inflight_cells = std::max(stats.peak_inflight_cells, inflight_cells);
@@ -360,47 +323,112 @@ uint64_t Queue::Publish(const Prepared& p) {
   __atomic_store_n(&host.host()->tail, tail + 1, __ATOMIC_RELEASE);
   ++tail;
   EnsureRunning(false);
+  lk.unlock();
+  cv_work.notify_one();
   return r->id;
 }
 
-void Queue::Wait(const std::shared_ptr<QRequest>& r, double timeout_ms) {
-  const auto start = Clock::now();
-  auto check = start;
+// The completing thread: while requests are in flight it polls their flags
+// and marks (yielding the core between looks), copies each served request's
+// cells to the caller's buffer, then under the lock frees the arena blocks
+// and ring slots of all it served and wakes their waiters; every
+// kLivenessMs without progress it makes sure the kernel runs.  The requests
+// it watches are its own list: the lock is taken to take over newly
+// published requests (when the published count moved) and to release.
+void Queue::CompleteLoop() {
+  static_cast<void>(hipSetDevice(ctx->device));
+  std::deque<std::shared_ptr<QRequest>> mine;  // oldest first
+  std::vector<std::shared_ptr<QRequest>> served;
+  uint64_t taken_over = 0;
+  auto check = Clock::now();
   for (;;) {
-    // the flag is read without the lock; a request collected meanwhile by
-    // another thread is seen complete under it
-    if (r->complete.load(std::memory_order_acquire)) return;
-    if (Ready(*r)) {
-      std::lock_guard<std::mutex> lk(mu);
-      if (!r->complete.load(std::memory_order_relaxed)) Collect(r);
-      return;
+    if (mine.empty() || incoming_count.load(std::memory_order_acquire) != taken_over) {
+      std::unique_lock<std::mutex> lk(mu);
+      if (mine.empty()) cv_work.wait(lk, [&] { return !incoming.empty() || stopping; });
+      if (mine.empty() && incoming.empty()) return;  // stopping with nothing in flight
+      for (auto& r : incoming) mine.push_back(std::move(r));
+      taken_over += incoming.size();
+      incoming.clear();
+    }
+    // the oldest requests first: tickets go out in entry order
+    const size_t n = std::min(mine.size(), kCollectWindow);
+    for (size_t i = 0; i < n; ++i) {
+      QRequest& r = *mine[i];
+      if (!Ready(r)) continue;
+      // the blocks stay allocated until the request is released below
+      if ((FlagOf(r) & kQueueRefused) == 0) {
+        std::memcpy(r.out, cost_arena.host() + r.cost_at, r.cells * sizeof(int64_t));
+      }
+      served.push_back(mine[i]);
     }
     const auto now = Clock::now();
-    if (timeout_ms >= 0 && Ms(start, now) > timeout_ms) {
-      throw std::runtime_error("request did not complete before the timeout");
+    if (served.empty()) {
+      if (Ms(check, now) > kLivenessMs) {
+        std::lock_guard<std::mutex> lk(mu);
+        EnsureRunning(true);
+        check = now;
+      }
+      std::this_thread::yield();
+      continue;
     }
-    if (Ms(check, now) > kLivenessMs) {
+    {
       std::lock_guard<std::mutex> lk(mu);
-      if (!r->complete.load(std::memory_order_relaxed) && !Ready(*r)) EnsureRunning(true);
-      check = now;
+      const auto done_at = Clock::now();
+      for (auto& rp : served) {
+        QRequest& r = *rp;
+        if ((FlagOf(r) & kQueueRefused) != 0) {
+          r.status = EE_ERR_RUNTIME;
+          r.error = "the device queue refused the request's entry";
+          broken = r.error;
+        }
+        task_ring.Free(r.task_at);
+        cost_ring.Free(r.cost_at);
+        by_slot[r.slot].reset();
+        inflight_cells -= r.cells;
+        r.done = done_at;
+        ++stats.requests;
+        stats.cells += r.cells;
+        ++completed;
+        r.complete.store(true, std::memory_order_release);
+      }
     }
-    std::this_thread::yield();
+    cv_room.notify_all();
+    cv_done.notify_all();
+    mine.erase(std::remove_if(mine.begin(), mine.end(),
+                              [](const std::shared_ptr<QRequest>& r) {
+                                return r->complete.load(std::memory_order_relaxed);
+                              }),
+               mine.end());
+    served.clear();
+    check = now;
   }
 }
 
-// Serves everything published, then stops the kernel.
+// Polls the request's completion (yielding) for a short while, then blocks.
+void Queue::Wait(const std::shared_ptr<QRequest>& r, double timeout_ms) {
+  const auto start = Clock::now();
+  while (Ms(start, Clock::now()) < 0.2) {
+    if (r->complete.load(std::memory_order_acquire)) return;
+    std::this_thread::yield();
+  }
+  std::unique_lock<std::mutex> lk(mu);
+  const auto ready = [&] { return r->complete.load(std::memory_order_acquire); };
+  if (timeout_ms < 0) {
+    cv_done.wait(lk, ready);
+  } else if (!cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready)) {
+    throw std::runtime_error("request did not complete before the timeout");
+  }
+}
+
+// Serves everything published, stops the completer, then the kernel.
 void Queue::Drain() {
-  std::vector<std::shared_ptr<QRequest>> pending;
   {
-    std::lock_guard<std::mutex> lk(mu);
-    pending.assign(inflight.begin(), inflight.end());
+    std::unique_lock<std::mutex> lk(mu);
+    cv_room.wait_for(lk, std::chrono::seconds(30), [&] { return completed == tail || !broken.empty(); });
+    stopping = true;
   }
-  for (auto& r : pending) {
-    try {
-      Wait(r, 10000.0);
-    } catch (...) {
-    }
-  }
+  cv_work.notify_all();
+  if (completer.joinable()) completer.join();
   host.host()->stop = 1;
   std::atomic_thread_fence(std::memory_order_seq_cst);
   static_cast<void>(hipStreamSynchronize(stream));
@@ -473,9 +501,11 @@ ee_status ee_queue_create(ee_ctx* handle, const ee_queue_config* config, ee_queu
     q->dev.Allocate(1, dev);
     q->mirror.Allocate(c.ring_entries, dev);
     q->done.Allocate(c.ring_entries, dev);
+    q->tickets.Allocate(c.task_rows, dev);  // a tile has at least one row
     ee::CheckHip(hipMemsetAsync(q->dev.get(), 0, sizeof(ee::KoptQueueDevice), ctx->stream), "hipMemsetAsync");
     ee::CheckHip(hipMemsetAsync(q->mirror.get(), 0, q->mirror.bytes(), ctx->stream), "hipMemsetAsync");
     ee::CheckHip(hipMemsetAsync(q->done.get(), 0, q->done.bytes(), ctx->stream), "hipMemsetAsync");
+    ee::CheckHip(hipMemsetAsync(q->tickets.get(), 0, q->tickets.bytes(), ctx->stream), "hipMemsetAsync");
     ctx->Sync();
     q->slot_tiles.assign(c.ring_entries, 0);
     q->by_slot.resize(c.ring_entries);
@@ -491,6 +521,8 @@ ee_status ee_queue_create(ee_ctx* handle, const ee_queue_config* config, ee_queu
     q->life_ticks = static_cast<uint64_t>(ee::kLifeMs * khz);
     ee::CheckHip(hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking), "hipStreamCreate");
     ee::CheckHip(hipStreamCreateWithFlags(&q->copy_stream, hipStreamNonBlocking), "hipStreamCreate");
+    ee::Queue* raw = q.get();
+    raw->completer = std::thread([raw] { raw->CompleteLoop(); });
     *out = reinterpret_cast<ee_queue*>(q.release());
   });
 }
