@@ -44,6 +44,8 @@ constexpr double kLifeMs = 2000.0;
 constexpr double kLivenessMs = 1.0;
 // in-flight requests the completing thread looks at per pass, oldest first
 constexpr size_t kCollectWindow = 512;
+// how long the completing thread polls for a submit before it blocks
+constexpr double kIdleSpinMs = 5.0;
 
 double Ms(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); }
 
@@ -152,6 +154,7 @@ struct Queue {
   KoptTuning tuning;
   std::array<KoptPlan, 3> plans;
   DeviceArray<KoptPlan> d_plans;
+  std::mutex graphs_mu;  // instances, coords and uploads into the pool
   std::vector<KoptInstance> instances;
   DeviceArray<longlong2> d_xy;
   uint64_t coords = 0;
@@ -181,7 +184,7 @@ struct Queue {
   std::condition_variable cv_room;  // publishers: arena space or a ring slot came free
   std::condition_variable cv_done;  // waiters: a request completed
   std::thread completer;
-  bool stopping = false;
+  std::atomic<bool> stopping{false};
   uint32_t epoch = 0;  // the latest launch
   uint64_t tail = 0;   // entries published
   uint64_t next_id = 1;
@@ -258,7 +261,7 @@ Prepared Queue::Prepare(uint32_t graph, uint32_t k, const ee_kopt_task* tasks, u
   }
   KoptInstance inst{};
   {
-    std::lock_guard<std::mutex> lk(mu);
+    std::lock_guard<std::mutex> lk(graphs_mu);
     if (graph >= instances.size()) throw std::invalid_argument("unknown graph id");
     inst = instances[graph];
   }
@@ -334,7 +337,8 @@ uint64_t Queue::Publish(const Prepared& p) {
 // and ring slots of all it served and wakes their waiters; every
 // kLivenessMs without progress it makes sure the kernel runs.  The requests
 // it watches are its own list: the lock is taken to take over newly
-// published requests (when the published count moved) and to release.
+// published requests (when the published count moved) and to release.  With
+// nothing in flight it polls for kIdleSpinMs before it blocks.
 void Queue::CompleteLoop() {
   static_cast<void>(hipSetDevice(ctx->device));
   std::deque<std::shared_ptr<QRequest>> mine;  // oldest first
@@ -342,9 +346,18 @@ void Queue::CompleteLoop() {
   uint64_t taken_over = 0;
   auto check = Clock::now();
   for (;;) {
+    if (mine.empty()) {
+      // nothing in flight: poll for the next submit a while before blocking
+      // (a blocked thread wakes tens of microseconds after the notify)
+      const auto idle = Clock::now();
+      while (incoming_count.load(std::memory_order_acquire) == taken_over &&
+             !stopping.load(std::memory_order_acquire) && Ms(idle, Clock::now()) < kIdleSpinMs) {
+        std::this_thread::yield();
+      }
+    }
     if (mine.empty() || incoming_count.load(std::memory_order_acquire) != taken_over) {
       std::unique_lock<std::mutex> lk(mu);
-      if (mine.empty()) cv_work.wait(lk, [&] { return !incoming.empty() || stopping; });
+      if (mine.empty()) cv_work.wait(lk, [&] { return !incoming.empty() || stopping.load(); });
       if (mine.empty() && incoming.empty()) return;  // stopping with nothing in flight
       for (auto& r : incoming) mine.push_back(std::move(r));
       taken_over += incoming.size();
@@ -552,7 +565,7 @@ ee_status ee_queue_add_graph(ee_queue* queue, const int64_t* x, const int64_t* y
     }
     std::vector<longlong2> xy(static_cast<size_t>(dimension));
     for (int32_t i = 0; i < dimension; ++i) xy[i] = longlong2{x[i], y[i]};
-    std::lock_guard<std::mutex> lk(q.mu);
+    std::lock_guard<std::mutex> lk(q.graphs_mu);
     if (static_cast<uint64_t>(dimension) > q.d_xy.count() - q.coords) {
       throw std::invalid_argument("graph exceeds the queue's coordinate pool (max_nodes)");
     }
