@@ -240,9 +240,9 @@ def many_instance_leg(ctx, cpu: bool, sample: int = 16):
 
 def serve_stream(ctx, graphs, pools, requests, is_jv, arrivals, mode: str, burst: bool, keep=False):
     """One request stream against a fresh server of `mode` ("batched",
-    "plain", or "queue": the device work queue, leaf-cost queries only):
-    latency from each request's scheduled arrival to completion.  keep: also
-    return the burst's results (key "outputs")."""
+    "plain", or "queue": the device work queue's persistent kernel): latency
+    from each request's scheduled arrival to completion.  keep: also return
+    the burst's results (key "outputs")."""
     import threading
 
     count = len(requests)
@@ -250,13 +250,11 @@ def serve_stream(ctx, graphs, pools, requests, is_jv, arrivals, mode: str, burst
     if mode == "queue":
         server = DeviceQueue(ctx, task_rows=1 << 20, cost_cells=1 << 25, max_nodes=1 << 16,
                              ring_entries=1 << 15)
-        jids = None
     else:
         server = KoptServer(ctx, batching=(mode == "batched"), hbm_budget_bytes=2 << 30,
                             max_batch_cells=1 << 21)
     gids = [server.add_graph(g) for g in graphs]
-    if mode != "queue":
-        jids = [server.add_jv_graph(g) for g in graphs]
+    jids = [server.add_jv_graph(g) for g in graphs]
     warm = server.submit(gids[0], 3, pools[0][3][:8])
     server.wait(warm)
     latencies = np.zeros(count)
@@ -365,11 +363,12 @@ def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005, jv_share:
                                            keep=(qps == 0))
             if qps == 0:
                 # every path's burst results against the first path's
-                first = point[modes[0]].pop("outputs")
-                same = True
-                for mode in modes[1:]:
-                    same = same and all(np.array_equal(a, b) for a, b in zip(first, point[mode].pop("outputs")))
-                point["results_equal_across_paths"] = bool(same)
+                outs = {mode: point[mode].pop("outputs") for mode in modes}
+                differing = {mode: sum(not np.array_equal(a, b) for a, b in zip(outs[modes[0]], outs[mode]))
+                             for mode in modes[1:]}
+                point["results_equal_across_paths"] = not any(differing.values())
+                if any(differing.values()):
+                    point["requests_differing_from_" + modes[0]] = differing
             for mode in modes:
                 if mode != "plain":
                     point[f"{mode}_over_plain_cells_per_s"] = point[mode]["cells_per_s"] / point["plain"]["cells_per_s"]
@@ -397,10 +396,10 @@ def serving_leg(ctx, qps_list, seconds: float, seed: int = 0xEE000005, jv_share:
                 sat[f"{mode}_over_plain_qps"] = sat[mode]["saturation_qps"] / sat["plain"]["saturation_qps"]
         return sat
 
-    out["points"] = points(jv_share, ("batched", "plain"))
+    out["points"] = points(jv_share, ("queue", "batched", "plain"))
     if saturation_rates:
-        out["saturation"] = saturation(jv_share, ("batched", "plain"))
-    # leaf-cost queries alone, which the device work queue also serves
+        out["saturation"] = saturation(jv_share, ("queue", "batched", "plain"))
+    # leaf-cost queries alone
     lo = {"jv_share": 0.0, "points": points(0.0, ("queue", "batched", "plain"))}
     if saturation_rates:
         lo["saturation"] = saturation(0.0, ("queue", "batched", "plain"))

@@ -204,12 +204,14 @@ class ServerStats(C.Structure):
 
 class QueueConfig(C.Structure):
     _fields_ = [("task_rows", C.c_uint64), ("cost_cells", C.c_uint64), ("max_nodes", C.c_uint64),
-                ("ring_entries", C.c_uint32), ("workers", C.c_uint32), ("idle_exit_ms", C.c_double)]
+                ("ring_entries", C.c_uint32), ("workers", C.c_uint32), ("idle_exit_ms", C.c_double),
+                ("max_jv_nodes", C.c_uint64), ("max_jv_edges", C.c_uint64),
+                ("max_jv_adjacency", C.c_uint64)]
 
 
 class QueueStats(C.Structure):
     _fields_ = [("requests", C.c_uint64), ("cells", C.c_uint64), ("launches", C.c_uint64),
-                ("peak_inflight_cells", C.c_uint64)]
+                ("peak_inflight_cells", C.c_uint64), ("jv_edges", C.c_uint64)]
 
 
 P = C.c_void_p
@@ -273,6 +275,8 @@ PROTOTYPES = {
     "ee_queue_destroy": (None, [P]),
     "ee_queue_add_graph": (C.c_int, [P, P, P, I32, U8, C.POINTER(U32)]),
     "ee_queue_submit": (C.c_int, [P, U32, U32, P, U64, P, C.POINTER(U64)]),
+    "ee_queue_add_jv_graph": (C.c_int, [P, C.POINTER(GraphView), C.POINTER(U32)]),
+    "ee_queue_submit_jv": (C.c_int, [P, U32, P, U64, P, C.POINTER(U64)]),
     "ee_queue_submit_many": (C.c_int, [P, U32, P, P]),
     "ee_queue_wait": (C.c_int, [P, U64, DBL, C.POINTER(RequestResult)]),
     "ee_queue_wait_many": (C.c_int, [P, U32, P, DBL, P]),

@@ -181,6 +181,30 @@ void LaunchKoptResident(const KoptLaunch& finalized, const KoptLaunch* device_ar
                         const KoptTuning& tuning, size_t shared_bytes, hipStream_t stream,
                         unsigned long long* stamps = nullptr);
 
+// ------------------------------------------------------------------- JV
+// Edge slots edge_list[0..edge_count) (or 0..edge_count when edge_list is
+// NULL) are tested; item i's witness goes to witnesses[i].
+struct JvLaunch {
+  int32_t edge_count;
+  int32_t distance_type;
+  int32_t near;                 // NearExtent of the coordinates: FP64 distances
+  int32_t one_per_wave;         // 1: one edge slot per wavefront; 0: two when they fit
+  const int2* edge_uv;          // [m]
+  const int64_t* edge_weight;   // [m]
+  const int32_t* edge_active;   // [m]
+  const int32_t* row_offsets;   // [n+1]
+  const int32_t* neighbors;     // [A]
+  const int32_t* csr_edge_ids;  // [A]
+  const longlong2* xy;          // [n]
+  int32_t* witnesses;           // [m]
+  // optional, for several graphs in one launch: per edge slot, bit 0 the
+  // distance type and bit 1 the near flag of the slot's graph
+  const uint8_t* edge_mode;     // [m] or NULL
+  const int32_t* edge_list;     // [edge_count] or NULL
+};
+
+void LaunchJv(const JvLaunch& args, uint32_t grid, hipStream_t stream);
+
 // ------------------------------------------------------- k-opt work queue
 // A persistent kernel serves leaf-cost requests from a ring of entries in
 // coherent host memory.  Workgroup 0 (one wavefront) polls the ring's tail,
@@ -201,8 +225,13 @@ void LaunchKoptResident(const KoptLaunch& finalized, const KoptLaunch* device_ar
 // queue whose kernel has ended; the new poller first rewinds the ticket
 // index past the tickets the ended launch handed out but never filled.
 //
-// One entry (64 bytes): task rows [task_offset, task_offset + task_count) of
-// the task arena, cells from cost_offset of the cost arena.
+// One entry (64 bytes).  A leaf-cost query (k = 3..5): task rows
+// [task_offset, task_offset + task_count) of the task arena, cells from
+// cost_offset of the cost arena.  A JV recheck (k = 0): task_count edge
+// slots of the resident JV union as int32 from row task_offset of the task
+// arena (the request holds `tiles` rows there, one mark each), witnesses as
+// int32 from cell cost_offset of the cost arena; kQueueJvTileEdges slots per
+// tile.
 struct KoptQueueEntry {
   uint64_t seq;          // entry index; ring slot = seq % ring
   uint64_t task_offset;
@@ -242,6 +271,8 @@ struct KoptQueueDevice {
 };
 // Tiles of one entry and entries are numbered in a ticket's 24 and 40 bits.
 constexpr uint64_t kQueueMaxTiles = (uint64_t{1} << 24) - 1;
+// Edge slots per JV recheck tile: two pairs of edges on one wavefront.
+constexpr uint32_t kQueueJvTileEdges = 4;
 
 struct KoptQueueArgs {
   KoptQueueHost* host;             // device address of the host control words
@@ -259,6 +290,13 @@ struct KoptQueueArgs {
   uint64_t idle_ticks, life_ticks; // s_memrealtime ticks
   uint32_t ring_size;
   uint32_t epoch;                  // this launch, counted from 1
+  // JV rechecks: the resident union (edge_list, witnesses and edge_count are
+  // set per tile) and a device staging row per workgroup (kQueueJvTileEdges
+  // slots, then as many witnesses); jv_slot_count = 0 when the queue holds
+  // no JV graph
+  JvLaunch jv;
+  uint64_t jv_slot_count;
+  int32_t* jv_stage;
 };
 
 // LDS layout for the queue kernel: one wavefront per workgroup, the plans of
@@ -269,29 +307,5 @@ size_t KoptQueueLayout(KoptLaunch* a, const KoptPlan plans[3], const KoptTuning&
 // (costs).
 void LaunchKoptQueue(const KoptLaunch& a, const KoptQueueArgs& q, uint32_t workers, size_t shared_bytes,
                      hipStream_t stream);
-
-// ------------------------------------------------------------------- JV
-// Edge slots edge_list[0..edge_count) (or 0..edge_count when edge_list is
-// NULL) are tested; item i's witness goes to witnesses[i].
-struct JvLaunch {
-  int32_t edge_count;
-  int32_t distance_type;
-  int32_t near;                 // NearExtent of the coordinates: FP64 distances
-  int32_t one_per_wave;         // 1: one edge slot per wavefront; 0: two when they fit
-  const int2* edge_uv;          // [m]
-  const int64_t* edge_weight;   // [m]
-  const int32_t* edge_active;   // [m]
-  const int32_t* row_offsets;   // [n+1]
-  const int32_t* neighbors;     // [A]
-  const int32_t* csr_edge_ids;  // [A]
-  const longlong2* xy;          // [n]
-  int32_t* witnesses;           // [m]
-  // optional, for several graphs in one launch: per edge slot, bit 0 the
-  // distance type and bit 1 the near flag of the slot's graph
-  const uint8_t* edge_mode;     // [m] or NULL
-  const int32_t* edge_list;     // [edge_count] or NULL
-};
-
-void LaunchJv(const JvLaunch& args, uint32_t grid, hipStream_t stream);
 
 }  // namespace ee

@@ -35,6 +35,7 @@
 
 #include "ee_exact.h"
 #include "ee_internal.h"
+#include "ee_jv_body.h"
 
 namespace ee {
 namespace {
@@ -638,6 +639,15 @@ __device__ __forceinline__ KoptQueueEntry MirrorEntry(const KoptQueueArgs& q, ui
 // The entry's fields stay inside the arenas and the coordinate pool, so a
 // tile of it reads and writes in bounds whatever its task rows hold.
 __device__ __forceinline__ bool EntryInBounds(const KoptLaunch& a, const KoptQueueArgs& q, const KoptQueueEntry& e) {
+  if (e.k == 0) {
+    // a JV recheck: its tiles' rows (marks, and the slots, 12 per row) and
+    // its witnesses (two per cell) inside the arenas
+    const uint64_t tiles = (static_cast<uint64_t>(e.task_count) + kQueueJvTileEdges - 1) / kQueueJvTileEdges;
+    return q.jv_slot_count != 0 && e.task_count != 0 && e.tiles == tiles && tiles <= kQueueMaxTiles &&
+           e.task_offset <= q.task_capacity && tiles <= q.task_capacity - e.task_offset &&
+           e.cost_offset <= q.cost_capacity &&
+           (static_cast<uint64_t>(e.task_count) + 1) / 2 <= q.cost_capacity - e.cost_offset;
+  }
   if (e.k < 3 || e.k > 5 || e.task_count == 0 || e.node_count == 0) return false;
   const KoptKLayout& L = a.layout[e.k - 3];
   return e.task_offset <= q.task_capacity && e.task_count <= q.task_capacity - e.task_offset &&
@@ -775,11 +785,54 @@ __device__ __forceinline__ void QueueTile(const KoptLaunch& a, const KoptPointer
   }
 }
 
+// One JV recheck tile: up to kQueueJvTileEdges edge slots, copied from the
+// task arena into the workgroup's staging row (a slot outside the union reads
+// slot 0: rows written by another agent), tested in pairs with JvKernel's
+// bodies into the row's witness half, then stored into the cost arena with
+// system-scope stores.  Those write through with byte enables: tiles of one
+// request, on different XCDs, share the cost arena's cache lines.
+__device__ void QueueJvTile(const KoptLaunch& a, const KoptQueueArgs& q, const KoptQueueEntry& e, uint32_t tile,
+                            jvbody::JvShared& jsh, int lane) {
+  const uint32_t first = tile * kQueueJvTileEdges;
+  const uint32_t n = e.task_count - first < kQueueJvTileEdges ? e.task_count - first : kQueueJvTileEdges;
+  int32_t* const stage = q.jv_stage + static_cast<uint64_t>(blockIdx.x) * (2 * kQueueJvTileEdges);
+  const int32_t* const src = reinterpret_cast<const int32_t*>(a.tasks + e.task_offset) + first;
+  if (static_cast<uint32_t>(lane) < n) {
+    const int32_t v = src[lane];
+    stage[lane] = static_cast<uint64_t>(static_cast<uint32_t>(v)) < q.jv_slot_count ? v : 0;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  JvLaunch g = q.jv;
+  g.edge_list = stage;
+  g.edge_count = static_cast<int32_t>(n);
+  g.witnesses = stage + kQueueJvTileEdges;
+  for (int32_t pair = 0; pair < static_cast<int32_t>(n + 1) / 2; ++pair) {
+    const uint64_t rest = __ballot(jvbody::PairOfEdges(g, jsh, 0, lane, pair));
+    for (int h = 0; h < 2; ++h) {  // uniform
+      if (((rest >> (jvbody::kHalf * h)) & 1ull) != 0) {
+        const int32_t item = 2 * pair + h;
+        const int32_t result = jvbody::OneEdge(g, jsh, 0, lane, item);
+        if (lane == 0) g.witnesses[item] = result;
+      }
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  int32_t* const out = reinterpret_cast<int32_t*>(a.costs + e.cost_offset) + first;
+  if (static_cast<uint32_t>(lane) < n) {
+    __hip_atomic_store(out + lane, g.witnesses[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
 // One wavefront per workgroup.  Workgroup 0 polls the host ring; the others
 // take a ticket index, wait for the ticket and run its tile with the batched
 // kernel's tile code.
 __global__ void __launch_bounds__(kWave) KoptQueueKernel(KoptLaunch a, KoptQueueArgs q) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
+  __shared__ jvbody::JvShared jsh;  // JV tiles use wavefront slot 0
   const int lane = static_cast<int>(threadIdx.x);
   const uint64_t start = __builtin_amdgcn_s_memrealtime();
   if (blockIdx.x == 0) {
@@ -840,7 +893,9 @@ __global__ void __launch_bounds__(kWave) KoptQueueKernel(KoptLaunch a, KoptQueue
       continue;
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // the host's task rows
-    if (e.k == 3) {
+    if (e.k == 0) {
+      QueueJvTile(a, q, e, tile, jsh, lane);
+    } else if (e.k == 3) {
       QueueTile<3>(a, bufs, e, tile, smem, wave, lane);
     } else if (e.k == 4) {
       QueueTile<4>(a, bufs, e, tile, smem, wave, lane);

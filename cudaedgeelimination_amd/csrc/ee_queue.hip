@@ -30,9 +30,13 @@
 #include <unordered_map>
 #include <vector>
 
+#include "ee_jvhost.h"
 #include "ee_runtime.h"
 
 namespace ee {
+
+void CheckGraphView(const ee_graph_view* g);
+
 namespace {
 
 using Clock = std::chrono::steady_clock;
@@ -128,20 +132,32 @@ class RingArena {
   uint64_t capacity_ = 0;
 };
 
-// A validated request, not yet published.
+// A validated request, not yet published: a leaf-cost query (k = 3..5) or
+// a JV recheck (k = 0).
 struct Prepared {
-  const ee_kopt_task* tasks;
-  uint64_t count, cells, cell_span, tiles;
-  int64_t* out;
-  uint32_t k;
-  KoptInstance inst;
+  uint32_t k = 0;
+  uint64_t count = 0;      // task rows, or edge slots
+  uint64_t rows = 0;       // task-arena rows the request holds
+  uint64_t cell_span = 0;  // cost-arena cells it holds
+  uint64_t cells = 0;      // leaf: output cells
+  uint64_t tiles = 0;
+  const ee_kopt_task* tasks = nullptr;  // leaf
+  int64_t* out = nullptr;               // leaf
+  KoptInstance inst{};                  // leaf
+  std::vector<int32_t> slots;           // JV: union edge slots
+  int32_t* witnesses = nullptr;         // JV
+  int32_t node_base = 0;                // JV: the graph's first union node
 };
 
 struct QRequest {
   uint64_t id = 0, entry = 0, slot = 0;
-  uint64_t task_at = 0, cost_at = 0, cells = 0, tiles = 0;
+  uint64_t task_at = 0, cost_at = 0, cells = 0, tiles = 0, span = 0;
   uint64_t marked = 0;  // tiles whose marks the completer has seen
   int64_t* out = nullptr;
+  bool jv = false;
+  uint64_t edges = 0;
+  int32_t* witnesses = nullptr;
+  int32_t node_base = 0;
   Clock::time_point submit, done;
   std::atomic<bool> complete{false};  // collected: cells in the caller's buffer
   int32_t status = EE_OK;
@@ -154,10 +170,20 @@ struct Queue {
   KoptTuning tuning;
   std::array<KoptPlan, 3> plans;
   DeviceArray<KoptPlan> d_plans;
-  std::mutex graphs_mu;  // instances, coords and uploads into the pool
+  std::mutex graphs_mu;  // instances, coords, the JV union and uploads into them
   std::vector<KoptInstance> instances;
   DeviceArray<longlong2> d_xy;
   uint64_t coords = 0;
+  // JV snapshots back to back (node ids, edge slots and CSR offset by graph),
+  // sized at creation; zero-filled, so every slot below the capacity reads
+  // in-bounds data
+  DeviceArray<int2> jv_uv;
+  DeviceArray<int64_t> jv_weight;
+  DeviceArray<uint8_t> jv_mode;
+  DeviceArray<longlong2> jv_xy;
+  DeviceArray<int32_t> jv_active, jv_row_offsets, jv_neighbors, jv_edge_ids, jv_stage;
+  uint64_t jv_nodes = 0, jv_slots = 0, jv_adjacency = 0;
+  std::vector<int32_t> jv_node_base, jv_edge_base, jv_edge_count;
   CoherentHost<KoptQueueHost> host;
   CoherentHost<KoptQueueEntry> ring;
   CoherentHost<uint64_t> flags;
@@ -207,7 +233,9 @@ struct Queue {
   // under mu
   void Launch();
   void EnsureRunning(bool stream_check);
+  void StopKernel();
   Prepared Prepare(uint32_t graph, uint32_t k, const ee_kopt_task* tasks, uint64_t count, int64_t* costs);
+  Prepared PrepareJv(uint32_t jv_graph, const int32_t* edges, uint64_t count, int32_t* witnesses);
   uint64_t Publish(const Prepared& p);
   void CompleteLoop();
   void Wait(const std::shared_ptr<QRequest>& r, double timeout_ms);
@@ -231,10 +259,32 @@ void Queue::Launch() {
   q.idle_ticks = idle_ticks;
   q.life_ticks = life_ticks;
   q.ring_size = cfg.ring_entries;
+  if (jv_uv.count() > 0) {
+    q.jv.edge_uv = jv_uv.get();
+    q.jv.edge_weight = jv_weight.get();
+    q.jv.edge_active = jv_active.get();
+    q.jv.row_offsets = jv_row_offsets.get();
+    q.jv.neighbors = jv_neighbors.get();
+    q.jv.csr_edge_ids = jv_edge_ids.get();
+    q.jv.xy = jv_xy.get();
+    q.jv.edge_mode = jv_mode.get();
+    q.jv_slot_count = jv_uv.count();
+    q.jv_stage = jv_stage.get();
+  }
   host.host()->stop = 0;
   q.epoch = ++epoch;
   LaunchKoptQueue(layout, q, cfg.workers, lds, stream);
   ++stats.launches;
+}
+
+// Ends the running launch (its poller publishes what the host has queued and
+// its tile waves serve it), so that data uploaded next is read by a new
+// dispatch; the next submit launches again.
+void Queue::StopKernel() {
+  if (epoch == 0) return;
+  host.host()->stop = 1;
+  std::atomic_thread_fence(std::memory_order_seq_cst);
+  CheckHip(hipStreamSynchronize(stream), "hipStreamSynchronize (queue kernel)");
 }
 
 // Launches the kernel when the latest launch has announced its end (or, with
@@ -268,12 +318,53 @@ Prepared Queue::Prepare(uint32_t graph, uint32_t k, const ee_kopt_task* tasks, u
   ValidateTasks(static_cast<int32_t>(inst.n), k, tasks, count);
   const uint32_t rows = tuning.rows_per_tile[k - 3];
   if ((count + rows - 1) / rows > kQueueMaxTiles) throw std::invalid_argument("request has too many tiles");
-  return Prepared{tasks, count, cells, cell_span, (count + rows - 1) / rows, costs, k, inst};
+  Prepared p;
+  p.k = k;
+  p.count = count;
+  p.rows = count;
+  p.cell_span = cell_span;
+  p.cells = cells;
+  p.tiles = (count + rows - 1) / rows;
+  p.tasks = tasks;
+  p.out = costs;
+  p.inst = inst;
+  return p;
+}
+
+Prepared Queue::PrepareJv(uint32_t jv_graph, const int32_t* edges, uint64_t count, int32_t* witnesses) {
+  if (count == 0 || edges == nullptr || witnesses == nullptr) {
+    throw std::invalid_argument("a JV recheck needs at least one edge slot and an output buffer");
+  }
+  Prepared p;
+  p.k = 0;
+  p.count = count;
+  p.tiles = (count + kQueueJvTileEdges - 1) / kQueueJvTileEdges;
+  p.rows = p.tiles;  // one mark per tile; 12 slots fit a row, the tile takes 4
+  p.cell_span = ((count + 1) / 2 + kSegmentAlignCells - 1) / kSegmentAlignCells * kSegmentAlignCells;
+  if (p.tiles > kQueueMaxTiles || p.rows > task_arena.count() || p.cell_span > cost_arena.count()) {
+    throw std::invalid_argument("JV recheck exceeds the queue's arenas");
+  }
+  int32_t eb = 0, m = 0;
+  {
+    std::lock_guard<std::mutex> lk(graphs_mu);
+    if (jv_graph >= jv_node_base.size()) throw std::invalid_argument("unknown JV graph id");
+    eb = jv_edge_base[jv_graph];
+    m = jv_edge_count[jv_graph];
+    p.node_base = jv_node_base[jv_graph];
+  }
+  p.slots.resize(count);
+  for (uint64_t i = 0; i < count; ++i) {
+    if (edges[i] < 0 || edges[i] >= m) throw std::invalid_argument("JV recheck edge slot out of range");
+    p.slots[i] = eb + edges[i];
+  }
+  p.witnesses = witnesses;
+  return p;
 }
 
 uint64_t Queue::Publish(const Prepared& p) {
   const uint64_t count = p.count, cell_span = p.cell_span, tiles = p.tiles;
   const KoptInstance& inst = p.inst;
+  const bool jv = p.k == 0;
   std::unique_lock<std::mutex> lk(mu);
   auto r = std::make_shared<QRequest>();
   for (;;) {
@@ -282,7 +373,7 @@ uint64_t Queue::Publish(const Prepared& p) {
     const uint64_t slot = tail % cfg.ring_entries;
     if (!by_slot[slot]) {
       uint64_t task_at = 0, cost_at = 0;
-      if (task_ring.Allocate(count, &task_at)) {
+      if (task_ring.Allocate(p.rows, &task_at)) {
         if (cost_ring.Allocate(cell_span, &cost_at)) {
           r->slot = slot;
           r->task_at = task_at;
@@ -295,11 +386,20 @@ uint64_t Queue::Publish(const Prepared& p) {
     // no room: the completer frees space as requests complete
     cv_room.wait_for(lk, std::chrono::milliseconds(1));
   }
-  std::memcpy(task_arena.host() + r->task_at, p.tasks, count * sizeof(ee_kopt_task));
+  if (jv) {
+    std::memcpy(reinterpret_cast<int32_t*>(task_arena.host() + r->task_at), p.slots.data(), count * sizeof(int32_t));
+  } else {
+    std::memcpy(task_arena.host() + r->task_at, p.tasks, count * sizeof(ee_kopt_task));
+  }
   r->entry = tail;
   r->tiles = tiles;
   r->cells = p.cells;
+  r->span = cell_span;
   r->out = p.out;
+  r->jv = jv;
+  r->edges = jv ? count : 0;
+  r->witnesses = p.witnesses;
+  r->node_base = p.node_base;
   r->id = next_id++;
   r->submit = Clock::now();
   slot_tiles[r->slot] += tiles;
@@ -307,20 +407,22 @@ uint64_t Queue::Publish(const Prepared& p) {
   e.seq = r->entry;
   e.task_offset = r->task_at;
   e.cost_offset = r->cost_at;
-  e.coord_base = inst.coord_base;
   e.done_target = slot_tiles[r->slot];
   e.task_count = static_cast<uint32_t>(count);
   e.tiles = static_cast<uint32_t>(tiles);
   e.k = p.k;
-  e.node_count = inst.n;
-  e.distance_type = inst.distance_type;
-  e.near = inst.near;
+  if (!jv) {
+    e.coord_base = inst.coord_base;
+    e.node_count = inst.n;
+    e.distance_type = inst.distance_type;
+    e.near = inst.near;
+  }
   ring.host()[r->slot] = e;
   by_slot[r->slot] = r;
   incoming.push_back(r);
   incoming_count.fetch_add(1, std::memory_order_release);
   live[r->id] = r;
-  inflight_cells += p.cells;
+  inflight_cells += cell_span;
   stats.peak_inflight_cells = std::max(stats.peak_inflight_cells, inflight_cells);
   // the entry and the task rows are in place before the tail moves past them
   __atomic_store_n(&host.host()->tail, tail + 1, __ATOMIC_RELEASE);
@@ -370,7 +472,13 @@ void Queue::CompleteLoop() {
       if (!Ready(r)) continue;
       // the blocks stay allocated until the request is released below
       if ((FlagOf(r) & kQueueRefused) == 0) {
-        std::memcpy(r.out, cost_arena.host() + r.cost_at, r.cells * sizeof(int64_t));
+        if (r.jv) {
+          // union node ids back to the graph's own ids
+          const int32_t* w = reinterpret_cast<const int32_t*>(cost_arena.host() + r.cost_at);
+          for (uint64_t j = 0; j < r.edges; ++j) r.witnesses[j] = w[j] >= 0 ? w[j] - r.node_base : -1;
+        } else {
+          std::memcpy(r.out, cost_arena.host() + r.cost_at, r.cells * sizeof(int64_t));
+        }
       }
       served.push_back(mine[i]);
     }
@@ -397,10 +505,11 @@ void Queue::CompleteLoop() {
         task_ring.Free(r.task_at);
         cost_ring.Free(r.cost_at);
         by_slot[r.slot].reset();
-        inflight_cells -= r.cells;
+        inflight_cells -= r.span;
         r.done = done_at;
         ++stats.requests;
         stats.cells += r.cells;
+        stats.jv_edges += r.edges;
         ++completed;
         r.complete.store(true, std::memory_order_release);
       }
@@ -486,7 +595,9 @@ ee_status ee_queue_create(ee_ctx* handle, const ee_queue_config* config, ee_queu
     const ee_queue_config& c = *config;
     if (c.task_rows < 1 || c.task_rows > (uint64_t{1} << 32) || c.cost_cells < 16 || c.max_nodes < 1 ||
         c.ring_entries < 1 || c.ring_entries > (1u << 20) || c.workers < 1 || c.workers > 65536 ||
-        !(c.idle_exit_ms > 0.0) || c.idle_exit_ms > 1000.0) {
+        !(c.idle_exit_ms > 0.0) || c.idle_exit_ms > 1000.0 || c.max_jv_edges >= (uint64_t{1} << 31) ||
+        c.max_jv_nodes >= (uint64_t{1} << 31) || c.max_jv_adjacency >= (uint64_t{1} << 31) ||
+        (c.max_jv_edges > 0 && (c.max_jv_nodes == 0 || c.max_jv_adjacency == 0))) {
       throw std::invalid_argument("bad queue configuration");
     }
     auto* ctx = reinterpret_cast<ee::Context*>(handle);
@@ -519,6 +630,29 @@ ee_status ee_queue_create(ee_ctx* handle, const ee_queue_config* config, ee_queu
     ee::CheckHip(hipMemsetAsync(q->mirror.get(), 0, q->mirror.bytes(), ctx->stream), "hipMemsetAsync");
     ee::CheckHip(hipMemsetAsync(q->done.get(), 0, q->done.bytes(), ctx->stream), "hipMemsetAsync");
     ee::CheckHip(hipMemsetAsync(q->tickets.get(), 0, q->tickets.bytes(), ctx->stream), "hipMemsetAsync");
+    if (c.max_jv_edges > 0) {
+      q->jv_uv.Allocate(c.max_jv_edges, dev);
+      q->jv_weight.Allocate(c.max_jv_edges, dev);
+      q->jv_mode.Allocate(c.max_jv_edges, dev);
+      q->jv_active.Allocate(c.max_jv_edges, dev);
+      q->jv_xy.Allocate(c.max_jv_nodes, dev);
+      q->jv_row_offsets.Allocate(c.max_jv_nodes + 1, dev);
+      q->jv_neighbors.Allocate(c.max_jv_adjacency, dev);
+      q->jv_edge_ids.Allocate(c.max_jv_adjacency, dev);
+      q->jv_stage.Allocate((static_cast<size_t>(c.workers) + 1) * 2 * ee::kQueueJvTileEdges, dev);
+      const auto zero = [&](void* ptr, size_t bytes) {
+        ee::CheckHip(hipMemsetAsync(ptr, 0, bytes, ctx->stream), "hipMemsetAsync");
+      };
+      zero(q->jv_uv.get(), q->jv_uv.bytes());
+      zero(q->jv_weight.get(), q->jv_weight.bytes());
+      zero(q->jv_mode.get(), q->jv_mode.bytes());
+      zero(q->jv_active.get(), q->jv_active.bytes());
+      zero(q->jv_xy.get(), q->jv_xy.bytes());
+      zero(q->jv_row_offsets.get(), q->jv_row_offsets.bytes());
+      zero(q->jv_neighbors.get(), q->jv_neighbors.bytes());
+      zero(q->jv_edge_ids.get(), q->jv_edge_ids.bytes());
+      zero(q->jv_stage.get(), q->jv_stage.bytes());
+    }
     ctx->Sync();
     q->slot_tiles.assign(c.ring_entries, 0);
     q->by_slot.resize(c.ring_entries);
@@ -565,17 +699,86 @@ ee_status ee_queue_add_graph(ee_queue* queue, const int64_t* x, const int64_t* y
     }
     std::vector<longlong2> xy(static_cast<size_t>(dimension));
     for (int32_t i = 0; i < dimension; ++i) xy[i] = longlong2{x[i], y[i]};
+    std::lock_guard<std::mutex> run(q.mu);
     std::lock_guard<std::mutex> lk(q.graphs_mu);
     if (static_cast<uint64_t>(dimension) > q.d_xy.count() - q.coords) {
       throw std::invalid_argument("graph exceeds the queue's coordinate pool (max_nodes)");
     }
-    // into unused pool space, on a stream of its own: the kernel keeps running
+    // into unused pool space; a running launch ends first (serving what is
+    // queued), so that only a new dispatch reads the new coordinates
     q.ctx->Activate();
+    q.StopKernel();
     q.d_xy.UploadAt(q.coords, xy.data(), xy.size(), q.copy_stream);
     ee::CheckHip(hipStreamSynchronize(q.copy_stream), "hipStreamSynchronize");
     q.instances.push_back(ee::MakeKoptInstance(q.coords, x, y, dimension, distance_type));
     q.coords += static_cast<uint64_t>(dimension);
     if (graph) *graph = static_cast<uint32_t>(q.instances.size() - 1);
+  });
+}
+
+ee_status ee_queue_add_jv_graph(ee_queue* queue, const ee_graph_view* graph, uint32_t* jv_graph) {
+  return Guarded([&] {
+    ee::Queue& q = ee::AsQueue(queue);
+    if (q.jv_uv.count() == 0) throw std::invalid_argument("the queue was created without JV capacity");
+    ee::CheckGraphView(graph);
+    ee::JvHost h;
+    h.Load(graph);  // validates and builds the CSR
+    std::lock_guard<std::mutex> run(q.mu);
+    std::lock_guard<std::mutex> lk(q.graphs_mu);
+    const uint64_t n = static_cast<uint64_t>(h.n), m = static_cast<uint64_t>(h.m), adj = h.neighbors.size();
+    if (n > q.jv_xy.count() - q.jv_nodes || m > q.jv_uv.count() - q.jv_slots ||
+        adj > q.jv_neighbors.count() - q.jv_adjacency) {
+      throw std::invalid_argument("JV graph exceeds the queue's JV capacity");
+    }
+    const int32_t nb = static_cast<int32_t>(q.jv_nodes), eb = static_cast<int32_t>(q.jv_slots);
+    const int32_t ab = static_cast<int32_t>(q.jv_adjacency);
+    const uint8_t md = static_cast<uint8_t>((h.type & 1u) | (h.near ? 2u : 0u));
+    std::vector<int2> uv(m);
+    std::vector<uint8_t> mode(m, md);
+    std::vector<int32_t> active(m), rows(n), nbrs(adj), eids(adj);
+    std::vector<longlong2> xy(n);
+    for (uint64_t e = 0; e < m; ++e) {
+      uv[e] = make_int2(nb + h.u[e], nb + h.v[e]);
+      active[e] = h.active[e] ? 1 : 0;
+    }
+    for (uint64_t v = 0; v < n; ++v) {
+      xy[v] = longlong2{h.x[v], h.y[v]};
+      rows[v] = ab + h.row_offsets[v + 1];
+    }
+    for (uint64_t j = 0; j < adj; ++j) {
+      nbrs[j] = nb + h.neighbors[j];
+      eids[j] = eb + h.edge_ids[j];
+    }
+    // into unused union space; a running launch ends first, so that only a
+    // new dispatch reads the new snapshot
+    q.ctx->Activate();
+    q.StopKernel();
+    const hipStream_t s = q.copy_stream;
+    q.jv_uv.UploadAt(q.jv_slots, uv.data(), m, s);
+    q.jv_weight.UploadAt(q.jv_slots, h.weight.data(), m, s);
+    q.jv_mode.UploadAt(q.jv_slots, mode.data(), m, s);
+    q.jv_active.UploadAt(q.jv_slots, active.data(), m, s);
+    q.jv_xy.UploadAt(q.jv_nodes, xy.data(), n, s);
+    q.jv_row_offsets.UploadAt(q.jv_nodes + 1, rows.data(), n, s);
+    q.jv_neighbors.UploadAt(q.jv_adjacency, nbrs.data(), adj, s);
+    q.jv_edge_ids.UploadAt(q.jv_adjacency, eids.data(), adj, s);
+    ee::CheckHip(hipStreamSynchronize(s), "hipStreamSynchronize");
+    q.jv_node_base.push_back(nb);
+    q.jv_edge_base.push_back(eb);
+    q.jv_edge_count.push_back(h.m);
+    q.jv_nodes += n;
+    q.jv_slots += m;
+    q.jv_adjacency += adj;
+    if (jv_graph) *jv_graph = static_cast<uint32_t>(q.jv_node_base.size() - 1);
+  });
+}
+
+ee_status ee_queue_submit_jv(ee_queue* queue, uint32_t jv_graph, const int32_t* edges, uint64_t count,
+                             int32_t* witnesses, uint64_t* request_id) {
+  return Guarded([&] {
+    ee::Queue& q = ee::AsQueue(queue);
+    const uint64_t id = q.Publish(q.PrepareJv(jv_graph, edges, count, witnesses));
+    if (request_id) *request_id = id;
   });
 }
 
@@ -598,9 +801,15 @@ ee_status ee_queue_submit_many(ee_queue* queue, uint32_t count, const ee_server_
     prepared.reserve(count);
     for (uint32_t i = 0; i < count; ++i) {
       const ee_server_request& r = requests[i];
-      if (r.kind != 0) throw std::invalid_argument("the device queue serves leaf-cost queries (kind 0)");
-      prepared.push_back(q.Prepare(r.graph, r.k, static_cast<const ee_kopt_task*>(r.input), r.count,
-                                   static_cast<int64_t*>(r.output)));
+      if (r.kind == 0) {
+        prepared.push_back(q.Prepare(r.graph, r.k, static_cast<const ee_kopt_task*>(r.input), r.count,
+                                     static_cast<int64_t*>(r.output)));
+      } else if (r.kind == 1) {
+        prepared.push_back(q.PrepareJv(r.graph, static_cast<const int32_t*>(r.input), r.count,
+                                       static_cast<int32_t*>(r.output)));
+      } else {
+        throw std::invalid_argument("request kind must be 0 (leaf costs) or 1 (JV recheck)");
+      }
     }
     for (uint32_t i = 0; i < count; ++i) request_ids[i] = q.Publish(prepared[i]);
   });

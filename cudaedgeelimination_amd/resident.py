@@ -1,6 +1,7 @@
 """Device-resident forms of the two hot paths: many k-opt segments per launch
 (``KoptBatch``), multi-epoch JV with the snapshot kept on the device
-(``JvSession``, ``JvFleet``), served queries (``KoptServer``) and the
+(``JvSession``, ``JvFleet``), served queries (``KoptServer``, and
+``DeviceQueue`` through a persistent kernel) and the
 certified leaf pipeline (``LeafPipeline``).  Thin owners over the C ABI in
 include/ee_hip.h."""
 from __future__ import annotations
@@ -413,18 +414,19 @@ class KoptServer:
 
 
 class DeviceQueue:
-    """Leaf-cost queries through the device work queue (ee_queue_* in
-    ee_hip.h): a persistent kernel polls a request ring in coherent host
-    memory and raises a completion flag per request; no launch or copy per
-    request.  Same submit / wait surface as KoptServer (leaf-cost queries
-    only)."""
+    """Leaf-cost queries and JV rechecks through the device work queue
+    (ee_queue_* in ee_hip.h): a persistent kernel polls a request ring in
+    coherent host memory and raises a completion flag per request; no launch
+    or copy per request.  Same submit / wait surface as KoptServer."""
 
     def __init__(self, ctx: Context, task_rows: int = 1 << 18, cost_cells: int = 1 << 23,
                  max_nodes: int = 1 << 20, ring_entries: int = 4096, workers: int = 256,
-                 idle_exit_ms: float = 20.0):
+                 idle_exit_ms: float = 20.0, max_jv_nodes: int = 1 << 17, max_jv_edges: int = 1 << 20,
+                 max_jv_adjacency: int = 1 << 21):
         cfg = _lib.QueueConfig()
         cfg.task_rows, cfg.cost_cells, cfg.max_nodes = task_rows, cost_cells, max_nodes
         cfg.ring_entries, cfg.workers, cfg.idle_exit_ms = ring_entries, workers, idle_exit_ms
+        cfg.max_jv_nodes, cfg.max_jv_edges, cfg.max_jv_adjacency = max_jv_nodes, max_jv_edges, max_jv_adjacency
         handle = C.c_void_p()
         _lib.check(_lib.lib().ee_queue_create(ctx.handle, C.byref(cfg), C.byref(handle)))
         self._h = handle
@@ -460,6 +462,25 @@ class DeviceQueue:
         self._buffers[rid.value] = (tasks, out)
         return rid.value
 
+    def add_jv_graph(self, graph: GraphSnapshot) -> int:
+        """Makes a JV snapshot resident for recheck requests."""
+        gid = C.c_uint32()
+        view = graph.view()
+        _lib.check(_lib.lib().ee_queue_add_jv_graph(self._h, C.byref(view), C.byref(gid)))
+        return gid.value
+
+    def submit_jv(self, jv_graph: int, edges: np.ndarray, out: np.ndarray | None = None) -> int:
+        """Queues a JV recheck of the given edge slots; wait() returns the
+        witness per slot (-1: not eliminable)."""
+        edges = np.ascontiguousarray(edges, dtype=np.int32)
+        if out is None:
+            out = np.empty(len(edges), np.int32)
+        rid = C.c_uint64()
+        _lib.check(_lib.lib().ee_queue_submit_jv(self._h, jv_graph, _lib.ptr(edges), len(edges),
+                                                 _lib.ptr(out), C.byref(rid)))
+        self._buffers[rid.value] = (edges, out)
+        return rid.value
+
     def wait(self, request_id: int, timeout_ms: float = -1.0):
         """(costs, result dict) of a completed request."""
         res = _lib.RequestResult()
@@ -471,14 +492,19 @@ class DeviceQueue:
                      "latency_ms": res.latency_ms}
 
     def submit_many(self, requests) -> list[int]:
-        """Queues (graph, k, task rows) requests in order; the inputs are
-        copied into the task arena, the outputs kept until wait_many()."""
+        """Queues requests in order: (graph, k, task rows) leaf-cost queries
+        or (jv_graph, 0, int32 edge slots) JV rechecks; the inputs are copied
+        into the task arena, the outputs kept until wait_many()."""
         descs = (_lib.ServerRequest * max(len(requests), 1))()
         keep = []
         for i, (graph, k, data) in enumerate(requests):
-            data = np.ascontiguousarray(data, dtype=_lib.KOPT_TASK_DTYPE)
-            out = np.empty(len(data) * TEMPLATES_PER_K[k], np.int64)
-            descs[i].kind, descs[i].graph, descs[i].k = 0, graph, k
+            if k:
+                data = np.ascontiguousarray(data, dtype=_lib.KOPT_TASK_DTYPE)
+                out = np.empty(len(data) * TEMPLATES_PER_K[k], np.int64)
+            else:
+                data = np.ascontiguousarray(data, dtype=np.int32)
+                out = np.empty(len(data), np.int32)
+            descs[i].kind, descs[i].graph, descs[i].k = (0 if k else 1), graph, k
             descs[i].input, descs[i].count, descs[i].output = _lib.ptr(data), len(data), _lib.ptr(out)
             keep.append((data, out))
         ids = np.zeros(max(len(requests), 1), np.uint64)
@@ -501,7 +527,7 @@ class DeviceQueue:
         st = _lib.QueueStats()
         _lib.check(_lib.lib().ee_queue_stats_get(self._h, C.byref(st)))
         return {"requests": st.requests, "cells": st.cells, "launches": st.launches,
-                "peak_inflight_cells": st.peak_inflight_cells}
+                "peak_inflight_cells": st.peak_inflight_cells, "jv_edges": st.jv_edges}
 
 
 @dataclasses.dataclass

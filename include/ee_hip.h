@@ -709,16 +709,17 @@ ee_status ee_server_wait_many(ee_server* server, uint32_t count, const uint64_t*
 ee_status ee_server_stats_get(const ee_server* server, ee_server_stats* stats);
 
 /* ------------------------------------------------- device work queue */
-/* No reference counterpart.  Leaf-cost queries served by a persistent
- * kernel: requests go into a ring in coherent host memory, the kernel
- * (polling that ring) writes each request's cells into a host arena and
- * raises its completion flag there; ee_queue_wait polls the flag and copies
- * the cells to the caller's buffer.  No launch or copy command is issued per
- * request.  The kernel ends after idle_exit_ms without requests and is
+/* No reference counterpart.  Leaf-cost queries and JV rechecks served by a
+ * persistent kernel: requests go into a ring in coherent host memory, the
+ * kernel (polling that ring) writes each request's cells or witnesses into a
+ * host arena and raises its completion flag there; a completing thread
+ * copies them to the caller's buffer.  No launch or copy command is issued
+ * per request.  The kernel ends after idle_exit_ms without requests and is
  * launched again by the next submit.  Cells are bit-identical to
- * ee_kopt_eval_costs with the canonical templates.  While the kernel runs,
- * calls that wait for the whole device (hipDeviceSynchronize, hipFree) wait
- * for it to end. */
+ * ee_kopt_eval_costs with the canonical templates, witnesses to
+ * ee_jv_find_candidates on the same snapshot.  While the kernel runs, calls
+ * that wait for the whole device (hipDeviceSynchronize, hipFree) wait for it
+ * to end. */
 typedef struct ee_queue ee_queue;
 
 typedef struct {
@@ -728,27 +729,39 @@ typedef struct {
   uint32_t ring_entries;  /* requests in flight */
   uint32_t workers;       /* tile wavefronts of the kernel */
   double idle_exit_ms;    /* the kernel ends after this long without a request (0, 1000] */
+  uint64_t max_jv_nodes;      /* JV snapshots: nodes, edge slots and adjacency entries */
+  uint64_t max_jv_edges;      /* of all JV graphs (all 0: no JV rechecks) */
+  uint64_t max_jv_adjacency;
 } ee_queue_config;
 
 typedef struct {
   uint64_t requests;
   uint64_t cells;
   uint64_t launches;            /* kernel launches (one per busy period) */
-  uint64_t peak_inflight_cells;
+  uint64_t peak_inflight_cells; /* cost-arena cells held by requests in flight */
+  uint64_t jv_edges;            /* edge slots rechecked */
 } ee_queue_stats;
 
 ee_status ee_queue_create(ee_ctx* ctx, const ee_queue_config* config, ee_queue** out);
 /* Serves every queued request, stops the kernel and frees everything. */
 void ee_queue_destroy(ee_queue* queue);
-/* Uploads a graph's coordinates into the pool (before or while serving). */
+/* Uploads a graph's coordinates into the pool (before or while serving; a
+ * running kernel first serves what is queued and ends). */
 ee_status ee_queue_add_graph(ee_queue* queue, const int64_t* x, const int64_t* y, int32_t dimension,
                              uint8_t distance_type, uint32_t* graph);
+/* Makes a JV snapshot resident for rechecks, like ee_server_add_jv_graph. */
+ee_status ee_queue_add_jv_graph(ee_queue* queue, const ee_graph_view* graph, uint32_t* jv_graph);
+/* Queues a JV recheck of edge slots edges[0..count) of a resident JV graph;
+ * witnesses[i] receives slot edges[i]'s witness (graph node id) or -1. */
+ee_status ee_queue_submit_jv(ee_queue* queue, uint32_t jv_graph, const int32_t* edges, uint64_t count,
+                             int32_t* witnesses, uint64_t* request_id);
 /* Validates and queues a request (the rows are copied); waits for room when
  * the arenas or the ring are full.  costs (count x templates(k) cells) must
  * stay valid until the request is collected. */
 ee_status ee_queue_submit(ee_queue* queue, uint32_t graph, uint32_t k, const ee_kopt_task* tasks,
                           uint64_t count, int64_t* costs, uint64_t* request_id);
-/* ee_queue_submit for each request in order (kind 0 only). */
+/* ee_queue_submit / ee_queue_submit_jv for each request in order; every
+ * request is validated first (on an error none is queued). */
 ee_status ee_queue_submit_many(ee_queue* queue, uint32_t count, const ee_server_request* requests,
                                uint64_t* request_ids);
 /* Blocks until the request's cells are in the caller's buffer (timeout_ms <

@@ -1,9 +1,10 @@
-"""Leaf-cost queries through the device work queue: a persistent kernel polls
-a request ring in coherent host memory and raises a completion flag per
-request.  Every request's cells equal the CPU oracle's and the plain
-per-query server's, alone and in the mix, with concurrent submitters, with
-arenas and a ring small enough that submits wait for room, and across the
-kernel's idle exit and relaunch."""
+"""Leaf-cost queries and JV rechecks through the device work queue: a
+persistent kernel polls a request ring in coherent host memory and raises a
+completion flag per request.  Every request's cells and witnesses equal the
+CPU oracle's and the batched server's, alone and in the mix, with concurrent
+submitters, with arenas and a ring small enough that submits wait for room,
+across the kernel's idle exit and relaunch, and with graphs added while
+serving."""
 from __future__ import annotations
 
 import sys
@@ -183,30 +184,105 @@ def test_queue_rejects_bad_requests(ctx, graphs_and_rows):
     queue.close()
 
 
+def test_queue_jv_rechecks_mixed_with_leaf_costs(ctx, graphs_and_rows):
+    """JV rechecks of edge-slot subsets on resident snapshots (EUC_2D and
+    CEIL_2D, some edges inactive), mixed with leaf-cost queries and submitted
+    one by one: every witness equals the oracle's on that snapshot, every
+    cost row the oracle's; odd counts end in a half-filled pair."""
+    say("queue: JV rechecks mixed with leaf costs")
+    graphs, rows = graphs_and_rows
+    ceil = W.config1(seed=0xEE0005C1, n=500)
+    ceil = W.knn_graph(ceil.x, ceil.y, 1, 12)
+    sparse = graphs[1].copy()
+    rng = np.random.default_rng(7)
+    sparse.edge_active[rng.random(sparse.edge_count) < 0.1] = 0
+    sparse.rebuild_csr()
+    jv_graphs = [graphs[0], ceil, sparse]
+    want = [O.OGraph.from_snapshot(g).witnesses() for g in jv_graphs]
+    queue = DeviceQueue(ctx)
+    kids = [queue.add_graph(g) for g in graphs]
+    jids = [queue.add_jv_graph(g) for g in jv_graphs]
+    submitted = []
+    kopt_reqs = make_requests(rows, 60, seed=5)
+    for i in range(120):
+        if i % 2 == 0:
+            j = int(rng.integers(len(jv_graphs)))
+            m = jv_graphs[j].edge_count
+            size = 1 if i == 0 else int(rng.integers(1, min(m, 700)))
+            edges = rng.choice(m, size=size, replace=False).astype(np.int32)
+            submitted.append(("jv", queue.submit_jv(jids[j], edges), j, edges))
+        else:
+            g, k, t = kopt_reqs[i // 2]
+            submitted.append(("kopt", queue.submit(kids[g], k, t), g, (k, t)))
+    total_edges = 0
+    for kind, rid, g, payload in submitted:
+        out, _ = queue.wait(rid, 60000)
+        if kind == "jv":
+            assert np.array_equal(out, want[g][payload]), (g, len(payload))
+            total_edges += len(payload)
+        else:
+            k, t = payload
+            assert np.array_equal(out, expected(graphs, g, k, t))
+    assert queue.stats()["jv_edges"] == total_edges
+    assert any((w >= 0).any() for w in want)
+    with pytest.raises(_lib.InvalidArgument):
+        queue.submit_jv(jids[0], np.array([jv_graphs[0].edge_count], np.int32))
+    with pytest.raises(_lib.InvalidArgument):
+        queue.submit_jv(len(jv_graphs) + 3, np.array([0], np.int32))
+    queue.close()
+
+
+def test_queue_jv_graph_added_while_serving(ctx, graphs_and_rows):
+    """A JV graph added while requests are in flight: the running launch
+    serves them and ends, the next submit launches again, and every result
+    (before and after) is the oracle's."""
+    graphs, rows = graphs_and_rows
+    queue = DeviceQueue(ctx)
+    g0 = queue.add_graph(graphs[0])
+    j0 = queue.add_jv_graph(graphs[0])
+    first = [queue.submit(g0, 4, rows[0][4][:30]), queue.submit_jv(j0, np.arange(50, dtype=np.int32))]
+    launches = queue.stats()["launches"]
+    j1 = queue.add_jv_graph(graphs[2])
+    edges = np.arange(graphs[2].edge_count - 77, graphs[2].edge_count, dtype=np.int32)
+    later = queue.submit_jv(j1, edges)
+    c, _ = queue.wait(first[0])
+    w0, _ = queue.wait(first[1])
+    w1, _ = queue.wait(later)
+    assert np.array_equal(c, expected(graphs, 0, 4, rows[0][4][:30]))
+    assert np.array_equal(w0, O.OGraph.from_snapshot(graphs[0]).witnesses()[:50])
+    assert np.array_equal(w1, O.OGraph.from_snapshot(graphs[2]).witnesses()[edges])
+    assert queue.stats()["launches"] > launches
+    queue.close()
+
+
 def test_queue_served_graphs_at_full_size(ctx):
-    """The served workload's leaf-cost queries at full size (graphs of 1k,
-    5k, 15k and 30k nodes; 128 to 65,728 cells) queued in one call: every
-    result equals the oracle's and the batched server's."""
+    """The served workload at full size (graphs of 1k, 5k, 15k and 30k nodes;
+    leaf-cost queries of 128 to 65,728 cells; JV rechecks of 16 to 4,096 edge
+    slots) queued in one call: every result equals the oracle's and the
+    batched server's."""
     say("queue at full size: graphs 1k/5k/15k/30k")
     graphs = W.config5_graphs()
     pools = W.config5_row_pools(graphs)
     rng = np.random.default_rng(0xEE0005F6)
     requests, _ = W.config5_requests(graphs, pools, 500, rng)
-    requests = [r for r in requests if r[1]]  # leaf-cost queries
     requests.append((3, 5, pools[3][5][:316]))
     requests.append((2, 4, pools[2][4][:2621]))
-    want = [expected(graphs, g, k, t) for g, k, t in requests]
+    assert any(k == 0 for _, k, _ in requests)
+    want_w = [O.OGraph.from_snapshot(g).witnesses() for g in graphs]
+    want = [want_w[g][t] if k == 0 else expected(graphs, g, k, t) for g, k, t in requests]
     queue = DeviceQueue(ctx)
     qids = [queue.add_graph(g) for g in graphs]
-    outs = [o for o, _ in queue.wait_many(queue.submit_many([(qids[g], k, t) for g, k, t in requests]),
-                                          120000)]
+    jids = [queue.add_jv_graph(g) for g in graphs]
+    outs = [o for o, _ in queue.wait_many(
+        queue.submit_many([(qids[g], k, t) if k else (jids[g], 0, t) for g, k, t in requests]), 120000)]
     bad = [i for i, (o, w) in enumerate(zip(outs, want)) if not np.array_equal(o, w)]
     assert not bad, f"queue: requests differ from the oracle: {bad[:10]}"
     queue.close()
     server = KoptServer(ctx, batching=True, max_batch_cells=32768)
     sids = [server.add_graph(g) for g in graphs]
-    souts = [o for o, _ in server.wait_many(server.submit_many([(sids[g], k, t) for g, k, t in requests]),
-                                            120000)]
+    sjids = [server.add_jv_graph(g) for g in graphs]
+    souts = [o for o, _ in server.wait_many(
+        server.submit_many([(sids[g], k, t) if k else (sjids[g], 0, t) for g, k, t in requests]), 120000)]
     server.close()
     for a, b in zip(outs, souts):
         assert np.array_equal(a, b)
