@@ -50,6 +50,9 @@ constexpr double kLivenessMs = 1.0;
 constexpr size_t kCollectWindow = 512;
 // how long the completing thread polls for a submit before it blocks
 constexpr double kIdleSpinMs = 5.0;
+// requests published per lock hold, and the threads copying their rows
+constexpr size_t kPublishRun = 1024;
+constexpr unsigned kPublishCopyThreads = 8;
 
 double Ms(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); }
 
@@ -236,7 +239,7 @@ struct Queue {
   void StopKernel();
   Prepared Prepare(uint32_t graph, uint32_t k, const ee_kopt_task* tasks, uint64_t count, int64_t* costs);
   Prepared PrepareJv(uint32_t jv_graph, const int32_t* edges, uint64_t count, int32_t* witnesses);
-  uint64_t Publish(const Prepared& p);
+  void Publish(const Prepared* ps, size_t n, uint64_t* ids, unsigned copy_threads = 1);
   void CompleteLoop();
   void Wait(const std::shared_ptr<QRequest>& r, double timeout_ms);
   void Drain();
@@ -361,76 +364,100 @@ Prepared Queue::PrepareJv(uint32_t jv_graph, const int32_t* edges, uint64_t coun
   return p;
 }
 
-uint64_t Queue::Publish(const Prepared& p) {
-  const uint64_t count = p.count, cell_span = p.cell_span, tiles = p.tiles;
-  const KoptInstance& inst = p.inst;
-  const bool jv = p.k == 0;
-  std::unique_lock<std::mutex> lk(mu);
-  auto r = std::make_shared<QRequest>();
-  for (;;) {
-    if (!broken.empty()) throw std::runtime_error(broken);
-    if (stopping) throw std::runtime_error("queue is stopping");
-    const uint64_t slot = tail % cfg.ring_entries;
-    if (!by_slot[slot]) {
-      uint64_t task_at = 0, cost_at = 0;
-      if (task_ring.Allocate(p.rows, &task_at)) {
-        if (cost_ring.Allocate(cell_span, &cost_at)) {
-          r->slot = slot;
-          r->task_at = task_at;
-          r->cost_at = cost_at;
+// Publishes ps[0..n) in order: as many per lock hold as the ring and the
+// arenas take (waiting for room otherwise), their rows copied into the task
+// arena on up to `copy_threads` threads, then one tail store for the run.
+void Queue::Publish(const Prepared* ps, size_t n, uint64_t* ids, unsigned copy_threads) {
+  size_t done = 0;
+  std::vector<std::shared_ptr<QRequest>> run;
+  while (done < n) {
+    std::unique_lock<std::mutex> lk(mu);
+    run.clear();
+    for (;;) {
+      if (!broken.empty()) throw std::runtime_error(broken);
+      if (stopping) throw std::runtime_error("queue is stopping");
+      while (done + run.size() < n && run.size() < kPublishRun) {
+        const Prepared& p = ps[done + run.size()];
+        const uint64_t slot = (tail + run.size()) % cfg.ring_entries;
+        if (by_slot[slot]) break;
+        uint64_t task_at = 0, cost_at = 0;
+        if (!task_ring.Allocate(p.rows, &task_at)) break;
+        if (!cost_ring.Allocate(p.cell_span, &cost_at)) {
+          task_ring.Free(task_at);
           break;
         }
-        task_ring.Free(task_at);
+        auto r = std::make_shared<QRequest>();
+        r->slot = slot;
+        r->task_at = task_at;
+        r->cost_at = cost_at;
+        by_slot[slot] = r;  // holds the slot; published below
+        run.push_back(std::move(r));
       }
+      if (!run.empty()) break;
+      // no room: the completer frees space as requests complete
+      cv_room.wait_for(lk, std::chrono::milliseconds(1));
     }
-    // no room: the completer frees space as requests complete
-    cv_room.wait_for(lk, std::chrono::milliseconds(1));
+    const auto copy = [&](size_t j) {
+      const Prepared& p = ps[done + j];
+      if (p.k == 0) {
+        std::memcpy(reinterpret_cast<int32_t*>(task_arena.host() + run[j]->task_at), p.slots.data(),
+                    p.count * sizeof(int32_t));
+      } else {
+        std::memcpy(task_arena.host() + run[j]->task_at, p.tasks, p.count * sizeof(ee_kopt_task));
+      }
+    };
+    if (run.size() > 1 && copy_threads > 1) {
+      ParallelFor(run.size(), copy, copy_threads);
+    } else {
+      for (size_t j = 0; j < run.size(); ++j) copy(j);
+    }
+    const auto now = Clock::now();
+    for (size_t j = 0; j < run.size(); ++j) {
+      const Prepared& p = ps[done + j];
+      QRequest& r = *run[j];
+      const bool jv = p.k == 0;
+      r.entry = tail + j;
+      r.tiles = p.tiles;
+      r.cells = p.cells;
+      r.span = p.cell_span;
+      r.out = p.out;
+      r.jv = jv;
+      r.edges = jv ? p.count : 0;
+      r.witnesses = p.witnesses;
+      r.node_base = p.node_base;
+      r.id = next_id++;
+      r.submit = now;
+      slot_tiles[r.slot] += p.tiles;
+      KoptQueueEntry e{};
+      e.seq = r.entry;
+      e.task_offset = r.task_at;
+      e.cost_offset = r.cost_at;
+      e.done_target = slot_tiles[r.slot];
+      e.task_count = static_cast<uint32_t>(p.count);
+      e.tiles = static_cast<uint32_t>(p.tiles);
+      e.k = p.k;
+      if (!jv) {
+        e.coord_base = p.inst.coord_base;
+        e.node_count = p.inst.n;
+        e.distance_type = p.inst.distance_type;
+        e.near = p.inst.near;
+      }
+      ring.host()[r.slot] = e;
+      incoming.push_back(run[j]);
+      live[r.id] = run[j];
+      inflight_cells += p.cell_span;
+      if (ids) ids[done + j] = r.id;
+    }
+    stats.peak_inflight_cells = std::max(stats.peak_inflight_cells, inflight_cells);
+    incoming_count.fetch_add(run.size(), std::memory_order_release);
+    // the entries and their task rows are in place before the tail moves past them
+    __atomic_store_n(&host.host()->tail, tail + run.size(), __ATOMIC_RELEASE);
+    tail += run.size();
+    EnsureRunning(false);
+    lk.unlock();
+    cv_work.notify_one();
+    done += run.size();
   }
-  if (jv) {
-    std::memcpy(reinterpret_cast<int32_t*>(task_arena.host() + r->task_at), p.slots.data(), count * sizeof(int32_t));
-  } else {
-    std::memcpy(task_arena.host() + r->task_at, p.tasks, count * sizeof(ee_kopt_task));
-  }
-  r->entry = tail;
-  r->tiles = tiles;
-  r->cells = p.cells;
-  r->span = cell_span;
-  r->out = p.out;
-  r->jv = jv;
-  r->edges = jv ? count : 0;
-  r->witnesses = p.witnesses;
-  r->node_base = p.node_base;
-  r->id = next_id++;
-  r->submit = Clock::now();
-  slot_tiles[r->slot] += tiles;
-  KoptQueueEntry e{};
-  e.seq = r->entry;
-  e.task_offset = r->task_at;
-  e.cost_offset = r->cost_at;
-  e.done_target = slot_tiles[r->slot];
-  e.task_count = static_cast<uint32_t>(count);
-  e.tiles = static_cast<uint32_t>(tiles);
-  e.k = p.k;
-  if (!jv) {
-    e.coord_base = inst.coord_base;
-    e.node_count = inst.n;
-    e.distance_type = inst.distance_type;
-    e.near = inst.near;
-  }
-  ring.host()[r->slot] = e;
-  by_slot[r->slot] = r;
-  incoming.push_back(r);
-  incoming_count.fetch_add(1, std::memory_order_release);
-  live[r->id] = r;
-  inflight_cells += cell_span;
-  stats.peak_inflight_cells = std::max(stats.peak_inflight_cells, inflight_cells);
-  // the entry and the task rows are in place before the tail moves past them
-  __atomic_store_n(&host.host()->tail, tail + 1, __ATOMIC_RELEASE);
-  ++tail;
-  EnsureRunning(false);
-  lk.unlock();
-  cv_work.notify_one();
-  return r->id;
 }
 
 // The completing thread: while requests are in flight it polls their flags
@@ -777,7 +804,9 @@ ee_status ee_queue_submit_jv(ee_queue* queue, uint32_t jv_graph, const int32_t* 
                              int32_t* witnesses, uint64_t* request_id) {
   return Guarded([&] {
     ee::Queue& q = ee::AsQueue(queue);
-    const uint64_t id = q.Publish(q.PrepareJv(jv_graph, edges, count, witnesses));
+    const ee::Prepared p = q.PrepareJv(jv_graph, edges, count, witnesses);
+    uint64_t id = 0;
+    q.Publish(&p, 1, &id);
     if (request_id) *request_id = id;
   });
 }
@@ -786,7 +815,9 @@ ee_status ee_queue_submit(ee_queue* queue, uint32_t graph, uint32_t k, const ee_
                           uint64_t count, int64_t* costs, uint64_t* request_id) {
   return Guarded([&] {
     ee::Queue& q = ee::AsQueue(queue);
-    const uint64_t id = q.Publish(q.Prepare(graph, k, tasks, count, costs));
+    const ee::Prepared p = q.Prepare(graph, k, tasks, count, costs);
+    uint64_t id = 0;
+    q.Publish(&p, 1, &id);
     if (request_id) *request_id = id;
   });
 }
@@ -796,22 +827,21 @@ ee_status ee_queue_submit_many(ee_queue* queue, uint32_t count, const ee_server_
   return Guarded([&] {
     ee::Queue& q = ee::AsQueue(queue);
     if (count > 0 && (requests == nullptr || request_ids == nullptr)) throw std::invalid_argument("NULL argument");
-    // every request is validated before any is queued
-    std::vector<ee::Prepared> prepared;
-    prepared.reserve(count);
-    for (uint32_t i = 0; i < count; ++i) {
+    // every request is validated (spread over the host cores) before any is queued
+    std::vector<ee::Prepared> prepared(count);
+    ee::ParallelFor(count, [&](size_t i) {
       const ee_server_request& r = requests[i];
       if (r.kind == 0) {
-        prepared.push_back(q.Prepare(r.graph, r.k, static_cast<const ee_kopt_task*>(r.input), r.count,
-                                     static_cast<int64_t*>(r.output)));
+        prepared[i] = q.Prepare(r.graph, r.k, static_cast<const ee_kopt_task*>(r.input), r.count,
+                                static_cast<int64_t*>(r.output));
       } else if (r.kind == 1) {
-        prepared.push_back(q.PrepareJv(r.graph, static_cast<const int32_t*>(r.input), r.count,
-                                       static_cast<int32_t*>(r.output)));
+        prepared[i] = q.PrepareJv(r.graph, static_cast<const int32_t*>(r.input), r.count,
+                                  static_cast<int32_t*>(r.output));
       } else {
         throw std::invalid_argument("request kind must be 0 (leaf costs) or 1 (JV recheck)");
       }
-    }
-    for (uint32_t i = 0; i < count; ++i) request_ids[i] = q.Publish(prepared[i]);
+    });
+    q.Publish(prepared.data(), prepared.size(), request_ids, ee::kPublishCopyThreads);
   });
 }
 
