@@ -281,6 +281,7 @@ PROTOTYPES = {
     "ee_queue_wait": (C.c_int, [P, U64, DBL, C.POINTER(RequestResult)]),
     "ee_queue_wait_many": (C.c_int, [P, U32, P, DBL, P]),
     "ee_queue_stats_get": (C.c_int, [P, C.POINTER(QueueStats)]),
+    "ee_queue_inject_refusal": (C.c_int, [P, U32, P, P, C.POINTER(U64)]),
     "ee_jv_find_candidates": (C.c_int, [P, C.POINTER(GraphView), P, C.POINTER(JvUsage)]),
     "ee_jv_clear_cache": (C.c_int, [P]),
     "ee_kopt_build_templates": (C.c_int, [U32, P, U32, C.POINTER(U32), C.POINTER(U64)]),

@@ -150,6 +150,7 @@ struct Prepared {
   std::vector<int32_t> slots;           // JV: union edge slots
   int32_t* witnesses = nullptr;         // JV
   int32_t node_base = 0;                // JV: the graph's first union node
+  bool corrupt = false;                 // test hook: the entry's task offset past the arena
 };
 
 struct QRequest {
@@ -442,6 +443,7 @@ void Queue::Publish(const Prepared* ps, size_t n, uint64_t* ids, unsigned copy_t
         e.distance_type = p.inst.distance_type;
         e.near = p.inst.near;
       }
+      if (p.corrupt) e.task_offset = task_arena.count();
       ring.host()[r.slot] = e;
       incoming.push_back(run[j]);
       live[r.id] = run[j];
@@ -805,6 +807,18 @@ ee_status ee_queue_submit_jv(ee_queue* queue, uint32_t jv_graph, const int32_t* 
   return Guarded([&] {
     ee::Queue& q = ee::AsQueue(queue);
     const ee::Prepared p = q.PrepareJv(jv_graph, edges, count, witnesses);
+    uint64_t id = 0;
+    q.Publish(&p, 1, &id);
+    if (request_id) *request_id = id;
+  });
+}
+
+ee_status ee_queue_inject_refusal(ee_queue* queue, uint32_t graph, const ee_kopt_task* task, int64_t* costs,
+                                  uint64_t* request_id) {
+  return Guarded([&] {
+    ee::Queue& q = ee::AsQueue(queue);
+    ee::Prepared p = q.Prepare(graph, 3, task, 1, costs);
+    p.corrupt = true;
     uint64_t id = 0;
     q.Publish(&p, 1, &id);
     if (request_id) *request_id = id;

@@ -523,6 +523,18 @@ class DeviceQueue:
         _lib.check(status)
         return [(outs[i], res[i].latency_ms) for i in range(len(ids))]
 
+    def inject_refusal(self, graph: int, task: np.ndarray) -> int:
+        """Test hook: queues a one-row k = 3 request whose ring entry points
+        past the task arena; the kernel refuses it, wait() raises and the
+        queue refuses later submits."""
+        task = np.ascontiguousarray(task[:1], dtype=_lib.KOPT_TASK_DTYPE)
+        out = np.empty(TEMPLATES_PER_K[3], np.int64)
+        rid = C.c_uint64()
+        _lib.check(_lib.lib().ee_queue_inject_refusal(self._h, graph, _lib.ptr(task), _lib.ptr(out),
+                                                      C.byref(rid)))
+        self._buffers[rid.value] = (task, out)
+        return rid.value
+
     def stats(self) -> dict:
         st = _lib.QueueStats()
         _lib.check(_lib.lib().ee_queue_stats_get(self._h, C.byref(st)))

@@ -770,6 +770,13 @@ ee_status ee_queue_wait(ee_queue* queue, uint64_t request_id, double timeout_ms,
 ee_status ee_queue_wait_many(ee_queue* queue, uint32_t count, const uint64_t* request_ids, double timeout_ms,
                              ee_request_result* results);
 ee_status ee_queue_stats_get(const ee_queue* queue, ee_queue_stats* stats);
+/* No reference counterpart; a test hook for the fail-closed path.  Queues
+ * one k = 3 leaf-cost request (one valid task row) whose ring entry points
+ * past the task arena.  The kernel's bounds check refuses the entry before
+ * any tile of it runs; waiting for the request then reports EE_ERR_RUNTIME,
+ * and the queue refuses every later submit. */
+ee_status ee_queue_inject_refusal(ee_queue* queue, uint32_t graph, const ee_kopt_task* task, int64_t* costs,
+                                  uint64_t* request_id);
 
 #ifdef __cplusplus
 }

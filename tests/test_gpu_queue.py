@@ -184,6 +184,24 @@ def test_queue_rejects_bad_requests(ctx, graphs_and_rows):
     queue.close()
 
 
+def test_queue_refused_entry_fails_closed(ctx, graphs_and_rows):
+    """An entry pointing past the task arena (forced from the host) is
+    refused by the kernel's bounds check: its wait raises, requests served
+    before it are intact, and the queue takes no more work."""
+    graphs, rows = graphs_and_rows
+    queue = DeviceQueue(ctx)
+    g = queue.add_graph(graphs[0])
+    t = rows[0][3][:20]
+    costs, _ = queue.wait(queue.submit(g, 3, t), 60000)
+    assert np.array_equal(costs, expected(graphs, 0, 3, t))
+    bad = queue.inject_refusal(g, rows[0][3][:1])
+    with pytest.raises(_lib.EeError, match="refused"):
+        queue.wait(bad, 60000)
+    with pytest.raises(_lib.EeError, match="refused"):
+        queue.submit(g, 3, t)
+    queue.close()
+
+
 def test_queue_jv_rechecks_mixed_with_leaf_costs(ctx, graphs_and_rows):
     """JV rechecks of edge-slot subsets on resident snapshots (EUC_2D and
     CEIL_2D, some edges inactive), mixed with leaf-cost queries and submitted
