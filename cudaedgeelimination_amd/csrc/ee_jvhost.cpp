@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <exception>
 #include <mutex>
 #include <thread>
@@ -35,37 +36,123 @@ std::string Hex(uint64_t h) {
 
 }  // namespace
 
+namespace {
+
+unsigned AffinityCores() {
+  cpu_set_t set;
+  CPU_ZERO(&set);
+  const unsigned n = sched_getaffinity(0, sizeof(set), &set) == 0 ? static_cast<unsigned>(CPU_COUNT(&set)) : 1u;
+  return std::max(1u, std::min(n, 32u));
+}
+
+// Worker threads kept for the life of the process.  One job runs at a time;
+// the caller takes part in it.  Between jobs a worker polls for the next one
+// for a short while (yielding), then blocks.
+class ForPool {
+ public:
+  static ForPool& Get() {
+    static ForPool* pool = new ForPool(AffinityCores() - 1);  // never destroyed: its threads outlive exit
+    return *pool;
+  }
+  // False when another job holds the pool (the caller then runs serially).
+  bool Run(size_t n, const std::function<void(size_t)>& body, unsigned threads) {
+    std::unique_lock<std::mutex> job(job_mu_, std::try_to_lock);
+    if (!job.owns_lock()) return false;
+    const unsigned helpers = std::min<unsigned>(static_cast<unsigned>(workers_), threads - 1);
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      body_ = &body;
+      n_ = n;
+      next_.store(0, std::memory_order_relaxed);
+      first_ = nullptr;
+      wanted_ = helpers;
+      joined_ = 0;
+      active_ = 0;
+      accepting_ = true;
+      generation_.fetch_add(1, std::memory_order_release);
+    }
+    cv_.notify_all();
+    Work();
+    // every index is claimed: no later helper joins; wait for those working
+    std::unique_lock<std::mutex> lk(mu_);
+    accepting_ = false;
+    done_cv_.wait(lk, [&] { return active_ == 0; });
+    body_ = nullptr;
+    if (first_) std::rethrow_exception(first_);
+    return true;
+  }
+  static bool InWorker() { return in_worker_; }
+
+ private:
+  explicit ForPool(unsigned workers) : workers_(workers) {
+    for (unsigned w = 0; w < workers; ++w) std::thread([this] { Loop(); }).detach();
+  }
+  void Work() {
+    for (size_t i = next_.fetch_add(1); i < n_; i = next_.fetch_add(1)) {
+      try {
+        (*body_)(i);
+      } catch (...) {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (!first_) first_ = std::current_exception();
+      }
+    }
+  }
+  void Loop() {
+    in_worker_ = true;
+    uint64_t seen = generation_.load(std::memory_order_acquire);
+    for (;;) {
+      // poll a while for the next job (back-to-back epochs), then block
+      const auto until = std::chrono::steady_clock::now() + std::chrono::microseconds(kPollMicros);
+      while (generation_.load(std::memory_order_acquire) == seen && std::chrono::steady_clock::now() < until) {
+        std::this_thread::yield();
+      }
+      std::unique_lock<std::mutex> lk(mu_);
+      cv_.wait(lk, [&] { return generation_.load(std::memory_order_relaxed) != seen; });
+      seen = generation_.load(std::memory_order_relaxed);
+      if (!accepting_ || joined_ >= wanted_) continue;  // the job is done or has its helpers
+      ++joined_;
+      ++active_;
+      lk.unlock();
+      Work();
+      lk.lock();
+      if (--active_ == 0) done_cv_.notify_one();
+    }
+  }
+
+  static constexpr int kPollMicros = 100;
+  static thread_local bool in_worker_;
+  const size_t workers_;
+  std::mutex job_mu_;  // one job at a time
+  std::mutex mu_;
+  std::condition_variable cv_, done_cv_;
+  std::atomic<uint64_t> generation_{0};
+  const std::function<void(size_t)>* body_ = nullptr;
+  size_t n_ = 0;
+  std::atomic<size_t> next_{0};
+  unsigned wanted_ = 0, joined_ = 0, active_ = 0;
+  bool accepting_ = false;
+  std::exception_ptr first_;
+};
+thread_local bool ForPool::in_worker_ = false;
+
+}  // namespace
+
 void ParallelFor(size_t n, const std::function<void(size_t)>& body, unsigned threads) {
-  if (threads == 0) {
-    cpu_set_t set;
-    CPU_ZERO(&set);
-    threads = sched_getaffinity(0, sizeof(set), &set) == 0 ? static_cast<unsigned>(CPU_COUNT(&set)) : 1u;
-    threads = std::min(threads, 32u);
-  }
+  if (threads == 0) threads = AffinityCores();
   const size_t workers = std::min<size_t>(std::max(threads, 1u), n);
-  if (workers <= 1) {
-    for (size_t i = 0; i < n; ++i) body(i);
-    return;
-  }
-  std::atomic<size_t> next{0};
-  std::mutex mu;
-  std::exception_ptr first;
-  auto work = [&] {
-    for (size_t i = next.fetch_add(1); i < n; i = next.fetch_add(1)) {
+  // serially: one worker's worth, a call from inside a pool job, or a pool
+  // busy with another caller's job
+  if (workers <= 1 || ForPool::InWorker() || !ForPool::Get().Run(n, body, static_cast<unsigned>(workers))) {
+    std::exception_ptr first;
+    for (size_t i = 0; i < n; ++i) {
       try {
         body(i);
       } catch (...) {
-        std::lock_guard<std::mutex> lk(mu);
         if (!first) first = std::current_exception();
       }
     }
-  };
-  std::vector<std::thread> pool;
-  pool.reserve(workers - 1);
-  for (size_t t = 1; t < workers; ++t) pool.emplace_back(work);
-  work();
-  for (std::thread& t : pool) t.join();
-  if (first) std::rethrow_exception(first);
+    if (first) std::rethrow_exception(first);
+  }
 }
 
 void JvHost::Load(const ee_graph_view* g) {
