@@ -206,8 +206,50 @@ void JvHost::RebuildCsr() {
 
 // Removes the entries of edges that are no longer active, row by row.  Rows
 // keep their order, so the result equals RebuildCsr on the new flags (rows
-// are filled in edge-slot order there too).
+// are filled in edge-slot order there too).  Large CSRs are filtered on the
+// host cores in blocks of rows: each block counts what it keeps, the block
+// offsets follow from a prefix sum, and each block writes its rows into
+// fresh arrays, in the same order as one pass over all rows.
 void JvHost::DropInactiveFromCsr() {
+  constexpr size_t kParallelEntries = 65536, kBlocks = 64;
+  const size_t entries = static_cast<size_t>(row_offsets[static_cast<size_t>(n)]);
+  if (entries >= kParallelEntries && n >= static_cast<int32_t>(kBlocks)) {
+    const size_t rows_per = (static_cast<size_t>(n) + kBlocks - 1) / kBlocks;
+    std::vector<size_t> kept(kBlocks + 1, 0);
+    ParallelFor(kBlocks, [&](size_t b) {
+      const size_t r0 = std::min(static_cast<size_t>(n), b * rows_per);
+      const size_t r1 = std::min(static_cast<size_t>(n), r0 + rows_per);
+      size_t k = 0;
+      for (size_t s = static_cast<size_t>(row_offsets[r0]); s < static_cast<size_t>(row_offsets[r1]); ++s) {
+        k += active[static_cast<size_t>(edge_ids[s])] != 0;
+      }
+      kept[b + 1] = k;
+    });
+    for (size_t b = 0; b < kBlocks; ++b) kept[b + 1] += kept[b];
+    next_rows_.resize(static_cast<size_t>(n) + 1);
+    next_neighbors_.resize(kept[kBlocks]);
+    next_edge_ids_.resize(kept[kBlocks]);
+    ParallelFor(kBlocks, [&](size_t b) {
+      const size_t r0 = std::min(static_cast<size_t>(n), b * rows_per);
+      const size_t r1 = std::min(static_cast<size_t>(n), r0 + rows_per);
+      size_t out = kept[b];
+      for (size_t vtx = r0; vtx < r1; ++vtx) {
+        next_rows_[vtx] = static_cast<int32_t>(out);
+        for (size_t s = static_cast<size_t>(row_offsets[vtx]); s < static_cast<size_t>(row_offsets[vtx + 1]); ++s) {
+          if (active[static_cast<size_t>(edge_ids[s])] != 0) {
+            next_neighbors_[out] = neighbors[s];
+            next_edge_ids_[out] = edge_ids[s];
+            ++out;
+          }
+        }
+      }
+    });
+    next_rows_[static_cast<size_t>(n)] = static_cast<int32_t>(kept[kBlocks]);
+    row_offsets.swap(next_rows_);
+    neighbors.swap(next_neighbors_);
+    edge_ids.swap(next_edge_ids_);
+    return;
+  }
   int32_t out = 0, start = row_offsets[0];
   for (int32_t vtx = 0; vtx < n; ++vtx) {
     const int32_t end = row_offsets[static_cast<size_t>(vtx) + 1];
@@ -273,6 +315,8 @@ void JvHost::StartRun() {
   for (SnapHash& h : snaps_) {
     if (h.job.valid()) h.Get();
   }
+  live_edges_ = 0;
+  for (int32_t e = 0; e < m; ++e) live_edges_ += active[e] != 0;
   records_.clear();
   metrics_.clear();
   snaps_.clear();
@@ -284,7 +328,7 @@ void JvHost::StartRun() {
 void JvHost::BeginEpoch(uint32_t epoch) {
   cur_ = Metrics{};
   cur_.epoch = epoch;
-  for (int32_t e = 0; e < m; ++e) cur_.before += active[e] != 0;
+  cur_.before = live_edges_;
   snap_generation_ = generation_;
   snaps_.push_back(CurrentSnap());
   gated_.clear();
@@ -375,6 +419,7 @@ size_t JvHost::ApplyCommit() {
   }
   if (!gated_.empty()) {
     for (const auto& c : gated_) active[c.first] = 0;
+    live_edges_ -= gated_.size();  // each was verified active
     ++generation_;
     DropInactiveFromCsr();
   }

@@ -108,6 +108,9 @@ struct JvHost {
 
   std::vector<Record> records_;
   std::vector<Metrics> metrics_;
+  size_t live_edges_ = 0;  // active edge slots, kept through the run's commits
+  // the next CSR of a parallel DropInactiveFromCsr (swapped in; capacity kept)
+  std::vector<int32_t> next_rows_, next_neighbors_, next_edge_ids_;
   std::vector<SnapHash> snaps_;  // one per epoch of the run
   Metrics cur_{};
   std::vector<std::pair<int32_t, int32_t>> gated_;  // (edge, witness) of the pending commit
