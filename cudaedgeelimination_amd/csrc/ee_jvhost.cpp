@@ -348,7 +348,7 @@ const std::vector<int32_t>& JvHost::VerifyAndGate(const int32_t* witnesses, doub
 
   // host re-verification on the same snapshot
   auto t0 = std::chrono::steady_clock::now();
-  constexpr size_t kVerifyBlock = 256, kParallelVerify = 2048;
+  constexpr size_t kVerifyBlock = 64, kParallelVerify = 512;
   if (verify_threads != 1 && proposed.size() >= kParallelVerify) {
     std::atomic<size_t> rejected{0};
     const size_t blocks = (proposed.size() + kVerifyBlock - 1) / kVerifyBlock;
