@@ -69,18 +69,26 @@ constexpr int64_t kNearExtent = int64_t{1} << 25;
 // whose root floors to 0 as well.
 EE_HD int64_t ExactDistanceNear(int32_t dx, int32_t dy, int type) {
   const double fx = static_cast<double>(dx), fy = static_cast<double>(dy);
-  const double s = fma(fx, fx, fy * fy);
 #if defined(__HIP_DEVICE_COMPILE__)
+  const double s = fma(fx, fx, fy * fy);
   const double s4 = fmax(s, 0.25);  // rsq(0) is +inf
   const double y = __builtin_amdgcn_rsq(s4);
   const double g = s4 * y, h = 0.5 * y;
   const double r = floor(fma(g, fma(-h, g, 0.5), g));
-#else
-  const double r = floor(sqrt(s));
-#endif
   const double rem = fma(-r, r, s);
+  const double twice_r1 = fma(2.0, r, 1.0);
+#else
+  // Every value is an integer below 2^53 (squares below 2^50, s below 2^51,
+  // r * r below 2^52), so plain products and sums are exact and give what
+  // the fused forms give; without FMA code generation for the host, fma()
+  // would be a library call per use.
+  const double s = fx * fx + fy * fy;
+  const double r = floor(sqrt(s));
+  const double rem = s - r * r;
+  const double twice_r1 = 2.0 * r + 1.0;
+#endif
   const uint32_t up = type == 0 ? static_cast<uint32_t>(rem > r)
-                                : static_cast<uint32_t>(rem > 0.0) + static_cast<uint32_t>(rem > fma(2.0, r, 1.0));
+                                : static_cast<uint32_t>(rem > 0.0) + static_cast<uint32_t>(rem > twice_r1);
   return static_cast<int64_t>(static_cast<uint32_t>(r) + up);
 }
 

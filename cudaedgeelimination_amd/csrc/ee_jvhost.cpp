@@ -192,7 +192,12 @@ void JvHost::Load(const ee_graph_view* g) {
   RebuildCsr();
 }
 
-int64_t JvHost::Dist(int32_t a, int32_t b) const { return ExactDistance(x[a], y[a], x[b], y[b], type); }
+// Coordinate extents below 2^25 take the FP64 form (the same distance,
+// without the integer root's correction loops).
+int64_t JvHost::Dist(int32_t a, int32_t b) const {
+  if (near) return ExactDistanceNear(static_cast<int32_t>(x[a] - x[b]), static_cast<int32_t>(y[a] - y[b]), type);
+  return ExactDistance(x[a], y[a], x[b], y[b], type);
+}
 
 void JvHost::RebuildCsr() {
   row_offsets.assign(static_cast<size_t>(n) + 1, 0);

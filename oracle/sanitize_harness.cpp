@@ -12,14 +12,19 @@
 //    matrix, and a planted cell difference found at its index;
 //  - a JV host run (re-verification, commit rule, CSR rebuild, proof text,
 //    snapshot hashes on helper threads) fed each epoch with the oracle's
-//    witnesses, whose proof equals the oracle's own run.
+//    witnesses, whose proof equals the oracle's own run;
+//  - the host's two exact distance forms (the integer root, and the FP64
+//    form for extents below 2^25 that re-verification and certification
+//    use) against the oracle's restoring square root.
 #include <cstdint>
 #include <cstring>
 #include <iostream>
+#include <random>
 #include <sstream>
 #include <string>
 #include <vector>
 
+#include "ee_exact.h"
 #include "ee_gen.h"
 #include "ee_hip.h"
 #include "ee_jvhost.h"
@@ -190,9 +195,60 @@ void KoptChecks(const Graph& g) {
   }
 }
 
+// Both host distance forms against the oracle, both rounding types: random
+// differences, squares and their neighbours, roots just below an integer
+// (s = r^2 - 1 for large r), right triangles, and the extremes.
+void DistanceChecks() {
+  const int64_t lim = (int64_t{1} << 25) - 1;
+  uint64_t bad = 0;
+  const auto check = [&](int64_t dx, int64_t dy) {
+    for (int t = 0; t < 2; ++t) {
+      int64_t want = 0;
+      if (orc_exact_distance(0, 0, dx, dy, t, &want) != 0) {
+        ++bad;
+        continue;
+      }
+      const int64_t exact = ee::ExactDistance(3, -5, 3 + dx, -5 + dy, t);
+      const int64_t near = ee::ExactDistanceNear(static_cast<int32_t>(dx), static_cast<int32_t>(dy), t);
+      if ((exact != want || near != want) && bad++ < 5) {
+        std::cout << "distance dx=" << dx << " dy=" << dy << " type=" << t << ": oracle " << want
+                  << ", integer form " << exact << ", FP64 form " << near << '\n';
+      }
+    }
+  };
+  std::mt19937_64 rng(0xEE5D);
+  std::uniform_int_distribution<int64_t> u(-lim, lim);
+  for (int i = 0; i < 200000; ++i) check(u(rng), u(rng));
+  for (int64_t r = lim; r > lim - 2000; --r) {
+    check(r, 0);
+    check(r, 1);   // s = r^2 + 1
+    check(r - 1, 0);
+  }
+  for (int64_t r = 2; r < 3000; ++r) {
+    for (int64_t d = -2; d <= 2; ++d) check(r, d);  // s within 4 of r^2
+  }
+  // EUC_2D's half-way points: s - m^2 = m (m = q^2, dy = q; not rounded up)
+  // and s - m^2 = m + 1 (m = q^2 - 1, dy = q; rounded up)
+  for (int64_t q = 1; q * q <= lim; ++q) {
+    check(q * q, q);
+    check(q * q - 1, q);
+  }
+  for (int64_t k = 1; 5 * k <= lim; k += 4099) {
+    check(3 * k, 4 * k);
+    check(3 * k, 4 * k - 1);
+    check(3 * k + 1, 4 * k);
+  }
+  check(0, 0);
+  check(lim, lim);
+  check(-lim, lim);
+  check(lim, -lim);
+  Expect(bad == 0, "host distance forms equal the oracle's");
+}
+
 }  // namespace
 
 int main() {
+  DistanceChecks();
   // uniform points, EUC_2D, 10-NN
   {
     std::vector<int64_t> x(600), y(600);
