@@ -276,10 +276,10 @@ uint32_t CertifyRow(const CertifyPlan& plan, const int64_t* dist, bool plain, co
 }  // namespace
 
 CertifyResult CertifyCells(HostPool& pool, const CertifyPlan& plan, const int64_t* x, const int64_t* y,
-                           int32_t n, uint8_t distance_type, const ee_kopt_task* tasks, size_t rows,
+                           int32_t n, uint8_t distance_type, bool near, const ee_kopt_task* tasks, size_t rows,
                            const int64_t* cells, uint32_t* improving, const std::function<void()>* side) {
+  static_cast<void>(n);
   const uint32_t k = plan.k, ports = 2 * k, T = plan.templates;
-  const bool near = NearExtent(x, y, n);
   const uint32_t mask_words = (T + 31) / 32;
   const size_t chunks = (rows + kChunkRows - 1) / kChunkRows;
   std::atomic<uint64_t> mismatch{UINT64_MAX};
@@ -378,7 +378,8 @@ extern "C" ee_status ee_kopt_certify(const int64_t* x, const int64_t* y, int32_t
     static std::mutex serial;  // one certification at a time on the shared pool
     std::lock_guard<std::mutex> lk(serial);
     const ee::CertifyResult r =
-        ee::CertifyCells(ee::SharedPool(), plan, x, y, dimension, distance_type, tasks, task_count, cells, improving);
+        ee::CertifyCells(ee::SharedPool(), plan, x, y, dimension, distance_type, ee::NearExtent(x, y, dimension),
+                         tasks, task_count, cells, improving);
     if (mismatch) *mismatch = r.mismatch;
     if (improving_count) *improving_count = r.improving;
     if (r.mismatch != UINT64_MAX) {

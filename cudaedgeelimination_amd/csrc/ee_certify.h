@@ -76,13 +76,15 @@ struct CertifyResult {
 
 // Certifies rows [0, rows) of `cells` (rows x plan.templates, row-major)
 // against the exact CPU matrix of `tasks` on the n coordinates x, y (distance
-// safe; the rows pass ValidateKOptCostTask).  improving (may be null) receives
+// safe; the rows pass ValidateKOptCostTask); near is NearExtent(x, y, n),
+// which callers that certify many blocks of one graph compute once.
+// improving (may be null) receives
 // per row ceil(templates / 32) words, bit j set iff the row's certified cell
 // j is below its deleted_cost.  The words of rows from the first mismatching
 // row on are unspecified.  side (may be null) runs once on one of the pool's
 // threads alongside the rows; an exception it throws is rethrown.
 CertifyResult CertifyCells(HostPool& pool, const CertifyPlan& plan, const int64_t* x, const int64_t* y,
-                           int32_t n, uint8_t distance_type, const ee_kopt_task* tasks, size_t rows,
+                           int32_t n, uint8_t distance_type, bool near, const ee_kopt_task* tasks, size_t rows,
                            const int64_t* cells, uint32_t* improving,
                            const std::function<void()>* side = nullptr);
 

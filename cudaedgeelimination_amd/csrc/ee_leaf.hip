@@ -392,7 +392,7 @@ struct Leaf {
       if (inject_cell < static_cast<uint64_t>(s.rows) * plan.templates) s.h_costs.get()[inject_cell] = inject_value;
     }
     const std::function<void()> rows_check = [&] { CheckCursorRows(s); };
-    const CertifyResult r = CertifyCells(pool, plan, x.data(), y.data(), n, type, s.h_tasks.get(), s.rows,
+    const CertifyResult r = CertifyCells(pool, plan, x.data(), y.data(), n, type, near, s.h_tasks.get(), s.rows,
                                          s.h_costs.get(), s.improving.data(), s.cursor ? &rows_check : nullptr);
     stats.certify_ms += MsSince(t0);
     if (r.mismatch != UINT64_MAX) {
