@@ -416,12 +416,7 @@ void HamiltonReplies(Context& ctx, const ee_graph_view* graph, int32_t target_u,
 
   t0 = std::chrono::steady_clock::now();
   const uint64_t copy = std::min(*total, capacity);
-  if (copy > 0) {
-    CheckHip(hipMemcpyAsync(replies, c.d_replies.get(), copy * sizeof(ee_ht_neighbor_pair),
-                            hipMemcpyDeviceToHost, ctx.stream),
-             "hipMemcpyAsync HT replies");
-    ctx.Sync();
-  }
+  if (copy > 0) DownloadToCaller(ctx, replies, c.d_replies.get(), copy * sizeof(ee_ht_neighbor_pair));
   u.d2h_ms = MsSince(t0);
   u.resident_bytes = HtResidentBytes(c);
   if (usage) *usage = u;
@@ -438,10 +433,7 @@ void CopyReplies(Context& ctx, uint64_t first, uint64_t count, void* out) {
   if (out == nullptr) throw std::invalid_argument("HT copy replies: NULL buffer");
   ctx.Activate();
   const size_t record = c.last_kind == 1 ? sizeof(ee_ht_neighbor_pair) : sizeof(ee_node_edge);
-  CheckHip(hipMemcpyAsync(out, reinterpret_cast<const char*>(c.d_replies.get()) + first * record,
-                          count * record, hipMemcpyDeviceToHost, ctx.stream),
-           "hipMemcpyAsync HT replies");
-  ctx.Sync();
+  DownloadToCaller(ctx, out, reinterpret_cast<const char*>(c.d_replies.get()) + first * record, count * record);
 }
 
 void EndReplies(Context& ctx, const ee_graph_view* graph, const ee_ht_end_task* tasks, size_t count,
@@ -495,12 +487,7 @@ void EndReplies(Context& ctx, const ee_graph_view* graph, const ee_ht_end_task* 
   c.last_total = *total;
   t0 = std::chrono::steady_clock::now();
   const uint64_t copy = std::min(*total, capacity);
-  if (copy > 0) {
-    CheckHip(hipMemcpyAsync(replies, c.d_replies.get(), copy * sizeof(ee_node_edge),
-                            hipMemcpyDeviceToHost, ctx.stream),
-             "hipMemcpyAsync HT end replies");
-    ctx.Sync();
-  }
+  if (copy > 0) DownloadToCaller(ctx, replies, c.d_replies.get(), copy * sizeof(ee_node_edge));
   u.d2h_ms = MsSince(t0);
   u.resident_bytes = HtResidentBytes(c);
   if (usage) *usage = u;

@@ -258,7 +258,18 @@ struct Context {
   KoptCache kopt;
   KoptCursorCache cursor;
   HtCache ht;
+  // DownloadToCaller's two page-locked chunks and their events, made on
+  // first use
+  Pinned<char> stage[2];
+  hipEvent_t stage_ready[2] = {nullptr, nullptr};
 };
+
+// Copies `bytes` of device memory to caller memory and waits, after the work
+// queued before it on the context's stream.  A large copy goes through the
+// context's two page-locked chunks: while one chunk is in flight over PCIe
+// the other is copied out on host threads.  A copy straight into pageable
+// caller memory is staged by the runtime at about half that rate.
+void DownloadToCaller(Context& ctx, void* dst, const void* src, size_t bytes);
 
 // The deletion-set producer of ee_cursor.hip: the number of sets of size k
 // the reference cursor enumerates over `selectable_count` positions, the
