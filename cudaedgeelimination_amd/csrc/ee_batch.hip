@@ -703,8 +703,7 @@ ee_status ee_kopt_batch_download(ee_kopt_batch* batch, uint64_t first_cell, uint
     }
     if (cell_count > 0 && costs == nullptr) throw std::invalid_argument("costs is NULL");
     b.ctx->Activate();
-    b.d_costs.DownloadAt(first_cell, costs, cell_count, b.ctx->stream);
-    b.ctx->Sync();
+    ee::DownloadRange(*b.ctx, b.d_costs, first_cell, cell_count, costs);
   });
 }
 
@@ -721,14 +720,13 @@ ee_status ee_kopt_batch_download_pool(ee_kopt_batch* batch, uint32_t pool, uint6
     if (cell_count > 0 && costs == nullptr) throw std::invalid_argument("costs is NULL");
     b.ctx->Activate();
     if (pool == 0) {
-      b.d_costs.DownloadAt(first_cell, costs, cell_count, b.ctx->stream);
+      ee::DownloadRange(*b.ctx, b.d_costs, first_cell, cell_count, costs);
     } else {
       ee::Batch::PipeLane& l = b.lanes[pool - 1];
       // the lane's launches ran on its own stream; wait for them first
       ee::CheckHip(hipStreamSynchronize(l.stream), "hipStreamSynchronize");
-      l.costs.DownloadAt(first_cell, costs, cell_count, b.ctx->stream);
+      ee::DownloadRange(*b.ctx, l.costs, first_cell, cell_count, costs);
     }
-    b.ctx->Sync();
   });
 }
 
@@ -764,8 +762,7 @@ ee_status ee_kopt_batch_download_masks(ee_kopt_batch* batch, uint64_t first_row,
     if (row_count > 0 && masks == nullptr) throw std::invalid_argument("masks is NULL");
     const uint64_t w = b.MaskWords();
     b.ctx->Activate();
-    b.d_masks.DownloadAt(first_row * w, masks, row_count * w, b.ctx->stream);
-    b.ctx->Sync();
+    ee::DownloadRange(*b.ctx, b.d_masks, first_row * w, row_count * w, masks);
   });
 }
 

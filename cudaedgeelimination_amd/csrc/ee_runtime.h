@@ -271,6 +271,15 @@ struct Context {
 // caller memory is staged by the runtime at about half that rate.
 void DownloadToCaller(Context& ctx, void* dst, const void* src, size_t bytes);
 
+// Elements [offset, offset + count) of a device array into caller memory, as
+// DownloadToCaller.
+template <typename T>
+void DownloadRange(Context& ctx, const DeviceArray<T>& a, size_t offset, size_t count, T* host) {
+  if (count == 0) return;
+  if (offset > a.count() || count > a.count() - offset) throw std::logic_error("D2H beyond device buffer");
+  DownloadToCaller(ctx, host, a.get() + offset, count * sizeof(T));
+}
+
 // The deletion-set producer of ee_cursor.hip: the number of sets of size k
 // the reference cursor enumerates over `selectable_count` positions, the
 // checks of a tour view, and the row-building launch (sets [first, first +
