@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstring>
 #include <exception>
 #include <limits>
@@ -30,6 +31,14 @@
 namespace ee {
 
 // ------------------------------------------------------------- HostPool
+namespace {
+using PoolClock = std::chrono::steady_clock;
+// how long an idle worker, and a caller waiting for the last iterations,
+// poll before blocking: the pipeline certifies a block every few hundred
+// microseconds, and a wake-up through the condition variable costs tens
+constexpr auto kPoolSpin = std::chrono::microseconds(100);
+}  // namespace
+
 HostPool::HostPool(unsigned threads) {
   if (threads == 0) {
     cpu_set_t set;
@@ -42,16 +51,23 @@ HostPool::HostPool(unsigned threads) {
     workers_.emplace_back([this] {
       uint64_t seen = 0;
       for (;;) {
+        const auto idle = PoolClock::now();
+        while (generation_.load(std::memory_order_acquire) == seen && !stop_.load(std::memory_order_acquire) &&
+               PoolClock::now() - idle < kPoolSpin) {
+          std::this_thread::yield();
+        }
+        std::shared_ptr<Job> job;
         {
           std::unique_lock<std::mutex> lk(mu_);
-          wake_.wait(lk, [&] { return stop_ || generation_ != seen; });
-          if (stop_) return;
-          seen = generation_;
-          ++busy_;
+          wake_.wait(lk, [&] { return stop_.load() || generation_.load() != seen; });
+          if (stop_.load()) return;
+          seen = generation_.load();
+          job = job_;
+          busy_.fetch_add(1);
         }
-        Drain();
+        Drain(*job);
         std::lock_guard<std::mutex> lk(mu_);
-        if (--busy_ == 0) done_.notify_all();
+        if (busy_.fetch_sub(1) == 1) done_.notify_all();
       }
     });
   }
@@ -60,25 +76,21 @@ HostPool::HostPool(unsigned threads) {
 HostPool::~HostPool() {
   {
     std::lock_guard<std::mutex> lk(mu_);
-    stop_ = true;
+    stop_.store(true);
   }
   wake_.notify_all();
   for (std::thread& t : workers_) t.join();
 }
 
-void HostPool::Drain() {
+void HostPool::Drain(Job& job) {
   for (;;) {
-    size_t i;
-    {
-      std::lock_guard<std::mutex> lk(mu_);
-      if (next_ >= n_) return;
-      i = next_++;
-    }
+    const size_t i = job.next.fetch_add(1, std::memory_order_relaxed);
+    if (i >= job.n) return;
     try {
-      (*body_)(i);
+      (*job.body)(i);
     } catch (...) {
       std::lock_guard<std::mutex> lk(mu_);
-      if (!error_) error_ = std::current_exception();
+      if (!job.error) job.error = std::current_exception();
     }
   }
 }
@@ -86,29 +98,43 @@ void HostPool::Drain() {
 void HostPool::Run(size_t n, const std::function<void(size_t)>& body) {
   if (n == 0) return;
   if (workers_.empty() || n == 1) {
-    for (size_t i = 0; i < n; ++i) body(i);
+    // every iteration runs, as on the workers; the first exception follows
+    std::exception_ptr error;
+    for (size_t i = 0; i < n; ++i) {
+      try {
+        body(i);
+      } catch (...) {
+        if (!error) error = std::current_exception();
+      }
+    }
+    if (error) std::rethrow_exception(error);
     return;
   }
+  auto job = std::make_shared<Job>();
+  job->body = &body;
+  job->n = n;
   {
     std::lock_guard<std::mutex> lk(mu_);
-    body_ = &body;
-    n_ = n;
-    next_ = 0;
-    error_ = nullptr;
-    ++busy_;  // the caller
-    ++generation_;
+    job_ = job;
+    busy_.fetch_add(1);  // the caller
+    generation_.fetch_add(1);
   }
   wake_.notify_all();
-  Drain();
+  Drain(*job);
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    busy_.fetch_sub(1);
+  }
+  // every iteration has been claimed; wait for the workers still running one
+  const auto start = PoolClock::now();
+  while (busy_.load(std::memory_order_acquire) != 0 && PoolClock::now() - start < kPoolSpin) {
+    std::this_thread::yield();
+  }
   std::exception_ptr error;
   {
     std::unique_lock<std::mutex> lk(mu_);
-    --busy_;
-    // every iteration has been claimed; wait for the workers still running one
-    done_.wait(lk, [&] { return busy_ == 0; });
-    body_ = nullptr;
-    error = error_;
-    error_ = nullptr;
+    done_.wait(lk, [&] { return busy_.load() == 0; });
+    error = job->error;
   }
   if (error) std::rethrow_exception(error);
 }

@@ -10,9 +10,11 @@
 // compares it with the device cell, and sets the row's improving-template bit.
 #pragma once
 
+#include <atomic>
 #include <condition_variable>
 #include <cstdint>
 #include <functional>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -37,19 +39,23 @@ class HostPool {
   void Run(size_t n, const std::function<void(size_t)>& body);
 
  private:
-  void Work(uint64_t generation);
-  void Drain();
+  // One Run: its iterations are claimed from `next`; a worker holds the job
+  // it registered for, so a late worker never claims another Run's index.
+  struct Job {
+    const std::function<void(size_t)>* body = nullptr;
+    size_t n = 0;
+    std::atomic<size_t> next{0};
+    std::exception_ptr error;  // under the pool's mutex
+  };
+  void Drain(Job& job);
 
   std::vector<std::thread> workers_;
   std::mutex mu_;
   std::condition_variable wake_, done_;
-  uint64_t generation_ = 0;
-  bool stop_ = false;
-  const std::function<void(size_t)>* body_ = nullptr;
-  size_t n_ = 0;
-  size_t next_ = 0;
-  size_t busy_ = 0;
-  std::exception_ptr error_;
+  std::atomic<uint64_t> generation_{0};  // written under mu_
+  std::atomic<bool> stop_{false};        // written under mu_
+  std::shared_ptr<Job> job_;             // the latest Run's; under mu_
+  std::atomic<size_t> busy_{0};          // written under mu_
 };
 
 // A reconnect table as the CPU scorer walks it: per template, its k added
