@@ -15,15 +15,22 @@
 //    witnesses, whose proof equals the oracle's own run;
 //  - the host's two exact distance forms (the integer root, and the FP64
 //    form for extents below 2^25 that re-verification and certification
-//    use) against the oracle's restoring square root.
+//    use) against the oracle's restoring square root;
+//  - the certification pool: every iteration exactly once and the first
+//    exception rethrown after all of them, for pools of 1 to 16 threads,
+//    two of them driven at once.
+#include <atomic>
 #include <cstdint>
 #include <cstring>
 #include <iostream>
 #include <random>
 #include <sstream>
+#include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
+#include "ee_certify.h"
 #include "ee_exact.h"
 #include "ee_gen.h"
 #include "ee_hip.h"
@@ -245,10 +252,42 @@ void DistanceChecks() {
   Expect(bad == 0, "host distance forms equal the oracle's");
 }
 
+void PoolChecks() {
+  std::atomic<int> bad{0};
+  const auto drive = [&](unsigned threads, unsigned seed, int runs) {
+    ee::HostPool pool(threads);
+    std::mt19937 rng(seed);
+    for (int r = 0; r < runs; ++r) {
+      const size_t n = rng() % 200;
+      std::vector<std::atomic<int>> hits(n);
+      for (auto& h : hits) h.store(0);
+      const bool plant = n > 0 && rng() % 5 == 0;
+      const size_t at = plant ? rng() % n : 0;
+      bool caught = false;
+      try {
+        pool.Run(n, [&](size_t i) {
+          hits[i].fetch_add(1);
+          if (plant && i == at) throw std::runtime_error("planted");
+        });
+      } catch (const std::runtime_error&) {
+        caught = true;
+      }
+      for (size_t i = 0; i < n; ++i) bad += hits[i].load() != 1;
+      bad += caught != plant;
+    }
+  };
+  std::thread other(drive, 4u, 0xEE61u, 300);
+  drive(16, 0xEE62, 300);
+  other.join();
+  drive(1, 0xEE63, 100);
+  Expect(bad.load() == 0, "certification pool runs every iteration once and rethrows");
+}
+
 }  // namespace
 
 int main() {
   DistanceChecks();
+  PoolChecks();
   // uniform points, EUC_2D, 10-NN
   {
     std::vector<int64_t> x(600), y(600);
