@@ -183,7 +183,7 @@ CertifyPlan MakeCertifyPlan(uint32_t k, const ee_endpoint_matching* templates, u
 namespace {
 
 constexpr int64_t kInvalidCell = std::numeric_limits<int64_t>::max();
-constexpr uint32_t kChunkRows = 32;
+constexpr uint32_t kChunkRows = 64;
 constexpr uint32_t kPatternSlots = 64;
 constexpr uint32_t kMaxWords = (945 + 63) / 64;
 
